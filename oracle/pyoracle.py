@@ -1,0 +1,130 @@
+"""ctypes view of the CPU oracle ``libcm1oracle.so`` (test infrastructure only).
+
+Only tests, ``__graft_entry__.smoke()`` and the bench's ``cpu_baseline`` leg
+use this module; the product never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcm1oracle.so")
+
+FIELDS = ["ua", "u3d", "uten", "uten1", "va", "v3d", "vten", "vten1",
+          "wa", "w3d", "wten", "wten1", "rru", "rrv", "rrw",
+          "ppi", "pp3d", "ppten", "sten", "sadv", "ppx",
+          "tha", "th3d", "thten", "thten1", "thterm",
+          "qpten", "qtten", "qvten", "qcten",
+          "rho", "rr", "rf", "prs", "divx",
+          "dum1", "dum2", "dum3", "dum4", "dum5", "dum6", "dum7", "dum8", "dum9",
+          "t11", "t12", "t13", "t22", "t23", "t33",
+          "qa0", "qa1", "qa2", "q3d0", "q3d1", "q3d2", "qten0", "qten1", "qten2"]
+FID = {n: i for i, n in enumerate(FIELDS)}
+F_PTA0 = len(FIELDS)
+for _n in range(8):
+    FID[f"pta{_n}"] = F_PTA0 + _n
+    FID[f"pt3d{_n}"] = F_PTA0 + 8 + _n
+    FID[f"ptten{_n}"] = F_PTA0 + 16 + _n
+KFIELDS = ["mh", "rmh", "mf", "rmf", "c1", "c2", "zh", "zf", "pi0", "th0", "prs0",
+           "qv0", "rho0", "rr0", "rf0", "rrf0", "thv0", "rth0", "qc0", "qi0", "u0", "v0"]
+KID = {n: i for i, n in enumerate(KFIELDS)}
+D2 = {"rain": 0, "prate": 1, "radbcw": 2, "radbce": 3, "radbcs": 4, "radbcn": 5}
+
+
+class Config(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "ni", "nj", "nk", "wbc", "ebc", "sbc", "nbc", "imoist", "ptype", "psolver",
+        "advwenos", "advwenov", "apmasscon", "irbc", "nsound", "npt", "pdtra")] + \
+        [(n, C.c_float) for n in ("dx", "dy", "dz", "dt", "alph", "kdiv", "csmax")]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} not built (run make -C oracle)")
+    lib = C.CDLL(LIB_PATH)
+    vp, fp = C.c_void_p, C.POINTER(C.c_float)
+    lib.cm1o_create.restype = vp
+    lib.cm1o_create.argtypes = [C.POINTER(Config)]
+    lib.cm1o_destroy.argtypes = [vp]
+    for name in ("cm1o_field", "cm1o_kfield", "cm1o_2dfield", "cm1o_hfield"):
+        getattr(lib, name).restype = fp
+        getattr(lib, name).argtypes = [vp, C.c_int]
+    lib.cm1o_set_extrap.argtypes = [vp, fp]
+    lib.cm1o_dims.argtypes = [vp, C.POINTER(C.c_int64)] * 1 + [C.POINTER(C.c_int64)] * 2
+    lib.cm1o_scalars.restype = C.POINTER(C.c_double)
+    lib.cm1o_scalars.argtypes = [vp]
+    lib.cm1o_base_derive.argtypes = [vp]
+    lib.cm1o_getset.argtypes = [vp, C.c_int]
+    lib.cm1o_step.argtypes = [vp]
+    lib.cm1o_bc.argtypes = [vp, C.c_int, C.c_int]
+    lib.cm1o_advs.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+    for name in ("cm1o_advu", "cm1o_advv", "cm1o_advw"):
+        getattr(lib, name).argtypes = [vp, C.c_int]
+    lib.cm1o_sound.argtypes = [vp, C.c_int, C.c_float]
+    lib.cm1o_kessler_branch.argtypes = [vp]
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = _load()
+    return _LIB
+
+
+def make_config(**kw):
+    d = dict(ni=64, nj=64, nk=40, wbc=2, ebc=2, sbc=2, nbc=2, imoist=1, ptype=1,
+             psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4, nsound=4, npt=0,
+             pdtra=1, dx=1000.0, dy=1000.0, dz=500.0, dt=6.0, alph=0.6, kdiv=0.1,
+             csmax=350.0)
+    d.update(kw)
+    return Config(**d)
+
+
+class Oracle:
+    """One oracle model; arrays are numpy views shaped (k, j, i) with
+    index offsets i0=2, j0=2 (i=-2 at index 0), k0=0."""
+
+    def __init__(self, cfg):
+        self.L = lib()
+        self.cfg = cfg
+        self.h = self.L.cm1o_create(C.byref(cfg))
+        px, py, pz = C.c_int64(), C.c_int64(), C.c_int64()
+        self.L.cm1o_dims(self.h, C.byref(px), C.byref(py), C.byref(pz))
+        self.px, self.py, self.pz = px.value, py.value, pz.value
+
+    def __del__(self):
+        try:
+            self.L.cm1o_destroy(self.h)
+        except Exception:
+            pass
+
+    def f(self, name):
+        p = self.L.cm1o_field(self.h, FID[name])
+        return np.ctypeslib.as_array(p, shape=(self.pz, self.py, self.px))
+
+    def kf(self, name):
+        p = self.L.cm1o_kfield(self.h, KID[name])
+        return np.ctypeslib.as_array(p, shape=(self.pz + 1,))
+
+    def d2(self, name):
+        p = self.L.cm1o_2dfield(self.h, D2[name])
+        return np.ctypeslib.as_array(p, shape=(self.py, self.px))
+
+    def scalars(self):
+        p = self.L.cm1o_scalars(self.h)
+        return dict(mass1=p[0], train=p[1], cflmax=p[2], mass2=p[3])
+
+    def base_derive(self):
+        self.L.cm1o_base_derive(self.h)
+
+    def getset(self, fresh=True):
+        self.L.cm1o_getset(self.h, 1 if fresh else 0)
+
+    def step(self, n=1):
+        for _ in range(n):
+            self.L.cm1o_step(self.h)

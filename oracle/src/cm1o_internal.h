@@ -1,0 +1,96 @@
+/* Internal declarations of the CPU oracle (test infrastructure only). */
+#ifndef CM1O_INTERNAL_H
+#define CM1O_INTERNAL_H
+#include "cm1o.h"
+#include <math.h>
+
+typedef __float128 quad;
+
+/* Physical constants, set exactly as set_constants() does for testcase=0
+ * (src/constants.F:69-170): single-precision arithmetic at run time. */
+typedef struct {
+  float g, rd, rv, cp, cv, cpv, cvv, cpdcv, rovcp, rddcv, cpdrd, eps, reps,
+        repsm1, cpl, cpi, lv1, lv2, xlv;
+  float p00, rp00, th0r, smeps, tsmall, qsmall;
+  float epsilon;
+} cm1o_const;
+
+struct cm1o_model {
+  cm1o_config c;
+  cm1o_const k;
+  int ni, nj, nk;
+  long px, py, pz, pxy, ntot;
+  int ibw, ibe, ibs, ibn;
+  float rdx, rdy, rdz;
+  float cgs1, cgs2, cgs3, cgt1, cgt2, cgt3;
+  float *f[F_NFIELDS];
+  float *kf[K_NKFIELDS];
+  float *h[6];           /* uh, ruh, uf, vh, rvh, vf */
+  float *d2[D_N2D];
+  float *dtu, *dtv, *dtu0, *dtv0;  /* 2-D (px*py) */
+  /* scratch */
+  float *s1, *s2, *s3, *s4, *s5, *s6, *s7, *s8;
+  unsigned char *flag;
+  double mass1, mass2, train, cflmax;
+  float qmag[3];
+};
+
+#define IX(m, i, j, k) ((size_t)((i) + 2) + (size_t)((j) + 2) * (size_t)(m)->px + (size_t)(k) * (size_t)(m)->pxy)
+#define IX2(m, i, j) ((size_t)((i) + 2) + (size_t)((j) + 2) * (size_t)(m)->px)
+
+/* accessor helpers used by the routine files */
+#define A3(arr, i, j, k) (arr)[IX(M, i, j, k)]
+#define UH(i)  M->h[0][(i) + 2]
+#define RUH(i) M->h[1][(i) + 2]
+#define UF(i)  M->h[2][(i) + 2]
+#define VH(j)  M->h[3][(j) + 2]
+#define RVH(j) M->h[4][(j) + 2]
+#define VF(j)  M->h[5][(j) + 2]
+
+/* bc.F */
+void o_bcs(cm1o_model *M, float *s);
+void o_bcp(cm1o_model *M, float *p);
+void o_bcu(cm1o_model *M, float *u);
+void o_bcv(cm1o_model *M, float *v);
+void o_bcw(cm1o_model *M, float *w, int flag);
+void o_radbcew4(cm1o_model *M, float *radbcw, float *radbce, const float *u1, const float *u2, float dt);
+void o_radbcns4(cm1o_model *M, float *radbcs, float *radbcn, const float *v1, const float *v2, float dt);
+
+/* adv.F / adv_routines.F / pdef.F */
+float o_flx3(float s1, float s2, float s3);
+float o_flx5(float s1, float s2, float s3, float s4, float s5);
+float o_weno5(float s1, float s2, float s3, float s4, float s5, double weps);
+void o_advs(cm1o_model *M, int nrk, const float *rru, const float *rrv, const float *rrw,
+            const float *divx, const float *s0, const float *s, float *sten,
+            int pdef, int pdefweno, float dt, double weps);
+void o_advu(cm1o_model *M, int nrk, const float *rru, const float *u3d, float *uten,
+            const float *rrv, const float *rrw, const float *divx);
+void o_advv(cm1o_model *M, int nrk, const float *rru, const float *rrv, const float *v3d,
+            float *vten, const float *rrw, const float *divx);
+void o_advw(cm1o_model *M, int nrk, const float *rru, const float *rrv, const float *rrw,
+            const float *w3d, float *wten, const float *divx);
+
+/* sound.F */
+void o_sound(cm1o_model *M, int nrk, float dttmp);
+
+/* misclibs.F */
+void o_getdiv(cm1o_model *M, const float *u, const float *v, const float *w, float *div);
+void o_getdivx(cm1o_model *M, float *rru, float *rrv, float *rrw, float *divx);
+void o_calccflquick(cm1o_model *M, float dt, const float *ua, const float *va, const float *wa);
+void o_pdefq(cm1o_model *M, float rmax, const float *rho, float *q3d);
+
+/* kessler.F */
+void o_k_fallout(cm1o_model *M, const float *rho, const float *qr3d, float *vr);
+void o_fallout(cm1o_model *M, float dt, const float *rho, float *q3d, float *vq);
+void o_kessler(cm1o_model *M, float dt, float *tmp, float *rho, float *pp3d, float *th3d,
+               float *prs, float *qv3d, float *qc3d, float *qr3d);
+int  o_satadj(cm1o_model *M, int nrk, float dt, float *rho, float *pp3d, float *prs,
+              float *th3d, float *qv, float *qc, float *qr);
+
+/* solve*.F */
+void o_solve1(cm1o_model *M);
+void o_solve2(cm1o_model *M);
+void o_mp_driver(cm1o_model *M);
+void o_solve3(cm1o_model *M);
+
+#endif
