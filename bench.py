@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Throughput of the CM1 dynamical-core large step on one MI355X.
+
+Metric: grid-cell updates per second per full RK3 large step (FP32),
+``nx*ny*nz / t_step``, on the 1024x1024x96 supercell workload:
+dx=dy=250 m, stretched dz 50-500 m (stretch_z=1), dt=1.5 s, 8 acoustic
+substeps per large step, Kessler microphysics, 5th-order advection, open
+lateral boundaries, eight seeded +-2 K bubbles; every array resident in HBM.
+Synthetic initial state (Weisman-Klemp sounding, u = 30 tanh(z/3 km)).
+
+One JSON line is printed with the roofline of the dominant kernel and a
+CPU baseline: the reference CM1 OpenMP build (oracle/_ref/cm1_sp.exe) timed
+on a bounded sample of the same case on this host's cores.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+STRETCH = dict(dz_bot=50.0, dz_top=500.0, str_bot=500.0, str_top=17000.0, ztop=30000.0)
+
+
+def workload_config1(ni=1024, nj=1024, nk=96, seed=2024):
+    from cm1_amd import init as hinit
+    dx = dy = 250.0
+    rng = np.random.default_rng(seed)
+    bubbles = []
+    for _ in range(8):
+        bubbles.append(dict(x=float(rng.uniform(0.1, 0.9) * ni * dx),
+                            y=float(rng.uniform(0.1, 0.9) * nj * dy), z=1400.0,
+                            amp=float(rng.choice([-2.0, 2.0])), rh=10000.0, rv=1400.0))
+    cols, fields, dzn = hinit.initial_state(ni, nj, nk, dx, dy, 500.0, stretch=STRETCH,
+                                            umax=30.0, bubbles=bubbles)
+    cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=2, ebc=2, sbc=2, nbc=2, imoist=1, ptype=1,
+                psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4, nsound=8, npt=0,
+                pdtra=1, dx=dx, dy=dy, dz=dzn, dt=1.5, alph=0.6, kdiv=0.1, csmax=350.0)
+    return cfgd, cols, fields
+
+
+def cpu_baseline(threads):
+    """Reference CM1 (FP32 OpenMP build) on a 256x256x96 sample of the case."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    try:
+        import refrun
+    except Exception as e:  # pragma: no cover
+        return dict(value=None, unit="cell-updates/s", cores=threads, kind="reference",
+                    sample=f"unavailable: {e}")
+    if not os.path.exists(refrun.REF_EXE):
+        return dict(value=None, unit="cell-updates/s", cores=threads, kind="reference",
+                    sample="oracle/_ref/cm1_sp.exe not built")
+    ni, nj, nk, nsteps = 256, 256, 96, 6
+    ov = dict(nx=ni, ny=nj, nz=nk, dx=250.0, dy=250.0, dz=500.0, dtl=1.5, timax=nsteps * 1.5,
+              tapfrq=1.0e9, stretch_z=1, ztop=STRETCH["ztop"], str_bot=STRETCH["str_bot"],
+              str_top=STRETCH["str_top"], dz_bot=STRETCH["dz_bot"], dz_top=STRETCH["dz_top"])
+    wd = os.path.join("/tmp", f"cm1_cpu_baseline_{os.getpid()}")
+    try:
+        out, wall = refrun.run_reference(wd, ov, threads=threads, timeout=900,
+                                         extra_env={"OMP_PROC_BIND": "spread", "OMP_PLACES": "cores"})
+    except Exception as e:
+        return dict(value=None, unit="cell-updates/s", cores=threads, kind="reference",
+                    sample=f"run failed: {str(e)[:200]}")
+    ts = refrun.step_times(out)
+    if len(ts) > 1:
+        ts = ts[1:]
+    t = float(np.median(ts)) if ts else float("nan")
+    return dict(value=ni * nj * nk / t, unit="cell-updates/s", cores=threads, kind="reference",
+                sample=(f"reference CM1 FP32 OpenMP build, {ni}x{nj}x{nk} subdomain of the same case "
+                        f"(dx=250 m, stretched dz, dt=1.5 s, Kessler, open BCs), {nsteps} steps, "
+                        f"median per-step time {t:.3f} s excluding step 1; nsound chosen by the "
+                        f"reference (4, vs 8 on the GPU)"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ni", type=int, default=1024)
+    ap.add_argument("--nj", type=int, default=1024)
+    ap.add_argument("--nk", type=int, default=96)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
+              "the multi-GPU decomposition is described in DESIGN.md", file=sys.stderr)
+        return 2
+
+    import torch
+    from cm1_amd import model
+    import cm1_amd._native as N
+    import ctypes as C
+
+    cfgd, cols, fields = workload_config1(args.ni, args.nj, args.nk)
+    cfg = model.make_config(**cfgd)
+    m = model.setup(cfg, cols, fields)
+    del fields
+    for _ in range(args.warmup):
+        m.step(1)
+    m.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cfl = 0.0
+    for _ in range(args.steps):
+        cfl = m.step(1)
+    m.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1000.0 / args.steps
+    cells = cfgd["ni"] * cfgd["nj"] * cfgd["nk"]
+    value = cells / (ms / 1000.0)
+
+    # roofline of the dominant kernel, timed with HIP events on the model's stream
+    roof = None
+    L = N.lib()
+    if hasattr(L, "cm1g_time_kernel"):
+        L.cm1g_time_kernel.restype = C.c_int
+        L.cm1g_time_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float),
+                                       C.POINTER(C.c_double), C.c_char_p, C.c_int]
+        avg = C.c_float()
+        nbytes = C.c_double()
+        name = C.create_string_buffer(64)
+        rc = L.cm1g_time_kernel(m.h, 0, 20, C.byref(avg), C.byref(nbytes), name, 64)
+        if rc == 0 and avg.value > 0:
+            gbs = nbytes.value / (avg.value * 1e-3) / 1e9
+            roof = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
+                        kernel=name.value.decode(), avg_ms=round(avg.value, 4),
+                        algorithmic_bytes=nbytes.value)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(threads)
+
+    line = {
+        "metric": "cell-updates/s per RK3 step (FP32)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"supercell {cfgd['ni']}x{cfgd['nj']}x{cfgd['nk']} dx=250m stretched-dz "
+                               f"dt=1.5s nsound=8 Kessler open-BC 8 bubbles",
+                   "nx": cfgd["ni"], "ny": cfgd["nj"], "nz": cfgd["nk"], "nsound": cfgd["nsound"],
+                   "cflmax": cfl},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    m.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,112 @@
+"""ctypes binding of the cm1gpu C ABI (``include/cm1gpu.h``).
+
+The library is the only compute path: if ``libcm1gpu.so`` is missing or
+fails to load, :func:`lib` raises; nothing falls back to a CPU
+implementation.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("CM1GPU_LIB", os.path.join(HERE, "libcm1gpu.so"))
+
+FIELDS = ["ua", "u3d", "uten", "uten1", "va", "v3d", "vten", "vten1",
+          "wa", "w3d", "wten", "wten1", "rru", "rrv", "rrw",
+          "ppi", "pp3d", "ppten", "sten", "sadv", "ppx",
+          "tha", "th3d", "thten", "thten1", "thterm",
+          "qpten", "qtten", "qvten", "qcten",
+          "rho", "rr", "rf", "prs", "divx",
+          "dum1", "dum2", "dum3", "dum4", "dum5", "dum6", "dum7", "dum8", "dum9",
+          "t11", "t12", "t13", "t22", "t23", "t33",
+          "qa0", "qa1", "qa2", "q3d0", "q3d1", "q3d2", "qten0", "qten1", "qten2"]
+FID = {n: i for i, n in enumerate(FIELDS)}
+_PTA0 = len(FIELDS)
+for _n in range(8):
+    FID[f"pta{_n}"] = _PTA0 + _n
+    FID[f"pt3d{_n}"] = _PTA0 + 8 + _n
+    FID[f"ptten{_n}"] = _PTA0 + 16 + _n
+KFIELDS = ["mh", "rmh", "mf", "rmf", "c1", "c2", "zh", "zf", "pi0", "th0", "prs0",
+           "qv0", "rho0", "rr0", "rf0", "rrf0", "thv0", "rth0", "qc0", "qi0", "u0", "v0"]
+KID = {n: i for i, n in enumerate(KFIELDS)}
+D2 = {"rain": 0, "prate": 1, "radbcw": 2, "radbce": 3, "radbcs": 4, "radbcn": 5}
+BC = {"s": 0, "p": 1, "u": 2, "v": 3, "w": 4, "w0": 5}
+
+ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "device allocation failed",
+          4: "unsupported", 5: "satadj did not converge", 6: "CFL >= 1.5"}
+
+
+class Config(C.Structure):
+    """cm1g_config: the namelist subset the hot path reads."""
+    _fields_ = [(n, C.c_int32) for n in (
+        "ni", "nj", "nk", "wbc", "ebc", "sbc", "nbc", "imoist", "ptype", "psolver",
+        "advwenos", "advwenov", "apmasscon", "irbc", "nsound", "npt", "pdtra")] + \
+        [(n, C.c_float) for n in ("dx", "dy", "dz", "dt", "alph", "kdiv", "csmax")]
+
+
+class CM1GError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+# exported symbols and their signatures (checked by the CPU test suite)
+SIGNATURES = {
+    "cm1g_create": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_void_p)]),
+    "cm1g_destroy": (C.c_int, [C.c_void_p]),
+    "cm1g_last_error": (C.c_char_p, []),
+    "cm1g_dims": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "cm1g_put_column": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "cm1g_get_column": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "cm1g_metrics": (C.c_int, [C.c_void_p]),
+    "cm1g_base_derive": (C.c_int, [C.c_void_p]),
+    "cm1g_put_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "cm1g_get_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "cm1g_put_2d": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "cm1g_get_2d": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "cm1g_field_ptr": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "cm1g_sync": (C.c_int, [C.c_void_p]),
+    "cm1g_getset": (C.c_int, [C.c_void_p, C.c_int]),
+    "cm1g_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "cm1g_solve1": (C.c_int, [C.c_void_p]),
+    "cm1g_solve2": (C.c_int, [C.c_void_p]),
+    "cm1g_mp_driver": (C.c_int, [C.c_void_p]),
+    "cm1g_solve3": (C.c_int, [C.c_void_p]),
+    "cm1g_advs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                            C.c_float, C.c_double]),
+    "cm1g_advu": (C.c_int, [C.c_void_p, C.c_int]),
+    "cm1g_advv": (C.c_int, [C.c_void_p, C.c_int]),
+    "cm1g_advw": (C.c_int, [C.c_void_p, C.c_int]),
+    "cm1g_sound": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
+    "cm1g_kessler_branch": (C.c_int, [C.c_void_p]),
+    "cm1g_pdefq": (C.c_int, [C.c_void_p, C.c_float, C.c_int]),
+    "cm1g_bc": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "cm1g_calccflquick": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_int, C.c_int,
+                                    C.POINTER(C.c_float)]),
+    "cm1g_getdiv": (C.c_int, [C.c_void_p, C.c_int]),
+    "cm1g_getdivx": (C.c_int, [C.c_void_p]),
+    "cm1g_dtsmall": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_int32)]),
+    "cm1g_budgets": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "cm1g_scalars": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+}
+
+
+def lib():
+    """Load ``libcm1gpu.so`` (raises if it is missing)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise CM1GError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                            "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().cm1g_last_error().decode(errors="replace")
+        raise CM1GError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")

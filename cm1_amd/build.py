@@ -1,0 +1,54 @@
+"""Build the cm1gpu HIP library (``cm1_amd/libcm1gpu.so``) for gfx950.
+
+FP32 semantics follow the reference CPU build: no FMA contraction
+(``-ffp-contract=off``), IEEE division and square root, denormals kept.
+Packed FP32 instructions are disabled (``-packed-fp32-ops``).
+"""
+import glob
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libcm1gpu.so")
+
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+               "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-Wno-unused-result", "-Wno-unused-value"]
+
+
+def build(verbose=False, jobs=8):
+    srcs = sorted(glob.glob(os.path.join(SRC, "*.hip")))
+    objs = []
+    procs = []
+    for s in srcs:
+        o = os.path.join(SRC, "_obj_" + os.path.basename(s) + ".o")
+        cmd = ["hipcc"] + [f for f in HIPCC_FLAGS if f != "-shared"] + ["-c", s, "-o", o]
+        if verbose:
+            print(" ".join(cmd))
+        procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                          text=True)))
+        objs.append(o)
+    err = []
+    for s, p in procs:
+        out = p.communicate()[0]
+        lines = [l for l in out.splitlines() if "packed-fp32-ops" not in l]
+        if p.returncode != 0:
+            err.append(f"{s}:\n" + "\n".join(lines))
+        elif verbose and lines:
+            print("\n".join(lines))
+    if err:
+        raise RuntimeError("hipcc failed:\n" + "\n".join(err))
+    cmd = ["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,452 @@
+// Flux-form advection on the device.
+// flx3/flx5: src/adv_routines.F:5971-5984, :6006-6019.  hadv_flx5
+// :1335-1476, vadv_flx5 :3269-3388 (+ vadv_lwr_flx_23 :4817-4995,
+// vadv_lwr_flx_35 :5001-5169).  advs: src/adv.F:14-903, advu :911-1839,
+// advv :1847-2770, advw :2778-3610; pdef: src/pdef.F:15-581.
+//
+// Every flux is evaluated with the reference's operation order in FP32
+// without contraction, so results are bitwise those of the reference.
+#include "cm1g_internal.h"
+
+__device__ __forceinline__ float flx3(float s1, float s2, float s3) {
+  return (-1.0f / 6.0f) * s1 + (5.0f / 6.0f) * s2 + (2.0f / 6.0f) * s3;
+}
+__device__ __forceinline__ float flx5(float s1, float s2, float s3, float s4, float s5) {
+  return (2.0f / 60.0f) * s1 + (-13.0f / 60.0f) * s2 + (47.0f / 60.0f) * s3 +
+         (27.0f / 60.0f) * s4 + (-3.0f / 60.0f) * s5;
+}
+
+// upwind 5th-order face value between a(m) (left) and a(m+1) given the
+// five-point stencils: positive flow uses a(m-2..m+2), negative a(m+3..m-1)
+#define FLXP(A, o1, o2, o3, o4, o5) flx5(A(o1), A(o2), A(o3), A(o4), A(o5))
+
+// ------------------------------------------------------------ horizontal
+// stag 1: scalar; 2: u; 3: v; 4: w.  Writes dumx/dumy over the reference
+// loop ranges.
+static void hadv5(cm1g_model *m, int stag, const float *rru, const float *rrv, float *dumx,
+                  float *dumy, const float *a) {
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  if (stag == 1) {
+    for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+      auto S = [&](int ii, int jj) { return a[IXD(P, ii, jj, k)]; };
+      float ru = rru[IXD(P, i, j, k)], rv = rrv[IXD(P, i, j, k)];
+      dumx[IXD(P, i, j, k)] = (ru >= 0.0f)
+          ? ru * flx5(S(i - 3, j), S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j))
+          : ru * flx5(S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j), S(i - 2, j));
+      dumy[IXD(P, i, j, k)] = (rv >= 0.0f)
+          ? rv * flx5(S(i, j - 3), S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1))
+          : rv * flx5(S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1), S(i, j - 2));
+    });
+  } else if (stag == 2) {
+    int i1 = P.ibw == 1 ? 2 : 1, i2 = P.ibe == 1 ? ni : ni + 1;
+    for3(m->stream, i1 - 1, i2, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+      auto S = [&](int ii, int jj) { return a[IXD(P, ii, jj, k)]; };
+      float ubar = 0.5f * (rru[IXD(P, i, j, k)] + rru[IXD(P, i + 1, j, k)]);
+      dumx[IXD(P, i, j, k)] = (ubar >= 0.0f)
+          ? ubar * flx5(S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j), S(i + 2, j))
+          : ubar * flx5(S(i + 3, j), S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j));
+      float vbar = 0.5f * (rrv[IXD(P, i, j, k)] + rrv[IXD(P, i - 1, j, k)]);
+      dumy[IXD(P, i, j, k)] = (vbar >= 0.0f)
+          ? vbar * flx5(S(i, j - 3), S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1))
+          : vbar * flx5(S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1), S(i, j - 2));
+    });
+  } else if (stag == 3) {
+    int j1 = P.ibs == 1 ? 2 : 1, j2 = P.ibn == 1 ? nj : nj + 1;
+    for3(m->stream, 1, ni + 1, j1 - 1, j2, 1, nk, [=] __device__(int i, int j, int k) {
+      auto S = [&](int ii, int jj) { return a[IXD(P, ii, jj, k)]; };
+      float ubar = 0.5f * (rru[IXD(P, i, j, k)] + rru[IXD(P, i, j - 1, k)]);
+      dumx[IXD(P, i, j, k)] = (ubar >= 0.0f)
+          ? ubar * flx5(S(i - 3, j), S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j))
+          : ubar * flx5(S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j), S(i - 2, j));
+      float vbar = 0.5f * (rrv[IXD(P, i, j, k)] + rrv[IXD(P, i, j + 1, k)]);
+      dumy[IXD(P, i, j, k)] = (vbar >= 0.0f)
+          ? vbar * flx5(S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1), S(i, j + 2))
+          : vbar * flx5(S(i, j + 3), S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1));
+    });
+  } else {
+    for3(m->stream, 1, ni + 1, 1, nj + 1, 2, nk, [=] __device__(int i, int j, int k) {
+      auto S = [&](int ii, int jj) { return a[IXD(P, ii, jj, k)]; };
+      float cc2 = 0.5f * (P.c2[k] + P.c2[k]);
+      float cc1 = 1.0f - cc2;
+      float ubar = cc2 * rru[IXD(P, i, j, k)] + cc1 * rru[IXD(P, i, j, k - 1)];
+      dumx[IXD(P, i, j, k)] = (ubar >= 0.0f)
+          ? ubar * flx5(S(i - 3, j), S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j))
+          : ubar * flx5(S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j), S(i - 2, j));
+      float vbar = cc2 * rrv[IXD(P, i, j, k)] + cc1 * rrv[IXD(P, i, j, k - 1)];
+      dumy[IXD(P, i, j, k)] = (vbar >= 0.0f)
+          ? vbar * flx5(S(i, j - 3), S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1))
+          : vbar * flx5(S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1), S(i, j - 2));
+    });
+  }
+}
+
+__device__ __forceinline__ float wface(const KP &P, int stag, const float *rrw, int i, int j, int k) {
+  if (stag == 1) return rrw[IXD(P, i, j, k)];
+  if (stag == 2) return 0.5f * (rrw[IXD(P, i, j, k)] + rrw[IXD(P, i - 1, j, k)]);
+  if (stag == 3) return 0.5f * (rrw[IXD(P, i, j, k)] + rrw[IXD(P, i, j - 1, k)]);
+  return 0.5f * (rrw[IXD(P, i, j, k)] + rrw[IXD(P, i, j, k + 1)]);
+}
+
+// vertical face flux at level k for staggering stag (interior 5th order and
+// the lower-order boundary fluxes of vadv_lwr_flx_23/35)
+__device__ __forceinline__ float vflux5(const KP &P, int stag, const float *rrw, const float *a,
+                                       int i, int j, int k) {
+  const int nk = P.nk;
+  auto S = [&](int kk) { return a[IXD(P, i, j, kk)]; };
+  float wb = wface(P, stag, rrw, i, j, k);
+  if (stag != 4) {
+    if (k >= 4 && k <= nk - 2)
+      return (wb >= 0.0f) ? wb * flx5(S(k - 3), S(k - 2), S(k - 1), S(k), S(k + 1))
+                          : wb * flx5(S(k + 2), S(k + 1), S(k), S(k - 1), S(k - 2));
+    if (k == 3)
+      return (wb >= 0.0f) ? wb * flx3(S(k - 2), S(k - 1), S(k))
+                          : wb * flx5(S(k + 2), S(k + 1), S(k), S(k - 1), S(k - 2));
+    if (k == nk - 1)
+      return (wb >= 0.0f) ? wb * flx5(S(k - 3), S(k - 2), S(k - 1), S(k), S(k + 1))
+                          : wb * flx3(S(k + 1), S(k), S(k - 1));
+    // k == 2 or k == nk
+    float lin;
+    if (stag == 1) lin = P.c1[k] * S(k - 1) + P.c2[k] * S(k);
+    else {
+      float cc2 = 0.5f * (P.c2[k] + P.c2[k]);
+      float cc1 = 1.0f - cc2;
+      lin = cc1 * S(k - 1) + cc2 * S(k);
+    }
+    if (k == 2) return (wb >= 0.0f) ? wb * lin : wb * flx3(S(k + 1), S(k), S(k - 1));
+    return (wb >= 0.0f) ? wb * flx3(S(k - 2), S(k - 1), S(k)) : wb * lin;
+  }
+  if (k >= 3 && k <= nk - 2)
+    return (wb >= 0.0f) ? wb * flx5(S(k - 2), S(k - 1), S(k), S(k + 1), S(k + 2))
+                        : wb * flx5(S(k + 3), S(k + 2), S(k + 1), S(k), S(k - 1));
+  if (k == 1)
+    return (wb >= 0.0f) ? wb * 0.5f * (S(k) + S(k + 1)) : wb * flx3(S(k + 2), S(k + 1), S(k));
+  if (k == nk)
+    return (wb >= 0.0f) ? wb * flx3(S(k - 1), S(k), S(k + 1)) : wb * 0.5f * (S(k) + S(k + 1));
+  if (k == 2)
+    return (wb >= 0.0f) ? wb * flx3(S(k - 1), S(k), S(k + 1))
+                        : wb * flx5(S(k + 3), S(k + 2), S(k + 1), S(k), S(k - 1));
+  // k == nk-1
+  return (wb >= 0.0f) ? wb * flx5(S(k - 2), S(k - 1), S(k), S(k + 1), S(k + 2))
+                      : wb * flx3(S(k + 2), S(k + 1), S(k));
+}
+
+static void vadv5(cm1g_model *m, int stag, const float *rrw, float *dumz, const float *a) {
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  int ilo = 1, ihi = ni, jlo = 1, jhi = nj, klo = 2, khi = nk;
+  if (stag == 2) { ilo = P.ibw == 1 ? 2 : 1; ihi = P.ibe == 1 ? ni : ni + 1; }
+  if (stag == 3) { jlo = P.ibs == 1 ? 2 : 1; jhi = P.ibn == 1 ? nj : nj + 1; }
+  if (stag == 4) { klo = 1; khi = nk; }
+  for3(m->stream, ilo, ihi, jlo, jhi, klo, khi, [=] __device__(int i, int j, int k) {
+    dumz[IXD(P, i, j, k)] = vflux5(P, stag, rrw, a, i, j, k);
+  });
+}
+
+// ------------------------------------------------------------------ pdef
+// One direction of the positive-definite correction (pdefx1+pdefx2,
+// pdefy1+pdefy2 or pdefz).  The reference scatters mass to i-1,i,i+1
+// inside a sequential loop; here each cell gathers the same terms in the
+// same order (left neighbour, itself, right neighbour), which reproduces
+// the sequential sums bit for bit.
+static void pdef_dir(cm1g_model *m, int dir, const float *vel, const float *adv, float *advout,
+                     float *dum, const float *s0, const float *s, float dt) {
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  // pdefx1 / pdefy1 / first part of pdefz: dum
+  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    int c = IXD(P, i, j, k);
+    float div;
+    if (dir == 0) div = (vel[IXD(P, i + 1, j, k)] - vel[c]) * P.rdx * P.uh[i + 2];
+    else if (dir == 1) div = (vel[IXD(P, i, j + 1, k)] - vel[c]) * P.rdy * P.vh[j + 2];
+    else div = (vel[IXD(P, i, j, k + 1)] - vel[c]) * P.rdz * P.mh[k];
+    float d = P.rho0[k] * s0[c] + dt * (adv[c] + s[c] * div);
+    if (dir == 0) {
+      if (P.wbc == 2 && P.ibw == 1 && i == 1 && vel[IXD(P, 1, j, k)] >= 0.0f) d = P.rho0[k] * s0[c];
+      if (P.ebc == 2 && P.ibe == 1 && i == ni && vel[IXD(P, ni + 1, j, k)] <= 0.0f) d = P.rho0[k] * s0[c];
+    } else if (dir == 1) {
+      if (P.sbc == 2 && P.ibs == 1 && j == 1 && vel[IXD(P, i, 1, k)] >= 0.0f) d = P.rho0[k] * s0[c];
+      if (P.nbc == 2 && P.ibn == 1 && j == nj && vel[IXD(P, i, nj + 1, k)] <= 0.0f) d = P.rho0[k] * s0[c];
+    }
+    dum[c] = d;
+  });
+  if (dir != 2) {
+    bc_fill(m, BC_S, dum);
+  } else {
+    for2(m->stream, 1, ni, 1, nj, [=] __device__(int i, int j) {
+      dum[IXD(P, i, j, 0)] = 0.0f;
+      dum[IXD(P, i, j, nk + 1)] = 0.0f;
+    });
+  }
+  // second part: gather the mass exchanged with the two neighbours
+  const float rdt = 1.0f / dt;
+  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    const float smeps = P.K.smeps;
+    auto D = [&](int o) -> float {
+      if (dir == 0) return dum[IXD(P, i + o, j, k)];
+      if (dir == 1) return dum[IXD(P, i, j + o, k)];
+      return dum[IXD(P, i, j, k + o)];
+    };
+    // the reference visits cells 0..ni+1 (x, y) or 1..nk (z)
+    auto visited = [&](int o) -> bool {
+      if (dir == 0) { int ii = i + o; return ii >= 0 && ii <= ni + 1; }
+      if (dir == 1) { int jj = j + o; return jj >= 0 && jj <= nj + 1; }
+      int kk = k + o; return kk >= 1 && kk <= nk;
+    };
+    float mass = 0.0f;
+    bool flag = false;
+    float d0 = D(0);
+    // contribution from the left neighbour (cell -1 scatters foo2*foo3 to us)
+    if (visited(-1)) {
+      float dm = D(-1);
+      if (dm < 0.0f) {
+        float foo1 = fmaxf(0.0f, D(-2)), foo2 = fmaxf(0.0f, d0);
+        if (foo1 + foo2 > smeps) {
+          float foo3 = fmaxf(dm, -(foo1 + foo2)) / (foo1 + foo2);
+          mass = mass + foo2 * foo3;
+          if (d0 > smeps) flag = true;
+        }
+      }
+    }
+    if (d0 < 0.0f) {
+      float foo1 = fmaxf(0.0f, D(-1)), foo2 = fmaxf(0.0f, D(1));
+      if (foo1 + foo2 > smeps) {
+        float foo3 = fmaxf(d0, -(foo1 + foo2)) / (foo1 + foo2);
+        mass = mass - (foo1 + foo2) * foo3;
+        flag = true;
+      }
+    }
+    if (visited(1)) {
+      float dp = D(1);
+      if (dp < 0.0f) {
+        float foo1 = fmaxf(0.0f, d0), foo2 = fmaxf(0.0f, D(2));
+        if (foo1 + foo2 > smeps) {
+          float foo3 = fmaxf(dp, -(foo1 + foo2)) / (foo1 + foo2);
+          mass = mass + foo1 * foo3;
+          if (d0 > smeps) flag = true;
+        }
+      }
+    }
+    int c = IXD(P, i, j, k);
+    float out = adv[c];
+    if (flag) {
+      float div;
+      if (dir == 0) div = (vel[IXD(P, i + 1, j, k)] - vel[c]) * P.rdx * P.uh[i + 2];
+      else if (dir == 1) div = (vel[IXD(P, i, j + 1, k)] - vel[c]) * P.rdy * P.vh[j + 2];
+      else div = (vel[IXD(P, i, j, k + 1)] - vel[c]) * P.rdz * P.mh[k];
+      out = (d0 + mass - P.rho0[k] * s0[c]) * rdt - s[c] * div;
+    }
+    advout[c] = out;
+  });
+}
+
+// ------------------------------------------------------------------ advs
+// sadd: when nonzero, s(i,j,k) = (th0(k) - th0r) + s_in(i,j,k) is formed on
+// the fly (the theta equation's sadv, src/solve2.F:1354-1374).
+// sten0/thterm_div: initial tendency; with thterm_div != nullptr the
+// initial tendency is thten1 - t11*thterm (src/solve2.F:1320-1330), where
+// sten0 = thten1, thterm_div[0] = t11 and thterm_div[1] = thterm.
+// out_new, if given, receives s0 + dt*tendency with the smeps flush
+// (src/solve2.F:1416-1426, :1518-1528).
+void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float *s0,
+              const float *sten0, const float *thterm_div, float *out_sten, float *out_new,
+              int pdef, int pdefweno, float dt, double weps, int smeps_flush) {
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  const float *rru = m->rrus, *rrv = m->rrvs, *rrw = m->rrws;
+  const float *divx = m->f[CM1G_DIVX];
+  float *dumx = m->sx, *dumy = m->sy, *dumz = m->sz;
+  float *advx = m->tx, *advy = m->ty, *advz = m->tz;
+  (void)nrk; (void)pdefweno; (void)weps;
+  const float *a = s_in;
+  if (sadd != 0.0f) {
+    // materialise s = (th0-th0r) + th3d over the whole box (ib:ie, jb:je)
+    float *sv = m->f[CM1G_SADV];
+    for3(m->stream, -2, ni + 3, -2, nj + 3, 1, nk, [=] __device__(int i, int j, int k) {
+      float tem = P.th0[k] - P.K.th0r;
+      sv[IXD(P, i, j, k)] = tem + s_in[IXD(P, i, j, k)];
+    });
+    a = sv;
+  }
+  hadv5(m, 1, rru, rrv, dumx, dumy, a);
+  const bool doitw = (P.wbc == 2 && P.ibw == 1), doite = (P.ebc == 2 && P.ibe == 1);
+  const bool doits = (P.sbc == 2 && P.ibs == 1), doitn = (P.nbc == 2 && P.ibn == 1);
+  if (doitw || doite)
+    for2(m->stream, 1, nj, 1, nk, [=] __device__(int j, int k) {
+      if (doitw && rru[IXD(P, 1, j, k)] >= 0.0f) dumx[IXD(P, 1, j, k)] = dumx[IXD(P, 2, j, k)];
+      if (doite && rru[IXD(P, ni + 1, j, k)] <= 0.0f) dumx[IXD(P, ni + 1, j, k)] = dumx[IXD(P, ni, j, k)];
+    });
+  if (doits || doitn)
+    for2(m->stream, 1, ni, 1, nk, [=] __device__(int i, int k) {
+      if (doits && rrv[IXD(P, i, 1, k)] >= 0.0f) dumy[IXD(P, i, 1, k)] = dumy[IXD(P, i, 2, k)];
+      if (doitn && rrv[IXD(P, i, nj + 1, k)] <= 0.0f) dumy[IXD(P, i, nj + 1, k)] = dumy[IXD(P, i, nj, k)];
+    });
+  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    int c = IXD(P, i, j, k);
+    float ax = -(dumx[IXD(P, i + 1, j, k)] - dumx[c]) * P.rdx * P.uh[i + 2];
+    float ay = -(dumy[IXD(P, i, j + 1, k)] - dumy[c]) * P.rdy * P.vh[j + 2];
+    if (doitw && i == 1 && rru[IXD(P, 1, j, k)] >= 0.0f)
+      ax = ax - a[c] * (rru[IXD(P, 2, j, k)] - rru[c]) * P.rdx * P.uh[1 + 2];
+    if (doite && i == ni && rru[IXD(P, ni + 1, j, k)] <= 0.0f)
+      ax = ax - a[c] * (rru[IXD(P, ni + 1, j, k)] - rru[c]) * P.rdx * P.uh[ni + 2];
+    if (doits && j == 1 && rrv[IXD(P, i, 1, k)] >= 0.0f)
+      ay = ay - a[c] * (rrv[IXD(P, i, 2, k)] - rrv[c]) * P.rdy * P.vh[1 + 2];
+    if (doitn && j == nj && rrv[IXD(P, i, nj + 1, k)] <= 0.0f)
+      ay = ay - a[c] * (rrv[IXD(P, i, nj + 1, k)] - rrv[c]) * P.rdy * P.vh[nj + 2];
+    advx[c] = ax;
+    advy[c] = ay;
+  });
+  vadv5(m, 1, rrw, dumz, a);
+  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    int c = IXD(P, i, j, k);
+    float az;
+    if (k == 1) az = -dumz[IXD(P, i, j, 2)] * P.rdz * P.mh[1];
+    else if (k == nk) az = +dumz[c] * P.rdz * P.mh[nk];
+    else az = -(dumz[IXD(P, i, j, k + 1)] - dumz[c]) * P.rdz * P.mh[k];
+    advz[c] = az;
+  });
+  const float *ax = advx, *ay = advy, *az = advz;
+  if (pdef == 1) {
+    // each direction writes its corrected tendency into dumx/dumy/dumz
+    // (the fluxes are no longer needed); the work array is the other one.
+    pdef_dir(m, 0, rru, advx, dumx, m->scr1, s0, a, dt);
+    pdef_dir(m, 2, rrw, advz, dumz, m->scr1, s0, a, dt);
+    pdef_dir(m, 1, rrv, advy, dumy, m->scr1, s0, a, dt);
+    ax = dumx; ay = dumy; az = dumz;
+  }
+  const float *t11 = thterm_div ? (const float *)thterm_div : nullptr;
+  const float *thterm = thterm_div ? m->f[CM1G_THTERM] : nullptr;
+  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    int c = IXD(P, i, j, k);
+    float st = sten0[c];
+    if (t11) st = st - t11[c] * thterm[c];
+    st = st + ((ax[c] + ay[c]) + az[c] + a[c] * divx[c]) * P.rr0[k];
+    if (out_sten) out_sten[c] = st;
+    if (out_new) {
+      float v = s0[c] + dt * st;
+      if (smeps_flush && fabsf(v) < P.K.smeps) v = 0.0f;
+      out_new[c] = v;
+    }
+  });
+}
+
+// ----------------------------------------------------------- advu/v/w
+// The momentum tendencies use the pre-acoustic mass fluxes rru/rrv/rrw
+// (model fields) and divx; uten/vten/wten receive uten1 + advection.
+void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy) {
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  const float *rru = m->f[CM1G_RRU], *rrv = m->f[CM1G_RRV], *rrw = m->f[CM1G_RRW];
+  const float *divx = m->f[CM1G_DIVX];
+  float *dumx = m->sx, *dumy = m->sy, *dumz = m->sz;
+  (void)nrk;
+  const bool doitw = (P.wbc == 2 && P.ibw == 1), doite = (P.ebc == 2 && P.ibe == 1);
+  const bool doits = (P.sbc == 2 && P.ibs == 1), doitn = (P.nbc == 2 && P.ibn == 1);
+  // ---- u
+  if (mask & 1) {
+    const float *u3d = m->f[CM1G_U3D];
+    float *uten = m->f[CM1G_UTEN];
+    int i1 = P.ibw == 1 ? 2 : 1, i2 = P.ibe == 1 ? ni : ni + 1;
+    hadv5(m, 2, rru, rrv, dumx, dumy, u3d);
+    if (doits || doitn)
+      for2(m->stream, i1, i2, 1, nk, [=] __device__(int i, int k) {
+        if (doits && (rrv[IXD(P, i, 1, k)] + rrv[IXD(P, i - 1, 1, k)]) >= 0.0f)
+          dumy[IXD(P, i, 1, k)] = dumy[IXD(P, i, 2, k)];
+        if (doitn && (rrv[IXD(P, i, nj + 1, k)] + rrv[IXD(P, i - 1, nj + 1, k)]) <= 0.0f)
+          dumy[IXD(P, i, nj + 1, k)] = dumy[IXD(P, i, nj, k)];
+      });
+    vadv5(m, 2, rrw, dumz, u3d);
+    for3(m->stream, i1, i2, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      float ax = -(dumx[c] - dumx[IXD(P, i - 1, j, k)]) * P.rdx * P.uf[i + 2];
+      float ay = -(dumy[IXD(P, i, j + 1, k)] - dumy[c]) * P.rdy * P.vh[j + 2];
+      if (doits && j == 1 && (rrv[IXD(P, i, 1, k)] + rrv[IXD(P, i - 1, 1, k)]) >= 0.0f)
+        ay = ay - u3d[c] * 0.5f * ((rrv[IXD(P, i - 1, 2, k)] - rrv[IXD(P, i - 1, 1, k)]) +
+                                  (rrv[IXD(P, i, 2, k)] - rrv[IXD(P, i, 1, k)])) * P.rdy * P.vh[1 + 2];
+      if (doitn && j == nj && (rrv[IXD(P, i, nj + 1, k)] + rrv[IXD(P, i - 1, nj + 1, k)]) <= 0.0f)
+        ay = ay - u3d[c] * 0.5f * ((rrv[IXD(P, i - 1, nj + 1, k)] - rrv[IXD(P, i - 1, nj, k)]) +
+                                  (rrv[IXD(P, i, nj + 1, k)] - rrv[IXD(P, i, nj, k)])) * P.rdy * P.vh[nj + 2];
+      float az;
+      if (k == 1) az = -dumz[IXD(P, i, j, 2)] * P.rdz * P.mh[1];
+      else if (k == nk) az = +dumz[c] * P.rdz * P.mh[nk];
+      else az = -(dumz[IXD(P, i, j, k + 1)] - dumz[c]) * P.rdz * P.mh[k];
+      uten[c] = uten[c] + ((ax + ay) + az + u3d[c] * 0.5f * (divx[c] + divx[IXD(P, i - 1, j, k)])) * P.rr0[k];
+    });
+  }
+  // ---- v
+  if (mask & 2) {
+    const float *v3d = m->f[CM1G_V3D];
+    float *vten = m->f[CM1G_VTEN];
+    int j1 = P.ibs == 1 ? 2 : 1, j2 = P.ibn == 1 ? nj : nj + 1;
+    hadv5(m, 3, rru, rrv, dumx, dumy, v3d);
+    if (doitw || doite)
+      for2(m->stream, j1, j2, 1, nk, [=] __device__(int j, int k) {
+        if (doitw && (rru[IXD(P, 1, j, k)] + rru[IXD(P, 1, j - 1, k)]) >= 0.0f)
+          dumx[IXD(P, 1, j, k)] = dumx[IXD(P, 2, j, k)];
+        if (doite && (rru[IXD(P, ni + 1, j, k)] + rru[IXD(P, ni + 1, j - 1, k)]) <= 0.0f)
+          dumx[IXD(P, ni + 1, j, k)] = dumx[IXD(P, ni, j, k)];
+      });
+    vadv5(m, 3, rrw, dumz, v3d);
+    for3(m->stream, 1, ni, j1, j2, 1, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      float ax = -(dumx[IXD(P, i + 1, j, k)] - dumx[c]) * P.rdx * P.uh[i + 2];
+      float ay = -(dumy[c] - dumy[IXD(P, i, j - 1, k)]) * P.rdy * P.vf[j + 2];
+      if (doitw && i == 1 && (rru[IXD(P, 1, j, k)] + rru[IXD(P, 1, j - 1, k)]) >= 0.0f)
+        ax = ax - v3d[c] * 0.5f * ((rru[IXD(P, 2, j - 1, k)] - rru[IXD(P, 1, j - 1, k)]) +
+                                  (rru[IXD(P, 2, j, k)] - rru[IXD(P, 1, j, k)])) * P.rdx * P.uh[1 + 2];
+      if (doite && i == ni && (rru[IXD(P, ni + 1, j, k)] + rru[IXD(P, ni + 1, j - 1, k)]) <= 0.0f)
+        ax = ax - v3d[c] * 0.5f * ((rru[IXD(P, ni + 1, j - 1, k)] - rru[IXD(P, ni, j - 1, k)]) +
+                                  (rru[IXD(P, ni + 1, j, k)] - rru[IXD(P, ni, j, k)])) * P.rdx * P.uh[ni + 2];
+      float az;
+      if (k == 1) az = -dumz[IXD(P, i, j, 2)] * P.rdz * P.mh[1];
+      else if (k == nk) az = +dumz[c] * P.rdz * P.mh[nk];
+      else az = -(dumz[IXD(P, i, j, k + 1)] - dumz[c]) * P.rdz * P.mh[k];
+      vten[c] = vten[c] + ((ax + ay) + az + v3d[c] * 0.5f * (divx[c] + divx[IXD(P, i, j - 1, k)])) * P.rr0[k];
+    });
+  }
+  // ---- w (+ buoyancy, src/solve2.F:899-907: dum8 holds buoyancy)
+  if (mask & 4) {
+    const float *w3d = m->f[CM1G_W3D];
+    float *wten = m->f[CM1G_WTEN];
+    const float *dum8 = m->f[CM1G_DUM8];
+    hadv5(m, 4, rru, rrv, dumx, dumy, w3d);
+    if (doitw || doite)
+      for2(m->stream, 1, nj, 2, nk, [=] __device__(int j, int k) {
+        float cc2 = 0.5f * (P.c2[k] + P.c2[k]), cc1 = 1.0f - cc2;
+        if (doitw && (cc2 * rru[IXD(P, 1, j, k)] + cc1 * rru[IXD(P, 1, j, k - 1)]) >= 0.0f)
+          dumx[IXD(P, 1, j, k)] = dumx[IXD(P, 2, j, k)];
+        if (doite && (cc2 * rru[IXD(P, ni + 1, j, k)] + cc1 * rru[IXD(P, ni + 1, j, k - 1)]) <= 0.0f)
+          dumx[IXD(P, ni + 1, j, k)] = dumx[IXD(P, ni, j, k)];
+      });
+    if (doits || doitn)
+      for2(m->stream, 1, ni, 2, nk, [=] __device__(int i, int k) {
+        float cc2 = 0.5f * (P.c2[k] + P.c2[k]), cc1 = 1.0f - cc2;
+        if (doits && (cc2 * rrv[IXD(P, i, 1, k)] + cc1 * rrv[IXD(P, i, 1, k - 1)]) >= 0.0f)
+          dumy[IXD(P, i, 1, k)] = dumy[IXD(P, i, 2, k)];
+        if (doitn && (cc2 * rrv[IXD(P, i, nj + 1, k)] + cc1 * rrv[IXD(P, i, nj + 1, k - 1)]) <= 0.0f)
+          dumy[IXD(P, i, nj + 1, k)] = dumy[IXD(P, i, nj, k)];
+      });
+    vadv5(m, 4, rrw, dumz, w3d);
+    for3(m->stream, 1, ni, 1, nj, 2, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      float cc2 = 0.5f * (P.c2[k] + P.c2[k]), cc1 = 1.0f - cc2;
+      float ax = -(dumx[IXD(P, i + 1, j, k)] - dumx[c]) * P.rdx * P.uh[i + 2];
+      float ay = -(dumy[IXD(P, i, j + 1, k)] - dumy[c]) * P.rdy * P.vh[j + 2];
+      if (doitw && i == 1 && (cc2 * rru[IXD(P, 1, j, k)] + cc1 * rru[IXD(P, 1, j, k - 1)]) >= 0.0f)
+        ax = ax - w3d[c] * (P.c1[k] * (rru[IXD(P, 2, j, k - 1)] - rru[IXD(P, 1, j, k - 1)]) +
+                            P.c2[k] * (rru[IXD(P, 2, j, k)] - rru[IXD(P, 1, j, k)])) * P.rdx * P.uh[1 + 2];
+      if (doite && i == ni && (cc2 * rru[IXD(P, ni + 1, j, k)] + cc1 * rru[IXD(P, ni + 1, j, k - 1)]) <= 0.0f)
+        ax = ax - w3d[c] * (P.c1[k] * (rru[IXD(P, ni + 1, j, k - 1)] - rru[IXD(P, ni, j, k - 1)]) +
+                            P.c2[k] * (rru[IXD(P, ni + 1, j, k)] - rru[IXD(P, ni, j, k)])) * P.rdx * P.uh[ni + 2];
+      if (doits && j == 1 && (cc2 * rrv[IXD(P, i, 1, k)] + cc1 * rrv[IXD(P, i, 1, k - 1)]) >= 0.0f)
+        ay = ay - w3d[c] * (P.c1[k] * (rrv[IXD(P, i, 2, k - 1)] - rrv[IXD(P, i, 1, k - 1)]) +
+                            P.c2[k] * (rrv[IXD(P, i, 2, k)] - rrv[IXD(P, i, 1, k)])) * P.rdy * P.vh[1 + 2];
+      if (doitn && j == nj && (cc2 * rrv[IXD(P, i, nj + 1, k)] + cc1 * rrv[IXD(P, i, nj + 1, k - 1)]) <= 0.0f)
+        ay = ay - w3d[c] * (P.c1[k] * (rrv[IXD(P, i, nj + 1, k - 1)] - rrv[IXD(P, i, nj, k - 1)]) +
+                            P.c2[k] * (rrv[IXD(P, i, nj + 1, k)] - rrv[IXD(P, i, nj, k)])) * P.rdy * P.vh[nj + 2];
+      float az = -(dumz[c] - dumz[IXD(P, i, j, k - 1)]) * P.rdz * P.mf[k];
+      float t = wten[c] + ((ax + ay) + az + w3d[c] * (P.c2[k] * divx[c] + P.c1[k] * divx[IXD(P, i, j, k - 1)])) * P.rrf0[k];
+      wten[c] = buoy ? t + (P.c1[k] * dum8[IXD(P, i, j, k - 1)] + P.c2[k] * dum8[c]) : t;
+    });
+  }
+}

@@ -1,0 +1,143 @@
+// Internal declarations of the cm1gpu HIP library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include "../../include/cm1gpu.h"
+
+// Physical constants, single precision as set_constants() computes them
+// (src/constants.F:69-170, testcase=0).
+struct Consts {
+  float g, rd, rv, cp, cv, cpv, cvv, cpdcv, rovcp, rddcv, cpdrd, eps, reps, repsm1,
+      cpl, cpi, lv1, lv2, xlv;
+  float p00, rp00, th0r, smeps, tsmall, qsmall, epsilon;
+};
+
+// Everything a kernel needs about the grid: sizes, boundary switches,
+// horizontal metrics and the vertical (k) metric/base-state columns.
+struct KP {
+  int ni, nj, nk;
+  int px, py;
+  int pxy;
+  int wbc, ebc, sbc, nbc;
+  int ibw, ibe, ibs, ibn;
+  float rdx, rdy, rdz, dx, dy, dz;
+  float cgs1, cgs2, cgs3, cgt1, cgt2, cgt3;
+  Consts K;
+  const float *mh, *rmh, *mf, *c1, *c2, *pi0, *th0, *qv0, *rho0, *rr0, *rf0, *rrf0,
+      *thv0, *rth0, *qc0, *qi0;
+  const float *uh, *ruh, *uf, *vh, *rvh, *vf;
+};
+
+__device__ __host__ __forceinline__ int IXD(const KP &P, int i, int j, int k) {
+  return (i + 2) + (j + 2) * P.px + k * P.pxy;
+}
+__device__ __host__ __forceinline__ int IX2D(const KP &P, int i, int j) {
+  return (i + 2) + (j + 2) * P.px;
+}
+
+// bc kinds
+enum { BC_S = 0, BC_P = 1, BC_U = 2, BC_V = 3, BC_W1 = 4, BC_W0 = 5 };
+
+struct cm1g_model {
+  cm1g_config c;
+  KP P;                         // kernel parameters (device pointers inside)
+  float *f[CM1G_NFIELDS];       // 3-D device fields
+  float *kf[CM1G_K_NFIELDS];    // 1-D device columns
+  float *hk[CM1G_K_NFIELDS];    // host copies of the columns
+  float *h[6];                  // device uh, ruh, uf, vh, rvh, vf
+  float *d2[CM1G_D_NFIELDS];    // 2-D device fields
+  float *dtu0, *dtv0;           // 2-D open-boundary masks (src/param.F:9182-9218)
+  // scratch (device)
+  float *sx, *sy, *sz, *tx, *ty, *tz;   // advection fluxes / pdef work
+  float *rrus, *rrvs, *rrws;    // time-averaged mass fluxes scaled by rho0 (getdivx)
+  float *thb;                   // ping-pong partner for th3d / q3d updates
+  float *scr1, *scr2;           // Thomas coefficients
+  double *dred;                 // reduction workspace (per-k partial sums etc.)
+  float *fred;                  // float reduction workspace
+  double *dscal;                // device scalars: [0] mass1 [1] mass2 [2] tem(pp shift) [3] train
+  int *iflag;                   // device flags: [0] satadj failure
+  hipStream_t stream;
+  size_t ntot;
+  double mass1, train;
+  float cflmax;
+  int nth_buf;                  // unused
+};
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) {                                                        \
+      cm1g_set_error(std::string(#x) + ": " + hipGetErrorString(e_));              \
+      return CM1G_E_HIP;                                                           \
+    }                                                                              \
+  } while (0)
+
+void cm1g_set_error(const std::string &s);
+
+// ---------------------------------------------------------------- launching
+template <class F>
+__global__ void __launch_bounds__(256) k_for3(F f, int i0, int ni, int j0, int nj, int k0) {
+  int i = i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  int j = j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  int k = k0 + (int)blockIdx.z;
+  if (i < i0 + ni && j < j0 + nj) f(i, j, k);
+}
+
+// run f(i,j,k) over the inclusive box [ilo,ihi]x[jlo,jhi]x[klo,khi]
+template <class F>
+inline void for3(hipStream_t s, int ilo, int ihi, int jlo, int jhi, int klo, int khi, F f) {
+  int ni = ihi - ilo + 1, nj = jhi - jlo + 1, nk = khi - klo + 1;
+  if (ni <= 0 || nj <= 0 || nk <= 0) return;
+  dim3 b(64, 4, 1);
+  dim3 g((ni + 63) / 64, (nj + 3) / 4, nk);
+  hipLaunchKernelGGL(k_for3<F>, g, b, 0, s, f, ilo, ni, jlo, nj, klo);
+}
+
+template <class F>
+__global__ void __launch_bounds__(256) k_for2(F f, int i0, int ni, int j0, int nj) {
+  int i = i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  int j = j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i < i0 + ni && j < j0 + nj) f(i, j);
+}
+
+// run f(i,j) over [ilo,ihi]x[jlo,jhi] (column kernels loop k inside)
+template <class F>
+inline void for2(hipStream_t s, int ilo, int ihi, int jlo, int jhi, F f) {
+  int ni = ihi - ilo + 1, nj = jhi - jlo + 1;
+  if (ni <= 0 || nj <= 0) return;
+  dim3 b(64, 4, 1);
+  dim3 g((ni + 63) / 64, (nj + 3) / 4, 1);
+  hipLaunchKernelGGL(k_for2<F>, g, b, 0, s, f, ilo, ni, jlo, nj);
+}
+
+// ---------------------------------------------------------------- modules
+// bc fills (cm1g_bc.hip)
+void bc_fill(cm1g_model *m, int kind, float *a);
+void bc_fill_n(cm1g_model *m, int kind, float *const *arrs, int n);
+void radbc4(cm1g_model *m, const float *ua, const float *u3d, const float *va,
+            const float *v3d, float dt);
+// advection (cm1g_adv.hip)
+void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy);
+void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float *s0,
+              const float *sten0, const float *thterm_div, float *out_sten, float *out_new,
+              int pdef, int pdefweno, float dt, double weps, int smeps_flush);
+// acoustic (cm1g_sound.hip)
+void sound_run(cm1g_model *m, int nrk);
+void getdiv_both(cm1g_model *m, int want_t11);
+void cfl_run(cm1g_model *m, float dt, const float *u, const float *v, const float *w);
+// microphysics (cm1g_micro.hip)
+void pdefq_run(cm1g_model *m, float rmax, const float *rho, float *q);
+void kessler_branch_run(cm1g_model *m);
+// reductions
+void mass_sums(cm1g_model *m);           // dumk3/dumk4 -> dscal
+void budgets_run(cm1g_model *m, double *tmass, double *tmois);
+
+// FP32 exp and pow evaluated in double precision and rounded once: the
+// result is the correctly rounded FP32 value except in vanishingly rare
+// near-midpoint cases, which keeps agreement with the host C library's
+// expf/powf (what the reference's REAL exp() and ** lower to) at the
+// level of isolated 1-ulp differences.
+__device__ __forceinline__ float exp_f(float x) { return (float)exp((double)x); }
+__device__ __forceinline__ float pow_f(float x, float y) { return (float)pow((double)x, (double)y); }

@@ -1,0 +1,990 @@
+// cm1gpu model: device residency, C ABI and the large-step orchestration.
+// solve1: src/solve1.F:414-454,1700-1780; solve2: src/solve2.F:296-2052;
+// solve3: src/solve3.F:246-705; getset: src/init3d.F:2032-2872;
+// constants: src/constants.F:69-170; metrics: src/param.F:7790-8215,
+// :8590-8622, :8980-9060, :9182-9218; base state: src/base.F:2085-2198.
+#include "cm1g_internal.h"
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+static thread_local std::string g_err;
+void cm1g_set_error(const std::string &s) { g_err = s; }
+extern "C" const char *cm1g_last_error(void) { return g_err.c_str(); }
+
+#define ARGCHK(cond, msg)              \
+  do {                                 \
+    if (!(cond)) {                     \
+      cm1g_set_error(msg);             \
+      return CM1G_E_ARG;               \
+    }                                  \
+  } while (0)
+
+static void set_constants(Consts &K) {
+  K.g = 9.81f; K.rd = 287.04f; K.cp = 1005.7f; K.cpv = 1870.0f; K.xlv = 2501000.0f;
+  K.rv = 461.5f;
+  K.cv = K.cp - K.rd;
+  K.cvv = K.cpv - K.rv;
+  K.cpdcv = K.cp / K.cv;
+  K.rovcp = K.rd / K.cp;
+  K.rddcv = K.rd / K.cv;
+  K.cpdrd = K.cp / K.rd;
+  K.eps = K.rd / K.rv;
+  K.reps = K.rv / K.rd;
+  K.repsm1 = K.rv / K.rd - 1.0f;
+  K.cpl = 4190.0f;
+  K.cpi = 2106.0f;
+  K.lv1 = K.xlv + (K.cpl - K.cpv) * 273.15f;
+  K.lv2 = K.cpl - K.cpv;
+  K.p00 = 1.0e5f;
+  K.rp00 = 1.0f / K.p00;
+  K.th0r = 300.0f;
+  K.smeps = 1.0e-30f;
+  K.tsmall = 0.0001f;
+  K.qsmall = 1.0e-14f;
+  K.epsilon = 1.0e-18f;
+}
+
+static int dalloc(float **p, size_t n) {
+  hipError_t e = hipMalloc((void **)p, n * sizeof(float));
+  if (e != hipSuccess) {
+    cm1g_set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return CM1G_E_NOMEM;
+  }
+  e = hipMemset(*p, 0, n * sizeof(float));
+  if (e != hipSuccess) {
+    cm1g_set_error(std::string("hipMemset: ") + hipGetErrorString(e));
+    return CM1G_E_HIP;
+  }
+  return 0;
+}
+
+static int upload_column(cm1g_model *m, int id) {
+  HIPCHK(hipMemcpy(m->kf[id], m->hk[id], (size_t)(m->c.nk + 3) * sizeof(float), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int cm1g_dtsmall(const cm1g_config *c, int32_t *nsound) {
+  ARGCHK(c && nsound, "cm1g_dtsmall: null argument");
+  float dtsm;
+  if (c->psolver == 2) {
+    float mn = fminf(fminf(c->dx, c->dy), c->dz);
+    dtsm = 0.50f * mn / 360.0f;
+  } else {
+    dtsm = 0.60f * fminf(c->dx, c->dy) / 360.0f;
+  }
+  double dbldt = (double)c->dt;
+  int ns = (int)lround(dbldt / (double)dtsm);
+  if (ns < 4) ns = 4;
+  if (ns % 2 != 0) ns = ns + 1;
+  if (dbldt / (double)(float)ns > (double)dtsm) ns = ns + 2;
+  if (ns > 192) {
+    cm1g_set_error("cm1g_dtsmall: nsound > 192 (src/misclibs.F:887-909)");
+    return CM1G_E_ARG;
+  }
+  *nsound = ns;
+  return 0;
+}
+
+extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
+  ARGCHK(cfg && out, "cm1g_create: null argument");
+  ARGCHK(cfg->ni >= 8 && cfg->nj >= 8 && cfg->nk >= 6, "cm1g_create: grid too small (ni,nj >= 8, nk >= 6)");
+  ARGCHK(cfg->ni <= 8192 && cfg->nj <= 8192 && cfg->nk <= 510, "cm1g_create: grid too large");
+  ARGCHK((cfg->wbc == 1 || cfg->wbc == 2) && (cfg->ebc == 1 || cfg->ebc == 2) &&
+             (cfg->sbc == 1 || cfg->sbc == 2) && (cfg->nbc == 1 || cfg->nbc == 2),
+         "cm1g_create: lateral boundaries must be 1 (periodic) or 2 (open)");
+  ARGCHK((cfg->wbc == 1) == (cfg->ebc == 1) && (cfg->sbc == 1) == (cfg->nbc == 1),
+         "cm1g_create: periodic boundaries must be set on both sides");
+  ARGCHK(cfg->psolver == 3, "cm1g_create: only psolver=3 (sound.F) is built");
+  ARGCHK(cfg->advwenos == 0 && cfg->advwenov == 0, "cm1g_create: WENO advection is not built yet");
+  ARGCHK(cfg->imoist == 0 || cfg->imoist == 1, "cm1g_create: imoist must be 0 or 1");
+  ARGCHK(cfg->ptype == 0 || cfg->ptype == 1, "cm1g_create: ptype must be 0 or 1 (Kessler)");
+  ARGCHK(cfg->nsound >= 1 && cfg->nsound <= 192, "cm1g_create: nsound out of range");
+  ARGCHK(cfg->npt >= 0 && cfg->npt <= 8, "cm1g_create: npt must be 0..8");
+  ARGCHK(cfg->dx > 0 && cfg->dy > 0 && cfg->dz > 0 && cfg->dt > 0, "cm1g_create: dx,dy,dz,dt must be > 0");
+  cm1g_model *m = new cm1g_model();
+  memset((void *)m, 0, sizeof(cm1g_model));
+  m->c = *cfg;
+  if (m->c.csmax == 0.0f) m->c.csmax = 350.0f;
+  KP &P = m->P;
+  P.ni = cfg->ni; P.nj = cfg->nj; P.nk = cfg->nk;
+  P.px = ((cfg->ni + 7) + 63) / 64 * 64;
+  P.py = cfg->nj + 7;
+  P.pxy = P.px * P.py;
+  P.wbc = cfg->wbc; P.ebc = cfg->ebc; P.sbc = cfg->sbc; P.nbc = cfg->nbc;
+  P.ibw = cfg->wbc != 1; P.ibe = cfg->ebc != 1; P.ibs = cfg->sbc != 1; P.ibn = cfg->nbc != 1;
+  P.dx = cfg->dx; P.dy = cfg->dy; P.dz = cfg->dz;
+  P.rdx = 1.0f / cfg->dx; P.rdy = 1.0f / cfg->dy; P.rdz = 1.0f / cfg->dz;
+  set_constants(P.K);
+  const int pz = cfg->nk + 3;
+  m->ntot = (size_t)P.pxy * pz;
+  int rc = 0;
+  if (hipStreamCreate(&m->stream) != hipSuccess) {
+    cm1g_set_error("hipStreamCreate failed");
+    delete m;
+    return CM1G_E_HIP;
+  }
+  for (int i = 0; i < CM1G_NFIELDS && !rc; i++) {
+    bool need = true;
+    if (i >= CM1G_PTA0 && i < CM1G_NFIELDS) {
+      int t = (i - CM1G_PTA0) % 8;
+      need = t < cfg->npt;
+    }
+    if (need) rc = dalloc(&m->f[i], m->ntot);
+  }
+  for (int i = 0; i < CM1G_K_NFIELDS && !rc; i++) {
+    rc = dalloc(&m->kf[i], pz + 1);
+    m->hk[i] = (float *)calloc(pz + 1, sizeof(float));
+  }
+  for (int i = 0; i < 6 && !rc; i++) rc = dalloc(&m->h[i], (i < 3 ? P.px : P.py));
+  for (int i = 0; i < CM1G_D_NFIELDS && !rc; i++) rc = dalloc(&m->d2[i], (size_t)P.pxy + (size_t)(P.px + P.py) * pz);
+  if (!rc) rc = dalloc(&m->dtu0, P.pxy);
+  if (!rc) rc = dalloc(&m->dtv0, P.pxy);
+  float **scr[] = {&m->sx, &m->sy, &m->sz, &m->tx, &m->ty, &m->tz, &m->rrus, &m->rrvs, &m->rrws,
+                   &m->scr1, &m->scr2};
+  for (auto p : scr)
+    if (!rc) rc = dalloc(p, m->ntot);
+  if (!rc) {
+    size_t nd = (size_t)cfg->ni * cfg->nj + 4 * (size_t)pz + 64;
+    if (hipMalloc((void **)&m->dred, nd * sizeof(double)) != hipSuccess) rc = CM1G_E_NOMEM;
+    if (!rc && hipMalloc((void **)&m->dscal, 16 * sizeof(double)) != hipSuccess) rc = CM1G_E_NOMEM;
+    if (!rc && hipMalloc((void **)&m->fred, (pz + 8) * sizeof(float)) != hipSuccess) rc = CM1G_E_NOMEM;
+    if (!rc && hipMalloc((void **)&m->iflag, 16 * sizeof(int)) != hipSuccess) rc = CM1G_E_NOMEM;
+    if (!rc) {
+      hipMemset(m->dscal, 0, 16 * sizeof(double));
+      hipMemset(m->iflag, 0, 16 * sizeof(int));
+    }
+    if (rc) cm1g_set_error("device allocation failed");
+  }
+  if (rc) {
+    cm1g_destroy(m);
+    return rc;
+  }
+  // horizontal metrics of an unstretched grid (uh = dx/(xf(i+1)-xf(i)) = 1)
+  {
+    std::vector<float> one((size_t)(P.px > P.py ? P.px : P.py), 1.0f);
+    for (int i = 0; i < 6; i++)
+      hipMemcpy(m->h[i], one.data(), (i < 3 ? P.px : P.py) * sizeof(float), hipMemcpyHostToDevice);
+  }
+  // vertical metrics of an unstretched grid (src/param.F:7790-7810)
+  {
+    const int nk = cfg->nk;
+    float *zf = m->hk[CM1G_K_ZF], *zh = m->hk[CM1G_K_ZH], *mh = m->hk[CM1G_K_MH], *mf = m->hk[CM1G_K_MF];
+    for (int k = 0; k <= nk + 2; k++) { zf[k] = cfg->dz * (float)(k - 1); mf[k] = 1.0f; }
+    for (int k = 0; k <= nk + 1; k++) { zh[k] = 0.5f * (zf[k] + zf[k + 1]); mh[k] = 1.0f; }
+  }
+  // open-boundary masks (src/param.F:9182-9218)
+  {
+    std::vector<float> du((size_t)P.pxy, 0.0f), dv((size_t)P.pxy, 0.0f);
+    for (int j = 1; j <= P.nj; j++)
+      for (int i = 1; i <= P.ni + 1; i++) du[IX2D(P, i, j)] = 1.0f;
+    if (P.ibw) for (int j = 1; j <= P.nj; j++) du[IX2D(P, 1, j)] = 0.0f;
+    if (P.ibe) for (int j = 1; j <= P.nj; j++) du[IX2D(P, P.ni + 1, j)] = 0.0f;
+    for (int j = 1; j <= P.nj + 1; j++)
+      for (int i = 1; i <= P.ni; i++) dv[IX2D(P, i, j)] = 1.0f;
+    if (P.ibs) for (int i = 1; i <= P.ni; i++) dv[IX2D(P, i, 1)] = 0.0f;
+    if (P.ibn) for (int i = 1; i <= P.ni; i++) dv[IX2D(P, i, P.nj + 1)] = 0.0f;
+    hipMemcpy(m->dtu0, du.data(), du.size() * sizeof(float), hipMemcpyHostToDevice);
+    hipMemcpy(m->dtv0, dv.data(), dv.size() * sizeof(float), hipMemcpyHostToDevice);
+  }
+  P.mh = m->kf[CM1G_K_MH]; P.rmh = m->kf[CM1G_K_RMH]; P.mf = m->kf[CM1G_K_MF];
+  P.c1 = m->kf[CM1G_K_C1]; P.c2 = m->kf[CM1G_K_C2]; P.pi0 = m->kf[CM1G_K_PI0];
+  P.th0 = m->kf[CM1G_K_TH0]; P.qv0 = m->kf[CM1G_K_QV0]; P.rho0 = m->kf[CM1G_K_RHO0];
+  P.rr0 = m->kf[CM1G_K_RR0]; P.rf0 = m->kf[CM1G_K_RF0]; P.rrf0 = m->kf[CM1G_K_RRF0];
+  P.thv0 = m->kf[CM1G_K_THV0]; P.rth0 = m->kf[CM1G_K_RTH0]; P.qc0 = m->kf[CM1G_K_QC0];
+  P.qi0 = m->kf[CM1G_K_QI0];
+  P.uh = m->h[0]; P.ruh = m->h[1]; P.uf = m->h[2]; P.vh = m->h[3]; P.rvh = m->h[4]; P.vf = m->h[5];
+  rc = cm1g_metrics(m);
+  if (rc) {
+    cm1g_destroy(m);
+    return rc;
+  }
+  *out = m;
+  return 0;
+}
+
+extern "C" int cm1g_destroy(cm1g_model *m) {
+  if (!m) return 0;
+  if (m->stream) hipStreamSynchronize(m->stream);
+  for (int i = 0; i < CM1G_NFIELDS; i++) if (m->f[i]) hipFree(m->f[i]);
+  for (int i = 0; i < CM1G_K_NFIELDS; i++) { if (m->kf[i]) hipFree(m->kf[i]); free(m->hk[i]); }
+  for (int i = 0; i < 6; i++) if (m->h[i]) hipFree(m->h[i]);
+  for (int i = 0; i < CM1G_D_NFIELDS; i++) if (m->d2[i]) hipFree(m->d2[i]);
+  float *scr[] = {m->dtu0, m->dtv0, m->sx, m->sy, m->sz, m->tx, m->ty, m->tz, m->rrus, m->rrvs,
+                  m->rrws, m->scr1, m->scr2, m->fred};
+  for (auto p : scr) if (p) hipFree(p);
+  if (m->dred) hipFree(m->dred);
+  if (m->dscal) hipFree(m->dscal);
+  if (m->iflag) hipFree(m->iflag);
+  if (m->stream) hipStreamDestroy(m->stream);
+  delete m;
+  return 0;
+}
+
+extern "C" int cm1g_dims(const cm1g_model *m, int64_t *px, int64_t *py, int64_t *pz) {
+  ARGCHK(m && px && py && pz, "cm1g_dims: null argument");
+  *px = m->P.px; *py = m->P.py; *pz = m->P.nk + 3;
+  return 0;
+}
+
+extern "C" int cm1g_put_column(cm1g_model *m, int id, const float *host) {
+  ARGCHK(m && host && id >= 0 && id < CM1G_K_NFIELDS, "cm1g_put_column: bad argument");
+  memcpy(m->hk[id], host, (size_t)(m->c.nk + 3) * sizeof(float));
+  return upload_column(m, id);
+}
+
+extern "C" int cm1g_get_column(cm1g_model *m, int id, float *host) {
+  ARGCHK(m && host && id >= 0 && id < CM1G_K_NFIELDS, "cm1g_get_column: bad argument");
+  memcpy(host, m->hk[id], (size_t)(m->c.nk + 3) * sizeof(float));
+  return 0;
+}
+
+extern "C" int cm1g_metrics(cm1g_model *m) {
+  ARGCHK(m, "cm1g_metrics: null model");
+  const int nk = m->c.nk;
+  float *zf = m->hk[CM1G_K_ZF], *zh = m->hk[CM1G_K_ZH], *mh = m->hk[CM1G_K_MH], *mf = m->hk[CM1G_K_MF];
+  float *c1 = m->hk[CM1G_K_C1], *c2 = m->hk[CM1G_K_C2];
+  for (int k = 0; k <= nk + 1; k++) m->hk[CM1G_K_RMH][k] = 1.0f / mh[k];
+  for (int k = 0; k <= nk + 2; k++) m->hk[CM1G_K_RMF][k] = 1.0f / mf[k];
+  for (int k = 1; k <= nk + 1; k++) {
+    c2[k] = (zf[k] - zh[k - 1]) / (zh[k] - zh[k - 1]);
+    c1[k] = 1.0f - c2[k];
+  }
+  // three-point extrapolation weights to zf(1) from zh(1..3) and to
+  // zf(nk+1) from zh(nk..nk-2): the m=0 Fornberg weights of
+  // src/param.F:8980-9060, in double precision, stored as REAL.
+  float w6[6];
+  for (int nn = 0; nn < 2; nn++) {
+    double x0, al[3];
+    if (nn == 0) { x0 = zf[1]; al[0] = zh[1]; al[1] = zh[2]; al[2] = zh[3]; }
+    else { x0 = zf[nk + 1]; al[0] = zh[nk]; al[1] = zh[nk - 1]; al[2] = zh[nk - 2]; }
+    for (int a = 0; a < 3; a++) {
+      double p = 1.0;
+      for (int b = 0; b < 3; b++)
+        if (b != a) p *= (x0 - al[b]) / (al[a] - al[b]);
+      w6[nn * 3 + a] = (float)p;
+    }
+  }
+  m->P.cgs1 = w6[0]; m->P.cgs2 = w6[1]; m->P.cgs3 = w6[2];
+  m->P.cgt1 = w6[3]; m->P.cgt2 = w6[4]; m->P.cgt3 = w6[5];
+  int ids[] = {CM1G_K_ZF, CM1G_K_ZH, CM1G_K_MH, CM1G_K_MF, CM1G_K_RMH, CM1G_K_RMF, CM1G_K_C1, CM1G_K_C2};
+  for (int id : ids) {
+    int rc = upload_column(m, id);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+extern "C" int cm1g_base_derive(cm1g_model *m) {
+  ARGCHK(m, "cm1g_base_derive: null model");
+  const int nk = m->c.nk;
+  const Consts &K = m->P.K;
+  float *pi0 = m->hk[CM1G_K_PI0], *th0 = m->hk[CM1G_K_TH0], *qv0 = m->hk[CM1G_K_QV0];
+  float *prs0 = m->hk[CM1G_K_PRS0], *rho0 = m->hk[CM1G_K_RHO0], *rr0 = m->hk[CM1G_K_RR0];
+  float *rf0 = m->hk[CM1G_K_RF0], *rrf0 = m->hk[CM1G_K_RRF0], *thv0 = m->hk[CM1G_K_THV0];
+  float *rth0 = m->hk[CM1G_K_RTH0], *qc0 = m->hk[CM1G_K_QC0], *qi0 = m->hk[CM1G_K_QI0];
+  float *c1 = m->hk[CM1G_K_C1], *c2 = m->hk[CM1G_K_C2];
+  const KP &P = m->P;
+  for (int k = 1; k <= nk; k++) {
+    prs0[k] = K.p00 * powf(pi0[k], K.cpdrd);  // host libm, as the reference
+    rho0[k] = prs0[k] / (K.rd * th0[k] * pi0[k] * (1.0f + qv0[k] * K.reps));
+    if (m->c.imoist == 1)
+      thv0[k] = (th0[k] + 0.0f) * (1.0f + K.reps * fmaxf(0.0f, qv0[k])) /
+                (1.0f + fmaxf(0.0f, qv0[k]) + fmaxf(0.0f, qc0[k] + qi0[k]));
+    else
+      thv0[k] = th0[k];
+  }
+  rho0[0] = P.cgs1 * rho0[1] + P.cgs2 * rho0[2] + P.cgs3 * rho0[3];
+  rho0[nk + 1] = P.cgt1 * rho0[nk] + P.cgt2 * rho0[nk - 1] + P.cgt3 * rho0[nk - 2];
+  for (int k = 0; k <= nk + 1; k++) rr0[k] = 1.0f / rho0[k];
+  for (int k = 2; k <= nk; k++) rf0[k] = c1[k] * rho0[k - 1] + c2[k] * rho0[k];
+  rf0[1] = P.cgs1 * rho0[1] + P.cgs2 * rho0[2] + P.cgs3 * rho0[3];
+  rf0[0] = rf0[1];
+  rf0[nk + 1] = P.cgt1 * rho0[nk] + P.cgt2 * rho0[nk - 1] + P.cgt3 * rho0[nk - 2];
+  for (int k = 0; k <= nk + 1; k++) rrf0[k] = 1.0f / rf0[k];
+  th0[0] = th0[1];
+  th0[nk + 1] = th0[nk];
+  for (int k = 0; k <= nk + 1; k++) rth0[k] = 1.0f / th0[k];
+  int ids[] = {CM1G_K_PI0, CM1G_K_TH0, CM1G_K_QV0, CM1G_K_PRS0, CM1G_K_RHO0, CM1G_K_RR0, CM1G_K_RF0,
+               CM1G_K_RRF0, CM1G_K_THV0, CM1G_K_RTH0, CM1G_K_QC0, CM1G_K_QI0};
+  for (int id : ids) {
+    int rc = upload_column(m, id);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+static bool field_ok(const cm1g_model *m, int id) {
+  return m && id >= 0 && id < CM1G_NFIELDS && m->f[id] != nullptr;
+}
+
+extern "C" int cm1g_put_field(cm1g_model *m, int id, const float *host) {
+  ARGCHK(field_ok(m, id) && host, "cm1g_put_field: bad field id or null pointer");
+  HIPCHK(hipMemcpy(m->f[id], host, m->ntot * sizeof(float), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int cm1g_get_field(cm1g_model *m, int id, float *host) {
+  ARGCHK(field_ok(m, id) && host, "cm1g_get_field: bad field id or null pointer");
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(host, m->f[id], m->ntot * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int cm1g_put_2d(cm1g_model *m, int id, const float *host) {
+  ARGCHK(m && host && id >= 0 && id < CM1G_D_NFIELDS, "cm1g_put_2d: bad argument");
+  size_t n = (size_t)m->P.pxy + (size_t)(m->P.px + m->P.py) * (m->c.nk + 3);
+  HIPCHK(hipMemcpy(m->d2[id], host, n * sizeof(float), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int cm1g_get_2d(cm1g_model *m, int id, float *host) {
+  ARGCHK(m && host && id >= 0 && id < CM1G_D_NFIELDS, "cm1g_get_2d: bad argument");
+  size_t n = (size_t)m->P.pxy + (size_t)(m->P.px + m->P.py) * (m->c.nk + 3);
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(host, m->d2[id], n * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int cm1g_field_ptr(cm1g_model *m, int id, float **dptr) {
+  ARGCHK(field_ok(m, id) && dptr, "cm1g_field_ptr: bad field id");
+  *dptr = m->f[id];
+  return 0;
+}
+
+extern "C" int cm1g_sync(cm1g_model *m) {
+  ARGCHK(m, "cm1g_sync: null model");
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+static int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    cm1g_set_error(std::string("kernel launch: ") + hipGetErrorString(e));
+    return CM1G_E_HIP;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------ reductions
+// Per-level partial sums of the dry-air mass and of pi for the apmasscon
+// adjustment (src/solve2.F:1617-1656), in double precision.
+__global__ void __launch_bounds__(256) k_mass_levels(KP P, const float *rho, const float *pp3d, double *out) {
+  __shared__ double r3[256], r4[256];
+  const int k = 1 + (int)blockIdx.x;
+  double a3 = 0.0, a4 = 0.0;
+  for (int t = threadIdx.x; t < P.ni * P.nj; t += blockDim.x) {
+    int i = 1 + t % P.ni, j = 1 + t / P.ni;
+    int c = IXD(P, i, j, k);
+    a3 += (double)(rho[c] * P.ruh[i + 2] * P.rvh[j + 2] * P.rmh[k]);
+    a4 += (double)(P.pi0[k] + pp3d[c]);
+  }
+  r3[threadIdx.x] = a3;
+  r4[threadIdx.x] = a4;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      r3[threadIdx.x] += r3[threadIdx.x + s];
+      r4[threadIdx.x] += r4[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * k] = r3[0];
+    out[2 * k + 1] = r4[0];
+  }
+}
+
+// two-sum accumulation of the nk level sums (the reference accumulates them
+// in real(kind=qp)); then the pi' shift tem (src/solve2.F:1976-1995).
+__device__ inline void dd_add(double &hi, double &lo, double x) {
+  double s = hi + x;
+  double bb = s - hi;
+  double err = (hi - (s - bb)) + (x - bb);
+  hi = s;
+  lo += err;
+}
+
+__global__ void k_mass_final(int nk, int mode, float rddcv, double ntot, const double *lev, double *dscal) {
+  double h3 = 0.0, l3 = 0.0, h4 = 0.0, l4 = 0.0;
+  for (int k = 1; k <= nk; k++) {
+    dd_add(h3, l3, lev[2 * k]);
+    dd_add(h4, l4, lev[2 * k + 1]);
+  }
+  double mass = h3 + l3, p2 = h4 + l4;
+  if (mode == 0) {
+    dscal[0] = mass;  // mass1 (getset)
+  } else {
+    dscal[1] = mass;  // mass2
+    double tem = (pow(dscal[0] / mass, (double)rddcv) - 1.0) * p2 / ntot;
+    dscal[2] = (double)(float)tem;
+  }
+}
+
+static void mass_levels(cm1g_model *m, const float *rho, int mode) {
+  hipLaunchKernelGGL(k_mass_levels, dim3(m->P.nk), dim3(256), 0, m->stream, m->P, rho,
+                     m->f[CM1G_PP3D], m->dred);
+  double ntot = (double)((long long)m->P.ni * m->P.nj * m->P.nk);
+  hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(1), 0, m->stream, m->P.nk, mode, m->P.K.rddcv,
+                     ntot, m->dred, m->dscal);
+}
+
+// domain rain-accumulation budget (src/kessler.F:470-476): per-row sums in
+// i order, then rows in j order, as the reference does.
+__global__ void k_train(int ni, int nj, double tem, const double *colbud, double *dscal) {
+  double tr = dscal[3];
+  for (int j = 0; j < nj; j++) {
+    double b = 0.0;
+    for (int i = 0; i < ni; i++) b = b + colbud[i + j * ni];
+    tr = tr + b * tem;
+  }
+  dscal[3] = tr;
+}
+
+// ------------------------------------------------------------------ getset
+extern "C" int cm1g_getset(cm1g_model *m, int fresh_start) {
+  ARGCHK(m, "cm1g_getset: null model");
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  float **f = m->f;
+  bc_fill(m, BC_U, f[CM1G_UA]);
+  bc_fill(m, BC_V, f[CM1G_VA]);
+  bc_fill(m, BC_W1, f[CM1G_WA]);
+  bc_fill(m, BC_S, f[CM1G_PPI]);
+  bc_fill(m, BC_S, f[CM1G_THA]);
+  if (m->c.imoist == 1)
+    for (int n = 0; n < 3; n++) bc_fill(m, BC_S, f[CM1G_QA0 + n]);
+  for (int n = 0; n < m->c.npt; n++) bc_fill(m, BC_S, f[CM1G_PTA0 + n]);
+  const size_t bytes = m->ntot * sizeof(float);
+  HIPCHK(hipMemcpyAsync(f[CM1G_U3D], f[CM1G_UA], bytes, hipMemcpyDeviceToDevice, m->stream));
+  HIPCHK(hipMemcpyAsync(f[CM1G_V3D], f[CM1G_VA], bytes, hipMemcpyDeviceToDevice, m->stream));
+  HIPCHK(hipMemcpyAsync(f[CM1G_W3D], f[CM1G_WA], bytes, hipMemcpyDeviceToDevice, m->stream));
+  HIPCHK(hipMemcpyAsync(f[CM1G_PP3D], f[CM1G_PPI], bytes, hipMemcpyDeviceToDevice, m->stream));
+  HIPCHK(hipMemcpyAsync(f[CM1G_TH3D], f[CM1G_THA], bytes, hipMemcpyDeviceToDevice, m->stream));
+  if (m->c.imoist == 1)
+    for (int n = 0; n < 3; n++)
+      HIPCHK(hipMemcpyAsync(f[CM1G_Q3D0 + n], f[CM1G_QA0 + n], bytes, hipMemcpyDeviceToDevice, m->stream));
+  for (int n = 0; n < m->c.npt; n++)
+    HIPCHK(hipMemcpyAsync(f[CM1G_PT3D0 + n], f[CM1G_PTA0 + n], bytes, hipMemcpyDeviceToDevice, m->stream));
+  float *rho = f[CM1G_RHO], *prs = f[CM1G_PRS], *rr = f[CM1G_RR], *rf = f[CM1G_RF];
+  const float *pp3d = f[CM1G_PP3D], *th3d = f[CM1G_TH3D], *qv = f[CM1G_Q3D0];
+  const int moist = m->c.imoist;
+  if (fresh_start) {
+    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      const Consts &K = P.K;
+      int c = IXD(P, i, j, k);
+      float p = K.p00 * pow_f(P.pi0[k] + pp3d[c], K.cpdrd);
+      prs[c] = p;
+      if (moist)
+        rho[c] = p / ((P.th0[k] + th3d[c]) * (P.pi0[k] + pp3d[c]) * (K.rd + fmaxf(0.0f, qv[c]) * K.rv));
+      else
+        rho[c] = p / (K.rd * (P.th0[k] + th3d[c]) * (P.pi0[k] + pp3d[c]));
+    });
+  }
+  bc_fill(m, BC_S, rho);
+  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    int c = IXD(P, i, j, k);
+    rr[c] = 1.0f / rho[c];
+    rf[c] = P.c1[k] * rho[IXD(P, i, j, k - 1)] + P.c2[k] * rho[c];
+  });
+  bc_fill(m, BC_S, rr);
+  bc_fill(m, BC_S, rf);
+  for2(m->stream, 0, ni + 1, 0, nj + 1, [=] __device__(int i, int j) {
+    rf[IXD(P, i, j, 1)] = P.cgs1 * rho[IXD(P, i, j, 1)] + P.cgs2 * rho[IXD(P, i, j, 2)] + P.cgs3 * rho[IXD(P, i, j, 3)];
+    rf[IXD(P, i, j, nk + 1)] = P.cgt1 * rho[IXD(P, i, j, nk)] + P.cgt2 * rho[IXD(P, i, j, nk - 1)] +
+                               P.cgt3 * rho[IXD(P, i, j, nk - 2)];
+  });
+  if (fresh_start) {
+    mass_levels(m, rho, 0);
+    float *ppx = f[CM1G_PPX];
+    const float *ppi = f[CM1G_PPI];
+    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      ppx[IXD(P, i, j, k)] = ppi[IXD(P, i, j, k)];
+    });
+  }
+  bc_fill(m, BC_S, f[CM1G_PPX]);
+  int rc = check_launch();
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(&m->mass1, m->dscal, sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// ------------------------------------------------------------------ solve1
+extern "C" int cm1g_solve1(cm1g_model *m) {
+  ARGCHK(m, "cm1g_solve1: null model");
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  const float dt = m->c.dt;
+  float **f = m->f;
+  float *uten1 = f[CM1G_UTEN1], *vten1 = f[CM1G_VTEN1], *wten1 = f[CM1G_WTEN1], *thten1 = f[CM1G_THTEN1];
+  for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+    int c = IXD(P, i, j, k);
+    uten1[c] = 0.0f;
+    vten1[c] = 0.0f;
+    wten1[c] = 0.0f;
+    thten1[c] = 0.0f;
+  });
+  for (int n = 0; n < m->c.npt; n++) {
+    float *pt = f[CM1G_PTTEN0 + n];
+    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) { pt[IXD(P, i, j, k)] = 0.0f; });
+  }
+  float *ppten = f[CM1G_PPTEN];
+  const float *rho = f[CM1G_RHO], *tha = f[CM1G_THA], *ppi = f[CM1G_PPI];
+  const int moist = m->c.imoist, diab = (m->c.imoist == 1 && m->c.ptype >= 1);
+  float *qten0 = f[CM1G_QTEN0], *qten1 = f[CM1G_QTEN1], *qten2 = f[CM1G_QTEN2];
+  const float *qa0 = f[CM1G_QA0];
+  const float *qpten = f[CM1G_QPTEN], *qtten = f[CM1G_QTTEN], *qvten = f[CM1G_QVTEN], *qcten = f[CM1G_QCTEN];
+  const float rdt = 1.0f / dt;
+  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    const Consts &K = P.K;
+    const float tem = 0.0001f * K.tsmall;
+    int c = IXD(P, i, j, k);
+    float th1 = 0.0f;       // thten1 was just zeroed (irdamp = 0)
+    float pt;
+    if (moist) {
+      float qvt = 0.0f;     // qten(nqv) was just zeroed
+      if (fabsf(dt * th1) > tem || fabsf(dt * qvt) > K.qsmall) {
+        float thnew = tha[c] + dt * th1;
+        float qvnew = qa0[c] + dt * qvt;
+        float pinew = pow_f(rho[c] * (P.th0[k] + thnew) * (K.rd + K.rv * qvnew) * K.rp00, K.rddcv) - P.pi0[k];
+        pt = (pinew - ppi[c]) * rdt;
+      } else pt = 0.0f;
+      float q0 = 0.0f, q1 = 0.0f;
+      if (diab) {
+        pt = pt + qpten[c];
+        th1 = th1 + qtten[c];
+        q0 = q0 + qvten[c];
+        q1 = q1 + qcten[c];
+      }
+      qten0[c] = q0;
+      qten1[c] = q1;
+      qten2[c] = 0.0f;
+    } else {
+      if (fabsf(dt * th1) > tem) {
+        float thnew = tha[c] + dt * th1;
+        float pinew = pow_f(rho[c] * (P.th0[k] + thnew) * K.rd * K.rp00, K.rddcv) - P.pi0[k];
+        pt = (pinew - ppi[c]) * rdt;
+      } else pt = 0.0f;
+    }
+    ppten[c] = pt;
+    thten1[c] = th1;
+  });
+  return check_launch();
+}
+
+// ------------------------------------------------------------------ solve2
+extern "C" int cm1g_solve2(cm1g_model *m) {
+  ARGCHK(m, "cm1g_solve2: null model");
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  const float dt = m->c.dt;
+  float **f = m->f;
+  const int moist = m->c.imoist;
+  float *u3d = f[CM1G_U3D], *v3d = f[CM1G_V3D], *w3d = f[CM1G_W3D];
+  float *uten = f[CM1G_UTEN], *vten = f[CM1G_VTEN], *wten = f[CM1G_WTEN];
+  const float *uten1 = f[CM1G_UTEN1], *vten1 = f[CM1G_VTEN1], *wten1 = f[CM1G_WTEN1];
+  float *rru = f[CM1G_RRU], *rrv = f[CM1G_RRV], *rrw = f[CM1G_RRW], *divx = f[CM1G_DIVX];
+  float *th3d = f[CM1G_TH3D], *pp3d = f[CM1G_PP3D], *sadv = f[CM1G_SADV];
+  float *thterm = f[CM1G_THTERM], *dum8 = f[CM1G_DUM8], *t11 = f[CM1G_T11], *t22 = f[CM1G_T22];
+  float *rho = f[CM1G_RHO], *prs = f[CM1G_PRS];
+  const float *qv = f[CM1G_Q3D0], *qc = f[CM1G_Q3D1], *qr = f[CM1G_Q3D2];
+  for (int nrk = 1; nrk <= 3; nrk++) {
+    const float dttmp = dt / (float)(3 + 1 - nrk);
+    // thermodynamic prep (src/solve2.F:302-450)
+    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      const Consts &K = P.K;
+      int c = IXD(P, i, j, k);
+      float th = th3d[c];
+      if (moist) {
+        float ql = qc[c];
+        ql = ql + qr[c];
+        float qvv = fmaxf(qv[c], 0.0f);
+        float qli = fmaxf(0.0f, ql + 0.0f);
+        float cpli = K.cpl * fmaxf(0.0f, ql) + K.cpi * fmaxf(0.0f, 0.0f);
+        dum8[c] = K.g * (th * P.rth0[k] + K.repsm1 * (qvv - P.qv0[k]) - (qli - P.qc0[k] - P.qi0[k]));
+        t11[c] = (P.th0[k] + th) * (1.0f + K.reps * qvv) / (1.0f + qvv + qli);
+        float cpm = K.cp + K.cpv * qvv + cpli;
+        float cvm = 1.0f / (K.cv + K.cvv * qvv + cpli);
+        thterm[c] = (P.th0[k] + th) * (K.rd + K.rv * qvv - K.rovcp * cpm) * cvm;
+        t22[c] = (P.pi0[k] + pp3d[c]) * K.rovcp * cpm * cvm;
+      } else {
+        dum8[c] = K.g * th * P.rth0[k];
+        t11[c] = P.th0[k] + th;
+        t22[c] = (P.pi0[k] + pp3d[c]) * K.rddcv;
+      }
+    });
+    bc_fill(m, BC_P, t11);
+    // tendencies start from the pre-RK values (src/solve2.F:459-471)
+    for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      uten[c] = uten1[c];
+      vten[c] = vten1[c];
+      wten[c] = wten1[c];
+    });
+    // rru, rrv, rrw (src/solve2.F:490-545)
+    for3(m->stream, 0, ni + 2, 0, nj + 2, 1, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      if (j <= nj + 1) rru[c] = P.rho0[k] * u3d[c];
+      if (i <= ni + 1) rrv[c] = P.rho0[k] * v3d[c];
+      if (i <= ni + 1 && j <= nj + 1) {
+        if (k == 1) {
+          rrw[c] = 0.0f;
+          rrw[IXD(P, i, j, nk + 1)] = 0.0f;
+        } else {
+          rrw[c] = P.rf0[k] * w3d[c];
+        }
+      }
+    });
+    // divx (src/solve2.F:587-625)
+    for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      float d = ((rru[IXD(P, i + 1, j, k)] - rru[c]) * P.rdx * P.uh[i + 2] +
+                 (rrv[IXD(P, i, j + 1, k)] - rrv[c]) * P.rdy * P.vh[j + 2]) +
+                (rrw[IXD(P, i, j, k + 1)] - rrw[c]) * P.rdz * P.mh[k];
+      if (fabsf(d) < P.K.smeps) d = 0.0f;
+      divx[c] = d;
+    });
+    adv_uvw(m, nrk, 7, 1);
+    // sadv = pi0 + pi' (src/solve2.F:1027-1050)
+    for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      sadv[c] = P.pi0[k] + pp3d[c];
+    });
+    sound_run(m, nrk);
+    if (nrk == 3) cfl_run(m, dt, u3d, v3d, w3d);
+    if (m->c.irbc == 4 && (P.ibw || P.ibe || P.ibs || P.ibn))
+      radbc4(m, f[CM1G_UA], u3d, f[CM1G_VA], v3d, dttmp);
+    getdiv_both(m, moist);
+    // theta (src/solve2.F:1310-1428)
+    advs_run(m, nrk, th3d, 1.0f, f[CM1G_THA], f[CM1G_THTEN1], moist ? t11 : nullptr, nullptr,
+             th3d, 0, 0, dttmp, (double)(10.0f * P.K.epsilon), 1);
+    if (moist) {
+      for (int n = 0; n < 3; n++) {
+        advs_run(m, nrk, f[CM1G_Q3D0 + n], 0.0f, f[CM1G_QA0 + n], f[CM1G_QTEN0 + n], nullptr, nullptr,
+                 f[CM1G_Q3D0 + n], nrk == 3 ? 1 : 0, n == 0 ? 0 : 1, dttmp,
+                 (double)(0.01f * P.K.epsilon), 1);
+      }
+    }
+    // pressure and density (src/solve2.F:1536-1656)
+    if (nrk == 3) {
+      float *qvw = f[CM1G_Q3D0], *qcw = f[CM1G_Q3D1];
+      float *qpten = f[CM1G_QPTEN], *qtten = f[CM1G_QTTEN], *qvten = f[CM1G_QVTEN], *qcten = f[CM1G_QCTEN];
+      const int diab = (moist && m->c.ptype >= 1);
+      for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+        const Consts &K = P.K;
+        int c = IXD(P, i, j, k);
+        float pp = pp3d[c], th = th3d[c];
+        if (diab) {
+          pp = pp - dt * qpten[c];
+          qpten[c] = pp;
+          th = th - dt * qtten[c];
+          qtten[c] = th;
+          float q0 = qvw[c] - dt * qvten[c];
+          qvten[c] = q0;
+          qvw[c] = q0;
+          float q1 = qcw[c] - dt * qcten[c];
+          qcten[c] = q1;
+          qcw[c] = q1;
+          pp3d[c] = pp;
+          th3d[c] = th;
+        }
+        float p = K.p00 * pow_f(P.pi0[k] + pp, K.cpdrd);
+        prs[c] = p;
+        if (moist)
+          rho[c] = p / ((P.th0[k] + th) * (P.pi0[k] + pp) * (K.rd + fmaxf(0.0f, qvw[c]) * K.rv));
+        else
+          rho[c] = p / (K.rd * (P.th0[k] + th) * (P.pi0[k] + pp));
+      });
+      if (m->c.apmasscon == 1) mass_levels(m, rho, 1);
+    }
+    bc_fill(m, BC_U, u3d);
+    bc_fill(m, BC_V, v3d);
+    bc_fill(m, BC_W1, w3d);
+    if (nrk < 3) {
+      float *arr[5] = {rho, pp3d, nullptr, nullptr, nullptr};
+      bc_fill_n(m, BC_P, arr, 2);
+      float *sarr[4] = {th3d, f[CM1G_Q3D0], f[CM1G_Q3D1], f[CM1G_Q3D2]};
+      bc_fill_n(m, BC_S, sarr, moist ? 4 : 1);
+    }
+    // passive tracers (src/solve2.F:1902-1970)
+    for (int n = 0; n < m->c.npt; n++) {
+      advs_run(m, nrk, f[CM1G_PT3D0 + n], 0.0f, f[CM1G_PTA0 + n], f[CM1G_PTTEN0 + n], nullptr, nullptr,
+               f[CM1G_PT3D0 + n], (nrk == 3 && m->c.pdtra == 1) ? 1 : 0, 1, dttmp,
+               (double)(1.0f * P.K.epsilon), 0);
+      if (nrk <= 2) bc_fill(m, BC_S, f[CM1G_PT3D0 + n]);
+    }
+    // finish the dry-mass adjustment (src/solve2.F:1973-2030)
+    if (nrk == 3 && m->c.apmasscon == 1) {
+      const double *dscal = m->dscal;
+      float *qvw = f[CM1G_Q3D0];
+      for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+        const Consts &K = P.K;
+        int c = IXD(P, i, j, k);
+        const float tem = (float)dscal[2];
+        float pp = pp3d[c] + tem;
+        pp3d[c] = pp;
+        float p = K.p00 * pow_f(P.pi0[k] + pp, K.cpdrd);
+        prs[c] = p;
+        if (moist)
+          rho[c] = p / ((P.th0[k] + th3d[c]) * (P.pi0[k] + pp) * (K.rd + fmaxf(0.0f, qvw[c]) * K.rv));
+        else
+          rho[c] = p / (K.rd * (P.th0[k] + th3d[c]) * (P.pi0[k] + pp));
+      });
+    }
+  }
+  for (int n = 0; n < m->c.npt; n++) {
+    if (m->c.pdtra == 1) pdefq_run(m, 0.0f, rho, f[CM1G_PT3D0 + n]);
+    bc_fill(m, BC_S, f[CM1G_PT3D0 + n]);
+  }
+  return check_launch();
+}
+
+// --------------------------------------------------------------- mp_driver
+extern "C" int cm1g_mp_driver(cm1g_model *m) {
+  ARGCHK(m, "cm1g_mp_driver: null model");
+  if (m->c.imoist != 1) return 0;
+  if (m->c.ptype == 0) {
+    pdefq_run(m, 0.0f, m->f[CM1G_RHO], m->f[CM1G_Q3D0]);
+    return check_launch();
+  }
+  kessler_branch_run(m);
+  hipLaunchKernelGGL(k_train, dim3(1), dim3(1), 0, m->stream, m->P.ni, m->P.nj,
+                     (double)(m->c.dx * m->c.dy), m->dred, m->dscal);
+  return check_launch();
+}
+
+// ------------------------------------------------------------------ solve3
+extern "C" int cm1g_solve3(cm1g_model *m) {
+  ARGCHK(m, "cm1g_solve3: null model");
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  const float dt = m->c.dt;
+  float **f = m->f;
+  const int moist = m->c.imoist;
+  float *pp3d = f[CM1G_PP3D], *th3d = f[CM1G_TH3D], *ppx = f[CM1G_PPX], *rho = f[CM1G_RHO];
+  float *rr = f[CM1G_RR], *rf = f[CM1G_RF];
+  bc_fill(m, BC_P, pp3d);
+  {
+    float *sarr[4] = {th3d, f[CM1G_Q3D0], f[CM1G_Q3D1], f[CM1G_Q3D2]};
+    bc_fill_n(m, BC_S, sarr, moist ? 4 : 1);
+  }
+  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    int c = IXD(P, i, j, k);
+    ppx[c] = pp3d[c] + ppx[c];
+  });
+  {
+    float *arr[2] = {ppx, rho};
+    bc_fill_n(m, BC_P, arr, 2);
+  }
+  if (moist && m->c.ptype >= 1) {
+    const float rdt = 1.0f / dt;
+    float *qpten = f[CM1G_QPTEN], *qtten = f[CM1G_QTTEN], *qvten = f[CM1G_QVTEN], *qcten = f[CM1G_QCTEN];
+    const float *qv = f[CM1G_Q3D0], *qc = f[CM1G_Q3D1];
+    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      qpten[c] = (pp3d[c] - qpten[c]) * rdt;
+      qtten[c] = (th3d[c] - qtten[c]) * rdt;
+      qvten[c] = (qv[c] - qvten[c]) * rdt;
+      qcten[c] = (qc[c] - qcten[c]) * rdt;
+    });
+  }
+  {
+    float *ua = f[CM1G_UA], *va = f[CM1G_VA], *wa = f[CM1G_WA];
+    const float *u3d = f[CM1G_U3D], *v3d = f[CM1G_V3D], *w3d = f[CM1G_W3D];
+    for3(m->stream, 0, ni + 2, 0, nj + 2, 1, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      ua[c] = u3d[c];
+      va[c] = v3d[c];
+      wa[c] = w3d[c];
+    });
+  }
+  for (int n = 0; n < m->c.npt; n++) {
+    float *pta = f[CM1G_PTA0 + n];
+    const float *pt3d = f[CM1G_PT3D0 + n];
+    for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+      pta[IXD(P, i, j, k)] = pt3d[IXD(P, i, j, k)];
+    });
+  }
+  {
+    float *ppi = f[CM1G_PPI], *tha = f[CM1G_THA];
+    float *qa0 = f[CM1G_QA0], *qa1 = f[CM1G_QA1], *qa2 = f[CM1G_QA2];
+    const float *q0 = f[CM1G_Q3D0], *q1 = f[CM1G_Q3D1], *q2 = f[CM1G_Q3D2];
+    for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      ppi[c] = pp3d[c];
+      tha[c] = th3d[c];
+      if (moist) {
+        qa0[c] = q0[c];
+        qa1[c] = q1[c];
+        qa2[c] = q2[c];
+      }
+    });
+  }
+  for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+    int c = IXD(P, i, j, k);
+    rr[c] = 1.0f / rho[c];
+    if (k >= 2) rf[c] = P.c1[k] * rho[IXD(P, i, j, k - 1)] + P.c2[k] * rho[c];
+    else {
+      rf[c] = P.cgs1 * rho[IXD(P, i, j, 1)] + P.cgs2 * rho[IXD(P, i, j, 2)] + P.cgs3 * rho[IXD(P, i, j, 3)];
+      rf[IXD(P, i, j, nk + 1)] = P.cgt1 * rho[IXD(P, i, j, nk)] + P.cgt2 * rho[IXD(P, i, j, nk - 1)] +
+                                 P.cgt3 * rho[IXD(P, i, j, nk - 2)];
+    }
+  });
+  return check_launch();
+}
+
+extern "C" int cm1g_step(cm1g_model *m, float *cflmax) {
+  ARGCHK(m, "cm1g_step: null model");
+  int rc = cm1g_solve1(m);
+  if (!rc) rc = cm1g_solve2(m);
+  if (!rc) rc = cm1g_mp_driver(m);
+  if (!rc) rc = cm1g_solve3(m);
+  if (rc) return rc;
+  if (cflmax) {
+    HIPCHK(hipMemcpyAsync(&m->cflmax, m->fred, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    int flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, m->iflag, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    *cflmax = m->cflmax;
+    if (flag) {
+      cm1g_set_error("satadj: convergence cannot be reached (src/kessler.F:343-356)");
+      return CM1G_E_SATADJ;
+    }
+  }
+  return 0;
+}
+
+// --------------------------------------------------- individual routines
+extern "C" int cm1g_advs(cm1g_model *m, int nrk, int s_id, int s0_id, int sten_id, int pdef,
+                         int pdefweno, float dt, double weps) {
+  ARGCHK(field_ok(m, s_id) && field_ok(m, s0_id) && field_ok(m, sten_id), "cm1g_advs: bad field id");
+  ARGCHK(nrk >= 1 && nrk <= 3, "cm1g_advs: nrk must be 1..3");
+  // the model's rru/rrv/rrw are used as given (no rho0 scaling here)
+  const size_t bytes = m->ntot * sizeof(float);
+  HIPCHK(hipMemcpyAsync(m->rrus, m->f[CM1G_RRU], bytes, hipMemcpyDeviceToDevice, m->stream));
+  HIPCHK(hipMemcpyAsync(m->rrvs, m->f[CM1G_RRV], bytes, hipMemcpyDeviceToDevice, m->stream));
+  HIPCHK(hipMemcpyAsync(m->rrws, m->f[CM1G_RRW], bytes, hipMemcpyDeviceToDevice, m->stream));
+  float *sten = m->f[sten_id];
+  advs_run(m, nrk, m->f[s_id], 0.0f, m->f[s0_id], sten, nullptr, sten, nullptr, pdef, pdefweno, dt,
+           weps, 0);
+  return check_launch();
+}
+extern "C" int cm1g_advu(cm1g_model *m, int nrk) {
+  ARGCHK(m && nrk >= 1 && nrk <= 3, "cm1g_advu: bad argument");
+  adv_uvw(m, nrk, 1, 0);
+  return check_launch();
+}
+extern "C" int cm1g_advv(cm1g_model *m, int nrk) {
+  ARGCHK(m && nrk >= 1 && nrk <= 3, "cm1g_advv: bad argument");
+  adv_uvw(m, nrk, 2, 0);
+  return check_launch();
+}
+extern "C" int cm1g_advw(cm1g_model *m, int nrk) {
+  ARGCHK(m && nrk >= 1 && nrk <= 3, "cm1g_advw: bad argument");
+  adv_uvw(m, nrk, 4, 0);
+  return check_launch();
+}
+extern "C" int cm1g_sound(cm1g_model *m, int nrk, float dttmp) {
+  ARGCHK(m && nrk >= 1 && nrk <= 3, "cm1g_sound: bad argument");
+  (void)dttmp;
+  sound_run(m, nrk);
+  return check_launch();
+}
+extern "C" int cm1g_kessler_branch(cm1g_model *m) {
+  ARGCHK(m, "null model");
+  kessler_branch_run(m);
+  return check_launch();
+}
+extern "C" int cm1g_pdefq(cm1g_model *m, float rmax, int q_id) {
+  ARGCHK(field_ok(m, q_id), "cm1g_pdefq: bad field id");
+  pdefq_run(m, rmax, m->f[CM1G_RHO], m->f[q_id]);
+  return check_launch();
+}
+extern "C" int cm1g_bc(cm1g_model *m, int id, int kind) {
+  ARGCHK(field_ok(m, id) && kind >= 0 && kind <= 5, "cm1g_bc: bad argument");
+  bc_fill(m, kind, m->f[id]);
+  return check_launch();
+}
+extern "C" int cm1g_calccflquick(cm1g_model *m, float dt, int u_id, int v_id, int w_id, float *cflmax) {
+  ARGCHK(field_ok(m, u_id) && field_ok(m, v_id) && field_ok(m, w_id) && cflmax, "cm1g_calccflquick: bad argument");
+  cfl_run(m, dt, m->f[u_id], m->f[v_id], m->f[w_id]);
+  HIPCHK(hipMemcpyAsync(cflmax, m->fred, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return check_launch();
+}
+extern "C" int cm1g_getdiv(cm1g_model *m, int div_id) {
+  ARGCHK(field_ok(m, div_id), "cm1g_getdiv: bad field id");
+  getdiv_both(m, 1);
+  if (div_id != CM1G_T11)
+    HIPCHK(hipMemcpyAsync(m->f[div_id], m->f[CM1G_T11], m->ntot * sizeof(float), hipMemcpyDeviceToDevice, m->stream));
+  return check_launch();
+}
+extern "C" int cm1g_getdivx(cm1g_model *m) {
+  ARGCHK(m, "null model");
+  getdiv_both(m, 0);
+  return check_launch();
+}
+extern "C" int cm1g_scalars(cm1g_model *m, double *out4) {
+  ARGCHK(m && out4, "cm1g_scalars: bad argument");
+  double d[4];
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(d, m->dscal, 4 * sizeof(double), hipMemcpyDeviceToHost));
+  out4[0] = d[0];
+  out4[1] = d[3];
+  float c;
+  HIPCHK(hipMemcpy(&c, m->fred, sizeof(float), hipMemcpyDeviceToHost));
+  out4[2] = c;
+  out4[3] = d[1];
+  return 0;
+}
+
+// totals of dry-air mass and total water (calcmass/totmois analogues)
+__global__ void __launch_bounds__(256) k_budget(KP P, const float *rho, const float *qv, const float *qc,
+                                                const float *qr, int moist, double *out) {
+  __shared__ double r1[256], r2[256];
+  const int k = 1 + (int)blockIdx.x;
+  double a1 = 0.0, a2 = 0.0;
+  for (int t = threadIdx.x; t < P.ni * P.nj; t += blockDim.x) {
+    int i = 1 + t % P.ni, j = 1 + t / P.ni;
+    int c = IXD(P, i, j, k);
+    double vol = (double)P.ruh[i + 2] * (double)P.rvh[j + 2] * (double)P.rmh[k];
+    a1 += (double)rho[c] * vol;
+    if (moist) a2 += (double)rho[c] * vol * ((double)qv[c] + (double)qc[c] + (double)qr[c]);
+  }
+  r1[threadIdx.x] = a1;
+  r2[threadIdx.x] = a2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      r1[threadIdx.x] += r1[threadIdx.x + s];
+      r2[threadIdx.x] += r2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * k] = r1[0];
+    out[2 * k + 1] = r2[0];
+  }
+}
+
+extern "C" int cm1g_budgets(cm1g_model *m, double *tmass, double *tmois) {
+  ARGCHK(m && tmass && tmois, "cm1g_budgets: bad argument");
+  const int nk = m->P.nk;
+  double *buf = m->dred + (size_t)m->P.ni * m->P.nj;
+  hipLaunchKernelGGL(k_budget, dim3(nk), dim3(256), 0, m->stream, m->P, m->f[CM1G_RHO], m->f[CM1G_QA0],
+                     m->f[CM1G_QA1], m->f[CM1G_QA2], m->c.imoist, buf);
+  int rc = check_launch();
+  if (rc) return rc;
+  std::vector<double> h(2 * (nk + 1));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(h.data(), buf, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double vol = (double)m->c.dx * (double)m->c.dy * (double)m->c.dz;
+  double s1 = 0.0, s2 = 0.0;
+  for (int k = 1; k <= nk; k++) { s1 += h[2 * k]; s2 += h[2 * k + 1]; }
+  *tmass = s1 * vol;
+  *tmois = s2 * vol;
+  return 0;
+}

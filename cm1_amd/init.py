@@ -1,0 +1,172 @@
+"""Host-side set-up of grids, base states and initial perturbations.
+
+This is initialisation code (it runs once, on the host), restating the
+parts of the reference's preprocessors the supported configurations use:
+
+* vertical grid: unstretched, or the analytic stretching ``stretch_z=1``
+  (``src/param.F:7790-7960``), plus the metric terms zh, mh, mf
+  (``src/param.F:8140-8165``);
+* Weisman-Klemp analytic sounding ``isnd=5`` (``src/base.F:363-455``) with
+  the hydrostatic iteration and ``rslf`` (``src/cm1libs.F:16-62``);
+* wind profile ``iwnd=4``, ``u0 = umax*tanh(z/3 km)`` (``src/base.F:2396-2408``);
+  the reference hard-codes ``umax=35``; the value is a parameter here;
+* warm bubbles of the ``iinit=1`` form (``src/init3d.F:450-487``) with
+  configurable amplitude, centres and radii, and seeded uniform theta noise.
+
+Arrays are returned in the library's logical layout: 3-D fields as
+``(nk+3, nj+7, ni+7)`` for k=0..nk+2, j=-2..nj+4, i=-2..ni+4; columns as
+``(nk+3,)`` for k=0..nk+2.
+"""
+import math
+
+import numpy as np
+
+F32 = np.float32
+
+# constants (testcase=0 values of set_constants, src/constants.F:69-170)
+G, RD, RV, CP = F32(9.81), F32(287.04), F32(461.5), F32(1005.7)
+P00 = F32(1.0e5)
+EPS = RD / RV
+REPS = RV / RD
+
+
+def vertical_grid(nk, dz, stretch=None):
+    """zf, zh, mh, mf columns (k=0..nk+2) and the nominal dz.
+
+    ``stretch`` = dict(dz_bot, dz_top, str_bot, str_top, ztop) selects
+    stretch_z=1."""
+    zf = np.zeros(nk + 3, np.float64)
+    if stretch is None:
+        for k in range(0, nk + 3):
+            zf[k] = float(F32(dz) * F32(k - 1))
+        dzn = dz
+    else:
+        dzb, dzt = stretch["dz_bot"], stretch["dz_top"]
+        sb, st, zt = stretch["str_bot"], stretch["str_top"], stretch["ztop"]
+        nk1 = int(round(sb / dzb))
+        nk3 = int(round((zt - st) / dzt))
+        nk2 = nk - (nk1 + nk3)
+        if nk2 <= 1 or nk1 < 0 or nk3 < 0:
+            raise ValueError("inconsistent stretched grid")
+        nominal = (st - sb) / nk2
+        c2 = (nominal - dzb) / (nominal * nominal * float(nk2 - 1))
+        c1 = (dzb / nominal) - c2 * nominal
+        for k in range(1, nk1 + 2):
+            zf[k] = (k - 1) * dzb
+        for k in range(nk1 + 1, nk1 + nk2 + 2):
+            zf[k] = zf[nk1 + 1] + (c1 + c2 * float(k - 1 - nk1) * nominal) * float(k - 1 - nk1) * nominal
+        for k in range(nk1 + nk2 + 2, nk1 + nk2 + nk3 + 2):
+            zf[k] = zf[k - 1] + dzt
+        dzn = zt / nk
+        zf[0] = -zf[2]
+        zf[nk + 2] = zf[nk + 1] + (zf[nk + 1] - zf[nk])
+    zf = zf.astype(F32)
+    zh = np.zeros(nk + 3, F32)
+    mh = np.ones(nk + 3, F32)
+    mf = np.ones(nk + 3, F32)
+    for k in range(0, nk + 2):
+        zh[k] = F32(0.5) * (zf[k] + zf[k + 1])
+    if stretch is not None:
+        for k in range(0, nk + 2):
+            mh[k] = F32(dzn) / (zf[k + 1] - zf[k])
+        zh[0] = -zh[1]
+        zh[nk + 1] = zh[nk] + F32(2.0) * (zf[nk + 1] - zh[nk])
+        for k in range(1, nk + 2):
+            mf[k] = F32(dzn) / (zh[k] - zh[k - 1])
+        mf[0] = mf[1]
+        mf[nk + 2] = mf[nk + 1]
+    return dict(zf=zf, zh=zh, mh=mh, mf=mf), float(dzn)
+
+
+def rslf(p, t):
+    esl = 611.2 * math.exp(17.67 * (t - 273.15) / (t - 29.65))
+    esl = min(esl, p * 0.5)
+    return float(EPS) * esl / (p - esl)
+
+
+def wk_sounding(zh, nk, imoist=1, qv_pbl=0.014, qv_scale=1.0):
+    """Weisman-Klemp base state (src/base.F:363-455): th0, pi0, qv0 columns."""
+    z_trop, th_trop, t_trop, th_sfc, prs_sfc = 12000.0, 343.0, 213.0, 300.0, 100000.0
+    g, cp, rd = float(G), float(CP), float(RD)
+    reps = float(REPS)
+    pi_sfc = (prs_sfc / 1.0e5) ** (rd / cp)
+    qv_sfc = rslf(prs_sfc, th_sfc * pi_sfc)
+    thv_sfc = th_sfc * (1.0 + qv_sfc * reps) / (1.0 + qv_sfc)
+    th0 = np.zeros(nk + 3)
+    rh0 = np.zeros(nk + 3)
+    for k in range(1, nk + 1):
+        z = float(zh[k])
+        if z < z_trop:
+            th0[k] = th_sfc + (th_trop - th_sfc) * ((z / z_trop) ** 1.25)
+            rh0[k] = (1.0 - 0.75 * ((z / z_trop) ** 1.25)) if imoist else 0.0
+        else:
+            th0[k] = th_trop * math.exp((g / (t_trop * cp)) * (z - z_trop))
+            rh0[k] = 0.25 if imoist else 0.0
+    th0[0], th0[nk + 1] = th0[1], th0[nk]
+    qv0 = np.zeros(nk + 3)
+    pi0 = np.zeros(nk + 3)
+    for _ in range(20):
+        thv0 = th0 * (1.0 + reps * qv0) / (1.0 + qv0)
+        pi0[1] = pi_sfc - g * float(zh[1]) / (cp * 0.5 * (thv_sfc + thv0[1]))
+        for k in range(2, nk + 1):
+            pi0[k] = pi0[k - 1] - g * float(zh[k] - zh[k - 1]) / (cp * 0.5 * (thv0[k] + thv0[k - 1]))
+        for k in range(1, nk + 1):
+            prs = 1.0e5 * pi0[k] ** (cp / rd)
+            q = rh0[k] * rslf(prs, th0[k] * pi0[k]) * qv_scale
+            qv0[k] = min(q, qv_pbl) if imoist else 0.0
+    return dict(th0=th0.astype(F32), pi0=pi0.astype(F32), qv0=qv0.astype(F32))
+
+
+def wind_iwnd4(zh, nk, umax=35.0):
+    """u0(k) = umax*tanh(z/3 km) at u points (src/base.F:2396-2408)."""
+    u0 = np.zeros(nk + 3, F32)
+    for k in range(1, nk + 1):
+        u0[k] = F32(umax * math.tanh(float(zh[k]) / 3000.0))
+    return u0
+
+
+def blank(ni, nj, nk):
+    return np.zeros((nk + 3, nj + 7, ni + 7), F32)
+
+
+def initial_state(ni, nj, nk, dx, dy, dz, *, stretch=None, imoist=1, umax=35.0,
+                  bubbles=None, noise_amp=0.0, noise_seed=0, noise_zmax=None,
+                  qv_scale=1.0, npt=0, tracers=None):
+    """Columns and t=0 fields for a WK supercell-type case.
+
+    bubbles: list of dict(x, y, z, amp, rh, rv) in metres/K (iinit=1 form);
+    default one 1 K bubble at the domain centre, 10 km / 1.4 km radii.
+    """
+    cols, dzn = vertical_grid(nk, dz, stretch)
+    snd = wk_sounding(cols["zh"], nk, imoist=imoist, qv_scale=qv_scale)
+    cols.update(snd)
+    cols["u0"] = wind_iwnd4(cols["zh"], nk, umax)
+    f = {n: blank(ni, nj, nk) for n in ("ua", "va", "wa", "tha", "ppi", "qa0", "qa1", "qa2")}
+    for k in range(1, nk + 1):
+        f["ua"][k, :, :] = cols["u0"][k]
+        if imoist:
+            f["qa0"][k, :, :] = cols["qv0"][k]
+    if bubbles is None:
+        bubbles = [dict(x=0.5 * ni * dx, y=0.5 * nj * dy, z=1400.0, amp=1.0, rh=10000.0, rv=1400.0)]
+    xh = (np.arange(1, ni + 1) - 0.5) * dx
+    yh = (np.arange(1, nj + 1) - 0.5) * dy
+    zh = cols["zh"][1:nk + 1].astype(np.float64)
+    th = np.zeros((nk, nj, ni))
+    for b in bubbles:
+        beta = np.sqrt(((xh[None, None, :] - b["x"]) / b["rh"]) ** 2 +
+                       ((yh[None, :, None] - b["y"]) / b["rh"]) ** 2 +
+                       ((zh[:, None, None] - b["z"]) / b["rv"]) ** 2)
+        th += np.where(beta < 1.0, b["amp"] * np.cos(0.5 * np.pi * beta) ** 2, 0.0)
+    if noise_amp > 0.0:
+        rng = np.random.default_rng(noise_seed)
+        nz = rng.uniform(-noise_amp, noise_amp, size=th.shape)
+        if noise_zmax is not None:
+            nz *= (zh < noise_zmax)[:, None, None]
+        th += nz
+    f["tha"][1:nk + 1, 3:nj + 3, 3:ni + 3] = th.astype(F32)
+    for n in range(npt):
+        a = blank(ni, nj, nk)
+        if tracers is not None:
+            a[1:nk + 1, 3:nj + 3, 3:ni + 3] = tracers[n]
+        f[f"pta{n}"] = a
+    return cols, f, dzn

@@ -1,0 +1,129 @@
+"""Python host mirror of the CM1 driver around the cm1gpu C ABI.
+
+``Model`` owns one device-resident model (``cm1g_create``); fields move
+between host and device only through :meth:`Model.put` / :meth:`Model.get`
+(the residency protocol a Fortran host uses around its non-ported code).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+def make_config(**kw):
+    d = dict(ni=64, nj=64, nk=40, wbc=2, ebc=2, sbc=2, nbc=2, imoist=1, ptype=1,
+             psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4, nsound=0, npt=0,
+             pdtra=1, dx=1000.0, dy=1000.0, dz=500.0, dt=6.0, alph=0.6, kdiv=0.1,
+             csmax=350.0)
+    d.update(kw)
+    cfg = N.Config(**d)
+    if cfg.nsound <= 0:
+        ns = C.c_int32()
+        N.check(N.lib().cm1g_dtsmall(C.byref(cfg), C.byref(ns)), "cm1g_dtsmall")
+        cfg.nsound = ns.value
+    return cfg
+
+
+class Model:
+    """One device-resident CM1 dynamical core."""
+
+    def __init__(self, cfg):
+        self.L = N.lib()
+        self.cfg = cfg
+        h = C.c_void_p()
+        N.check(self.L.cm1g_create(C.byref(cfg), C.byref(h)), "cm1g_create")
+        self.h = h
+        px, py, pz = C.c_int64(), C.c_int64(), C.c_int64()
+        N.check(self.L.cm1g_dims(self.h, C.byref(px), C.byref(py), C.byref(pz)), "cm1g_dims")
+        self.px, self.py, self.pz = px.value, py.value, pz.value
+        self.ni, self.nj, self.nk = cfg.ni, cfg.nj, cfg.nk
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cm1g_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- columns
+    def put_column(self, name, col):
+        a = np.zeros(self.nk + 3, np.float32)
+        col = np.asarray(col, np.float32)
+        a[:min(len(col), self.nk + 3)] = col[:self.nk + 3]
+        N.check(self.L.cm1g_put_column(self.h, N.KID[name], a.ctypes.data), "put_column " + name)
+
+    def get_column(self, name):
+        a = np.zeros(self.nk + 3, np.float32)
+        N.check(self.L.cm1g_get_column(self.h, N.KID[name], a.ctypes.data), "get_column " + name)
+        return a
+
+    def set_columns(self, cols):
+        for n in ("zf", "zh", "mh", "mf"):
+            if n in cols:
+                self.put_column(n, cols[n])
+        N.check(self.L.cm1g_metrics(self.h), "cm1g_metrics")
+        for n in ("pi0", "th0", "qv0", "u0", "v0"):
+            if n in cols:
+                self.put_column(n, cols[n])
+        N.check(self.L.cm1g_base_derive(self.h), "cm1g_base_derive")
+
+    # ---- 3-D fields in the logical (nk+3, nj+7, ni+7) box
+    def put(self, name, arr):
+        full = np.zeros((self.pz, self.py, self.px), np.float32)
+        full[:, :, :self.ni + 7] = arr
+        N.check(self.L.cm1g_put_field(self.h, N.FID[name], full.ctypes.data), "put " + name)
+
+    def get(self, name):
+        full = np.empty((self.pz, self.py, self.px), np.float32)
+        N.check(self.L.cm1g_get_field(self.h, N.FID[name], full.ctypes.data), "get " + name)
+        return full[:, :, :self.ni + 7].copy()
+
+    def get_2d(self, name):
+        n = self.px * self.py + (self.px + self.py) * self.pz
+        a = np.empty(n, np.float32)
+        N.check(self.L.cm1g_get_2d(self.h, N.D2[name], a.ctypes.data), "get_2d " + name)
+        return a
+
+    def interior(self, name, stag=None):
+        a = self.get(name)
+        ni, nj, nk = self.ni, self.nj, self.nk
+        ei, ej, ek = (1 if stag == "u" else 0), (1 if stag == "v" else 0), (1 if stag == "w" else 0)
+        return a[1:nk + 1 + ek, 3:nj + 3 + ej, 3:ni + 3 + ei]
+
+    # ---- stepping
+    def getset(self, fresh=True):
+        N.check(self.L.cm1g_getset(self.h, 1 if fresh else 0), "cm1g_getset")
+
+    def step(self, n=1):
+        cfl = C.c_float()
+        for _ in range(n):
+            N.check(self.L.cm1g_step(self.h, C.byref(cfl)), "cm1g_step")
+        return cfl.value
+
+    def sync(self):
+        N.check(self.L.cm1g_sync(self.h), "cm1g_sync")
+
+    def budgets(self):
+        a, b = C.c_double(), C.c_double()
+        N.check(self.L.cm1g_budgets(self.h, C.byref(a), C.byref(b)), "cm1g_budgets")
+        return a.value, b.value
+
+    def scalars(self):
+        a = (C.c_double * 4)()
+        N.check(self.L.cm1g_scalars(self.h, a), "cm1g_scalars")
+        return dict(mass1=a[0], train=a[1], cflmax=a[2], mass2=a[3])
+
+
+def setup(cfg, cols, fields, fresh=True):
+    """Create a model, load columns and t=0 fields, run getset."""
+    m = Model(cfg)
+    m.set_columns(cols)
+    for name, arr in fields.items():
+        m.put(name, arr)
+    m.getset(fresh)
+    return m

@@ -1,0 +1,178 @@
+/*
+ * cm1gpu: MI355X (gfx950) implementation of the CM1 r21.1 dynamical-core
+ * time step behind a plain C ABI.
+ *
+ * Every prognostic and scratch array of the step lives in HBM for the life
+ * of a model handle.  The host hands arrays over once (cm1g_put_field),
+ * runs steps or individual routines, and reads arrays back only when a
+ * host-side routine needs them (cm1g_get_field), which is the residency
+ * protocol a Fortran host uses around its non-ported code (statistics,
+ * output, restart).
+ *
+ * Array layout on the device: every 3-D array (scalar, u, v and w
+ * staggerings alike) is one padded box i = -2..ni+4, j = -2..nj+4,
+ * k = 0..nk+2, i fastest, element (i,j,k) at
+ *     (i + 2) + (j + 2) * px + k * px * py,
+ * with px >= ni + 7 (rounded up for 256-byte rows) and py = nj + 7.  This is
+ * a superset of the reference bounds ib:ie(+1), jb:je(+1), kb:ke(+1) with
+ * ngxy = 3 and ngz = 1 (src/cm1.F:747-777); cm1g_put_field/cm1g_get_field
+ * convert from/to the caller's own Fortran-ordered box.
+ *
+ * All entry points return 0 on success and a nonzero CM1G_E_* code on
+ * failure; cm1g_last_error() returns the message.  The reference has no
+ * return codes: its fatal paths call stopcm1 (src/stopcm1.F), which a
+ * Fortran wrapper does on a nonzero return.  Calls on one handle must not
+ * overlap; the library is single-stream per handle.
+ */
+#ifndef CM1GPU_H
+#define CM1GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CM1G_OK = 0,
+  CM1G_E_ARG = 1,      /* invalid argument or configuration */
+  CM1G_E_HIP = 2,      /* HIP runtime error */
+  CM1G_E_NOMEM = 3,    /* device allocation failed */
+  CM1G_E_UNSUPPORTED = 4,
+  CM1G_E_SATADJ = 5,   /* saturation adjustment did not converge (src/kessler.F:253-257) */
+  CM1G_E_CFL = 6       /* cflmax >= 1.5 (src/cm1.F:3020-3024) */
+};
+
+/* Namelist subset read by the hot path (module input, src/input.F:261-351). */
+typedef struct cm1g_config {
+  int32_t ni, nj, nk;             /* nx, ny, nz of one rank (single GPU: the whole domain) */
+  int32_t wbc, ebc, sbc, nbc;     /* 1 periodic, 2 open-radiative (src/bc.F) */
+  int32_t imoist;                 /* 0 dry, 1 moist */
+  int32_t ptype;                  /* 0 none, 1 Kessler */
+  int32_t psolver;                /* 3 vertically implicit (sound.F); 2 explicit (sounde.F) */
+  int32_t advwenos, advwenov;     /* 0 flux-form 5th order; 1 WENO5 every stage; 2 last stage */
+  int32_t apmasscon;              /* 1: dry-mass conserving pi' adjustment */
+  int32_t irbc;                   /* 4: radbcew4/radbcns4 */
+  int32_t nsound;                 /* acoustic steps per large step (T1: cm1g_dtsmall) */
+  int32_t npt;                    /* passive tracers */
+  int32_t pdtra;                  /* positive-definite tracers */
+  float dx, dy, dz, dt;
+  float alph, kdiv;               /* param3 */
+  float csmax;                    /* 350 (src/constants.F) */
+} cm1g_config;
+
+/* 3-D field ids (names follow the reference's arrays). */
+enum {
+  CM1G_UA, CM1G_U3D, CM1G_UTEN, CM1G_UTEN1, CM1G_VA, CM1G_V3D, CM1G_VTEN, CM1G_VTEN1,
+  CM1G_WA, CM1G_W3D, CM1G_WTEN, CM1G_WTEN1, CM1G_RRU, CM1G_RRV, CM1G_RRW,
+  CM1G_PPI, CM1G_PP3D, CM1G_PPTEN, CM1G_STEN, CM1G_SADV, CM1G_PPX,
+  CM1G_THA, CM1G_TH3D, CM1G_THTEN, CM1G_THTEN1, CM1G_THTERM,
+  CM1G_QPTEN, CM1G_QTTEN, CM1G_QVTEN, CM1G_QCTEN,
+  CM1G_RHO, CM1G_RR, CM1G_RF, CM1G_PRS, CM1G_DIVX,
+  CM1G_DUM1, CM1G_DUM2, CM1G_DUM3, CM1G_DUM4, CM1G_DUM5, CM1G_DUM6, CM1G_DUM7,
+  CM1G_DUM8, CM1G_DUM9, CM1G_T11, CM1G_T12, CM1G_T13, CM1G_T22, CM1G_T23, CM1G_T33,
+  CM1G_QA0, CM1G_QA1, CM1G_QA2, CM1G_Q3D0, CM1G_Q3D1, CM1G_Q3D2,
+  CM1G_QTEN0, CM1G_QTEN1, CM1G_QTEN2,
+  CM1G_PTA0, CM1G_PT3D0 = CM1G_PTA0 + 8, CM1G_PTTEN0 = CM1G_PT3D0 + 8,
+  CM1G_NFIELDS = CM1G_PTTEN0 + 8
+};
+
+/* 1-D vertical metric and base-state columns, index k = 0..nk+2. */
+enum {
+  CM1G_K_MH, CM1G_K_RMH, CM1G_K_MF, CM1G_K_RMF, CM1G_K_C1, CM1G_K_C2, CM1G_K_ZH, CM1G_K_ZF,
+  CM1G_K_PI0, CM1G_K_TH0, CM1G_K_PRS0, CM1G_K_QV0, CM1G_K_RHO0, CM1G_K_RR0, CM1G_K_RF0,
+  CM1G_K_RRF0, CM1G_K_THV0, CM1G_K_RTH0, CM1G_K_QC0, CM1G_K_QI0, CM1G_K_U0, CM1G_K_V0,
+  CM1G_K_NFIELDS
+};
+
+/* 2-D fields: rain, prate are (i,j); radbcw/e are (j,k); radbcs/n are (i,k). */
+enum { CM1G_D_RAIN, CM1G_D_PRATE, CM1G_D_RADBCW, CM1G_D_RADBCE, CM1G_D_RADBCS,
+       CM1G_D_RADBCN, CM1G_D_NFIELDS };
+
+typedef struct cm1g_model cm1g_model;
+
+/* --- set-up, residency ------------------------------------------------ */
+
+/* Replaces the module-scalar hand-over of param()/set_constants()
+ * (src/param.F:189-277, src/constants.F:69-170) and the device allocation
+ * of every array of src/cm1.F:779-1700.  Vertical metrics default to an
+ * unstretched grid (src/param.F:7790-7810); override them with
+ * cm1g_put_column and call cm1g_metrics. */
+int cm1g_create(const cm1g_config *cfg, cm1g_model **out);
+int cm1g_destroy(cm1g_model *m);
+const char *cm1g_last_error(void);
+int cm1g_dims(const cm1g_model *m, int64_t *px, int64_t *py, int64_t *pz);
+
+/* Copy a host column (k = 0..nk+2, nk+3 values) into the model. */
+int cm1g_put_column(cm1g_model *m, int id, const float *host);
+int cm1g_get_column(cm1g_model *m, int id, float *host);
+/* Recompute rmh, rmf, c1, c2 and the extrapolation coefficients from
+ * zh, zf, mh, mf (src/param.F:8590-8622, :8980-9060). */
+int cm1g_metrics(cm1g_model *m);
+/* rho0, rr0, rf0, rrf0, thv0, rth0, prs0 from pi0, th0, qv0
+ * (src/base.F:2085-2198). */
+int cm1g_base_derive(cm1g_model *m);
+
+/* Copy a whole padded host box (px*py*pz floats, device layout) in/out. */
+int cm1g_put_field(cm1g_model *m, int id, const float *host);
+int cm1g_get_field(cm1g_model *m, int id, float *host);
+int cm1g_put_2d(cm1g_model *m, int id, const float *host);
+int cm1g_get_2d(cm1g_model *m, int id, float *host);
+/* Device address of a field (for zero-copy use by a caller that already
+ * holds device memory, e.g. a Fortran host with its own device mirrors). */
+int cm1g_field_ptr(cm1g_model *m, int id, float **dptr);
+/* Wait for all queued work of this handle. */
+int cm1g_sync(cm1g_model *m);
+
+/* --- the time step ---------------------------------------------------- */
+
+/* getset: halo fills, 3d = a copies, rho/prs, rr/rf, mass1, ppx
+ * (src/init3d.F:2032-2872).  fresh_start=0 is the restart path. */
+int cm1g_getset(cm1g_model *m, int fresh_start);
+/* One large step: solve1 + solve2 + mp_driver + solve3
+ * (src/cm1.F:3313-3476).  Sets *cflmax (may be NULL). */
+int cm1g_step(cm1g_model *m, float *cflmax);
+int cm1g_solve1(cm1g_model *m);   /* src/solve1.F:77-1791 (ten1, ppten) */
+int cm1g_solve2(cm1g_model *m);   /* src/solve2.F:77-2094 (RK3 loop) */
+int cm1g_mp_driver(cm1g_model *m);/* src/mp_driver.F:17-1706 (ptype 0/1) */
+int cm1g_solve3(cm1g_model *m);   /* src/solve3.F:77-1143 */
+
+/* --- individual routines (same argument meaning as the reference) ------ */
+
+/* advs(nrk,...,s0,s,sten,pdef,pdefweno,dt,weps,...), src/adv.F:14-903.
+ * Uses the model's rru/rrv/rrw/divx; the s, s0 and sten fields are given
+ * by id. */
+int cm1g_advs(cm1g_model *m, int nrk, int s_id, int s0_id, int sten_id,
+              int pdef, int pdefweno, float dt, double weps);
+int cm1g_advu(cm1g_model *m, int nrk);   /* src/adv.F:911-1839  */
+int cm1g_advv(cm1g_model *m, int nrk);   /* src/adv.F:1847-2770 */
+int cm1g_advw(cm1g_model *m, int nrk);   /* src/adv.F:2778-3610 */
+/* sound(dt,...,nrk,dttmp,...), src/sound.F:10-859, with the solve2 call's
+ * arrays: thv=t11, ppterm=t22, piadv=sadv (src/solve2.F:1122-1136). */
+int cm1g_sound(cm1g_model *m, int nrk, float dttmp);
+/* Kessler branch of mp_driver: pdefq x3, k_fallout, fallout, kessler,
+ * satadj (src/mp_driver.F:190-232, src/kessler.F). */
+int cm1g_kessler_branch(cm1g_model *m);
+/* pdefq(rmax,asq,...,rho,q3d), src/misclibs.F:1037-1117 */
+int cm1g_pdefq(cm1g_model *m, float rmax, int q_id);
+/* Boundary fills, src/bc.F: kind 0 bcs :180, 1 bcp :372, 2 bcu :528,
+ * 3 bcv :722, 4 bcw(flag=1) :942, 5 bcw(flag=0). */
+int cm1g_bc(cm1g_model *m, int id, int kind);
+/* calccflquick(dt,uh,vh,mh,ua,va,wa,...), src/misclibs.F:2726-2814 */
+int cm1g_calccflquick(cm1g_model *m, float dt, int u_id, int v_id, int w_id, float *cflmax);
+/* getdiv / getdivx on the model's rru/rrv/rrw, src/misclibs.F:108-370 */
+int cm1g_getdiv(cm1g_model *m, int div_id);
+int cm1g_getdivx(cm1g_model *m);
+/* dtsmall: nsound for dt (src/misclibs.F:818-915) */
+int cm1g_dtsmall(const cm1g_config *cfg, int32_t *nsound);
+/* Domain budgets in double precision: dry-air mass (calcmass,
+ * src/misclibs.F:3134-3191) and total water qv+qc+qr (totmois,
+ * src/misclibs.F:3199-3264). */
+int cm1g_budgets(cm1g_model *m, double *tmass, double *tmois);
+/* Scalars carried between steps: [0] mass1, [1] train (domain rain
+ * accumulation, src/kessler.F:472), [2] cflmax, [3] mass2. */
+int cm1g_scalars(cm1g_model *m, double *out4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,133 @@
+"""Shared helpers of the parity tests: one initial state loaded into both the
+HIP model and the CPU oracle, and field comparisons."""
+import ctypes as C
+
+import numpy as np
+
+import pyoracle
+from cm1_amd import init as hinit
+
+# prognostic fields compared after stepping: (gpu/oracle name, staggering)
+PROG = [("ua", "u"), ("va", "v"), ("wa", "w"), ("tha", None), ("ppi", None),
+        ("qa0", None), ("qa1", None), ("qa2", None), ("rho", None), ("prs", None)]
+
+
+def case(ni=40, nj=36, nk=30, dx=1000.0, dy=1000.0, dz=500.0, dt=6.0, bc=2, imoist=1,
+         ptype=1, nsound=6, amp=3.0, umax=25.0, stretch=None, noise=0.1, seed=7, npt=0,
+         bfx=0.45, bfy=0.55, brh=None):
+    cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=bc, ebc=bc, sbc=bc, nbc=bc, imoist=imoist,
+                ptype=ptype, psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4,
+                nsound=nsound, npt=npt, pdtra=1, dx=dx, dy=dy, dz=dz, dt=dt, alph=0.6,
+                kdiv=0.1, csmax=350.0)
+    bub = [dict(x=bfx * ni * dx, y=bfy * nj * dy, z=1400.0, amp=amp,
+                rh=(8.0 * dx if brh is None else brh), rv=1400.0)]
+    cols, fields, dzn = hinit.initial_state(ni, nj, nk, dx, dy, dz, stretch=stretch,
+                                            imoist=imoist, umax=umax, bubbles=bub,
+                                            noise_amp=noise, noise_seed=seed, npt=npt)
+    if stretch is not None:
+        cfgd["dz"] = dzn
+    return cfgd, cols, fields
+
+
+def make_oracle(cfgd, cols, fields):
+    cfg = pyoracle.make_config(**cfgd)
+    o = pyoracle.Oracle(cfg)
+    for n in ("zf", "zh", "mh", "mf"):
+        o.kf(n)[:cfgd["nk"] + 3] = cols[n]
+    o.L.cm1o_set_extrap(o.h, None)
+    for n in ("pi0", "th0", "qv0"):
+        o.kf(n)[:cfgd["nk"] + 3] = cols[n]
+    o.base_derive()
+    for name, arr in fields.items():
+        o.f(name)[:, :, :] = arr
+    o.getset(True)
+    return o
+
+
+def make_gpu(cfgd, cols, fields):
+    from cm1_amd import model
+    cfg = model.make_config(**cfgd)
+    return model.setup(cfg, cols, fields)
+
+
+def interior(a, nk, nj, ni, stag=None):
+    ei, ej, ek = (1 if stag == "u" else 0), (1 if stag == "v" else 0), (1 if stag == "w" else 0)
+    return a[1:nk + 1 + ek, 3:nj + 3 + ej, 3:ni + 3 + ei]
+
+
+def rel_l2(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    den = np.sqrt((b * b).sum())
+    num = np.sqrt(((a - b) ** 2).sum())
+    if den == 0.0:
+        return 0.0 if num == 0.0 else np.inf
+    return num / den
+
+
+def load_fixture(path):
+    """Reference t=0 state -> (cfgd, cols, fields), plus the t=N outputs."""
+    import os
+    z = np.load(path)
+    ni, nj, nk, nsteps, bc = [int(v) for v in z["meta"]]
+    dt = float(z["dt"][0])
+    cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=bc, ebc=bc, sbc=bc, nbc=bc, imoist=1, ptype=1,
+                psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4, nsound=0, npt=0,
+                pdtra=1, dx=1000.0, dy=1000.0, dz=500.0, dt=dt, alph=0.6, kdiv=0.1, csmax=350.0)
+    cols, _ = hinit.vertical_grid(nk, 500.0)
+    for n in ("pi0", "th0", "qv0"):
+        c = np.zeros(nk + 3, np.float32)
+        c[1:nk + 1] = z[n]
+        cols[n] = c
+    f = {n: hinit.blank(ni, nj, nk) for n in ("ua", "va", "wa", "tha", "ppi", "qa0", "qa1", "qa2")}
+    f["ua"][1:nk + 1, 3:nj + 3, 3:ni + 4] = z["t0_u"]
+    f["va"][1:nk + 1, 3:nj + 4, 3:ni + 3] = z["t0_v"]
+    f["wa"][1:nk + 2, 3:nj + 3, 3:ni + 3] = z["t0_w"]
+    f["tha"][1:nk + 1, 3:nj + 3, 3:ni + 3] = z["t0_th"]
+    f["ppi"][1:nk + 1, 3:nj + 3, 3:ni + 3] = z["t0_pi"]
+    f["qa0"][1:nk + 1, 3:nj + 3, 3:ni + 3] = z["t0_qv"]
+    f["qa1"][1:nk + 1, 3:nj + 3, 3:ni + 3] = z["t0_qc"]
+    f["qa2"][1:nk + 1, 3:nj + 3, 3:ni + 3] = z["t0_qr"]
+    target = {"ua": z["tN_u"], "va": z["tN_v"], "wa": z["tN_w"], "tha": z["tN_th"],
+              "ppi": z["tN_pi"], "qa0": z["tN_qv"], "qa1": z["tN_qc"], "qa2": z["tN_qr"],
+              "rho": z["tN_rho"], "prs": z["tN_prs"]}
+    return cfgd, cols, f, nsteps, target, z
+
+
+def nsound_for(cfgd):
+    """dtsmall (src/misclibs.F:818-915) for psolver=3, in Python."""
+    dtsm = np.float32(0.60) * np.float32(min(cfgd["dx"], cfgd["dy"])) / np.float32(360.0)
+    ns = max(int(round(cfgd["dt"] / float(dtsm))), 4)
+    if ns % 2:
+        ns += 1
+    if cfgd["dt"] / float(np.float32(ns)) > float(dtsm):
+        ns += 2
+    return ns
+
+
+def ulp_floor(cfgd, cols, fields, nsteps, make=None):
+    """The FP32 noise floor of a case: per-field relative L2 between two
+    oracle runs of nsteps that differ only by a one-ulp change of pi' in
+    every cell after the first step (the size of change one misrounded
+    pow/exp makes through the dry-mass pressure adjustment)."""
+    ni, nj, nk = cfgd["ni"], cfgd["nj"], cfgd["nk"]
+    a = make_oracle(cfgd, cols, fields)
+    b = make_oracle(cfgd, cols, fields)
+    a.step(nsteps)
+    b.step(1)
+    pp = b.f("pp3d")
+    pp[:] = np.nextafter(pp, np.float32(1.0))
+    b.f("ppi")[:] = pp
+    b.step(nsteps - 1)
+    out = {}
+    for name, stag in PROG:
+        x = interior(a.f(name), nk, nj, ni, stag)
+        y = interior(b.f(name), nk, nj, ni, stag)
+        out[name] = rel_l2(y, x)
+    return out
+
+
+def parity_bound(floor, name, tol=1.0e-5):
+    """Per-field bound: the north-star 1e-5, or three times the case's FP32
+    noise floor where that floor is itself above 1e-5."""
+    return max(tol, 3.0 * floor.get(name, 0.0))

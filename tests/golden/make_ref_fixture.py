@@ -1,0 +1,57 @@
+"""Generate golden vectors from the reference CM1 executable.
+
+Runs ``oracle/_ref/cm1_sp.exe`` (built from the reference's own Fortran
+sources by ``oracle/build_ref.py``) on a small Kessler supercell case and
+stores its GrADS output at t=0 and after N large steps as float32 arrays:
+
+* ``t0_*``: the initial state (u, v, w, theta', pi', qv, qc, qr) and the
+  base state (pi0, th0, qv0, prs0) the reference wrote at t=0;
+* ``tN_*``: the same prognostics plus rho and prs after N steps.
+
+Usage: python tests/golden/make_ref_fixture.py [name ni nj nk nsteps dt wbc]
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import refrun  # noqa: E402
+
+
+def make(name, ni, nj, nk, nsteps, dt, bc, workdir="/tmp/cm1_fixture"):
+    ov = dict(nx=ni, ny=nj, nz=nk, dtl=dt, timax=nsteps * dt, tapfrq=nsteps * dt,
+              wbc=bc, ebc=bc, sbc=bc, nbc=bc)
+    refrun.run_reference(workdir, ov, threads=1)
+    S = refrun.read_grads(workdir, "cm1out_s.ctl")
+    U = refrun.read_grads(workdir, "cm1out_u.ctl")
+    V = refrun.read_grads(workdir, "cm1out_v.ctl")
+    W = refrun.read_grads(workdir, "cm1out_w.ctl")
+    out = dict(meta=np.array([ni, nj, nk, nsteps, bc], np.int32), dt=np.array([dt], np.float32))
+    for t, tag in ((0, "t0"), (1, "tN")):
+        out[f"{tag}_u"] = U["u"][t]
+        out[f"{tag}_v"] = V["v"][t]
+        out[f"{tag}_w"] = W["w"][t]
+        out[f"{tag}_th"] = S["thpert"][t]
+        out[f"{tag}_pi"] = S["pipert"][t]
+        out[f"{tag}_qv"] = S["qv"][t]
+        out[f"{tag}_qc"] = S["qc"][t]
+        out[f"{tag}_qr"] = S["qr"][t]
+        out[f"{tag}_rho"] = S["rho"][t]
+        out[f"{tag}_prs"] = S["prs"][t]
+    for n in ("pi0", "th0", "qv0", "prs0"):
+        out[n] = S[n][0, :, 0, 0]
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **{k: np.ascontiguousarray(v) for k, v in out.items()})
+    return path
+
+
+if __name__ == "__main__":
+    if len(sys.argv) > 1:
+        a = sys.argv[1:]
+        print(make(a[0], int(a[1]), int(a[2]), int(a[3]), int(a[4]), float(a[5]), int(a[6])))
+    else:
+        print(make("ref_open_24x20x18_10", 24, 20, 18, 10, 6.0, 2))
+        print(make("ref_periodic_20x24x18_10", 20, 24, 18, 10, 6.0, 1))
