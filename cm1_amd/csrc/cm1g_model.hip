@@ -146,10 +146,10 @@ extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
   for (auto p : scr)
     if (!rc) rc = dalloc(p, m->ntot);
   if (!rc) {
-    size_t nd = (size_t)cfg->ni * cfg->nj + 4 * (size_t)pz + 64;
+    size_t nd = (size_t)cfg->ni * cfg->nj + (size_t)cfg->nj + 2 * 32 * (size_t)pz + 4 * (size_t)pz + 64;
     if (hipMalloc((void **)&m->dred, nd * sizeof(double)) != hipSuccess) rc = CM1G_E_NOMEM;
     if (!rc && hipMalloc((void **)&m->dscal, 16 * sizeof(double)) != hipSuccess) rc = CM1G_E_NOMEM;
-    if (!rc && hipMalloc((void **)&m->fred, (pz + 8) * sizeof(float)) != hipSuccess) rc = CM1G_E_NOMEM;
+    if (!rc && hipMalloc((void **)&m->fred, 32 * (pz + 8) * sizeof(float)) != hipSuccess) rc = CM1G_E_NOMEM;
     if (!rc && hipMalloc((void **)&m->iflag, 16 * sizeof(int)) != hipSuccess) rc = CM1G_E_NOMEM;
     if (!rc) {
       hipMemset(m->dscal, 0, 16 * sizeof(double));
@@ -371,16 +371,17 @@ static int check_launch() {
 // ------------------------------------------------------------ reductions
 // Per-level partial sums of the dry-air mass and of pi for the apmasscon
 // adjustment (src/solve2.F:1617-1656), in double precision.
+#define MASS_NB 32
 __global__ void __launch_bounds__(256) k_mass_levels(KP P, const float *rho, const float *pp3d, double *out) {
   __shared__ double r3[256], r4[256];
-  const int k = 1 + (int)blockIdx.x;
+  const int k = 1 + (int)blockIdx.x, b = (int)blockIdx.y;
   double a3 = 0.0, a4 = 0.0;
-  for (int t = threadIdx.x; t < P.ni * P.nj; t += blockDim.x) {
-    int i = 1 + t % P.ni, j = 1 + t / P.ni;
-    int c = IXD(P, i, j, k);
-    a3 += (double)(rho[c] * P.ruh[i + 2] * P.rvh[j + 2] * P.rmh[k]);
-    a4 += (double)(P.pi0[k] + pp3d[c]);
-  }
+  for (int j = 1 + b; j <= P.nj; j += MASS_NB)
+    for (int i = 1 + (int)threadIdx.x; i <= P.ni; i += blockDim.x) {
+      int c = IXD(P, i, j, k);
+      a3 += (double)(rho[c] * P.ruh[i + 2] * P.rvh[j + 2] * P.rmh[k]);
+      a4 += (double)(P.pi0[k] + pp3d[c]);
+    }
   r3[threadIdx.x] = a3;
   r4[threadIdx.x] = a4;
   __syncthreads();
@@ -392,8 +393,8 @@ __global__ void __launch_bounds__(256) k_mass_levels(KP P, const float *rho, con
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[2 * k] = r3[0];
-    out[2 * k + 1] = r4[0];
+    out[2 * (k * MASS_NB + b)] = r3[0];
+    out[2 * (k * MASS_NB + b) + 1] = r4[0];
   }
 }
 
@@ -410,8 +411,13 @@ __device__ inline void dd_add(double &hi, double &lo, double x) {
 __global__ void k_mass_final(int nk, int mode, float rddcv, double ntot, const double *lev, double *dscal) {
   double h3 = 0.0, l3 = 0.0, h4 = 0.0, l4 = 0.0;
   for (int k = 1; k <= nk; k++) {
-    dd_add(h3, l3, lev[2 * k]);
-    dd_add(h4, l4, lev[2 * k + 1]);
+    double s3 = 0.0, s4 = 0.0;
+    for (int b = 0; b < MASS_NB; b++) {
+      s3 += lev[2 * (k * MASS_NB + b)];
+      s4 += lev[2 * (k * MASS_NB + b) + 1];
+    }
+    dd_add(h3, l3, s3);
+    dd_add(h4, l4, s4);
   }
   double mass = h3 + l3, p2 = h4 + l4;
   if (mode == 0) {
@@ -424,7 +430,7 @@ __global__ void k_mass_final(int nk, int mode, float rddcv, double ntot, const d
 }
 
 static void mass_levels(cm1g_model *m, const float *rho, int mode) {
-  hipLaunchKernelGGL(k_mass_levels, dim3(m->P.nk), dim3(256), 0, m->stream, m->P, rho,
+  hipLaunchKernelGGL(k_mass_levels, dim3(m->P.nk, MASS_NB), dim3(256), 0, m->stream, m->P, rho,
                      m->f[CM1G_PP3D], m->dred);
   double ntot = (double)((long long)m->P.ni * m->P.nj * m->P.nk);
   hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(1), 0, m->stream, m->P.nk, mode, m->P.K.rddcv,
@@ -433,13 +439,16 @@ static void mass_levels(cm1g_model *m, const float *rho, int mode) {
 
 // domain rain-accumulation budget (src/kessler.F:470-476): per-row sums in
 // i order, then rows in j order, as the reference does.
-__global__ void k_train(int ni, int nj, double tem, const double *colbud, double *dscal) {
+__global__ void k_train_rows(int ni, int nj, const double *colbud, double *rows) {
+  int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j >= nj) return;
+  double b = 0.0;
+  for (int i = 0; i < ni; i++) b = b + colbud[i + j * ni];
+  rows[j] = b;
+}
+__global__ void k_train(int nj, double tem, const double *rows, double *dscal) {
   double tr = dscal[3];
-  for (int j = 0; j < nj; j++) {
-    double b = 0.0;
-    for (int i = 0; i < ni; i++) b = b + colbud[i + j * ni];
-    tr = tr + b * tem;
-  }
+  for (int j = 0; j < nj; j++) tr = tr + rows[j] * tem;
   dscal[3] = tr;
 }
 
@@ -751,8 +760,11 @@ extern "C" int cm1g_mp_driver(cm1g_model *m) {
     return check_launch();
   }
   kessler_branch_run(m);
-  hipLaunchKernelGGL(k_train, dim3(1), dim3(1), 0, m->stream, m->P.ni, m->P.nj,
-                     (double)(m->c.dx * m->c.dy), m->dred, m->dscal);
+  double *rows = m->dred + (size_t)m->P.ni * m->P.nj;
+  hipLaunchKernelGGL(k_train_rows, dim3((m->P.nj + 63) / 64), dim3(64), 0, m->stream, m->P.ni,
+                     m->P.nj, m->dred, rows);
+  hipLaunchKernelGGL(k_train, dim3(1), dim3(1), 0, m->stream, m->P.nj,
+                     (double)(m->c.dx * m->c.dy), rows, m->dscal);
   return check_launch();
 }
 

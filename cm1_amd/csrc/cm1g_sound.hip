@@ -248,36 +248,38 @@ void getdiv_both(cm1g_model *m, int want_t11) {
 }
 
 // calccflquick: per-level maximum of the Courant magnitude, then the max.
+#define CFL_NB 32
 __global__ void __launch_bounds__(256) k_cfl(KP P, float dt, const float *ua, const float *va,
                                              const float *wa, float *out) {
   __shared__ float red[256];
-  const int k = 1 + (int)blockIdx.x;
+  const int k = 1 + (int)blockIdx.x, b = (int)blockIdx.y;
   const float dtdx = 0.5f * dt * P.rdx, dtdy = 0.5f * dt * P.rdy, dtdz = 0.5f * dt * P.rdz;
   float cfl = 0.0f;
-  for (int t = threadIdx.x; t < P.ni * P.nj; t += blockDim.x) {
-    int i = 1 + t % P.ni, j = 1 + t / P.ni;
-    float a = (ua[IXD(P, i, j, k)] + ua[IXD(P, i + 1, j, k)]) * dtdx * P.uh[i + 2];
-    float b = (va[IXD(P, i, j, k)] + va[IXD(P, i, j + 1, k)]) * dtdy * P.vh[j + 2];
-    float c = (wa[IXD(P, i, j, k)] + wa[IXD(P, i, j, k + 1)]) * dtdz * P.mh[k];
-    float wsp = sqrtf((a * a + b * b) + c * c);
-    cfl = fmaxf(cfl, wsp);
-  }
+  for (int j = 1 + b; j <= P.nj; j += CFL_NB)
+    for (int i = 1 + (int)threadIdx.x; i <= P.ni; i += blockDim.x) {
+      float a = (ua[IXD(P, i, j, k)] + ua[IXD(P, i + 1, j, k)]) * dtdx * P.uh[i + 2];
+      float bb = (va[IXD(P, i, j, k)] + va[IXD(P, i, j + 1, k)]) * dtdy * P.vh[j + 2];
+      float c = (wa[IXD(P, i, j, k)] + wa[IXD(P, i, j, k + 1)]) * dtdz * P.mh[k];
+      float wsp = sqrtf((a * a + bb * bb) + c * c);
+      cfl = fmaxf(cfl, wsp);
+    }
   red[threadIdx.x] = cfl;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[k] = red[0];
+  if (threadIdx.x == 0) out[1 + (k - 1) * CFL_NB + b] = red[0];
 }
 
-__global__ void k_cfl_final(int nk, float *out) {
+// maximum over the per-level partial maxima (exact in any order)
+__global__ void k_cfl_final(int n, float *out) {
   float fmax = -99999999.0f;
-  for (int k = 1; k <= nk; k++) fmax = fmaxf(fmax, out[k]);
+  for (int t = 1; t <= n; t++) fmax = fmaxf(fmax, out[t]);
   out[0] = fmax;
 }
 
 void cfl_run(cm1g_model *m, float dt, const float *u, const float *v, const float *w) {
-  hipLaunchKernelGGL(k_cfl, dim3(m->P.nk), dim3(256), 0, m->stream, m->P, dt, u, v, w, m->fred);
-  hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(1), 0, m->stream, m->P.nk, m->fred);
+  hipLaunchKernelGGL(k_cfl, dim3(m->P.nk, CFL_NB), dim3(256), 0, m->stream, m->P, dt, u, v, w, m->fred);
+  hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(1), 0, m->stream, m->P.nk * CFL_NB, m->fred);
 }
