@@ -88,16 +88,27 @@ def main():
     ap.add_argument("--nj", type=int, default=1024)
     ap.add_argument("--nk", type=int, default=96)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
-              "the multi-GPU decomposition is described in DESIGN.md", file=sys.stderr)
+              "multi-GPU horizontal decomposition is not implemented", file=sys.stderr)
         return 2
 
     import torch
     from cm1_amd import model
     import cm1_amd._native as N
     import ctypes as C
+    from cm1_amd import bytes_model
+
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     cfgd, cols, fields = workload_config1(args.ni, args.nj, args.nk)
     cfg = model.make_config(**cfgd)
@@ -106,6 +117,10 @@ def main():
     for _ in range(args.warmup):
         m.step(1)
     m.sync()
+    L = N.lib()
+    # HIP events around every launch of the acoustic substep kernel on the
+    # model's stream during the timed steps
+    kt_ok = L.cm1g_ktimer_start(m.h, N.KT_SOUND_SUBSTEP) == 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     cfl = 0.0
@@ -118,23 +133,23 @@ def main():
     cells = cfgd["ni"] * cfgd["nj"] * cfgd["nk"]
     value = cells / (ms / 1000.0)
 
-    # roofline of the dominant kernel, timed with HIP events on the model's stream
     roof = None
-    L = N.lib()
-    if hasattr(L, "cm1g_time_kernel"):
-        L.cm1g_time_kernel.restype = C.c_int
-        L.cm1g_time_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float),
-                                       C.POINTER(C.c_double), C.c_char_p, C.c_int]
-        avg = C.c_float()
+    if kt_ok:
+        tot = C.c_double()
+        nl = C.c_int64()
         nbytes = C.c_double()
         name = C.create_string_buffer(64)
-        rc = L.cm1g_time_kernel(m.h, 0, 20, C.byref(avg), C.byref(nbytes), name, 64)
-        if rc == 0 and avg.value > 0:
-            gbs = nbytes.value / (avg.value * 1e-3) / 1e9
+        rc = L.cm1g_ktimer_stop(m.h, C.byref(tot), C.byref(nl), C.byref(nbytes), name, 64)
+        if rc == 0 and nl.value > 0 and tot.value > 0:
+            avg_ms = tot.value / nl.value
+            gbs = nbytes.value / (avg_ms * 1e-3) / 1e9
             roof = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
-                        kernel=name.value.decode(), avg_ms=round(avg.value, 4),
-                        algorithmic_bytes=nbytes.value)
+                        kernel=name.value.decode(), avg_ms=round(avg_ms, 4),
+                        launches=nl.value, bytes_per_launch=nbytes.value)
+    # whole-step algorithmic rate (frozen byte table)
+    step_bytes = cells * bytes_model.bytes_per_cell_step(cfgd["nsound"])
+    step_gbs = step_bytes / (ms * 1e-3) / 1e9
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -157,7 +172,9 @@ def main():
         "config": {"workload": f"supercell {cfgd['ni']}x{cfgd['nj']}x{cfgd['nk']} dx=250m stretched-dz "
                                f"dt=1.5s nsound=8 Kessler open-BC 8 bubbles",
                    "nx": cfgd["ni"], "ny": cfgd["nj"], "nz": cfgd["nk"], "nsound": cfgd["nsound"],
-                   "cflmax": cfl},
+                   "cflmax": cfl,
+                   "step_algorithmic_GBps": round(step_gbs, 1),
+                   "step_bytes_per_cell": bytes_model.bytes_per_cell_step(cfgd["nsound"])},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -29,6 +29,7 @@ KFIELDS = ["mh", "rmh", "mf", "rmf", "c1", "c2", "zh", "zf", "pi0", "th0", "prs0
            "qv0", "rho0", "rr0", "rf0", "rrf0", "thv0", "rth0", "qc0", "qi0", "u0", "v0"]
 KID = {n: i for i, n in enumerate(KFIELDS)}
 D2 = {"rain": 0, "prate": 1, "radbcw": 2, "radbce": 3, "radbcs": 4, "radbcn": 5}
+KT_SOUND_SUBSTEP = 1
 BC = {"s": 0, "p": 1, "u": 2, "v": 3, "w": 4, "w0": 5}
 
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "device allocation failed",
@@ -78,6 +79,9 @@ SIGNATURES = {
     "cm1g_advw": (C.c_int, [C.c_void_p, C.c_int]),
     "cm1g_sound": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
     "cm1g_kessler_branch": (C.c_int, [C.c_void_p]),
+    "cm1g_ktimer_start": (C.c_int, [C.c_void_p, C.c_int]),
+    "cm1g_ktimer_stop": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_double), C.c_char_p, C.c_int]),
     "cm1g_pdefq": (C.c_int, [C.c_void_p, C.c_float, C.c_int]),
     "cm1g_bc": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "cm1g_calccflquick": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_int, C.c_int,

@@ -89,12 +89,11 @@ __device__ __forceinline__ float wface(const KP &P, int stag, const float *rrw, 
 }
 
 // vertical face flux at level k for staggering stag (interior 5th order and
-// the lower-order boundary fluxes of vadv_lwr_flx_23/35)
-__device__ __forceinline__ float vflux5(const KP &P, int stag, const float *rrw, const float *a,
-                                       int i, int j, int k) {
+// the lower-order boundary fluxes of vadv_lwr_flx_23/35); S(kk) gives the
+// advected value at level kk of the column, wb the face mass flux.
+template <class SF>
+__device__ __forceinline__ float vflux5_of(const KP &P, int stag, float wb, SF S, int k) {
   const int nk = P.nk;
-  auto S = [&](int kk) { return a[IXD(P, i, j, kk)]; };
-  float wb = wface(P, stag, rrw, i, j, k);
   if (stag != 4) {
     if (k >= 4 && k <= nk - 2)
       return (wb >= 0.0f) ? wb * flx5(S(k - 3), S(k - 2), S(k - 1), S(k), S(k + 1))
@@ -129,6 +128,12 @@ __device__ __forceinline__ float vflux5(const KP &P, int stag, const float *rrw,
   // k == nk-1
   return (wb >= 0.0f) ? wb * flx5(S(k - 2), S(k - 1), S(k), S(k + 1), S(k + 2))
                       : wb * flx3(S(k + 2), S(k + 1), S(k));
+}
+
+__device__ __forceinline__ float vflux5(const KP &P, int stag, const float *rrw, const float *a,
+                                       int i, int j, int k) {
+  auto S = [&](int kk) { return a[IXD(P, i, j, kk)]; };
+  return vflux5_of(P, stag, wface(P, stag, rrw, i, j, k), S, k);
 }
 
 static void vadv5(cm1g_model *m, int stag, const float *rrw, float *dumz, const float *a) {
@@ -240,6 +245,118 @@ static void pdef_dir(cm1g_model *m, int dir, const float *vel, const float *adv,
   });
 }
 
+// ------------------------------------------------------ fused advs column
+// Upwind 5th-order face value for the face between cells m-1 and m along one
+// horizontal line, from the seven values S(m-3..m+3) (only six are used).
+__device__ __forceinline__ float hface5(float r, float sm3, float sm2, float sm1, float s0, float sp1,
+                                        float sp2) {
+  return (r >= 0.0f) ? r * flx5(sm3, sm2, sm1, s0, sp1) : r * flx5(sp2, sp1, s0, sm1, sm2);
+}
+
+struct AdvsArgs {
+  const float *s;                 // advected scalar
+  float sadd;                     // nonzero: advect (th0(k)-th0r) + s
+  const float *s0, *sten0, *t11, *thterm, *rru, *rrv, *rrw, *divx;
+  float *out_sten, *out_new;      // tendency and/or s0 + dt*tendency
+  float *advx, *advy, *advz;      // mode 1: the three advective tendencies
+  float dt;
+  int smeps_flush, mode;          // mode 0: final tendency; 1: advx/advy/advz only
+};
+
+// One column (i,j) of advs (src/adv.F:14-658 with hadv_flx5/vadv_flx5):
+// face fluxes at i, i+1, j, j+1 with the open-boundary inflow fixes
+// (src/adv.F:211-340), the vertical fluxes marched up the column with a
+// six-level register window, and the tendency assembly.  Each flux is the
+// same FP32 expression as the reference's, so results are bitwise equal
+// to separate flux / divergence passes.
+__global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a) {
+  const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  if (i > ni || j > nj) return;
+  const bool doitw = (P.wbc == 2 && P.ibw == 1), doite = (P.ebc == 2 && P.ibe == 1);
+  const bool doits = (P.sbc == 2 && P.ibs == 1), doitn = (P.nbc == 2 && P.ibn == 1);
+  const float *s = a.s;
+  const bool sadd = (a.sadd != 0.0f);
+  const int px = P.px, pxy = P.pxy;
+  auto val = [&](int kk) -> float {  // column value at level kk (1..nk)
+    if (kk < 1 || kk > nk) return 0.0f;
+    const float v = s[IXD(P, i, j, kk)];
+    return sadd ? (P.th0[kk] - P.K.th0r) + v : v;
+  };
+  // window: wv[0..5] = S(k-2 .. k+3)
+  float wv0 = val(-1), wv1 = val(0), wv2 = val(1), wv3 = val(2), wv4 = val(3), wv5 = val(4);
+  float fz_lo = 0.0f;
+  for (int k = 1; k <= nk; k++) {
+    const int c = IXD(P, i, j, k);
+    const float tem = sadd ? (P.th0[k] - P.K.th0r) : 0.0f;
+    auto H = [&](int off) -> float {
+      const float v = s[c + off];
+      return sadd ? tem + v : v;
+    };
+    const float sc = wv2;
+    // x faces i and i+1
+    const float xm3 = H(-3), xm2 = H(-2), xm1 = H(-1), xp1 = H(1), xp2 = H(2), xp3 = H(3);
+    const float ru0 = a.rru[c], ru1 = a.rru[c + 1];
+    float fx0 = hface5(ru0, xm3, xm2, xm1, sc, xp1, xp2);
+    float fx1 = hface5(ru1, xm2, xm1, sc, xp1, xp2, xp3);
+    const float fx0_raw = fx0;
+    if (doitw && i == 1 && ru0 >= 0.0f) fx0 = fx1;
+    if (doite && i == ni && ru1 <= 0.0f) fx1 = fx0_raw;
+    // y faces j and j+1
+    const float ym3 = H(-3 * px), ym2 = H(-2 * px), ym1 = H(-px), yp1 = H(px), yp2 = H(2 * px),
+                yp3 = H(3 * px);
+    const float rv0 = a.rrv[c], rv1 = a.rrv[c + px];
+    float fy0 = hface5(rv0, ym3, ym2, ym1, sc, yp1, yp2);
+    float fy1 = hface5(rv1, ym2, ym1, sc, yp1, yp2, yp3);
+    const float fy0_raw = fy0;
+    if (doits && j == 1 && rv0 >= 0.0f) fy0 = fy1;
+    if (doitn && j == nj && rv1 <= 0.0f) fy1 = fy0_raw;
+    float ax = -(fx1 - fx0) * P.rdx * P.uh[i + 2];
+    float ay = -(fy1 - fy0) * P.rdy * P.vh[j + 2];
+    if (doitw && i == 1 && ru0 >= 0.0f) ax = ax - sc * (ru1 - ru0) * P.rdx * P.uh[1 + 2];
+    if (doite && i == ni && ru1 <= 0.0f) ax = ax - sc * (ru1 - ru0) * P.rdx * P.uh[ni + 2];
+    if (doits && j == 1 && rv0 >= 0.0f) ay = ay - sc * (rv1 - rv0) * P.rdy * P.vh[1 + 2];
+    if (doitn && j == nj && rv1 <= 0.0f) ay = ay - sc * (rv1 - rv0) * P.rdy * P.vh[nj + 2];
+    // vertical face k+1 from S(k-2..k+3)
+    float fz_hi = 0.0f;
+    if (k + 1 <= nk) {
+      const int kf = k + 1;
+      auto S = [&](int kk) -> float {
+        const int o = kk - kf;  // -3..2
+        return o == -3 ? wv0 : o == -2 ? wv1 : o == -1 ? wv2 : o == 0 ? wv3 : o == 1 ? wv4 : wv5;
+      };
+      fz_hi = vflux5_of(P, 1, a.rrw[c + pxy], S, kf);
+    }
+    float az;
+    if (k == 1) az = -fz_hi * P.rdz * P.mh[1];
+    else if (k == nk) az = +fz_lo * P.rdz * P.mh[nk];
+    else az = -(fz_hi - fz_lo) * P.rdz * P.mh[k];
+    if (a.mode == 1) {
+      a.advx[c] = ax;
+      a.advy[c] = ay;
+      a.advz[c] = az;
+    } else {
+      float st = a.sten0[c];
+      if (a.t11) st = st - a.t11[c] * a.thterm[c];
+      st = st + ((ax + ay) + az + sc * a.divx[c]) * P.rr0[k];
+      if (a.out_sten) a.out_sten[c] = st;
+      if (a.out_new) {
+        float v = a.s0[c] + a.dt * st;
+        if (a.smeps_flush && fabsf(v) < P.K.smeps) v = 0.0f;
+        a.out_new[c] = v;
+      }
+    }
+    fz_lo = fz_hi;
+    wv0 = wv1;
+    wv1 = wv2;
+    wv2 = wv3;
+    wv3 = wv4;
+    wv4 = wv5;
+    wv5 = val(k + 4);
+  }
+}
+
 // ------------------------------------------------------------------ advs
 // sadd: when nonzero, s(i,j,k) = (th0(k) - th0r) + s_in(i,j,k) is formed on
 // the fly (the theta equation's sadv, src/solve2.F:1354-1374).
@@ -253,81 +370,76 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
               int pdef, int pdefweno, float dt, double weps, int smeps_flush) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
-  const float *rru = m->rrus, *rrv = m->rrvs, *rrw = m->rrws;
-  const float *divx = m->f[CM1G_DIVX];
-  float *dumx = m->sx, *dumy = m->sy, *dumz = m->sz;
-  float *advx = m->tx, *advy = m->ty, *advz = m->tz;
   (void)nrk; (void)pdefweno; (void)weps;
-  const float *a = s_in;
-  if (sadd != 0.0f) {
-    // materialise s = (th0-th0r) + th3d over the whole box (ib:ie, jb:je)
-    float *sv = m->f[CM1G_SADV];
-    for3(m->stream, -2, ni + 3, -2, nj + 3, 1, nk, [=] __device__(int i, int j, int k) {
-      float tem = P.th0[k] - P.K.th0r;
-      sv[IXD(P, i, j, k)] = tem + s_in[IXD(P, i, j, k)];
-    });
-    a = sv;
-  }
-  hadv5(m, 1, rru, rrv, dumx, dumy, a);
-  const bool doitw = (P.wbc == 2 && P.ibw == 1), doite = (P.ebc == 2 && P.ibe == 1);
-  const bool doits = (P.sbc == 2 && P.ibs == 1), doitn = (P.nbc == 2 && P.ibn == 1);
-  if (doitw || doite)
-    for2(m->stream, 1, nj, 1, nk, [=] __device__(int j, int k) {
-      if (doitw && rru[IXD(P, 1, j, k)] >= 0.0f) dumx[IXD(P, 1, j, k)] = dumx[IXD(P, 2, j, k)];
-      if (doite && rru[IXD(P, ni + 1, j, k)] <= 0.0f) dumx[IXD(P, ni + 1, j, k)] = dumx[IXD(P, ni, j, k)];
-    });
-  if (doits || doitn)
-    for2(m->stream, 1, ni, 1, nk, [=] __device__(int i, int k) {
-      if (doits && rrv[IXD(P, i, 1, k)] >= 0.0f) dumy[IXD(P, i, 1, k)] = dumy[IXD(P, i, 2, k)];
-      if (doitn && rrv[IXD(P, i, nj + 1, k)] <= 0.0f) dumy[IXD(P, i, nj + 1, k)] = dumy[IXD(P, i, nj, k)];
-    });
-  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-    int c = IXD(P, i, j, k);
-    float ax = -(dumx[IXD(P, i + 1, j, k)] - dumx[c]) * P.rdx * P.uh[i + 2];
-    float ay = -(dumy[IXD(P, i, j + 1, k)] - dumy[c]) * P.rdy * P.vh[j + 2];
-    if (doitw && i == 1 && rru[IXD(P, 1, j, k)] >= 0.0f)
-      ax = ax - a[c] * (rru[IXD(P, 2, j, k)] - rru[c]) * P.rdx * P.uh[1 + 2];
-    if (doite && i == ni && rru[IXD(P, ni + 1, j, k)] <= 0.0f)
-      ax = ax - a[c] * (rru[IXD(P, ni + 1, j, k)] - rru[c]) * P.rdx * P.uh[ni + 2];
-    if (doits && j == 1 && rrv[IXD(P, i, 1, k)] >= 0.0f)
-      ay = ay - a[c] * (rrv[IXD(P, i, 2, k)] - rrv[c]) * P.rdy * P.vh[1 + 2];
-    if (doitn && j == nj && rrv[IXD(P, i, nj + 1, k)] <= 0.0f)
-      ay = ay - a[c] * (rrv[IXD(P, i, nj + 1, k)] - rrv[c]) * P.rdy * P.vh[nj + 2];
-    advx[c] = ax;
-    advy[c] = ay;
-  });
-  vadv5(m, 1, rrw, dumz, a);
-  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-    int c = IXD(P, i, j, k);
-    float az;
-    if (k == 1) az = -dumz[IXD(P, i, j, 2)] * P.rdz * P.mh[1];
-    else if (k == nk) az = +dumz[c] * P.rdz * P.mh[nk];
-    else az = -(dumz[IXD(P, i, j, k + 1)] - dumz[c]) * P.rdz * P.mh[k];
-    advz[c] = az;
-  });
-  const float *ax = advx, *ay = advy, *az = advz;
-  if (pdef == 1) {
+  AdvsArgs g;
+  g.s = s_in;
+  g.sadd = sadd;
+  g.s0 = s0;
+  g.sten0 = sten0;
+  g.t11 = thterm_div;
+  g.thterm = thterm_div ? m->f[CM1G_THTERM] : nullptr;
+  g.rru = m->rrus;
+  g.rrv = m->rrvs;
+  g.rrw = m->rrws;
+  g.divx = m->f[CM1G_DIVX];
+  g.out_sten = out_sten;
+  g.out_new = out_new;
+  g.advx = m->tx;
+  g.advy = m->ty;
+  g.advz = m->tz;
+  g.dt = dt;
+  g.smeps_flush = smeps_flush;
+  const dim3 blk(64, 4, 1), grd((ni + 63) / 64, (nj + 3) / 4, 1);
+  // an in-place update (out_new aliasing the advected field) goes through
+  // a partner array, since neighbouring columns still read the old values
+  const bool inplace = (out_new != nullptr && out_new == s_in);
+  if (inplace) g.out_new = m->thb;
+  if (pdef != 1) {
+    g.mode = 0;
+    hipLaunchKernelGGL(k_advs_col, grd, blk, 0, m->stream, P, g);
+  } else {
+    g.mode = 1;
+    hipLaunchKernelGGL(k_advs_col, grd, blk, 0, m->stream, P, g);
+    float *advx = m->tx, *advy = m->ty, *advz = m->tz;
+    float *dumx = m->sx, *dumy = m->sy, *dumz = m->sz;
+    const float *rru = m->rrus, *rrv = m->rrvs, *rrw = m->rrws;
+    const float *a = s_in;
+    if (sadd != 0.0f) {
+      // the positive-definite pass needs the advected values themselves
+      float *sv = m->f[CM1G_SADV];
+      for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+        float tem = P.th0[k] - P.K.th0r;
+        sv[IXD(P, i, j, k)] = tem + s_in[IXD(P, i, j, k)];
+      });
+      a = sv;
+    }
     // each direction writes its corrected tendency into dumx/dumy/dumz
-    // (the fluxes are no longer needed); the work array is the other one.
     pdef_dir(m, 0, rru, advx, dumx, m->scr1, s0, a, dt);
     pdef_dir(m, 2, rrw, advz, dumz, m->scr1, s0, a, dt);
     pdef_dir(m, 1, rrv, advy, dumy, m->scr1, s0, a, dt);
-    ax = dumx; ay = dumy; az = dumz;
+    const float *ax = dumx, *ay = dumy, *az = dumz;
+    const float *t11 = thterm_div, *thterm = g.thterm, *divx = g.divx;
+    float *onew = g.out_new;
+    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      float st = sten0[c];
+      if (t11) st = st - t11[c] * thterm[c];
+      st = st + ((ax[c] + ay[c]) + az[c] + a[c] * divx[c]) * P.rr0[k];
+      if (out_sten) out_sten[c] = st;
+      if (onew) {
+        float v = s0[c] + dt * st;
+        if (smeps_flush && fabsf(v) < P.K.smeps) v = 0.0f;
+        onew[c] = v;
+      }
+    });
   }
-  const float *t11 = thterm_div ? (const float *)thterm_div : nullptr;
-  const float *thterm = thterm_div ? m->f[CM1G_THTERM] : nullptr;
-  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-    int c = IXD(P, i, j, k);
-    float st = sten0[c];
-    if (t11) st = st - t11[c] * thterm[c];
-    st = st + ((ax[c] + ay[c]) + az[c] + a[c] * divx[c]) * P.rr0[k];
-    if (out_sten) out_sten[c] = st;
-    if (out_new) {
-      float v = s0[c] + dt * st;
-      if (smeps_flush && fabsf(v) < P.K.smeps) v = 0.0f;
-      out_new[c] = v;
-    }
-  });
+  if (inplace) {
+    const float *src = m->thb;
+    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      out_new[c] = src[c];
+    });
+  }
 }
 
 // ----------------------------------------------------------- advu/v/w

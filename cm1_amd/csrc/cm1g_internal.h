@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 #include "../../include/cm1gpu.h"
 
 // Physical constants, single precision as set_constants() computes them
@@ -63,7 +64,14 @@ struct cm1g_model {
   double mass1, train;
   float cflmax;
   int nth_buf;                  // unused
+  // kernel timer: selected kernel id and the event pairs recorded so far
+  int kt_which;
+  std::vector<hipEvent_t> kt_ev;
+  size_t kt_used;
 };
+
+// bracket a launch of kernel `which` with timer events (no-op unless selected)
+void ktimer_mark(cm1g_model *m, int which);
 
 #define HIPCHK(x)                                                                  \
   do {                                                                             \

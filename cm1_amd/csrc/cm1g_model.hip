@@ -142,7 +142,7 @@ extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
   if (!rc) rc = dalloc(&m->dtu0, P.pxy);
   if (!rc) rc = dalloc(&m->dtv0, P.pxy);
   float **scr[] = {&m->sx, &m->sy, &m->sz, &m->tx, &m->ty, &m->tz, &m->rrus, &m->rrvs, &m->rrws,
-                   &m->scr1, &m->scr2};
+                   &m->scr1, &m->scr2, &m->thb};
   for (auto p : scr)
     if (!rc) rc = dalloc(p, m->ntot);
   if (!rc) {
@@ -212,11 +212,12 @@ extern "C" int cm1g_destroy(cm1g_model *m) {
   for (int i = 0; i < 6; i++) if (m->h[i]) hipFree(m->h[i]);
   for (int i = 0; i < CM1G_D_NFIELDS; i++) if (m->d2[i]) hipFree(m->d2[i]);
   float *scr[] = {m->dtu0, m->dtv0, m->sx, m->sy, m->sz, m->tx, m->ty, m->tz, m->rrus, m->rrvs,
-                  m->rrws, m->scr1, m->scr2, m->fred};
+                  m->rrws, m->scr1, m->scr2, m->thb, m->fred};
   for (auto p : scr) if (p) hipFree(p);
   if (m->dred) hipFree(m->dred);
   if (m->dscal) hipFree(m->dscal);
   if (m->iflag) hipFree(m->iflag);
+  for (hipEvent_t e : m->kt_ev) hipEventDestroy(e);
   if (m->stream) hipStreamDestroy(m->stream);
   delete m;
   return 0;
@@ -998,5 +999,55 @@ extern "C" int cm1g_budgets(cm1g_model *m, double *tmass, double *tmois) {
   for (int k = 1; k <= nk; k++) { s1 += h[2 * k]; s2 += h[2 * k + 1]; }
   *tmass = s1 * vol;
   *tmois = s2 * vol;
+  return 0;
+}
+
+// ------------------------------------------------------------ kernel timer
+void ktimer_mark(cm1g_model *m, int which) {
+  if (m->kt_which != which || which == CM1G_KT_NONE) return;
+  if (m->kt_used == m->kt_ev.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    m->kt_ev.push_back(e);
+  }
+  hipEventRecord(m->kt_ev[m->kt_used++], m->stream);
+}
+
+extern "C" int cm1g_ktimer_start(cm1g_model *m, int which) {
+  ARGCHK(m, "cm1g_ktimer_start: null model");
+  ARGCHK(which == CM1G_KT_NONE || which == CM1G_KT_SOUND_SUBSTEP, "cm1g_ktimer_start: unknown kernel id");
+  m->kt_which = which;
+  m->kt_used = 0;
+  return 0;
+}
+
+extern "C" int cm1g_ktimer_stop(cm1g_model *m, double *total_ms, int64_t *launches,
+                                double *bytes_per_launch, char *name, int name_len) {
+  ARGCHK(m && total_ms && launches && bytes_per_launch, "cm1g_ktimer_stop: null argument");
+  HIPCHK(hipStreamSynchronize(m->stream));
+  double tot = 0.0;
+  const size_t npair = m->kt_used / 2;
+  for (size_t p = 0; p < npair; p++) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, m->kt_ev[2 * p], m->kt_ev[2 * p + 1]));
+    tot += ms;
+  }
+  const double ncell = (double)m->P.ni * m->P.nj * m->P.nk;
+  const char *nm = "";
+  double fields = 0.0;
+  if (m->kt_which == CM1G_KT_SOUND_SUBSTEP) {
+    // u, v, w, ppd, pp3d, rru, rrv, rrw read and written (16); uten, vten,
+    // wten, thv, ppten, ppterm, piadv read (7)
+    fields = 23.0;
+    nm = "k_sound_sub";
+  }
+  *total_ms = tot;
+  *launches = (int64_t)npair;
+  *bytes_per_launch = fields * 4.0 * ncell;
+  if (name && name_len > 0) {
+    snprintf(name, (size_t)name_len, "%s", nm);
+  }
+  m->kt_which = CM1G_KT_NONE;
+  m->kt_used = 0;
   return 0;
 }

@@ -27,6 +27,266 @@ static void sound_nloop(int nrk, int nsound, float dt, int *nloop_out, float *dt
   *dts_out = dts;
 }
 
+// Arguments of one fused acoustic substep.
+struct SubArgs {
+  float dts, alph, beta, asq, kdiv, tavg;
+  float tem1, tem2, temx, temy;   // cp*rdx/2, cp*rdy/2, dts*rdx/2, dts*rdy/2
+  int last, opw, ope, ops, opn;
+  const float *uX, *vX;           // u, v at the start of the substep
+  float *uY, *vY;                 // u, v after the substep
+  const float *ppdX;              // pi' + divergence damping from the previous substep
+  float *ppdY;                    // ... written by this substep
+  const float *uten, *vten, *wten, *thv, *piadv, *ppterm, *ppten, *dtu0, *dtv0;
+  const float *radbcw, *radbce, *radbcs, *radbcn;
+  float *w3d, *pp3d, *fpk, *dum1, *dum2, *rru, *rrv, *rrw;
+};
+
+// new u at (i,j,k): radiative open-boundary update (ssopenbcw/e,
+// src/bc.F:2392-2530) then the pressure-gradient step (src/sound.F:307-325)
+__device__ __forceinline__ float sub_u(const KP &P, const SubArgs &a, int i, int j, int k) {
+  const int c = IXD(P, i, j, k);
+  float u = a.uX[c];
+  if (j >= 1 && j <= P.nj) {
+    if (a.opw && i == 1)
+      u = u + a.dts * (-a.radbcw[(j + 2) + k * P.py] * (a.uX[IXD(P, 2, j, k)] - u) * P.rdx * P.uh[1 + 2] +
+                       a.uten[c]);
+    if (a.ope && i == P.ni + 1)
+      u = u + a.dts * (-a.radbce[(j + 2) + k * P.py] * (u - a.uX[IXD(P, P.ni, j, k)]) * P.rdx * P.uh[P.ni + 2] +
+                       a.uten[c]);
+  }
+  const float dtu = a.dts * a.dtu0[IX2D(P, i, j)];
+  return u + dtu * (a.uten[c] - a.tem1 * (a.ppdX[c] - a.ppdX[c - 1]) * P.uf[i + 2] * (a.thv[c] + a.thv[c - 1]));
+}
+
+// new v at (i,j,k) (ssopenbcs/n, src/bc.F:2538-2684; src/sound.F:327-345)
+__device__ __forceinline__ float sub_v(const KP &P, const SubArgs &a, int i, int j, int k) {
+  const int c = IXD(P, i, j, k);
+  float v = a.vX[c];
+  if (i >= 1 && i <= P.ni) {
+    if (a.ops && j == 1)
+      v = v + a.dts * (-a.radbcs[(i + 2) + k * P.px] * (a.vX[IXD(P, i, 2, k)] - v) * P.rdy * P.vh[1 + 2] +
+                       a.vten[c]);
+    if (a.opn && j == P.nj + 1)
+      v = v + a.dts * (-a.radbcn[(i + 2) + k * P.px] * (v - a.vX[IXD(P, i, P.nj, k)]) * P.rdy * P.vh[P.nj + 2] +
+                       a.vten[c]);
+  }
+  const float dtv = a.dts * a.dtv0[IX2D(P, i, j)];
+  const int cs = c - P.px;
+  return v + dtv * (a.vten[c] - a.tem2 * (a.ppdX[c] - a.ppdX[cs]) * P.vf[j + 2] * (a.thv[c] + a.thv[cs]));
+}
+
+// store a new u (v) and fold it into the substep average (src/sound.F:806-835)
+__device__ __forceinline__ void put_u(const SubArgs &a, int c, float u) {
+  a.uY[c] = u;
+  a.rru[c] = a.last ? (a.rru[c] + u) * a.tavg : a.rru[c] + u;
+}
+__device__ __forceinline__ void put_v(const SubArgs &a, int c, float v) {
+  a.vY[c] = v;
+  a.rrv[c] = a.last ? (a.rrv[c] + v) * a.tavg : a.rrv[c] + v;
+}
+
+// The averaging ring of one substep: u and v on the faces i = ni+1 and
+// j = nj+1 (unchanged by the step where the open-boundary masks are zero,
+// advanced otherwise) and the w average there (w is not advanced at those
+// points).  Thread t walks the ring (i = 1..ni+1 at j = nj+1, then
+// j = 1..nj at i = ni+1); blockIdx.y is the level.  Reads uX/vX/ppdX only,
+// so it may run before or after k_sound_sub.
+__global__ void __launch_bounds__(256) k_sound_ring(KP P, SubArgs a) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int k = 1 + (int)blockIdx.y;
+  const int ni = P.ni, nj = P.nj;
+  int i, j;
+  if (t <= ni) {
+    i = 1 + t;
+    j = nj + 1;
+  } else if (t <= ni + nj) {
+    i = ni + 1;
+    j = t - ni;
+  } else {
+    return;
+  }
+  const int c = IXD(P, i, j, k);
+  put_u(a, c, sub_u(P, a, i, j, k));
+  put_v(a, c, sub_v(P, a, i, j, k));
+  a.rrw[c] = a.last ? (a.rrw[c] + a.w3d[c] * a.alph) * a.tavg : a.rrw[c] + a.w3d[c];
+}
+
+// One acoustic substep for a column (i,j): u/v forward step (also the
+// faces i+1 / j+1 the divergence needs, recomputed from the same inputs),
+// explicit pi forcing fpk (src/sound.F:484-505), Thomas forward sweep of the
+// implicit w-pi system (src/sound.F:659-717), back substitution (:720-733),
+// pi update with divergence damping (:739-765) and the w average.  The
+// solver coefficients qk, pk1, pk2, mm (src/sound.F:117-180) are evaluated
+// on the fly with the same expressions.  u and v are read from uX/vX and
+// written to uY/vY (pi'+damping likewise ppdX/ppdY) so that neighbouring
+// columns see the old values.
+//
+// Each level issues all of its loads before any store, so a level costs one
+// memory round trip; the downward sweep also prefetches the next level's
+// operands before storing the current one.
+__global__ void __launch_bounds__(256) k_sound_sub(KP P, SubArgs a) {
+  const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  if (i > ni || j > nj) return;
+  const float dts = a.dts, rdz = P.rdz, cp = P.K.cp, smeps = P.K.smeps;
+  const float rdx = P.rdx, rdy = P.rdy;
+  const int px = P.px, pxy = P.pxy;
+  const bool bw = a.opw && i == 1, be = a.ope && i == ni, bs = a.ops && j == 1, bn = a.opn && j == nj;
+  // k-invariant factors
+  const float dtu_c = dts * a.dtu0[IX2D(P, i, j)], dtu_e = dts * a.dtu0[IX2D(P, i + 1, j)];
+  const float dtv_c = dts * a.dtv0[IX2D(P, i, j)], dtv_n = dts * a.dtv0[IX2D(P, i, j + 1)];
+  const float uf_c = P.uf[i + 2], uf_e = P.uf[i + 3], vf_c = P.vf[j + 2], vf_n = P.vf[j + 3];
+  const float uh_c = P.uh[i + 2], vh_c = P.vh[j + 2];
+  const float tem1 = a.tem1, tem2 = a.tem2, temx = a.temx, temy = a.temy;
+  const float alph = a.alph, beta = a.beta, asq = a.asq;
+
+  // level k-1 quantities carried up the column
+  float A_m = 0.0f, B_m = 0.0f, fpk_m = 0.0f, pp_m = 0.0f;
+  float d1p = 0.0f, d2p = 0.0f, w_top = 0.0f, w_bot = 0.0f, w_nk1 = 0.0f;
+  int c = IXD(P, i, j, 1);
+  float pa_m = a.piadv[c - pxy], pa_c = a.piadv[c];
+  float th_m = a.thv[c - pxy];
+  float w_c = a.w3d[c];
+  for (int k = 1; k <= nk; k++, c += pxy) {
+    // ---- loads
+    const float uXc = a.uX[c], uXe = a.uX[c + 1], utc = a.uten[c], ute = a.uten[c + 1];
+    const float vXc = a.vX[c], vXn = a.vX[c + px], vtc = a.vten[c], vtn = a.vten[c + px];
+    const float pdc = a.ppdX[c], pdw = a.ppdX[c - 1], pde = a.ppdX[c + 1], pds = a.ppdX[c - px],
+                pdn = a.ppdX[c + px];
+    const float thc = a.thv[c], thw = a.thv[c - 1], the = a.thv[c + 1], ths = a.thv[c - px],
+                thn = a.thv[c + px];
+    const float rruc = a.rru[c], rrvc = a.rrv[c];
+    const float paw = a.piadv[c - 1], pae = a.piadv[c + 1], pas = a.piadv[c - px], pan = a.piadv[c + px];
+    const float pa_p = a.piadv[c + pxy];
+    const float ppterm = a.ppterm[c], ppten = a.ppten[c], wten = a.wten[c], pp_c = a.pp3d[c];
+    const float w_p = a.w3d[c + pxy];
+    float rw = 0.0f, re = 0.0f, rs = 0.0f, rn = 0.0f;
+    if (bw) rw = a.radbcw[(j + 2) + k * P.py];
+    if (be) re = a.radbce[(j + 2) + k * P.py];
+    if (bs) rs = a.radbcs[(i + 2) + k * px];
+    if (bn) rn = a.radbcn[(i + 2) + k * px];
+    // ---- u, v (ssopenbc* then src/sound.F:307-345)
+    float uo0 = uXc, uo1 = uXe, vo0 = vXc, vo1 = vXn;
+    if (bw) uo0 = uo0 + dts * (-rw * (uXe - uo0) * rdx * P.uh[1 + 2] + utc);
+    if (be) uo1 = uo1 + dts * (-re * (uo1 - uXc) * rdx * P.uh[ni + 2] + ute);
+    if (bs) vo0 = vo0 + dts * (-rs * (vXn - vo0) * rdy * P.vh[1 + 2] + vtc);
+    if (bn) vo1 = vo1 + dts * (-rn * (vo1 - vXc) * rdy * P.vh[nj + 2] + vtn);
+    const float un0 = uo0 + dtu_c * (utc - tem1 * (pdc - pdw) * uf_c * (thc + thw));
+    const float un1 = uo1 + dtu_e * (ute - tem1 * (pde - pdc) * uf_e * (the + thc));
+    const float vn0 = vo0 + dtv_c * (vtc - tem2 * (pdc - pds) * vf_c * (thc + ths));
+    const float vn1 = vo1 + dtv_n * (vtn - tem2 * (pdn - pdc) * vf_n * (thn + thc));
+    // ---- coefficients (src/sound.F:117-180)
+    const float tem = dts * rdz * P.mh[k];
+    const float r2 = dts * rdz * P.mh[k] * P.rr0[k] * P.rf0[k + 1];
+    const float r1 = dts * rdz * P.mh[k] * P.rr0[k] * P.rf0[k];
+    const float temm = dts * rdz * cp * P.mf[k];
+    const float c1a = P.c1[k], c1b = P.c1[k + 1], c2a = P.c2[k], c2b = P.c2[k + 1];
+    const float qk = tem * ppterm;
+    const float pk2 = (k == nk) ? 0.0f : r2 * (-c2b * pa_p + (1.0f - c1b) * pa_c);
+    const float pk1 = (k == 1) ? 0.0f : r1 * (+c1a * pa_m + (c2a - 1.0f) * pa_c);
+    const float mm = temm * (c2a * thc + c1a * th_m);
+    const float A = pk1 + qk, B = pk2 - qk;
+    // ---- explicit pi forcing
+    float div = (un1 - un0) * rdx * uh_c + (vn1 - vn0) * rdy * vh_c;
+    if (fabsf(div) < smeps) div = 0.0f;
+    const float fpk = dts * (ppten - ppterm * div) +
+                      (-(un0 * (pa_c - paw) + un1 * (pae - pa_c)) * temx * uh_c -
+                       (vn0 * (pa_c - pas) + vn1 * (pan - pa_c)) * temy * vh_c) +
+                      beta * (w_c * (pk1 + qk) + w_p * (pk2 - qk));
+    // ---- Thomas forward sweep
+    if (k == 1) {
+      w_bot = w_c;
+    } else {
+      const float aa = -asq * (+mm * A_m);
+      const float cc = -asq * (-mm * B);
+      const float bb = 1.0f - asq * (+mm * (-A + B_m));
+      const float fwk = dts * wten + beta * (-mm * (pp_c - pp_m));
+      const float dd = w_c + fwk + alph * (-mm * ((pp_c + fpk) - (pp_m + fpk_m)));
+      if (k == 2) {
+        const float rr = 1.0f / bb;
+        d1p = -cc * rr;
+        d2p = dd * rr;
+      } else {
+        const float rr = 1.0f / (aa * d1p + bb);
+        d1p = -cc * rr;
+        d2p = (dd - aa * d2p) * rr;
+      }
+      if (k == nk) w_top = d2p;
+    }
+    // ---- stores
+    a.uY[c] = un0;
+    a.rru[c] = a.last ? (rruc + un0) * a.tavg : rruc + un0;
+    a.vY[c] = vn0;
+    a.rrv[c] = a.last ? (rrvc + vn0) * a.tavg : rrvc + vn0;
+    a.fpk[c] = fpk;
+    if (k >= 2 && k < nk) {
+      a.dum1[c] = d1p;
+      a.dum2[c] = d2p;
+    }
+    if (k == nk) w_nk1 = w_p;
+    A_m = A;
+    B_m = B;
+    fpk_m = fpk;
+    pp_m = pp_c;
+    pa_m = pa_c;
+    pa_c = pa_p;
+    th_m = thc;
+    w_c = w_p;
+  }
+
+  // back substitution, pi update, w average (top down)
+  c = IXD(P, i, j, nk);
+  float w_above = w_nk1;
+  float pa_p = a.piadv[c + pxy];
+  float pa_c2 = a.piadv[c];
+  // operands of the current level
+  float q_d1 = 0.0f, q_d2 = 0.0f;
+  float q_pp = a.pp3d[c], q_fpk = a.fpk[c], q_pm = a.piadv[c - pxy], q_term = a.ppterm[c], q_rrw = a.rrw[c];
+  for (int k = nk; k >= 1; k--, c -= pxy) {
+    const float p_old = q_pp, fk = q_fpk, pa_m2 = q_pm, pterm = q_term, rrwc = q_rrw;
+    float wk;
+    if (k == nk) {
+      wk = w_top;
+    } else if (k >= 2) {
+      wk = q_d1 * w_above + q_d2;
+    } else {
+      wk = w_bot;
+    }
+    // prefetch the level below
+    if (k > 1) {
+      const int cb = c - pxy;
+      if (k - 1 >= 2) {
+        q_d1 = a.dum1[cb];
+        q_d2 = a.dum2[cb];
+      }
+      q_pp = a.pp3d[cb];
+      q_fpk = a.fpk[cb];
+      q_pm = a.piadv[cb - pxy];
+      q_term = a.ppterm[cb];
+      q_rrw = a.rrw[cb];
+    }
+    const float tem = dts * rdz * P.mh[k];
+    const float r2 = dts * rdz * P.mh[k] * P.rr0[k] * P.rf0[k + 1];
+    const float r1 = dts * rdz * P.mh[k] * P.rr0[k] * P.rf0[k];
+    const float c1a = P.c1[k], c1b = P.c1[k + 1], c2a = P.c2[k], c2b = P.c2[k + 1];
+    const float qk = tem * pterm;
+    const float pk2 = (k == nk) ? 0.0f : r2 * (-c2b * pa_p + (1.0f - c1b) * pa_c2);
+    const float pk1 = (k == 1) ? 0.0f : r1 * (+c1a * pa_m2 + (c2a - 1.0f) * pa_c2);
+    float pn = p_old + fk + alph * (wk * (pk1 + qk) + w_above * (pk2 - qk));
+    if (fabsf(pn) < smeps) pn = 0.0f;
+    const float d = a.kdiv * (pn - p_old);
+    if (k >= 2) a.w3d[c] = wk;
+    a.pp3d[c] = pn;
+    if (a.last) a.dum1[c] = d;
+    a.ppdY[c] = pn + d;
+    a.rrw[c] = a.last ? (rrwc + wk * alph) * a.tavg : rrwc + wk;
+    w_above = wk;
+    pa_p = pa_c2;
+    pa_c2 = pa_m2;
+  }
+}
+
 void sound_run(cm1g_model *m, int nrk) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
@@ -35,35 +295,13 @@ void sound_run(cm1g_model *m, int nrk) {
   float dts;
   sound_nloop(nrk, m->c.nsound, dt, &nloop, &dts);
   const float beta = 1.0f - alph;
-  const float asq = alph * alph;
-  float *dum1 = m->f[CM1G_DUM1], *dum2 = m->f[CM1G_DUM2], *mm = m->f[CM1G_DUM5];
-  float *ppd = m->f[CM1G_DUM7], *fpk = m->f[CM1G_DUM8];
-  float *qk = m->f[CM1G_T12], *pk1 = m->f[CM1G_T13], *pk2 = m->f[CM1G_T23];
-  const float *piadv = m->f[CM1G_SADV], *thv = m->f[CM1G_T11], *ppterm = m->f[CM1G_T22];
+  float *dum1 = m->f[CM1G_DUM1];
+  float *ppd = m->f[CM1G_DUM7];
   float *u3d = m->f[CM1G_U3D], *v3d = m->f[CM1G_V3D], *w3d = m->f[CM1G_W3D];
   const float *ua = m->f[CM1G_UA], *va = m->f[CM1G_VA], *wa = m->f[CM1G_WA];
-  const float *uten = m->f[CM1G_UTEN], *vten = m->f[CM1G_VTEN], *wten = m->f[CM1G_WTEN];
   float *rru = m->f[CM1G_RRU], *rrv = m->f[CM1G_RRV], *rrw = m->f[CM1G_RRW];
-  const float *ppi = m->f[CM1G_PPI], *ppten = m->f[CM1G_PPTEN];
+  const float *ppi = m->f[CM1G_PPI];
   float *pp3d = m->f[CM1G_PP3D], *ppx = m->f[CM1G_PPX];
-  const float *radbcw = m->d2[CM1G_D_RADBCW], *radbce = m->d2[CM1G_D_RADBCE];
-  const float *radbcs = m->d2[CM1G_D_RADBCS], *radbcn = m->d2[CM1G_D_RADBCN];
-  const float *dtu0 = m->dtu0, *dtv0 = m->dtv0;
-  const float rdx = P.rdx, rdy = P.rdy, rdz = P.rdz, cp = P.K.cp, smeps = P.K.smeps;
-
-  // coefficients of the implicit solver (src/sound.F:117-180)
-  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-    float tem = dts * rdz * P.mh[k];
-    float r2 = dts * rdz * P.mh[k] * P.rr0[k] * P.rf0[k + 1];
-    float r1 = dts * rdz * P.mh[k] * P.rr0[k] * P.rf0[k];
-    float tem1 = dts * rdz * cp * P.mf[k];
-    float c1a = P.c1[k], c1b = P.c1[k + 1], c2a = P.c2[k], c2b = P.c2[k + 1];
-    int c = IXD(P, i, j, k);
-    qk[c] = tem * ppterm[c];
-    pk2[c] = (k == nk) ? 0.0f : r2 * (-c2b * piadv[IXD(P, i, j, k + 1)] + (1.0f - c1b) * piadv[c]);
-    pk1[c] = (k == 1) ? 0.0f : r1 * (+c1a * piadv[IXD(P, i, j, k - 1)] + (c2a - 1.0f) * piadv[c]);
-    mm[c] = tem1 * (c2a * thv[c] + c1a * thv[IXD(P, i, j, k - 1)]);
-  });
 
   // prep copies (src/sound.F:191-250)
   if (nrk == 1) {
@@ -88,120 +326,69 @@ void sound_run(cm1g_model *m, int nrk) {
     });
   }
 
-  const bool opw = (P.wbc == 2 && P.ibw == 1), ope = (P.ebc == 2 && P.ibe == 1);
-  const bool ops = (P.sbc == 2 && P.ibs == 1), opn = (P.nbc == 2 && P.ibn == 1);
+  SubArgs a;
+  a.dts = dts;
+  a.alph = alph;
+  a.beta = beta;
+  a.asq = alph * alph;
+  a.kdiv = kdiv;
+  a.tavg = 1.0f / (float)nloop;
+  a.tem1 = P.rdx * P.K.cp * 0.5f;
+  a.tem2 = P.rdy * P.K.cp * 0.5f;
+  a.temx = dts * 0.5f * P.rdx;
+  a.temy = dts * 0.5f * P.rdy;
+  a.opw = (P.wbc == 2 && P.ibw == 1);
+  a.ope = (P.ebc == 2 && P.ibe == 1);
+  a.ops = (P.sbc == 2 && P.ibs == 1);
+  a.opn = (P.nbc == 2 && P.ibn == 1);
+  a.uten = m->f[CM1G_UTEN];
+  a.vten = m->f[CM1G_VTEN];
+  a.wten = m->f[CM1G_WTEN];
+  a.thv = m->f[CM1G_T11];
+  a.piadv = m->f[CM1G_SADV];
+  a.ppterm = m->f[CM1G_T22];
+  a.ppten = m->f[CM1G_PPTEN];
+  a.dtu0 = m->dtu0;
+  a.dtv0 = m->dtv0;
+  a.radbcw = m->d2[CM1G_D_RADBCW];
+  a.radbce = m->d2[CM1G_D_RADBCE];
+  a.radbcs = m->d2[CM1G_D_RADBCS];
+  a.radbcn = m->d2[CM1G_D_RADBCN];
+  a.w3d = w3d;
+  a.pp3d = pp3d;
+  a.fpk = m->f[CM1G_DUM8];
+  a.dum1 = dum1;
+  a.dum2 = m->f[CM1G_DUM2];
+  a.rru = rru;
+  a.rrv = rrv;
+  a.rrw = rrw;
+  // u, v and ppd alternate between the state arrays and scratch arrays
+  float *ub[2] = {u3d, m->scr1}, *vb[2] = {v3d, m->scr2}, *pb[2] = {ppd, m->tx};
+  const dim3 blk(64, 4, 1), grd((ni + 63) / 64, (nj + 3) / 4, 1);
+  const dim3 ring_grd((ni + nj + 1 + 255) / 256, nk, 1);
   for (int n = 1; n <= nloop; n++) {
-    // open-boundary normal velocities (ssopenbcw/e/s/n, src/bc.F:2392-2684)
-    if (opw || ope)
-      for2(m->stream, 1, nj, 1, nk, [=] __device__(int j, int k) {
-        if (opw)
-          u3d[IXD(P, 1, j, k)] = u3d[IXD(P, 1, j, k)] + dts * (-radbcw[(j + 2) + k * P.py] *
-              (u3d[IXD(P, 2, j, k)] - u3d[IXD(P, 1, j, k)]) * rdx * P.uh[1 + 2] + uten[IXD(P, 1, j, k)]);
-        if (ope)
-          u3d[IXD(P, ni + 1, j, k)] = u3d[IXD(P, ni + 1, j, k)] + dts * (-radbce[(j + 2) + k * P.py] *
-              (u3d[IXD(P, ni + 1, j, k)] - u3d[IXD(P, ni, j, k)]) * rdx * P.uh[ni + 2] + uten[IXD(P, ni + 1, j, k)]);
-      });
-    if (ops || opn)
-      for2(m->stream, 1, ni, 1, nk, [=] __device__(int i, int k) {
-        if (ops)
-          v3d[IXD(P, i, 1, k)] = v3d[IXD(P, i, 1, k)] + dts * (-radbcs[(i + 2) + k * P.px] *
-              (v3d[IXD(P, i, 2, k)] - v3d[IXD(P, i, 1, k)]) * rdy * P.vh[1 + 2] + vten[IXD(P, i, 1, k)]);
-        if (opn)
-          v3d[IXD(P, i, nj + 1, k)] = v3d[IXD(P, i, nj + 1, k)] + dts * (-radbcn[(i + 2) + k * P.px] *
-              (v3d[IXD(P, i, nj + 1, k)] - v3d[IXD(P, i, nj, k)]) * rdy * P.vh[nj + 2] + vten[IXD(P, i, nj + 1, k)]);
-      });
-    // u, v forward (src/sound.F:307-345)
-    {
-      const float tem1 = rdx * cp * 0.5f, tem2 = rdy * cp * 0.5f;
-      for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
-        int c = IXD(P, i, j, k);
-        float dtu = dts * dtu0[IX2D(P, i, j)];
-        float dtv = dts * dtv0[IX2D(P, i, j)];
-        u3d[c] = u3d[c] + dtu * (uten[c] - tem1 * (ppd[c] - ppd[IXD(P, i - 1, j, k)]) * P.uf[i + 2] *
-                                              (thv[c] + thv[IXD(P, i - 1, j, k)]));
-        v3d[c] = v3d[c] + dtv * (vten[c] - tem2 * (ppd[c] - ppd[IXD(P, i, j - 1, k)]) * P.vf[j + 2] *
-                                              (thv[c] + thv[IXD(P, i, j - 1, k)]));
-      });
-    }
-    // explicit pi forcing (src/sound.F:484-505)
-    {
-      const float temx = dts * 0.5f * rdx, temy = dts * 0.5f * rdy;
-      for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-        int c = IXD(P, i, j, k);
-        float div = (u3d[IXD(P, i + 1, j, k)] - u3d[c]) * rdx * P.uh[i + 2] +
-                    (v3d[IXD(P, i, j + 1, k)] - v3d[c]) * rdy * P.vh[j + 2];
-        if (fabsf(div) < smeps) div = 0.0f;
-        fpk[c] = dts * (ppten[c] - ppterm[c] * div) +
-                 (-(u3d[c] * (piadv[c] - piadv[IXD(P, i - 1, j, k)]) +
-                    u3d[IXD(P, i + 1, j, k)] * (piadv[IXD(P, i + 1, j, k)] - piadv[c])) * temx * P.uh[i + 2] -
-                  (v3d[c] * (piadv[c] - piadv[IXD(P, i, j - 1, k)]) +
-                   v3d[IXD(P, i, j + 1, k)] * (piadv[IXD(P, i, j + 1, k)] - piadv[c])) * temy * P.vh[j + 2]) +
-                 beta * (w3d[c] * (pk1[c] + qk[c]) + w3d[IXD(P, i, j, k + 1)] * (pk2[c] - qk[c]));
-      });
-    }
-    // vertically implicit solve for w (src/sound.F:659-733)
-    for2(m->stream, 1, ni, 1, nj, [=] __device__(int i, int j) {
-      float d1p = 0.0f, d2p = 0.0f;
-      for (int k = 2; k <= nk; k++) {
-        int c = IXD(P, i, j, k), cm = IXD(P, i, j, k - 1);
-        float m_ = mm[c];
-        float aa = -asq * (+m_ * (pk1[cm] + qk[cm]));
-        float cc = -asq * (-m_ * (pk2[c] - qk[c]));
-        float bb = 1.0f - asq * (+m_ * (-(pk1[c] + qk[c]) + (pk2[cm] - qk[cm])));
-        float fwk = dts * wten[c] + beta * (-m_ * (pp3d[c] - pp3d[cm]));
-        float dd = w3d[c] + fwk + alph * (-m_ * ((pp3d[c] + fpk[c]) - (pp3d[cm] + fpk[cm])));
-        if (k == 2) {
-          float r1 = 1.0f / bb;
-          d1p = -cc * r1;
-          d2p = dd * r1;
-          dum1[c] = d1p;
-          dum2[c] = d2p;
-        } else if (k < nk) {
-          float r1 = 1.0f / (aa * d1p + bb);
-          d1p = -cc * r1;
-          d2p = (dd - aa * d2p) * r1;
-          dum1[c] = d1p;
-          dum2[c] = d2p;
-        } else {
-          float r1 = 1.0f / (aa * d1p + bb);
-          w3d[c] = (dd - aa * d2p) * r1;
-        }
-      }
-      float wk1 = w3d[IXD(P, i, j, nk)];
-      for (int k = nk - 1; k >= 2; k--) {
-        int c = IXD(P, i, j, k);
-        wk1 = dum1[c] * wk1 + dum2[c];
-        w3d[c] = wk1;
-      }
-    });
-    // new pi and divergence damping (src/sound.F:739-765)
-    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    const int src = (n - 1) & 1;
+    a.uX = ub[src];
+    a.vX = vb[src];
+    a.uY = ub[src ^ 1];
+    a.vY = vb[src ^ 1];
+    a.ppdX = pb[src];
+    a.ppdY = pb[src ^ 1];
+    a.last = (n == nloop);
+    ktimer_mark(m, CM1G_KT_SOUND_SUBSTEP);
+    hipLaunchKernelGGL(k_sound_ring, ring_grd, dim3(256), 0, m->stream, P, a);
+    hipLaunchKernelGGL(k_sound_sub, grd, blk, 0, m->stream, P, a);
+    ktimer_mark(m, CM1G_KT_SOUND_SUBSTEP);
+    if (n < nloop) bc_fill(m, BC_P, pb[src ^ 1]);
+  }
+  if (nloop & 1) {
+    // the last substep wrote the scratch pair: copy the updated range back
+    const float *us = m->scr1, *vs = m->scr2;
+    for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
       int c = IXD(P, i, j, k);
-      float p_old = pp3d[c];
-      float pn = p_old + fpk[c] + alph * (w3d[c] * (pk1[c] + qk[c]) + w3d[IXD(P, i, j, k + 1)] * (pk2[c] - qk[c]));
-      if (fabsf(pn) < smeps) pn = 0.0f;
-      pp3d[c] = pn;
-      float d = kdiv * (pn - p_old);
-      dum1[c] = d;
-      ppd[c] = pn + d;
+      u3d[c] = us[c];
+      v3d[c] = vs[c];
     });
-    if (n < nloop) bc_fill(m, BC_P, ppd);
-    if (n < nloop) {
-      for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
-        int c = IXD(P, i, j, k);
-        rru[c] = rru[c] + u3d[c];
-        rrv[c] = rrv[c] + v3d[c];
-        rrw[c] = rrw[c] + w3d[c];
-      });
-    } else {
-      const float tavg = 1.0f / (float)nloop;
-      for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
-        int c = IXD(P, i, j, k);
-        rru[c] = (rru[c] + u3d[c]) * tavg;
-        rrv[c] = (rrv[c] + v3d[c]) * tavg;
-        rrw[c] = (rrw[c] + w3d[c] * alph) * tavg;
-      });
-    }
   }
   if (nrk == 3)
     for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {

@@ -52,7 +52,7 @@ typedef struct cm1g_config {
   int32_t advwenos, advwenov;     /* 0 flux-form 5th order; 1 WENO5 every stage; 2 last stage */
   int32_t apmasscon;              /* 1: dry-mass conserving pi' adjustment */
   int32_t irbc;                   /* 4: radbcew4/radbcns4 */
-  int32_t nsound;                 /* acoustic steps per large step (T1: cm1g_dtsmall) */
+  int32_t nsound;                 /* acoustic steps per large step (see cm1g_dtsmall) */
   int32_t npt;                    /* passive tracers */
   int32_t pdtra;                  /* positive-definite tracers */
   float dx, dy, dz, dt;
@@ -171,6 +171,17 @@ int cm1g_budgets(cm1g_model *m, double *tmass, double *tmois);
 /* Scalars carried between steps: [0] mass1, [1] train (domain rain
  * accumulation, src/kessler.F:472), [2] cflmax, [3] mass2. */
 int cm1g_scalars(cm1g_model *m, double *out4);
+
+/* Kernel timer (measurement hook, no counterpart in the reference): while
+ * started, every launch of the selected kernel is bracketed by HIP events
+ * on the model's stream.  cm1g_ktimer_stop synchronizes the stream and
+ * returns the summed kernel time, the launch count and the algorithmic
+ * bytes of one launch (distinct full-size fields read plus written, 4 B
+ * per interior cell each, scratch excluded). */
+enum { CM1G_KT_NONE = 0, CM1G_KT_SOUND_SUBSTEP = 1 };
+int cm1g_ktimer_start(cm1g_model *m, int which);
+int cm1g_ktimer_stop(cm1g_model *m, double *total_ms, int64_t *launches,
+                     double *bytes_per_launch, char *name, int name_len);
 
 #ifdef __cplusplus
 }
