@@ -60,6 +60,7 @@ void o_radbcns4(cm1o_model *M, float *radbcs, float *radbcn, const float *v1, co
 float o_flx3(float s1, float s2, float s3);
 float o_flx5(float s1, float s2, float s3, float s4, float s5);
 float o_weno5(float s1, float s2, float s3, float s4, float s5, double weps);
+float o_upstrpd(float s1, float s2, float s3, double weps);
 void o_advs(cm1o_model *M, int nrk, const float *rru, const float *rrv, const float *rrw,
             const float *divx, const float *s0, const float *s, float *sten,
             int pdef, int pdefweno, float dt, double weps);
@@ -72,6 +73,7 @@ void o_advw(cm1o_model *M, int nrk, const float *rru, const float *rrv, const fl
 
 /* sound.F */
 void o_sound(cm1o_model *M, int nrk, float dttmp);
+void o_sounde(cm1o_model *M, int nrk, float dttmp);
 
 /* misclibs.F */
 void o_getdiv(cm1o_model *M, const float *u, const float *v, const float *w, float *div);

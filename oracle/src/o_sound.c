@@ -1,6 +1,6 @@
 /* Oracle: split-explicit acoustic substeps with the vertically implicit
- * w-pi solve, psolver=3, no terrain, Cartesian (test infrastructure only).
- * Follows src/sound.F:10-859; the sound() argument mapping used by
+ * w-pi solve, psolver=3, and the explicit solver sounde, psolver=2; no
+ * terrain, Cartesian (test infrastructure only).  Follows src/sound.F:10-859; the sound() argument mapping used by
  * src/solve2.F:1122-1136 is: mm=dum5, ppd=dum7, fpk=dum8, qk=t12,
  * pk1=t13, pk2=t23, piadv=sadv, thv=t11, ppterm=t22.
  * Also getdiv/getdivx/calccflquick/pdefq from src/misclibs.F. */
@@ -226,6 +226,188 @@ void o_sound(cm1o_model *M, int nrk, float dttmp) {
             A3(rru, i, j, k) = (A3(rru, i, j, k) + A3(u3d, i, j, k)) * tavg;
             A3(rrv, i, j, k) = (A3(rrv, i, j, k) + A3(v3d, i, j, k)) * tavg;
             A3(rrw, i, j, k) = (A3(rrw, i, j, k) + A3(w3d, i, j, k) * alph) * tavg;
+          }
+    }
+  }
+
+  if (nrk == 3)
+    for (k = 1; k <= nk; k++)
+      for (j = 1; j <= nj; j++)
+        for (i = 1; i <= ni; i++) A3(ppx, i, j, k) = A3(dum1, i, j, k);
+}
+
+/* Explicit acoustic substeps, psolver=2, no terrain, Cartesian:
+ * src/sounde.F:10-1026 (w :548-563, pi :775-790).  Same argument mapping
+ * as o_sound (src/solve2.F:1100-1117); pk1=t13, pk2=t23. */
+void o_sounde(cm1o_model *M, int nrk, float dttmp) {
+  const int ni = M->ni, nj = M->nj, nk = M->nk;
+  const float dt = M->c.dt;
+  const float rdx = M->rdx, rdy = M->rdy, rdz = M->rdz, cp = M->k.cp;
+  const float kdiv = M->c.kdiv, smeps = M->k.smeps;
+  const float *mh = M->kf[K_MH], *mf = M->kf[K_MF], *rr0 = M->kf[K_RR0], *rf0 = M->kf[K_RF0];
+  const float *c1 = M->kf[K_C1], *c2 = M->kf[K_C2];
+  float *dum1 = M->f[F_DUM1], *ppd = M->f[F_DUM7];
+  float *pk1 = M->f[F_T13], *pk2 = M->f[F_T23];
+  float *piadv = M->f[F_SADV], *thv = M->f[F_T11], *ppterm = M->f[F_T22];
+  float *u3d = M->f[F_U3D], *v3d = M->f[F_V3D], *w3d = M->f[F_W3D];
+  float *ua = M->f[F_UA], *va = M->f[F_VA], *wa = M->f[F_WA];
+  float *uten = M->f[F_UTEN], *vten = M->f[F_VTEN], *wten = M->f[F_WTEN];
+  float *rru = M->f[F_RRU], *rrv = M->f[F_RRV], *rrw = M->f[F_RRW];
+  float *ppi = M->f[F_PPI], *pp3d = M->f[F_PP3D], *ppten = M->f[F_PPTEN], *ppx = M->f[F_PPX];
+  float *radbcw = M->d2[D_RADBCW], *radbce = M->d2[D_RADBCE];
+  float *radbcs = M->d2[D_RADBCS], *radbcn = M->d2[D_RADBCN];
+  int i, j, k, n, nloop;
+  float dts;
+  const int nsound = M->c.nsound;
+  (void)dttmp;
+
+  if (nrk == 1) {
+    nloop = (int)lroundf((float)nsound / 3.0f);
+    dts = dt / ((float)nloop * 3.0f);
+    if (dts > (dt / (float)nsound)) {
+      nloop = nloop + 1;
+      dts = dt / ((float)nloop * 3.0f);
+    }
+  } else if (nrk == 2) {
+    nloop = (int)(0.5f * (float)nsound);
+    dts = dt / (float)nsound;
+  } else {
+    nloop = nsound;
+    dts = dt / (float)nsound;
+  }
+
+  for (j = 1; j <= nj + 1; j++)
+    for (i = 1; i <= ni + 1; i++) {
+      M->dtu[IX2(M, i, j)] = dts * M->dtu0[IX2(M, i, j)];
+      M->dtv[IX2(M, i, j)] = dts * M->dtv0[IX2(M, i, j)];
+    }
+
+  /* pi0 vertical-advection coefficients (src/sounde.F:117-136, 160-170) */
+  for (k = 1; k <= nk; k++) {
+    float p1 = dts * rdz * mh[k] * rr0[k] * rf0[k + 1] * (1.0f - c1[k + 1]);
+    float p2 = -(dts * rdz * mh[k] * rr0[k] * rf0[k + 1] * c2[k + 1]);
+    float p3 = dts * rdz * mh[k] * rr0[k] * rf0[k] * c1[k];
+    float p4 = dts * rdz * mh[k] * rr0[k] * rf0[k] * (c2[k] - 1.0f);
+    for (j = 1; j <= nj; j++)
+      for (i = 1; i <= ni; i++) {
+        A3(pk2, i, j, k) = p2 * A3(piadv, i, j, k + 1) + p1 * A3(piadv, i, j, k);
+        A3(pk1, i, j, k) = p3 * A3(piadv, i, j, k - 1) + p4 * A3(piadv, i, j, k);
+      }
+  }
+  for (j = 1; j <= nj; j++)
+    for (i = 1; i <= ni; i++) {
+      A3(pk1, i, j, 1) = 0.0f;
+      A3(pk2, i, j, nk) = 0.0f;
+    }
+
+  if (nrk == 1) {
+    for (k = 1; k <= nk; k++)
+      for (j = 0; j <= nj + 1; j++)
+        for (i = 0; i <= ni + 1; i++) {
+          A3(ppd, i, j, k) = A3(ppx, i, j, k);
+          A3(rru, i, j, k) = 0.0f;
+          A3(rrv, i, j, k) = 0.0f;
+          A3(rrw, i, j, k) = 0.0f;
+        }
+  } else {
+    for (k = 1; k <= nk; k++)
+      for (j = 0; j <= nj + 1; j++)
+        for (i = 0; i <= ni + 1; i++) {
+          A3(u3d, i, j, k) = A3(ua, i, j, k);
+          A3(rru, i, j, k) = 0.0f;
+          A3(v3d, i, j, k) = A3(va, i, j, k);
+          A3(rrv, i, j, k) = 0.0f;
+          A3(w3d, i, j, k) = A3(wa, i, j, k);
+          A3(rrw, i, j, k) = 0.0f;
+          A3(pp3d, i, j, k) = A3(ppi, i, j, k);
+          A3(ppd, i, j, k) = A3(ppx, i, j, k);
+        }
+  }
+
+  for (n = 1; n <= nloop; n++) {
+    if (M->c.wbc == 2 && M->ibw == 1)
+      for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+          A3(u3d, 1, j, k) = A3(u3d, 1, j, k) + dts * (-radbcw[(j + 2) + (size_t)k * M->py] *
+              (A3(u3d, 2, j, k) - A3(u3d, 1, j, k)) * rdx * UH(1) + A3(uten, 1, j, k));
+    if (M->c.ebc == 2 && M->ibe == 1)
+      for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+          A3(u3d, ni + 1, j, k) = A3(u3d, ni + 1, j, k) + dts * (-radbce[(j + 2) + (size_t)k * M->py] *
+              (A3(u3d, ni + 1, j, k) - A3(u3d, ni, j, k)) * rdx * UH(ni) + A3(uten, ni + 1, j, k));
+    if (M->c.sbc == 2 && M->ibs == 1)
+      for (k = 1; k <= nk; k++)
+        for (i = 1; i <= ni; i++)
+          A3(v3d, i, 1, k) = A3(v3d, i, 1, k) + dts * (-radbcs[(i + 2) + (size_t)k * M->px] *
+              (A3(v3d, i, 2, k) - A3(v3d, i, 1, k)) * rdy * VH(1) + A3(vten, i, 1, k));
+    if (M->c.nbc == 2 && M->ibn == 1)
+      for (k = 1; k <= nk; k++)
+        for (i = 1; i <= ni; i++)
+          A3(v3d, i, nj + 1, k) = A3(v3d, i, nj + 1, k) + dts * (-radbcn[(i + 2) + (size_t)k * M->px] *
+              (A3(v3d, i, nj + 1, k) - A3(v3d, i, nj, k)) * rdy * VH(nj) + A3(vten, i, nj + 1, k));
+
+    /* u, v (src/sounde.F:333-352) */
+    {
+      const float tem1 = rdx * cp * 0.5f, tem2 = rdy * cp * 0.5f;
+      for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj + 1; j++)
+          for (i = 1; i <= ni + 1; i++) {
+            A3(u3d, i, j, k) = A3(u3d, i, j, k) + M->dtu[IX2(M, i, j)] * (A3(uten, i, j, k)
+                - tem1 * UF(i) * (A3(ppd, i, j, k) - A3(ppd, i - 1, j, k)) * (A3(thv, i, j, k) + A3(thv, i - 1, j, k)));
+            A3(v3d, i, j, k) = A3(v3d, i, j, k) + M->dtv[IX2(M, i, j)] * (A3(vten, i, j, k)
+                - tem2 * VF(j) * (A3(ppd, i, j, k) - A3(ppd, i, j - 1, k)) * (A3(thv, i, j, k) + A3(thv, i, j - 1, k)));
+          }
+    }
+    /* w (src/sounde.F:548-563) */
+    for (k = 2; k <= nk; k++) {
+      const float tem1 = rdz * cp * mf[k];
+      const float c1a = c1[k], c2a = c2[k];
+      for (j = 1; j <= nj; j++)
+        for (i = 1; i <= ni; i++)
+          A3(w3d, i, j, k) = A3(w3d, i, j, k) + dts * (A3(wten, i, j, k)
+              - tem1 * (A3(ppd, i, j, k) - A3(ppd, i, j, k - 1))
+                     * (c2a * A3(thv, i, j, k) + c1a * A3(thv, i, j, k - 1)));
+    }
+    /* pi (src/sounde.F:770-795) */
+    {
+      const float temx = dts * 0.5f * rdx, temy = dts * 0.5f * rdy;
+      for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj; j++)
+          for (i = 1; i <= ni; i++) {
+            float div = ((A3(u3d, i + 1, j, k) - A3(u3d, i, j, k)) * rdx * UH(i)
+                       + (A3(v3d, i, j + 1, k) - A3(v3d, i, j, k)) * rdy * VH(j))
+                       + (A3(w3d, i, j, k + 1) - A3(w3d, i, j, k)) * rdz * mh[k];
+            if (fabsf(div) < smeps) div = 0.0f;
+            float delp = dts * (A3(ppten, i, j, k) - A3(ppterm, i, j, k) * div)
+                + (-(A3(u3d, i, j, k) * (A3(piadv, i, j, k) - A3(piadv, i - 1, j, k))
+                     + A3(u3d, i + 1, j, k) * (A3(piadv, i + 1, j, k) - A3(piadv, i, j, k))) * temx * UH(i)
+                   - (A3(v3d, i, j, k) * (A3(piadv, i, j, k) - A3(piadv, i, j - 1, k))
+                     + A3(v3d, i, j + 1, k) * (A3(piadv, i, j + 1, k) - A3(piadv, i, j, k))) * temy * VH(j))
+                + (A3(w3d, i, j, k) * A3(pk1, i, j, k) + A3(w3d, i, j, k + 1) * A3(pk2, i, j, k));
+            A3(pp3d, i, j, k) = A3(pp3d, i, j, k) + delp;
+            if (fabsf(A3(pp3d, i, j, k)) < smeps) A3(pp3d, i, j, k) = 0.0f;
+            A3(dum1, i, j, k) = kdiv * delp;
+            A3(ppd, i, j, k) = A3(pp3d, i, j, k) + A3(dum1, i, j, k);
+          }
+    }
+    if (n < nloop) o_bcp(M, ppd);
+
+    if (n < nloop) {
+      for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj + 1; j++)
+          for (i = 1; i <= ni + 1; i++) {
+            A3(rru, i, j, k) = A3(rru, i, j, k) + A3(u3d, i, j, k);
+            A3(rrv, i, j, k) = A3(rrv, i, j, k) + A3(v3d, i, j, k);
+            A3(rrw, i, j, k) = A3(rrw, i, j, k) + A3(w3d, i, j, k);
+          }
+    } else {
+      const float tavg = 1.0f / (float)nloop;
+      for (k = 1; k <= nk; k++)
+        for (j = 1; j <= nj + 1; j++)
+          for (i = 1; i <= ni + 1; i++) {
+            A3(rru, i, j, k) = (A3(rru, i, j, k) + A3(u3d, i, j, k)) * tavg;
+            A3(rrv, i, j, k) = (A3(rrv, i, j, k) + A3(v3d, i, j, k)) * tavg;
+            A3(rrw, i, j, k) = (A3(rrw, i, j, k) + A3(w3d, i, j, k)) * tavg;
           }
     }
   }

@@ -1,6 +1,6 @@
 /* Oracle: time-step orchestration and C API (test infrastructure only).
  * solve1: src/solve1.F:414-454,1700-1780 (irdamp=0, no turbulence).
- * solve2: src/solve2.F:296-2052 (psolver=3, no terrain, Cartesian).
+ * solve2: src/solve2.F:296-2052 (psolver=2 or 3, no terrain, Cartesian).
  * solve3: src/solve3.F:246-705.  getset: src/init3d.F:2032-2872.
  * Constants: src/constants.F:69-170; metrics: src/param.F:7790-8215,
  * :8590-8622, :8980-9060, :9182-9218; base state: src/base.F:2085-2198. */
@@ -408,7 +408,8 @@ void o_solve2(cm1o_model *M) {
         for (i = 0; i <= ni + 1; i++) A3(sadv, i, j, k) = tem + A3(pp3d, i, j, k);
     }
 
-    o_sound(M, nrk, dttmp);
+    if (M->c.psolver == 2) o_sounde(M, nrk, dttmp);
+    else o_sound(M, nrk, dttmp);
 
     if (nrk == 3) o_calccflquick(M, dt, u3d, v3d, w3d);
 
@@ -649,5 +650,8 @@ void cm1o_advs(cm1o_model *M, int nrk, int s_id, int s0_id, int sten_id, int pde
 void cm1o_advu(cm1o_model *M, int nrk) { o_advu(M, nrk, M->f[F_RRU], M->f[F_U3D], M->f[F_UTEN], M->f[F_RRV], M->f[F_RRW], M->f[F_DIVX]); }
 void cm1o_advv(cm1o_model *M, int nrk) { o_advv(M, nrk, M->f[F_RRU], M->f[F_RRV], M->f[F_V3D], M->f[F_VTEN], M->f[F_RRW], M->f[F_DIVX]); }
 void cm1o_advw(cm1o_model *M, int nrk) { o_advw(M, nrk, M->f[F_RRU], M->f[F_RRV], M->f[F_RRW], M->f[F_W3D], M->f[F_WTEN], M->f[F_DIVX]); }
-void cm1o_sound(cm1o_model *M, int nrk, float dttmp) { o_sound(M, nrk, dttmp); }
+void cm1o_sound(cm1o_model *M, int nrk, float dttmp) {
+  if (M->c.psolver == 2) o_sounde(M, nrk, dttmp);
+  else o_sound(M, nrk, dttmp);
+}
 void cm1o_kessler_branch(cm1o_model *M) { o_mp_driver(M); }

@@ -71,9 +71,11 @@ def load_fixture(path):
     z = np.load(path)
     ni, nj, nk, nsteps, bc = [int(v) for v in z["meta"]]
     dt = float(z["dt"][0])
+    psolver, advwenos, advwenov = [int(v) for v in z["opts"]] if "opts" in z else (3, 0, 0)
     cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=bc, ebc=bc, sbc=bc, nbc=bc, imoist=1, ptype=1,
-                psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4, nsound=0, npt=0,
-                pdtra=1, dx=1000.0, dy=1000.0, dz=500.0, dt=dt, alph=0.6, kdiv=0.1, csmax=350.0)
+                psolver=psolver, advwenos=advwenos, advwenov=advwenov, apmasscon=1, irbc=4,
+                nsound=0, npt=0, pdtra=1, dx=1000.0, dy=1000.0, dz=500.0, dt=dt, alph=0.6,
+                kdiv=0.1, csmax=350.0)
     cols, _ = hinit.vertical_grid(nk, 500.0)
     for n in ("pi0", "th0", "qv0"):
         c = np.zeros(nk + 3, np.float32)
@@ -95,8 +97,12 @@ def load_fixture(path):
 
 
 def nsound_for(cfgd):
-    """dtsmall (src/misclibs.F:818-915) for psolver=3, in Python."""
-    dtsm = np.float32(0.60) * np.float32(min(cfgd["dx"], cfgd["dy"])) / np.float32(360.0)
+    """dtsmall (src/misclibs.F:818-915) for psolver=2 or 3 (3-D), in Python."""
+    if cfgd.get("psolver", 3) == 2:
+        mn = min(cfgd["dx"], cfgd["dy"], float(np.min(cfgd["dz"])))
+        dtsm = np.float32(0.50) * np.float32(mn) / np.float32(360.0)
+    else:
+        dtsm = np.float32(0.60) * np.float32(min(cfgd["dx"], cfgd["dy"])) / np.float32(360.0)
     ns = max(int(round(cfgd["dt"] / float(dtsm))), 4)
     if ns % 2:
         ns += 1

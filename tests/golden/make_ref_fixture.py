@@ -8,7 +8,11 @@ stores its GrADS output at t=0 and after N large steps as float32 arrays:
   base state (pi0, th0, qv0, prs0) the reference wrote at t=0;
 * ``tN_*``: the same prognostics plus rho and prs after N steps.
 
-Usage: python tests/golden/make_ref_fixture.py [name ni nj nk nsteps dt wbc]
+* ``opts``: the dynamical-core switches the run used (psolver, advwenos,
+  advwenov).
+
+Usage: python tests/golden/make_ref_fixture.py [name ni nj nk nsteps dt wbc
+[psolver advwenos advwenov]]
 """
 import os
 import sys
@@ -21,15 +25,25 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import refrun  # noqa: E402
 
 
-def make(name, ni, nj, nk, nsteps, dt, bc, workdir="/tmp/cm1_fixture"):
+OPT_KEYS = ("psolver", "advwenos", "advwenov")
+OPT_DEFAULTS = dict(psolver=3, advwenos=0, advwenov=0)
+
+
+def make(name, ni, nj, nk, nsteps, dt, bc, workdir="/tmp/cm1_fixture", opts=None, threads=1,
+         outdir=HERE):
+    """Run the reference and write <outdir>/<name>.npz.  opts: namelist
+    switches of the dynamical core (psolver, advwenos, advwenov)."""
+    o = dict(OPT_DEFAULTS)
+    o.update(opts or {})
     ov = dict(nx=ni, ny=nj, nz=nk, dtl=dt, timax=nsteps * dt, tapfrq=nsteps * dt,
-              wbc=bc, ebc=bc, sbc=bc, nbc=bc)
-    refrun.run_reference(workdir, ov, threads=1)
+              wbc=bc, ebc=bc, sbc=bc, nbc=bc, **o)
+    refrun.run_reference(workdir, ov, threads=threads)
     S = refrun.read_grads(workdir, "cm1out_s.ctl")
     U = refrun.read_grads(workdir, "cm1out_u.ctl")
     V = refrun.read_grads(workdir, "cm1out_v.ctl")
     W = refrun.read_grads(workdir, "cm1out_w.ctl")
-    out = dict(meta=np.array([ni, nj, nk, nsteps, bc], np.int32), dt=np.array([dt], np.float32))
+    out = dict(meta=np.array([ni, nj, nk, nsteps, bc], np.int32), dt=np.array([dt], np.float32),
+               opts=np.array([o[k] for k in OPT_KEYS], np.int32))
     for t, tag in ((0, "t0"), (1, "tN")):
         out[f"{tag}_u"] = U["u"][t]
         out[f"{tag}_v"] = V["v"][t]
@@ -43,7 +57,7 @@ def make(name, ni, nj, nk, nsteps, dt, bc, workdir="/tmp/cm1_fixture"):
         out[f"{tag}_prs"] = S["prs"][t]
     for n in ("pi0", "th0", "qv0", "prs0"):
         out[n] = S[n][0, :, 0, 0]
-    path = os.path.join(HERE, f"{name}.npz")
+    path = os.path.join(outdir, f"{name}.npz")
     np.savez_compressed(path, **{k: np.ascontiguousarray(v) for k, v in out.items()})
     return path
 
@@ -51,7 +65,9 @@ def make(name, ni, nj, nk, nsteps, dt, bc, workdir="/tmp/cm1_fixture"):
 if __name__ == "__main__":
     if len(sys.argv) > 1:
         a = sys.argv[1:]
-        print(make(a[0], int(a[1]), int(a[2]), int(a[3]), int(a[4]), float(a[5]), int(a[6])))
+        opts = dict(zip(OPT_KEYS, (int(v) for v in a[7:10])))
+        print(make(a[0], int(a[1]), int(a[2]), int(a[3]), int(a[4]), float(a[5]), int(a[6]),
+                   opts=opts))
     else:
         print(make("ref_open_24x20x18_10", 24, 20, 18, 10, 6.0, 2))
         print(make("ref_periodic_20x24x18_10", 20, 24, 18, 10, 6.0, 1))

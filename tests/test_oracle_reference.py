@@ -4,7 +4,8 @@
   via tests/golden/make_ref_fixture.py): from the reference's own t=0
   state, the oracle must reproduce its state after 10 steps bit for bit.
 * Live (when oracle/_ref/cm1_sp.exe is built): the 64x64x40 Kessler
-  supercell of the reference for 60 steps, bit for bit.
+  supercell of the reference, bit for bit, with the implicit (sound) and
+  explicit (sounde) acoustic solvers and 5th-order or WENO5 advection.
 """
 import glob
 import os
@@ -40,29 +41,18 @@ REF_EXE = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "cm1_sp.exe")
 
 
 @pytest.mark.skipif(not os.path.exists(REF_EXE), reason="reference executable not built")
-@pytest.mark.parametrize("bc", [2, 1])
-def test_oracle_matches_reference_live(bc, tmp_path):
-    import refrun
-    from common import load_fixture as _unused  # noqa: F401
-    import tempfile
-    nsteps = 60 if bc == 2 else 20
-    ni, nj, nk, dt = 64, 64, 40, 6.0
-    wd = str(tmp_path / "ref")
-    refrun.run_reference(wd, dict(nx=ni, ny=nj, nz=nk, dtl=dt, timax=nsteps * dt,
-                                  tapfrq=nsteps * dt, wbc=bc, ebc=bc, sbc=bc, nbc=bc),
-                         threads=4)
-    S = refrun.read_grads(wd, "cm1out_s.ctl")
-    U = refrun.read_grads(wd, "cm1out_u.ctl")
-    V = refrun.read_grads(wd, "cm1out_v.ctl")
-    W = refrun.read_grads(wd, "cm1out_w.ctl")
-    fx = os.path.join(tempfile.mkdtemp(), "live.npz")
-    out = dict(meta=np.array([ni, nj, nk, nsteps, bc], np.int32), dt=np.array([dt], np.float32))
-    for t, tag in ((0, "t0"), (1, "tN")):
-        out.update({f"{tag}_u": U["u"][t], f"{tag}_v": V["v"][t], f"{tag}_w": W["w"][t],
-                    f"{tag}_th": S["thpert"][t], f"{tag}_pi": S["pipert"][t],
-                    f"{tag}_qv": S["qv"][t], f"{tag}_qc": S["qc"][t], f"{tag}_qr": S["qr"][t],
-                    f"{tag}_rho": S["rho"][t], f"{tag}_prs": S["prs"][t]})
-    for n in ("pi0", "th0", "qv0", "prs0"):
-        out[n] = S[n][0, :, 0, 0]
-    np.savez(fx, **out)
+@pytest.mark.parametrize("bc,nsteps,opts", [
+    (2, 60, {}),
+    (1, 20, {}),
+    (1, 20, dict(psolver=2, advwenos=1, advwenov=0)),
+    (2, 30, dict(psolver=3, advwenos=2, advwenov=2)),
+], ids=["open-psolver3", "periodic-psolver3", "periodic-sounde-weno", "open-weno-last-stage"])
+def test_oracle_matches_reference_live(bc, nsteps, opts, tmp_path):
+    """64x64x40 Kessler supercell run by the reference, then by the oracle
+    from the reference's t=0 state: bit for bit after nsteps."""
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    import make_ref_fixture
+    fx = make_ref_fixture.make("live", 64, 64, 40, nsteps, 6.0, bc, workdir=str(tmp_path / "ref"),
+                               opts=opts, threads=4, outdir=str(tmp_path))
     test_oracle_matches_reference_fixture(fx)
