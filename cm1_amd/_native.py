@@ -62,6 +62,8 @@ SIGNATURES = {
     "cm1g_base_derive": (C.c_int, [C.c_void_p]),
     "cm1g_put_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cm1g_get_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "cm1g_put_array": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "cm1g_get_array": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "cm1g_put_2d": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cm1g_get_2d": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cm1g_field_ptr": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),

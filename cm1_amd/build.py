@@ -47,7 +47,36 @@ def build(verbose=False, jobs=8):
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     for o in objs:
         os.remove(o)
+    build_fortran_host()
     return LIB
+
+
+FDIR = os.path.join(HERE, "fortran")
+FHOST = os.path.join(FDIR, "cm1g_host")
+
+
+def build_fortran_host():
+    """The ISO_C_BINDING module (fortran/cm1gpu.f90) and the Fortran host
+    program (fortran/cm1g_host.f90) with flang, linked against the HIP
+    library by rpath.  Skipped when no Fortran compiler is installed."""
+    fc = None
+    for cand in ("flang", "amdflang", "/opt/rocm/llvm/bin/flang"):
+        try:
+            subprocess.run([cand, "--version"], capture_output=True, check=True)
+            fc = cand
+            break
+        except (OSError, subprocess.CalledProcessError):
+            continue
+    if fc is None:
+        return None
+    mdir = os.path.join(FDIR, "_mod")
+    os.makedirs(mdir, exist_ok=True)
+    cmd = [fc, "-O2", "-J", mdir, os.path.join(FDIR, "cm1gpu.f90"), os.path.join(FDIR, "cm1g_host.f90"),
+           "-o", FHOST, "-L" + HERE, "-lcm1gpu", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
+    return FHOST
 
 
 if __name__ == "__main__":

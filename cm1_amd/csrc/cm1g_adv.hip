@@ -16,6 +16,52 @@ __device__ __forceinline__ float flx5(float s1, float s2, float s3, float s4, fl
          (27.0f / 60.0f) * s4 + (-3.0f / 60.0f) * s5;
 }
 
+// weno5 with the Borges et al. indicators (siform=2),
+// src/adv_routines.F:6166-6199: the indicators are formed in single
+// precision and widened, the weights and the combination are double
+// precision, the result is rounded to single once.
+__device__ __forceinline__ float weno5(float s1, float s2, float s3, float s4, float s5, double weps) {
+  const float a1 = s1 - 2.0f * s2 + s3, e1 = s1 - 4.0f * s2 + 3.0f * s3;
+  const float a2 = s2 - 2.0f * s3 + s4, e2 = s2 - s4;
+  const float a3 = s3 - 2.0f * s4 + s5, e3 = 3.0f * s3 - 4.0f * s4 + s5;
+  const double b1 = (double)((13.0f / 12.0f) * (a1 * a1) + 0.25f * (e1 * e1));
+  const double b2 = (double)((13.0f / 12.0f) * (a2 * a2) + 0.25f * (e2 * e2));
+  const double b3 = (double)((13.0f / 12.0f) * (a3 * a3) + 0.25f * (e3 * e3));
+  const double d13 = fabs(b1 - b3);
+  const double q1 = fmin(1.0e30, d13 / (b1 + weps));
+  const double q2 = fmin(1.0e30, d13 / (b2 + weps));
+  const double q3 = fmin(1.0e30, d13 / (b3 + weps));
+  const double w1 = (double)0.1f * ((double)1.0f + q1 * q1);
+  const double w2 = (double)0.6f * ((double)1.0f + q2 * q2);
+  const double w3 = (double)0.3f * ((double)1.0f + q3 * q3);
+  const float p1 = (2.0f / 6.0f) * s1 + (-7.0f / 6.0f) * s2 + (11.0f / 6.0f) * s3;
+  const float p2 = (-1.0f / 6.0f) * s2 + (5.0f / 6.0f) * s3 + (2.0f / 6.0f) * s4;
+  const float p3 = (2.0f / 6.0f) * s3 + (5.0f / 6.0f) * s4 + (-1.0f / 6.0f) * s5;
+  return (float)((w1 * p1 + w2 * p2 + w3 * p3) / (w1 + w2 + w3));
+}
+
+// positive-definite upstream value (Beets & Koren), src/adv_routines.F:6331-6348
+__device__ __forceinline__ float upstrpd(float s1, float s2, float s3, double weps) {
+  const float dd = s2 - s1;
+  const float rr = (s3 - s2) / (copysignf((float)weps, dd) + dd);
+  const float phi = fmaxf(0.0f, fminf(rr, fminf((1.0f / 6.0f) + (2.0f / 6.0f) * rr, 1.0f)));
+  return s2 + phi * (s2 - s1);
+}
+
+// the pdef magnitude test of hadv_weno5/vadv_weno5: false when all five
+// stencil values are within weps of zero (the flux is then zero)
+__device__ __forceinline__ bool weno_doit(float s1, float s2, float s3, float s4, float s5, double weps) {
+  const float bmax = fmaxf(fmaxf(fmaxf(fmaxf(fabsf(s1), fabsf(s2)), fabsf(s3)), fabsf(s4)), fabsf(s5));
+  return bmax > (float)weps;
+}
+
+// upwind face value with 5th-order fluxes or WENO5
+template <bool WENO>
+__device__ __forceinline__ float face5(float s1, float s2, float s3, float s4, float s5, double weps) {
+  if (WENO) return weno5(s1, s2, s3, s4, s5, weps);
+  return flx5(s1, s2, s3, s4, s5);
+}
+
 // upwind 5th-order face value between a(m) (left) and a(m+1) given the
 // five-point stencils: positive flow uses a(m-2..m+2), negative a(m+3..m-1)
 #define FLXP(A, o1, o2, o3, o4, o5) flx5(A(o1), A(o2), A(o3), A(o4), A(o5))
@@ -23,8 +69,9 @@ __device__ __forceinline__ float flx5(float s1, float s2, float s3, float s4, fl
 // ------------------------------------------------------------ horizontal
 // stag 1: scalar; 2: u; 3: v; 4: w.  Writes dumx/dumy over the reference
 // loop ranges.
-static void hadv5(cm1g_model *m, int stag, const float *rru, const float *rrv, float *dumx,
-                  float *dumy, const float *a) {
+template <bool WENO>
+static void hadv5t(cm1g_model *m, int stag, const float *rru, const float *rrv, float *dumx,
+                   float *dumy, const float *a, double weps) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   if (stag == 1) {
@@ -32,11 +79,11 @@ static void hadv5(cm1g_model *m, int stag, const float *rru, const float *rrv, f
       auto S = [&](int ii, int jj) { return a[IXD(P, ii, jj, k)]; };
       float ru = rru[IXD(P, i, j, k)], rv = rrv[IXD(P, i, j, k)];
       dumx[IXD(P, i, j, k)] = (ru >= 0.0f)
-          ? ru * flx5(S(i - 3, j), S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j))
-          : ru * flx5(S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j), S(i - 2, j));
+          ? ru * face5<WENO>(S(i - 3, j), S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j), weps)
+          : ru * face5<WENO>(S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j), S(i - 2, j), weps);
       dumy[IXD(P, i, j, k)] = (rv >= 0.0f)
-          ? rv * flx5(S(i, j - 3), S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1))
-          : rv * flx5(S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1), S(i, j - 2));
+          ? rv * face5<WENO>(S(i, j - 3), S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1), weps)
+          : rv * face5<WENO>(S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1), S(i, j - 2), weps);
     });
   } else if (stag == 2) {
     int i1 = P.ibw == 1 ? 2 : 1, i2 = P.ibe == 1 ? ni : ni + 1;
@@ -44,12 +91,12 @@ static void hadv5(cm1g_model *m, int stag, const float *rru, const float *rrv, f
       auto S = [&](int ii, int jj) { return a[IXD(P, ii, jj, k)]; };
       float ubar = 0.5f * (rru[IXD(P, i, j, k)] + rru[IXD(P, i + 1, j, k)]);
       dumx[IXD(P, i, j, k)] = (ubar >= 0.0f)
-          ? ubar * flx5(S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j), S(i + 2, j))
-          : ubar * flx5(S(i + 3, j), S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j));
+          ? ubar * face5<WENO>(S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j), S(i + 2, j), weps)
+          : ubar * face5<WENO>(S(i + 3, j), S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j), weps);
       float vbar = 0.5f * (rrv[IXD(P, i, j, k)] + rrv[IXD(P, i - 1, j, k)]);
       dumy[IXD(P, i, j, k)] = (vbar >= 0.0f)
-          ? vbar * flx5(S(i, j - 3), S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1))
-          : vbar * flx5(S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1), S(i, j - 2));
+          ? vbar * face5<WENO>(S(i, j - 3), S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1), weps)
+          : vbar * face5<WENO>(S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1), S(i, j - 2), weps);
     });
   } else if (stag == 3) {
     int j1 = P.ibs == 1 ? 2 : 1, j2 = P.ibn == 1 ? nj : nj + 1;
@@ -57,12 +104,12 @@ static void hadv5(cm1g_model *m, int stag, const float *rru, const float *rrv, f
       auto S = [&](int ii, int jj) { return a[IXD(P, ii, jj, k)]; };
       float ubar = 0.5f * (rru[IXD(P, i, j, k)] + rru[IXD(P, i, j - 1, k)]);
       dumx[IXD(P, i, j, k)] = (ubar >= 0.0f)
-          ? ubar * flx5(S(i - 3, j), S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j))
-          : ubar * flx5(S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j), S(i - 2, j));
+          ? ubar * face5<WENO>(S(i - 3, j), S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j), weps)
+          : ubar * face5<WENO>(S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j), S(i - 2, j), weps);
       float vbar = 0.5f * (rrv[IXD(P, i, j, k)] + rrv[IXD(P, i, j + 1, k)]);
       dumy[IXD(P, i, j, k)] = (vbar >= 0.0f)
-          ? vbar * flx5(S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1), S(i, j + 2))
-          : vbar * flx5(S(i, j + 3), S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1));
+          ? vbar * face5<WENO>(S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1), S(i, j + 2), weps)
+          : vbar * face5<WENO>(S(i, j + 3), S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1), weps);
     });
   } else {
     for3(m->stream, 1, ni + 1, 1, nj + 1, 2, nk, [=] __device__(int i, int j, int k) {
@@ -71,14 +118,22 @@ static void hadv5(cm1g_model *m, int stag, const float *rru, const float *rrv, f
       float cc1 = 1.0f - cc2;
       float ubar = cc2 * rru[IXD(P, i, j, k)] + cc1 * rru[IXD(P, i, j, k - 1)];
       dumx[IXD(P, i, j, k)] = (ubar >= 0.0f)
-          ? ubar * flx5(S(i - 3, j), S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j))
-          : ubar * flx5(S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j), S(i - 2, j));
+          ? ubar * face5<WENO>(S(i - 3, j), S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j), weps)
+          : ubar * face5<WENO>(S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j), S(i - 2, j), weps);
       float vbar = cc2 * rrv[IXD(P, i, j, k)] + cc1 * rrv[IXD(P, i, j, k - 1)];
       dumy[IXD(P, i, j, k)] = (vbar >= 0.0f)
-          ? vbar * flx5(S(i, j - 3), S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1))
-          : vbar * flx5(S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1), S(i, j - 2));
+          ? vbar * face5<WENO>(S(i, j - 3), S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1), weps)
+          : vbar * face5<WENO>(S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1), S(i, j - 2), weps);
     });
   }
+}
+
+// hadv_flx5 (src/adv_routines.F:1335-1476) or hadv_weno5 (:204-430, no
+// pdef test: the momentum equations)
+static void hadv5(cm1g_model *m, int stag, const float *rru, const float *rrv, float *dumx,
+                  float *dumy, const float *a, bool weno = false, double weps = 0.0) {
+  if (weno) hadv5t<true>(m, stag, rru, rrv, dumx, dumy, a, weps);
+  else hadv5t<false>(m, stag, rru, rrv, dumx, dumy, a, weps);
 }
 
 __device__ __forceinline__ float wface(const KP &P, int stag, const float *rrw, int i, int j, int k) {
@@ -130,22 +185,85 @@ __device__ __forceinline__ float vflux5_of(const KP &P, int stag, float wb, SF S
                       : wb * flx3(S(k + 2), S(k + 1), S(k));
 }
 
+// vertical face flux with WENO5 (vadv_weno5, src/adv_routines.F:2359-2557)
+// and its lower-order boundary fluxes (vadv_lwr_weno_23 :4038-4208,
+// vadv_lwr_weno_35 :4214-4406, vadv_lwr_weno_35_b :4412-4447).  pdefweno
+// (scalars only) adds the magnitude test and the one-sided upstream values
+// at k=3 and k=nk-1.
+template <class SF>
+__device__ __forceinline__ float vfluxw_of(const KP &P, int stag, float wb, SF S, int k, int pdefweno,
+                                           double weps) {
+  const int nk = P.nk;
+  if (stag == 4) {
+    if (k == 1)
+      return (wb >= 0.0f) ? wb * 0.5f * (S(k) + S(k + 1)) : wb * upstrpd(S(k + 2), S(k + 1), S(k), weps);
+    if (k == nk)
+      return (wb > 0.0f) ? wb * upstrpd(S(k - 1), S(k), S(k + 1), weps) : wb * 0.5f * (S(k) + S(k + 1));
+    if (k == 2)
+      return (wb >= 0.0f) ? wb * upstrpd(S(k - 1), S(k), S(k + 1), weps)
+                          : wb * weno5(S(k + 3), S(k + 2), S(k + 1), S(k), S(k - 1), weps);
+    if (k == nk - 1)
+      return (wb > 0.0f) ? wb * weno5(S(k - 2), S(k - 1), S(k), S(k + 1), S(k + 2), weps)
+                         : wb * upstrpd(S(k + 2), S(k + 1), S(k), weps);
+    return (wb >= 0.0f) ? wb * weno5(S(k - 2), S(k - 1), S(k), S(k + 1), S(k + 2), weps)
+                        : wb * weno5(S(k + 3), S(k + 2), S(k + 1), S(k), S(k - 1), weps);
+  }
+  if (k == 2 || k == nk) {
+    float lin;
+    if (stag == 1) lin = P.c1[k] * S(k - 1) + P.c2[k] * S(k);
+    else {
+      float cc2 = 0.5f * (P.c2[k] + P.c2[k]);
+      float cc1 = 1.0f - cc2;
+      lin = cc1 * S(k - 1) + cc2 * S(k);
+    }
+    if (k == 2) return (wb >= 0.0f) ? wb * lin : wb * upstrpd(S(k + 1), S(k), S(k - 1), weps);
+    return (wb > 0.0f) ? wb * upstrpd(S(k - 2), S(k - 1), S(k), weps) : wb * lin;
+  }
+  if (stag == 1 && pdefweno == 1) {
+    if (k == 3 && wb > 0.0f) return wb * upstrpd(S(k - 2), S(k - 1), S(k), weps);
+    if (k == nk - 1 && wb < 0.0f) return wb * upstrpd(S(k + 1), S(k), S(k - 1), weps);
+    float s1, s2, s3, s4, s5;
+    if (wb >= 0.0f) { s1 = S(k - 3); s2 = S(k - 2); s3 = S(k - 1); s4 = S(k); s5 = S(k + 1); }
+    else { s1 = S(k + 2); s2 = S(k + 1); s3 = S(k); s4 = S(k - 1); s5 = S(k - 2); }
+    return weno_doit(s1, s2, s3, s4, s5, weps) ? wb * weno5(s1, s2, s3, s4, s5, weps) : 0.0f;
+  }
+  if (k == 3)
+    return (wb >= 0.0f) ? wb * upstrpd(S(k - 2), S(k - 1), S(k), weps)
+                        : wb * weno5(S(k + 2), S(k + 1), S(k), S(k - 1), S(k - 2), weps);
+  if (k == nk - 1) {
+    const bool up = (stag == 1) ? (wb >= 0.0f) : (wb > 0.0f);
+    return up ? wb * weno5(S(k - 3), S(k - 2), S(k - 1), S(k), S(k + 1), weps)
+              : wb * upstrpd(S(k + 1), S(k), S(k - 1), weps);
+  }
+  return (wb >= 0.0f) ? wb * weno5(S(k - 3), S(k - 2), S(k - 1), S(k), S(k + 1), weps)
+                      : wb * weno5(S(k + 2), S(k + 1), S(k), S(k - 1), S(k - 2), weps);
+}
+
 __device__ __forceinline__ float vflux5(const KP &P, int stag, const float *rrw, const float *a,
                                        int i, int j, int k) {
   auto S = [&](int kk) { return a[IXD(P, i, j, kk)]; };
   return vflux5_of(P, stag, wface(P, stag, rrw, i, j, k), S, k);
 }
 
-static void vadv5(cm1g_model *m, int stag, const float *rrw, float *dumz, const float *a) {
+// vadv_flx5 (src/adv_routines.F:3269-3388) or vadv_weno5 (:2359-2557) for
+// the momentum equations (no pdef variant)
+static void vadv5(cm1g_model *m, int stag, const float *rrw, float *dumz, const float *a,
+                  bool weno = false, double weps = 0.0) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   int ilo = 1, ihi = ni, jlo = 1, jhi = nj, klo = 2, khi = nk;
   if (stag == 2) { ilo = P.ibw == 1 ? 2 : 1; ihi = P.ibe == 1 ? ni : ni + 1; }
   if (stag == 3) { jlo = P.ibs == 1 ? 2 : 1; jhi = P.ibn == 1 ? nj : nj + 1; }
   if (stag == 4) { klo = 1; khi = nk; }
-  for3(m->stream, ilo, ihi, jlo, jhi, klo, khi, [=] __device__(int i, int j, int k) {
-    dumz[IXD(P, i, j, k)] = vflux5(P, stag, rrw, a, i, j, k);
-  });
+  if (weno)
+    for3(m->stream, ilo, ihi, jlo, jhi, klo, khi, [=] __device__(int i, int j, int k) {
+      auto S = [&](int kk) { return a[IXD(P, i, j, kk)]; };
+      dumz[IXD(P, i, j, k)] = vfluxw_of(P, stag, wface(P, stag, rrw, i, j, k), S, k, 0, weps);
+    });
+  else
+    for3(m->stream, ilo, ihi, jlo, jhi, klo, khi, [=] __device__(int i, int j, int k) {
+      dumz[IXD(P, i, j, k)] = vflux5(P, stag, rrw, a, i, j, k);
+    });
 }
 
 // ------------------------------------------------------------------ pdef
@@ -248,9 +366,15 @@ static void pdef_dir(cm1g_model *m, int dir, const float *vel, const float *adv,
 // ------------------------------------------------------ fused advs column
 // Upwind 5th-order face value for the face between cells m-1 and m along one
 // horizontal line, from the seven values S(m-3..m+3) (only six are used).
+template <bool WENO>
 __device__ __forceinline__ float hface5(float r, float sm3, float sm2, float sm1, float s0, float sp1,
-                                        float sp2) {
-  return (r >= 0.0f) ? r * flx5(sm3, sm2, sm1, s0, sp1) : r * flx5(sp2, sp1, s0, sm1, sm2);
+                                        float sp2, int pdef, double weps) {
+  if (!WENO) return (r >= 0.0f) ? r * flx5(sm3, sm2, sm1, s0, sp1) : r * flx5(sp2, sp1, s0, sm1, sm2);
+  float s1, s2, s3, s4, s5;
+  if (r >= 0.0f) { s1 = sm3; s2 = sm2; s3 = sm1; s4 = s0; s5 = sp1; }
+  else { s1 = sp2; s2 = sp1; s3 = s0; s4 = sm1; s5 = sm2; }
+  if (pdef == 1 && !weno_doit(s1, s2, s3, s4, s5, weps)) return 0.0f;
+  return r * weno5(s1, s2, s3, s4, s5, weps);
 }
 
 struct AdvsArgs {
@@ -261,14 +385,18 @@ struct AdvsArgs {
   float *advx, *advy, *advz;      // mode 1: the three advective tendencies
   float dt;
   int smeps_flush, mode;          // mode 0: final tendency; 1: advx/advy/advz only
+  int pdef, pdefweno;             // WENO5 magnitude tests (horizontal / vertical)
+  double weps;
 };
 
-// One column (i,j) of advs (src/adv.F:14-658 with hadv_flx5/vadv_flx5):
+// One column (i,j) of advs (src/adv.F:14-658 with hadv_flx5/vadv_flx5, or
+// hadv_weno5/vadv_weno5 when WENO):
 // face fluxes at i, i+1, j, j+1 with the open-boundary inflow fixes
 // (src/adv.F:211-340), the vertical fluxes marched up the column with a
 // six-level register window, and the tendency assembly.  Each flux is the
 // same FP32 expression as the reference's, so results are bitwise equal
 // to separate flux / divergence passes.
+template <bool WENO>
 __global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a) {
   const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int j = 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
@@ -298,8 +426,8 @@ __global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a) {
     // x faces i and i+1
     const float xm3 = H(-3), xm2 = H(-2), xm1 = H(-1), xp1 = H(1), xp2 = H(2), xp3 = H(3);
     const float ru0 = a.rru[c], ru1 = a.rru[c + 1];
-    float fx0 = hface5(ru0, xm3, xm2, xm1, sc, xp1, xp2);
-    float fx1 = hface5(ru1, xm2, xm1, sc, xp1, xp2, xp3);
+    float fx0 = hface5<WENO>(ru0, xm3, xm2, xm1, sc, xp1, xp2, a.pdef, a.weps);
+    float fx1 = hface5<WENO>(ru1, xm2, xm1, sc, xp1, xp2, xp3, a.pdef, a.weps);
     const float fx0_raw = fx0;
     if (doitw && i == 1 && ru0 >= 0.0f) fx0 = fx1;
     if (doite && i == ni && ru1 <= 0.0f) fx1 = fx0_raw;
@@ -307,8 +435,8 @@ __global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a) {
     const float ym3 = H(-3 * px), ym2 = H(-2 * px), ym1 = H(-px), yp1 = H(px), yp2 = H(2 * px),
                 yp3 = H(3 * px);
     const float rv0 = a.rrv[c], rv1 = a.rrv[c + px];
-    float fy0 = hface5(rv0, ym3, ym2, ym1, sc, yp1, yp2);
-    float fy1 = hface5(rv1, ym2, ym1, sc, yp1, yp2, yp3);
+    float fy0 = hface5<WENO>(rv0, ym3, ym2, ym1, sc, yp1, yp2, a.pdef, a.weps);
+    float fy1 = hface5<WENO>(rv1, ym2, ym1, sc, yp1, yp2, yp3, a.pdef, a.weps);
     const float fy0_raw = fy0;
     if (doits && j == 1 && rv0 >= 0.0f) fy0 = fy1;
     if (doitn && j == nj && rv1 <= 0.0f) fy1 = fy0_raw;
@@ -326,7 +454,8 @@ __global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a) {
         const int o = kk - kf;  // -3..2
         return o == -3 ? wv0 : o == -2 ? wv1 : o == -1 ? wv2 : o == 0 ? wv3 : o == 1 ? wv4 : wv5;
       };
-      fz_hi = vflux5_of(P, 1, a.rrw[c + pxy], S, kf);
+      fz_hi = WENO ? vfluxw_of(P, 1, a.rrw[c + pxy], S, kf, a.pdefweno, a.weps)
+                   : vflux5_of(P, 1, a.rrw[c + pxy], S, kf);
     }
     float az;
     if (k == 1) az = -fz_hi * P.rdz * P.mh[1];
@@ -370,7 +499,8 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
               int pdef, int pdefweno, float dt, double weps, int smeps_flush) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
-  (void)nrk; (void)pdefweno; (void)weps;
+  // WENO5 on every stage (advwenos=1) or the last one (2), src/adv.F:75-78
+  const bool weno = (m->c.advwenos == 1) || (m->c.advwenos == 2 && nrk == 3);
   AdvsArgs g;
   g.s = s_in;
   g.sadd = sadd;
@@ -389,6 +519,10 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
   g.advz = m->tz;
   g.dt = dt;
   g.smeps_flush = smeps_flush;
+  g.pdef = pdef;
+  g.pdefweno = pdefweno;
+  g.weps = weps;
+  auto *kern = weno ? k_advs_col<true> : k_advs_col<false>;
   const dim3 blk(64, 4, 1), grd((ni + 63) / 64, (nj + 3) / 4, 1);
   // an in-place update (out_new aliasing the advected field) goes through
   // a partner array, since neighbouring columns still read the old values
@@ -396,10 +530,10 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
   if (inplace) g.out_new = m->thb;
   if (pdef != 1) {
     g.mode = 0;
-    hipLaunchKernelGGL(k_advs_col, grd, blk, 0, m->stream, P, g);
+    hipLaunchKernelGGL(kern, grd, blk, 0, m->stream, P, g);
   } else {
     g.mode = 1;
-    hipLaunchKernelGGL(k_advs_col, grd, blk, 0, m->stream, P, g);
+    hipLaunchKernelGGL(kern, grd, blk, 0, m->stream, P, g);
     float *advx = m->tx, *advy = m->ty, *advz = m->tz;
     float *dumx = m->sx, *dumy = m->sy, *dumz = m->sz;
     const float *rru = m->rrus, *rrv = m->rrvs, *rrw = m->rrws;
@@ -451,7 +585,9 @@ void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy) {
   const float *rru = m->f[CM1G_RRU], *rrv = m->f[CM1G_RRV], *rrw = m->f[CM1G_RRW];
   const float *divx = m->f[CM1G_DIVX];
   float *dumx = m->sx, *dumy = m->sy, *dumz = m->sz;
-  (void)nrk;
+  // WENO5 on every stage (advwenov=1) or the last one (2), src/adv.F:958-960
+  const bool weno = (m->c.advwenov == 1) || (m->c.advwenov == 2 && nrk == 3);
+  const double weps = (double)(100.0f * P.K.epsilon);
   const bool doitw = (P.wbc == 2 && P.ibw == 1), doite = (P.ebc == 2 && P.ibe == 1);
   const bool doits = (P.sbc == 2 && P.ibs == 1), doitn = (P.nbc == 2 && P.ibn == 1);
   // ---- u
@@ -459,7 +595,7 @@ void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy) {
     const float *u3d = m->f[CM1G_U3D];
     float *uten = m->f[CM1G_UTEN];
     int i1 = P.ibw == 1 ? 2 : 1, i2 = P.ibe == 1 ? ni : ni + 1;
-    hadv5(m, 2, rru, rrv, dumx, dumy, u3d);
+    hadv5(m, 2, rru, rrv, dumx, dumy, u3d, weno, weps);
     if (doits || doitn)
       for2(m->stream, i1, i2, 1, nk, [=] __device__(int i, int k) {
         if (doits && (rrv[IXD(P, i, 1, k)] + rrv[IXD(P, i - 1, 1, k)]) >= 0.0f)
@@ -467,7 +603,7 @@ void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy) {
         if (doitn && (rrv[IXD(P, i, nj + 1, k)] + rrv[IXD(P, i - 1, nj + 1, k)]) <= 0.0f)
           dumy[IXD(P, i, nj + 1, k)] = dumy[IXD(P, i, nj, k)];
       });
-    vadv5(m, 2, rrw, dumz, u3d);
+    vadv5(m, 2, rrw, dumz, u3d, weno, weps);
     for3(m->stream, i1, i2, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
       int c = IXD(P, i, j, k);
       float ax = -(dumx[c] - dumx[IXD(P, i - 1, j, k)]) * P.rdx * P.uf[i + 2];
@@ -490,7 +626,7 @@ void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy) {
     const float *v3d = m->f[CM1G_V3D];
     float *vten = m->f[CM1G_VTEN];
     int j1 = P.ibs == 1 ? 2 : 1, j2 = P.ibn == 1 ? nj : nj + 1;
-    hadv5(m, 3, rru, rrv, dumx, dumy, v3d);
+    hadv5(m, 3, rru, rrv, dumx, dumy, v3d, weno, weps);
     if (doitw || doite)
       for2(m->stream, j1, j2, 1, nk, [=] __device__(int j, int k) {
         if (doitw && (rru[IXD(P, 1, j, k)] + rru[IXD(P, 1, j - 1, k)]) >= 0.0f)
@@ -498,7 +634,7 @@ void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy) {
         if (doite && (rru[IXD(P, ni + 1, j, k)] + rru[IXD(P, ni + 1, j - 1, k)]) <= 0.0f)
           dumx[IXD(P, ni + 1, j, k)] = dumx[IXD(P, ni, j, k)];
       });
-    vadv5(m, 3, rrw, dumz, v3d);
+    vadv5(m, 3, rrw, dumz, v3d, weno, weps);
     for3(m->stream, 1, ni, j1, j2, 1, nk, [=] __device__(int i, int j, int k) {
       int c = IXD(P, i, j, k);
       float ax = -(dumx[IXD(P, i + 1, j, k)] - dumx[c]) * P.rdx * P.uh[i + 2];
@@ -521,7 +657,7 @@ void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy) {
     const float *w3d = m->f[CM1G_W3D];
     float *wten = m->f[CM1G_WTEN];
     const float *dum8 = m->f[CM1G_DUM8];
-    hadv5(m, 4, rru, rrv, dumx, dumy, w3d);
+    hadv5(m, 4, rru, rrv, dumx, dumy, w3d, weno, weps);
     if (doitw || doite)
       for2(m->stream, 1, nj, 2, nk, [=] __device__(int j, int k) {
         float cc2 = 0.5f * (P.c2[k] + P.c2[k]), cc1 = 1.0f - cc2;
@@ -538,7 +674,7 @@ void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy) {
         if (doitn && (cc2 * rrv[IXD(P, i, nj + 1, k)] + cc1 * rrv[IXD(P, i, nj + 1, k - 1)]) <= 0.0f)
           dumy[IXD(P, i, nj + 1, k)] = dumy[IXD(P, i, nj, k)];
       });
-    vadv5(m, 4, rrw, dumz, w3d);
+    vadv5(m, 4, rrw, dumz, w3d, weno, weps);
     for3(m->stream, 1, ni, 1, nj, 2, nk, [=] __device__(int i, int j, int k) {
       int c = IXD(P, i, j, k);
       float cc2 = 0.5f * (P.c2[k] + P.c2[k]), cc1 = 1.0f - cc2;

@@ -133,6 +133,7 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
               int pdef, int pdefweno, float dt, double weps, int smeps_flush);
 // acoustic (cm1g_sound.hip)
 void sound_run(cm1g_model *m, int nrk);
+void sounde_run(cm1g_model *m, int nrk);
 void getdiv_both(cm1g_model *m, int want_t11);
 void cfl_run(cm1g_model *m, float dt, const float *u, const float *v, const float *w);
 // microphysics (cm1g_micro.hip)

@@ -96,8 +96,10 @@ extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
          "cm1g_create: lateral boundaries must be 1 (periodic) or 2 (open)");
   ARGCHK((cfg->wbc == 1) == (cfg->ebc == 1) && (cfg->sbc == 1) == (cfg->nbc == 1),
          "cm1g_create: periodic boundaries must be set on both sides");
-  ARGCHK(cfg->psolver == 3, "cm1g_create: only psolver=3 (sound.F) is built");
-  ARGCHK(cfg->advwenos == 0 && cfg->advwenov == 0, "cm1g_create: WENO advection is not built yet");
+  ARGCHK(cfg->psolver == 2 || cfg->psolver == 3,
+         "cm1g_create: psolver must be 2 (sounde.F) or 3 (sound.F)");
+  ARGCHK(cfg->advwenos >= 0 && cfg->advwenos <= 2 && cfg->advwenov >= 0 && cfg->advwenov <= 2,
+         "cm1g_create: advwenos/advwenov must be 0, 1 or 2 (weno_order=5)");
   ARGCHK(cfg->imoist == 0 || cfg->imoist == 1, "cm1g_create: imoist must be 0 or 1");
   ARGCHK(cfg->ptype == 0 || cfg->ptype == 1, "cm1g_create: ptype must be 0 or 1 (Kessler)");
   ARGCHK(cfg->nsound >= 1 && cfg->nsound <= 192, "cm1g_create: nsound out of range");
@@ -330,6 +332,42 @@ extern "C" int cm1g_get_field(cm1g_model *m, int id, float *host) {
   ARGCHK(field_ok(m, id) && host, "cm1g_get_field: bad field id or null pointer");
   HIPCHK(hipStreamSynchronize(m->stream));
   HIPCHK(hipMemcpy(host, m->f[id], m->ntot * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// Host arrays in the reference's own shapes (src/cm1.F:747-777): scalars
+// (ib:ie,jb:je,kb:ke) with ib=-2, jb=-2, kb=0; u adds one point in i, v
+// one in j, w one in k.  One strided copy per k plane into the padded
+// device box.
+static int array_shape(const cm1g_model *m, int stag, size_t *nx, size_t *ny, size_t *nz) {
+  if (stag < CM1G_STAG_S || stag > CM1G_STAG_W) return 1;
+  *nx = (size_t)m->P.ni + 6 + (stag == CM1G_STAG_U ? 1 : 0);
+  *ny = (size_t)m->P.nj + 6 + (stag == CM1G_STAG_V ? 1 : 0);
+  *nz = (size_t)m->P.nk + 2 + (stag == CM1G_STAG_W ? 1 : 0);
+  return 0;
+}
+
+extern "C" int cm1g_put_array(cm1g_model *m, int id, int stag, const float *host) {
+  ARGCHK(field_ok(m, id) && host, "cm1g_put_array: bad field id or null pointer");
+  size_t nx, ny, nz;
+  ARGCHK(array_shape(m, stag, &nx, &ny, &nz) == 0, "cm1g_put_array: bad staggering");
+  HIPCHK(hipStreamSynchronize(m->stream));
+  const size_t dpitch = (size_t)m->P.px * sizeof(float), plane = (size_t)m->P.pxy;
+  for (size_t k = 0; k < nz; k++)
+    HIPCHK(hipMemcpy2D(m->f[id] + k * plane, dpitch, host + k * nx * ny, nx * sizeof(float),
+                       nx * sizeof(float), ny, hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int cm1g_get_array(cm1g_model *m, int id, int stag, float *host) {
+  ARGCHK(field_ok(m, id) && host, "cm1g_get_array: bad field id or null pointer");
+  size_t nx, ny, nz;
+  ARGCHK(array_shape(m, stag, &nx, &ny, &nz) == 0, "cm1g_get_array: bad staggering");
+  HIPCHK(hipStreamSynchronize(m->stream));
+  const size_t spitch = (size_t)m->P.px * sizeof(float), plane = (size_t)m->P.pxy;
+  for (size_t k = 0; k < nz; k++)
+    HIPCHK(hipMemcpy2D(host + k * nx * ny, nx * sizeof(float), m->f[id] + k * plane, spitch,
+                       nx * sizeof(float), ny, hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -663,7 +701,8 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
       int c = IXD(P, i, j, k);
       sadv[c] = P.pi0[k] + pp3d[c];
     });
-    sound_run(m, nrk);
+    if (m->c.psolver == 2) sounde_run(m, nrk);
+    else sound_run(m, nrk);
     if (nrk == 3) cfl_run(m, dt, u3d, v3d, w3d);
     if (m->c.irbc == 4 && (P.ibw || P.ibe || P.ibs || P.ibn))
       radbc4(m, f[CM1G_UA], u3d, f[CM1G_VA], v3d, dttmp);
@@ -903,7 +942,8 @@ extern "C" int cm1g_advw(cm1g_model *m, int nrk) {
 extern "C" int cm1g_sound(cm1g_model *m, int nrk, float dttmp) {
   ARGCHK(m && nrk >= 1 && nrk <= 3, "cm1g_sound: bad argument");
   (void)dttmp;
-  sound_run(m, nrk);
+  if (m->c.psolver == 2) sounde_run(m, nrk);
+  else sound_run(m, nrk);
   return check_launch();
 }
 extern "C" int cm1g_kessler_branch(cm1g_model *m) {

@@ -397,6 +397,152 @@ void sound_run(cm1g_model *m, int nrk) {
     });
 }
 
+// Explicit acoustic substeps, psolver=2 (sounde, src/sounde.F:10-1026;
+// Cartesian, no terrain).  Per substep: the radiative open-boundary
+// velocities, then one kernel that advances u, v, w from the previous
+// substep's ppd (src/sounde.F:333-352, 548-563) and folds them into the
+// averages, then one kernel for the pressure (src/sounde.F:770-795) whose
+// pi0 vertical-advection coefficients pk1/pk2 (src/sounde.F:117-136,
+// 160-170) are evaluated on the fly.
+void sounde_run(cm1g_model *m, int nrk) {
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  const float dt = m->c.dt, kdiv = m->c.kdiv;
+  int nloop;
+  float dts;
+  sound_nloop(nrk, m->c.nsound, dt, &nloop, &dts);
+  float *dum1 = m->f[CM1G_DUM1], *ppd = m->f[CM1G_DUM7];
+  const float *piadv = m->f[CM1G_SADV], *thv = m->f[CM1G_T11], *ppterm = m->f[CM1G_T22];
+  float *u3d = m->f[CM1G_U3D], *v3d = m->f[CM1G_V3D], *w3d = m->f[CM1G_W3D];
+  const float *ua = m->f[CM1G_UA], *va = m->f[CM1G_VA], *wa = m->f[CM1G_WA];
+  const float *uten = m->f[CM1G_UTEN], *vten = m->f[CM1G_VTEN], *wten = m->f[CM1G_WTEN];
+  float *rru = m->f[CM1G_RRU], *rrv = m->f[CM1G_RRV], *rrw = m->f[CM1G_RRW];
+  const float *ppi = m->f[CM1G_PPI], *ppten = m->f[CM1G_PPTEN];
+  float *pp3d = m->f[CM1G_PP3D], *ppx = m->f[CM1G_PPX];
+  const float *radbcw = m->d2[CM1G_D_RADBCW], *radbce = m->d2[CM1G_D_RADBCE];
+  const float *radbcs = m->d2[CM1G_D_RADBCS], *radbcn = m->d2[CM1G_D_RADBCN];
+  const float *dtu0 = m->dtu0, *dtv0 = m->dtv0;
+  const float rdx = P.rdx, rdy = P.rdy, rdz = P.rdz, cp = P.K.cp, smeps = P.K.smeps;
+
+  // prep copies (src/sounde.F:181-233)
+  if (nrk == 1) {
+    for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      ppd[c] = ppx[c];
+      rru[c] = 0.0f;
+      rrv[c] = 0.0f;
+      rrw[c] = 0.0f;
+    });
+  } else {
+    for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      u3d[c] = ua[c];
+      rru[c] = 0.0f;
+      v3d[c] = va[c];
+      rrv[c] = 0.0f;
+      w3d[c] = wa[c];
+      rrw[c] = 0.0f;
+      pp3d[c] = ppi[c];
+      ppd[c] = ppx[c];
+    });
+  }
+
+  const bool opw = (P.wbc == 2 && P.ibw == 1), ope = (P.ebc == 2 && P.ibe == 1);
+  const bool ops = (P.sbc == 2 && P.ibs == 1), opn = (P.nbc == 2 && P.ibn == 1);
+  const float tavg = 1.0f / (float)nloop;
+  for (int n = 1; n <= nloop; n++) {
+    const bool last = (n == nloop);
+    if (opw || ope)
+      for2(m->stream, 1, nj, 1, nk, [=] __device__(int j, int k) {
+        if (opw)
+          u3d[IXD(P, 1, j, k)] = u3d[IXD(P, 1, j, k)] + dts * (-radbcw[(j + 2) + k * P.py] *
+              (u3d[IXD(P, 2, j, k)] - u3d[IXD(P, 1, j, k)]) * rdx * P.uh[1 + 2] + uten[IXD(P, 1, j, k)]);
+        if (ope)
+          u3d[IXD(P, ni + 1, j, k)] = u3d[IXD(P, ni + 1, j, k)] + dts * (-radbce[(j + 2) + k * P.py] *
+              (u3d[IXD(P, ni + 1, j, k)] - u3d[IXD(P, ni, j, k)]) * rdx * P.uh[ni + 2] + uten[IXD(P, ni + 1, j, k)]);
+      });
+    if (ops || opn)
+      for2(m->stream, 1, ni, 1, nk, [=] __device__(int i, int k) {
+        if (ops)
+          v3d[IXD(P, i, 1, k)] = v3d[IXD(P, i, 1, k)] + dts * (-radbcs[(i + 2) + k * P.px] *
+              (v3d[IXD(P, i, 2, k)] - v3d[IXD(P, i, 1, k)]) * rdy * P.vh[1 + 2] + vten[IXD(P, i, 1, k)]);
+        if (opn)
+          v3d[IXD(P, i, nj + 1, k)] = v3d[IXD(P, i, nj + 1, k)] + dts * (-radbcn[(i + 2) + k * P.px] *
+              (v3d[IXD(P, i, nj + 1, k)] - v3d[IXD(P, i, nj, k)]) * rdy * P.vh[nj + 2] + vten[IXD(P, i, nj + 1, k)]);
+      });
+    // u, v, w forward and the velocity averages
+    {
+      const float tem1 = rdx * cp * 0.5f, tem2 = rdy * cp * 0.5f;
+      for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+        const int c = IXD(P, i, j, k);
+        const float dtu = dts * dtu0[IX2D(P, i, j)], dtv = dts * dtv0[IX2D(P, i, j)];
+        const float u = u3d[c] + dtu * (uten[c] - tem1 * P.uf[i + 2] * (ppd[c] - ppd[c - 1]) *
+                                                      (thv[c] + thv[c - 1]));
+        const float v = v3d[c] + dtv * (vten[c] - tem2 * P.vf[j + 2] * (ppd[c] - ppd[c - P.px]) *
+                                                      (thv[c] + thv[c - P.px]));
+        float w = w3d[c];
+        if (k >= 2 && i <= ni && j <= nj) {
+          const float temw = rdz * cp * P.mf[k];
+          w = w + dts * (wten[c] - temw * (ppd[c] - ppd[c - P.pxy]) *
+                                       (P.c2[k] * thv[c] + P.c1[k] * thv[c - P.pxy]));
+          w3d[c] = w;
+        }
+        u3d[c] = u;
+        v3d[c] = v;
+        if (last) {
+          rru[c] = (rru[c] + u) * tavg;
+          rrv[c] = (rrv[c] + v) * tavg;
+          rrw[c] = (rrw[c] + w) * tavg;
+        } else {
+          rru[c] = rru[c] + u;
+          rrv[c] = rrv[c] + v;
+          rrw[c] = rrw[c] + w;
+        }
+      });
+    }
+    // pressure
+    {
+      const float temx = dts * 0.5f * rdx, temy = dts * 0.5f * rdy;
+      for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+        const int c = IXD(P, i, j, k);
+        const float mh = P.mh[k], rr0 = P.rr0[k];
+        float pk1 = 0.0f, pk2 = 0.0f;
+        if (k < nk) {
+          const float p1 = dts * rdz * mh * rr0 * P.rf0[k + 1] * (1.0f - P.c1[k + 1]);
+          const float p2 = -(dts * rdz * mh * rr0 * P.rf0[k + 1] * P.c2[k + 1]);
+          pk2 = p2 * piadv[c + P.pxy] + p1 * piadv[c];
+        }
+        if (k > 1) {
+          const float p3 = dts * rdz * mh * rr0 * P.rf0[k] * P.c1[k];
+          const float p4 = dts * rdz * mh * rr0 * P.rf0[k] * (P.c2[k] - 1.0f);
+          pk1 = p3 * piadv[c - P.pxy] + p4 * piadv[c];
+        }
+        const float u0 = u3d[c], u1 = u3d[c + 1], v0 = v3d[c], v1 = v3d[c + P.px];
+        const float w0 = w3d[c], w1 = w3d[c + P.pxy];
+        const float pa = piadv[c];
+        float div = ((u1 - u0) * rdx * P.uh[i + 2] + (v1 - v0) * rdy * P.vh[j + 2]) + (w1 - w0) * rdz * mh;
+        if (fabsf(div) < smeps) div = 0.0f;
+        const float delp = dts * (ppten[c] - ppterm[c] * div) +
+                           (-(u0 * (pa - piadv[c - 1]) + u1 * (piadv[c + 1] - pa)) * temx * P.uh[i + 2] -
+                            (v0 * (pa - piadv[c - P.px]) + v1 * (piadv[c + P.px] - pa)) * temy * P.vh[j + 2]) +
+                           (w0 * pk1 + w1 * pk2);
+        float p = pp3d[c] + delp;
+        if (fabsf(p) < smeps) p = 0.0f;
+        pp3d[c] = p;
+        const float d = kdiv * delp;
+        if (last) dum1[c] = d;
+        ppd[c] = p + d;
+      });
+    }
+    if (n < nloop) bc_fill(m, BC_P, ppd);
+  }
+  if (nrk == 3)
+    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      int c = IXD(P, i, j, k);
+      ppx[c] = dum1[c];
+    });
+}
+
 // getdiv (t11 from the time-averaged velocities) and getdivx (mass fluxes
 // scaled by rho0/rf0 into rrus/rrvs/rrws, and divx from them).
 void getdiv_both(cm1g_model *m, int want_t11) {

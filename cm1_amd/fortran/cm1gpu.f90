@@ -1,0 +1,265 @@
+! ISO_C_BINDING interface of the cm1gpu C ABI (include/cm1gpu.h) for the
+! CM1 Fortran host.  A host that keeps its own namelist, allocation and
+! output code calls these in place of solve1/solve2/mp_driver/solve3
+! (src/cm1.F:3313-3476) or of the individual routines (advs, sound, ...);
+! every entry returns 0 or a CM1G_E_* code (message: cm1g_error_message).
+!
+! Array arguments of cm1g_put_array/cm1g_get_array use the reference's own
+! shapes: scalars (ib:ie,jb:je,kb:ke), u (ib:ie+1,...), v (...,jb:je+1,...),
+! w (...,kb:ke+1) with ngxy=3, ngz=1 (src/cm1.F:747-777).
+module cm1gpu
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private :: c_strlen_
+
+  type, bind(C) :: cm1g_config
+    integer(c_int32_t) :: ni, nj, nk
+    integer(c_int32_t) :: wbc, ebc, sbc, nbc
+    integer(c_int32_t) :: imoist, ptype, psolver, advwenos, advwenov
+    integer(c_int32_t) :: apmasscon, irbc, nsound, npt, pdtra
+    real(c_float) :: dx, dy, dz, dt, alph, kdiv, csmax
+  end type cm1g_config
+
+  ! return codes
+  integer(c_int), parameter :: CM1G_OK = 0, CM1G_E_ARG = 1, CM1G_E_HIP = 2, &
+                               CM1G_E_NOMEM = 3, CM1G_E_UNSUPPORTED = 4, &
+                               CM1G_E_SATADJ = 5, CM1G_E_CFL = 6
+  ! staggering of host arrays
+  integer(c_int), parameter :: CM1G_STAG_S = 0, CM1G_STAG_U = 1, CM1G_STAG_V = 2, &
+                               CM1G_STAG_W = 3
+
+  ! 3-D field ids
+  integer(c_int), parameter :: CM1G_UA       = 0
+  integer(c_int), parameter :: CM1G_U3D      = 1
+  integer(c_int), parameter :: CM1G_UTEN     = 2
+  integer(c_int), parameter :: CM1G_UTEN1    = 3
+  integer(c_int), parameter :: CM1G_VA       = 4
+  integer(c_int), parameter :: CM1G_V3D      = 5
+  integer(c_int), parameter :: CM1G_VTEN     = 6
+  integer(c_int), parameter :: CM1G_VTEN1    = 7
+  integer(c_int), parameter :: CM1G_WA       = 8
+  integer(c_int), parameter :: CM1G_W3D      = 9
+  integer(c_int), parameter :: CM1G_WTEN     = 10
+  integer(c_int), parameter :: CM1G_WTEN1    = 11
+  integer(c_int), parameter :: CM1G_RRU      = 12
+  integer(c_int), parameter :: CM1G_RRV      = 13
+  integer(c_int), parameter :: CM1G_RRW      = 14
+  integer(c_int), parameter :: CM1G_PPI      = 15
+  integer(c_int), parameter :: CM1G_PP3D     = 16
+  integer(c_int), parameter :: CM1G_PPTEN    = 17
+  integer(c_int), parameter :: CM1G_STEN     = 18
+  integer(c_int), parameter :: CM1G_SADV     = 19
+  integer(c_int), parameter :: CM1G_PPX      = 20
+  integer(c_int), parameter :: CM1G_THA      = 21
+  integer(c_int), parameter :: CM1G_TH3D     = 22
+  integer(c_int), parameter :: CM1G_THTEN    = 23
+  integer(c_int), parameter :: CM1G_THTEN1   = 24
+  integer(c_int), parameter :: CM1G_THTERM   = 25
+  integer(c_int), parameter :: CM1G_QPTEN    = 26
+  integer(c_int), parameter :: CM1G_QTTEN    = 27
+  integer(c_int), parameter :: CM1G_QVTEN    = 28
+  integer(c_int), parameter :: CM1G_QCTEN    = 29
+  integer(c_int), parameter :: CM1G_RHO      = 30
+  integer(c_int), parameter :: CM1G_RR       = 31
+  integer(c_int), parameter :: CM1G_RF       = 32
+  integer(c_int), parameter :: CM1G_PRS      = 33
+  integer(c_int), parameter :: CM1G_DIVX     = 34
+  integer(c_int), parameter :: CM1G_DUM1     = 35
+  integer(c_int), parameter :: CM1G_DUM2     = 36
+  integer(c_int), parameter :: CM1G_DUM3     = 37
+  integer(c_int), parameter :: CM1G_DUM4     = 38
+  integer(c_int), parameter :: CM1G_DUM5     = 39
+  integer(c_int), parameter :: CM1G_DUM6     = 40
+  integer(c_int), parameter :: CM1G_DUM7     = 41
+  integer(c_int), parameter :: CM1G_DUM8     = 42
+  integer(c_int), parameter :: CM1G_DUM9     = 43
+  integer(c_int), parameter :: CM1G_T11      = 44
+  integer(c_int), parameter :: CM1G_T12      = 45
+  integer(c_int), parameter :: CM1G_T13      = 46
+  integer(c_int), parameter :: CM1G_T22      = 47
+  integer(c_int), parameter :: CM1G_T23      = 48
+  integer(c_int), parameter :: CM1G_T33      = 49
+  integer(c_int), parameter :: CM1G_QA0      = 50
+  integer(c_int), parameter :: CM1G_QA1      = 51
+  integer(c_int), parameter :: CM1G_QA2      = 52
+  integer(c_int), parameter :: CM1G_Q3D0     = 53
+  integer(c_int), parameter :: CM1G_Q3D1     = 54
+  integer(c_int), parameter :: CM1G_Q3D2     = 55
+  integer(c_int), parameter :: CM1G_QTEN0    = 56
+  integer(c_int), parameter :: CM1G_QTEN1    = 57
+  integer(c_int), parameter :: CM1G_QTEN2    = 58
+  integer(c_int), parameter :: CM1G_PTA0     = 59
+  integer(c_int), parameter :: CM1G_PTA1     = 60
+  integer(c_int), parameter :: CM1G_PTA2     = 61
+  integer(c_int), parameter :: CM1G_PTA3     = 62
+  integer(c_int), parameter :: CM1G_PTA4     = 63
+  integer(c_int), parameter :: CM1G_PTA5     = 64
+  integer(c_int), parameter :: CM1G_PTA6     = 65
+  integer(c_int), parameter :: CM1G_PTA7     = 66
+  integer(c_int), parameter :: CM1G_PT3D0    = 67
+  integer(c_int), parameter :: CM1G_PT3D1    = 68
+  integer(c_int), parameter :: CM1G_PT3D2    = 69
+  integer(c_int), parameter :: CM1G_PT3D3    = 70
+  integer(c_int), parameter :: CM1G_PT3D4    = 71
+  integer(c_int), parameter :: CM1G_PT3D5    = 72
+  integer(c_int), parameter :: CM1G_PT3D6    = 73
+  integer(c_int), parameter :: CM1G_PT3D7    = 74
+  integer(c_int), parameter :: CM1G_PTTEN0   = 75
+  integer(c_int), parameter :: CM1G_PTTEN1   = 76
+  integer(c_int), parameter :: CM1G_PTTEN2   = 77
+  integer(c_int), parameter :: CM1G_PTTEN3   = 78
+  integer(c_int), parameter :: CM1G_PTTEN4   = 79
+  integer(c_int), parameter :: CM1G_PTTEN5   = 80
+  integer(c_int), parameter :: CM1G_PTTEN6   = 81
+  integer(c_int), parameter :: CM1G_PTTEN7   = 82
+
+  ! 1-D column ids (k = 0..nk+2)
+  integer(c_int), parameter :: CM1G_K_MH    = 0
+  integer(c_int), parameter :: CM1G_K_RMH   = 1
+  integer(c_int), parameter :: CM1G_K_MF    = 2
+  integer(c_int), parameter :: CM1G_K_RMF   = 3
+  integer(c_int), parameter :: CM1G_K_C1    = 4
+  integer(c_int), parameter :: CM1G_K_C2    = 5
+  integer(c_int), parameter :: CM1G_K_ZH    = 6
+  integer(c_int), parameter :: CM1G_K_ZF    = 7
+  integer(c_int), parameter :: CM1G_K_PI0   = 8
+  integer(c_int), parameter :: CM1G_K_TH0   = 9
+  integer(c_int), parameter :: CM1G_K_PRS0  = 10
+  integer(c_int), parameter :: CM1G_K_QV0   = 11
+  integer(c_int), parameter :: CM1G_K_RHO0  = 12
+  integer(c_int), parameter :: CM1G_K_RR0   = 13
+  integer(c_int), parameter :: CM1G_K_RF0   = 14
+  integer(c_int), parameter :: CM1G_K_RRF0  = 15
+  integer(c_int), parameter :: CM1G_K_THV0  = 16
+  integer(c_int), parameter :: CM1G_K_RTH0  = 17
+  integer(c_int), parameter :: CM1G_K_QC0   = 18
+  integer(c_int), parameter :: CM1G_K_QI0   = 19
+  integer(c_int), parameter :: CM1G_K_U0    = 20
+  integer(c_int), parameter :: CM1G_K_V0    = 21
+
+  ! 2-D field ids
+  integer(c_int), parameter :: CM1G_D_RAIN    = 0
+  integer(c_int), parameter :: CM1G_D_PRATE   = 1
+  integer(c_int), parameter :: CM1G_D_RADBCW  = 2
+  integer(c_int), parameter :: CM1G_D_RADBCE  = 3
+  integer(c_int), parameter :: CM1G_D_RADBCS  = 4
+  integer(c_int), parameter :: CM1G_D_RADBCN  = 5
+
+  interface
+    integer(c_int) function cm1g_create(cfg, m) bind(C, name="cm1g_create")
+      import :: c_int, c_ptr, cm1g_config
+      type(cm1g_config), intent(in) :: cfg
+      type(c_ptr), intent(out) :: m
+    end function
+    integer(c_int) function cm1g_destroy(m) bind(C, name="cm1g_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    type(c_ptr) function cm1g_last_error() bind(C, name="cm1g_last_error")
+      import :: c_ptr
+    end function
+    integer(c_int) function cm1g_dtsmall(cfg, nsound) bind(C, name="cm1g_dtsmall")
+      import :: c_int, c_int32_t, cm1g_config
+      type(cm1g_config), intent(in) :: cfg
+      integer(c_int32_t), intent(out) :: nsound
+    end function
+    integer(c_int) function cm1g_put_column(m, id, col) bind(C, name="cm1g_put_column")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      integer(c_int), value :: id
+      real(c_float), intent(in) :: col(*)
+    end function
+    integer(c_int) function cm1g_get_column(m, id, col) bind(C, name="cm1g_get_column")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      integer(c_int), value :: id
+      real(c_float), intent(out) :: col(*)
+    end function
+    integer(c_int) function cm1g_metrics(m) bind(C, name="cm1g_metrics")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_base_derive(m) bind(C, name="cm1g_base_derive")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_put_array(m, id, stag, a) bind(C, name="cm1g_put_array")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      integer(c_int), value :: id, stag
+      real(c_float), intent(in) :: a(*)
+    end function
+    integer(c_int) function cm1g_get_array(m, id, stag, a) bind(C, name="cm1g_get_array")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      integer(c_int), value :: id, stag
+      real(c_float), intent(out) :: a(*)
+    end function
+    integer(c_int) function cm1g_sync(m) bind(C, name="cm1g_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_getset(m, fresh_start) bind(C, name="cm1g_getset")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: fresh_start
+    end function
+    integer(c_int) function cm1g_step(m, cflmax) bind(C, name="cm1g_step")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      real(c_float), intent(out) :: cflmax
+    end function
+    integer(c_int) function cm1g_solve1(m) bind(C, name="cm1g_solve1")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_solve2(m) bind(C, name="cm1g_solve2")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_mp_driver(m) bind(C, name="cm1g_mp_driver")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_solve3(m) bind(C, name="cm1g_solve3")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_budgets(m, tmass, tmois) bind(C, name="cm1g_budgets")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: m
+      real(c_double), intent(out) :: tmass, tmois
+    end function
+  end interface
+
+contains
+
+  ! the message of the last failed call, as a Fortran string
+  function cm1g_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: p(:)
+    type(c_ptr) :: cp
+    integer :: n, i
+    cp = cm1g_last_error()
+    n = c_strlen_(cp)
+    call c_f_pointer(cp, p, [n])
+    allocate(character(len=n) :: msg)
+    do i = 1, n
+      msg(i:i) = p(i)
+    end do
+  end function
+
+  integer function c_strlen_(cp)
+    type(c_ptr), intent(in) :: cp
+    character(kind=c_char), pointer :: p(:)
+    integer :: i
+    c_strlen_ = 0
+    if (.not. c_associated(cp)) return
+    call c_f_pointer(cp, p, [4096])
+    do i = 1, 4096
+      if (p(i) == c_null_char) exit
+      c_strlen_ = i
+    end do
+  end function
+
+end module cm1gpu

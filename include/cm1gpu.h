@@ -115,6 +115,14 @@ int cm1g_base_derive(cm1g_model *m);
 /* Copy a whole padded host box (px*py*pz floats, device layout) in/out. */
 int cm1g_put_field(cm1g_model *m, int id, const float *host);
 int cm1g_get_field(cm1g_model *m, int id, float *host);
+/* The same for a host array in the reference's own shape (what a Fortran
+ * host holds, src/cm1.F:747-777): stag CM1G_STAG_S (ib:ie,jb:je,kb:ke),
+ * _U (ib:ie+1,...), _V (...,jb:je+1,...), _W (...,kb:ke+1), ngxy=3,
+ * ngz=1, i fastest.  Replaces the host<->device moves around non-ported
+ * code (statpack, writeout, restart). */
+enum { CM1G_STAG_S = 0, CM1G_STAG_U = 1, CM1G_STAG_V = 2, CM1G_STAG_W = 3 };
+int cm1g_put_array(cm1g_model *m, int id, int stag, const float *host);
+int cm1g_get_array(cm1g_model *m, int id, int stag, float *host);
 int cm1g_put_2d(cm1g_model *m, int id, const float *host);
 int cm1g_get_2d(cm1g_model *m, int id, float *host);
 /* Device address of a field (for zero-copy use by a caller that already
@@ -146,8 +154,9 @@ int cm1g_advs(cm1g_model *m, int nrk, int s_id, int s0_id, int sten_id,
 int cm1g_advu(cm1g_model *m, int nrk);   /* src/adv.F:911-1839  */
 int cm1g_advv(cm1g_model *m, int nrk);   /* src/adv.F:1847-2770 */
 int cm1g_advw(cm1g_model *m, int nrk);   /* src/adv.F:2778-3610 */
-/* sound(dt,...,nrk,dttmp,...), src/sound.F:10-859, with the solve2 call's
- * arrays: thv=t11, ppterm=t22, piadv=sadv (src/solve2.F:1122-1136). */
+/* sound(dt,...,nrk,dttmp,...), src/sound.F:10-859 (psolver=3), or
+ * sounde(...), src/sounde.F:10-1026 (psolver=2), with the solve2 calls'
+ * arrays: thv=t11, ppterm=t22, piadv=sadv (src/solve2.F:1100-1136). */
 int cm1g_sound(cm1g_model *m, int nrk, float dttmp);
 /* Kessler branch of mp_driver: pdefq x3, k_fallout, fallout, kessler,
  * satadj (src/mp_driver.F:190-232, src/kessler.F). */

@@ -14,9 +14,10 @@ PROG = [("ua", "u"), ("va", "v"), ("wa", "w"), ("tha", None), ("ppi", None),
 
 def case(ni=40, nj=36, nk=30, dx=1000.0, dy=1000.0, dz=500.0, dt=6.0, bc=2, imoist=1,
          ptype=1, nsound=6, amp=3.0, umax=25.0, stretch=None, noise=0.1, seed=7, npt=0,
-         bfx=0.45, bfy=0.55, brh=None):
+         bfx=0.45, bfy=0.55, brh=None, psolver=3, advwenos=0, advwenov=0):
     cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=bc, ebc=bc, sbc=bc, nbc=bc, imoist=imoist,
-                ptype=ptype, psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4,
+                ptype=ptype, psolver=psolver, advwenos=advwenos, advwenov=advwenov,
+                apmasscon=1, irbc=4,
                 nsound=nsound, npt=npt, pdtra=1, dx=dx, dy=dy, dz=dz, dt=dt, alph=0.6,
                 kdiv=0.1, csmax=350.0)
     bub = [dict(x=bfx * ni * dx, y=bfy * nj * dy, z=1400.0, amp=amp,

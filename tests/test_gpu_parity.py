@@ -48,11 +48,16 @@ def run_pair(nsteps, **kw):
     return out, o, g, cfgd
 
 
-@pytest.mark.parametrize("bc", [2, 1])
-def test_one_step_dynamics_bitwise(require_gpu, bc):
+@pytest.mark.parametrize("bc,opts", [
+    (2, {}), (1, {}),
+    (2, dict(advwenos=1, advwenov=1)),
+    (1, dict(psolver=2, advwenos=1, nsound=10)),
+    (2, dict(psolver=2, advwenos=2, advwenov=2, nsound=10)),
+], ids=["open", "periodic", "open-weno", "periodic-sounde-weno", "open-sounde-weno-last"])
+def test_one_step_dynamics_bitwise(require_gpu, bc, opts):
     """One large step: winds, theta and moisture are bit-identical; pi',
     rho, prs differ only through libm pow (ulp level)."""
-    res, _, _, _ = run_pair(1, bc=bc, ni=40, nj=36, nk=30, amp=3.0)
+    res, _, _, _ = run_pair(1, bc=bc, ni=40, nj=36, nk=30, amp=3.0, **opts)
     print(res)
     for name in ("ua", "va", "wa", "tha", "qa0", "qa1", "qa2"):
         assert res[name][1] == 0, (name, res[name])
@@ -81,6 +86,20 @@ def test_ten_steps_config0(require_gpu):
 
 def test_ten_steps_dry_periodic(require_gpu):
     kw = dict(bc=1, imoist=0, ptype=0, ni=48, nj=40, nk=32, amp=2.0)
+    res, _, _, _ = run_pair(10, **kw)
+    floor = ulp_floor(*case(**kw), 10)
+    print(res)
+    print("fp32 floor", floor)
+    for name, (err, nbad, n) in res.items():
+        assert err <= parity_bound(floor, name), (name, err, floor[name])
+
+
+@pytest.mark.parametrize("opts", [dict(advwenos=1, advwenov=1), dict(psolver=2, advwenos=1, nsound=10)],
+                         ids=["weno", "sounde-weno"])
+def test_ten_steps_weno_sounde(require_gpu, opts):
+    """WENO5 advection and the explicit acoustic solver over 10 steps of the
+    64x64x40 supercell."""
+    kw = dict(config0(), **opts)
     res, _, _, _ = run_pair(10, **kw)
     floor = ulp_floor(*case(**kw), 10)
     print(res)
