@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a) {
     float fz_hi = 0.0f;
     if (k + 1 <= nk) {
       const int kf = k + 1;
-      auto S = [&](int kk) -> float {
+      auto S = [=](int kk) -> float {
         const int o = kk - kf;  // -3..2
         return o == -3 ? wv0 : o == -2 ? wv1 : o == -1 ? wv2 : o == 0 ? wv3 : o == 1 ? wv4 : wv5;
       };

@@ -63,6 +63,7 @@ def _load():
         getattr(lib, name).argtypes = [vp, C.c_int]
     lib.cm1o_sound.argtypes = [vp, C.c_int, C.c_float]
     lib.cm1o_kessler_branch.argtypes = [vp]
+    lib.cm1o_set_libm.argtypes = [C.c_int]
     return lib
 
 
@@ -89,9 +90,13 @@ class Oracle:
     """One oracle model; arrays are numpy views shaped (k, j, i) with
     index offsets i0=2, j0=2 (i=-2 at index 0), k0=0."""
 
-    def __init__(self, cfg):
+    def __init__(self, cfg, libm=0):
+        """libm: 0 evaluates exp/pow with the host C library (the
+        reference's rounding), 1 in double precision rounded once (the HIP
+        kernels' rounding)."""
         self.L = lib()
         self.cfg = cfg
+        self.libm = libm
         self.h = self.L.cm1o_create(C.byref(cfg))
         px, py, pz = C.c_int64(), C.c_int64(), C.c_int64()
         self.L.cm1o_dims(self.h, C.byref(px), C.byref(py), C.byref(pz))
@@ -123,8 +128,10 @@ class Oracle:
         self.L.cm1o_base_derive(self.h)
 
     def getset(self, fresh=True):
+        self.L.cm1o_set_libm(self.libm)
         self.L.cm1o_getset(self.h, 1 if fresh else 0)
 
     def step(self, n=1):
+        self.L.cm1o_set_libm(self.libm)
         for _ in range(n):
             self.L.cm1o_step(self.h)

@@ -99,6 +99,9 @@ void cm1o_advu(cm1o_model *m, int nrk);
 void cm1o_advv(cm1o_model *m, int nrk);
 void cm1o_advw(cm1o_model *m, int nrk);
 void cm1o_sound(cm1o_model *m, int nrk, float dttmp);
+/* exp/pow evaluation: 0 host expf/powf (default, the reference), 1 double
+ * precision rounded once (the HIP kernels' evaluation). Process-wide. */
+void cm1o_set_libm(int mode);
 void cm1o_kessler_branch(cm1o_model *m);
 
 #ifdef __cplusplus

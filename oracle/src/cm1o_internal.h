@@ -6,6 +6,18 @@
 
 typedef __float128 quad;
 
+/* Single-precision exp and pow.  Mode 0 (default): the host C library's
+ * expf/powf, which is what the reference's REAL exp() and ** lower to;
+ * the oracle then reproduces the reference bit for bit.  Mode 1: the
+ * value evaluated in double precision and rounded once, which is how the
+ * HIP kernels evaluate them; used to separate the effect of libm rounding
+ * from everything else when the GPU is compared with the reference. */
+extern int cm1o_libm_mode;
+static inline float o_expf(float x) { return cm1o_libm_mode ? (float)exp((double)x) : expf(x); }
+static inline float o_powf(float x, float y) {
+  return cm1o_libm_mode ? (float)pow((double)x, (double)y) : powf(x, y);
+}
+
 /* Physical constants, set exactly as set_constants() does for testcase=0
  * (src/constants.F:69-170): single-precision arithmetic at run time. */
 typedef struct {

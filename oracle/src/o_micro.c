@@ -1,8 +1,9 @@
 /* Oracle: Kessler warm-rain microphysics (test infrastructure only).
  * kessler :8-138, satadj :146-360 (eqtset=2), k_fallout :368-396,
  * fallout :404-477 of src/kessler.F; driver branch ptype=1 of
- * src/mp_driver.F:100-232.  exp/pow are the C library's expf/powf, which
- * is what the reference's REAL exp() and ** lower to. */
+ * src/mp_driver.F:100-232.  exp/pow go through o_expf/o_powf (the C
+ * library's expf/powf by default, what the reference's REAL exp() and **
+ * lower to). */
 #include "cm1o_internal.h"
 #include <stdio.h>
 
@@ -14,7 +15,7 @@ void o_k_fallout(cm1o_model *M, const float *rho, const float *qr3d, float *vr) 
       for (i = 1; i <= ni; i++) {
         A3(vr, i, j, k) = 0.0f;
         if (A3(qr3d, i, j, k) > 1.0e-12f)
-          A3(vr, i, j, k) = 14.34f * powf(A3(rho, i, j, k) * A3(qr3d, i, j, k), 0.1346f) *
+          A3(vr, i, j, k) = 14.34f * o_powf(A3(rho, i, j, k) * A3(qr3d, i, j, k), 0.1346f) *
                             sqrtf(1.15f / A3(rho, i, j, k));
       }
 }
@@ -71,17 +72,17 @@ void o_kessler(cm1o_model *M, float dt, float *tmp, float *rho, float *pp3d, flo
           ar = ar * dt;
         }
         if (qcnew > 0.0f && qrnew > 0.0f) {
-          cr = 2.2f * qcnew * powf(qrnew, 0.875f);
+          cr = 2.2f * qcnew * o_powf(qrnew, 0.875f);
           cr = cr * dt;
         }
         if (qrnew > 0.0f) {
           float t = A3(tmp, i, j, k);
-          float esl = 611.2f * expf(17.67f * (t - 273.15f) / (t - 29.65f));
+          float esl = 611.2f * o_expf(17.67f * (t - 273.15f) / (t - 29.65f));
           esl = fminf(esl, A3(prs, i, j, k) * 0.5f);
           float qvs = K->eps * esl / (A3(prs, i, j, k) - esl);
           if (qvnew < qvs) {
             float rq = A3(rho, i, j, k) * qrnew;
-            er = (1.6f + 30.3922f * powf(rq, 0.2046f)) * (1.0f - (qvnew / qvs)) * powf(rq, 0.525f) /
+            er = (1.6f + 30.3922f * o_powf(rq, 0.2046f)) * (1.0f - (qvnew / qvs)) * o_powf(rq, 0.525f) /
                  ((2.03e4f + 9.584e6f / (qvs * A3(prs, i, j, k))) * A3(rho, i, j, k));
             er = fminf(er * dt, qrnew);
             if ((qvnew + er) > qvs) er = qvs - qvnew;
@@ -102,8 +103,8 @@ void o_kessler(cm1o_model *M, float dt, float *tmp, float *rho, float *pp3d, flo
           A3(th3d, i, j, k) = A3(th3d, i, j, k) - er * (
               (K->lv1 - K->lv2 * A3(tmp, i, j, k)) / (K->cpdcv * cvml * (pi0[k] + A3(pp3d, i, j, k)))
               - (th0[k] + A3(th3d, i, j, k)) * (K->rv / cvml) * (1.0f - K->rovcp * cpml / rm));
-          A3(pp3d, i, j, k) = powf(A3(rho, i, j, k) * rm * (th0[k] + A3(th3d, i, j, k)) * K->rp00, K->rddcv) - pi0[k];
-          A3(prs, i, j, k) = K->p00 * powf(pi0[k] + A3(pp3d, i, j, k), K->cpdrd);
+          A3(pp3d, i, j, k) = o_powf(A3(rho, i, j, k) * rm * (th0[k] + A3(th3d, i, j, k)) * K->rp00, K->rddcv) - pi0[k];
+          A3(prs, i, j, k) = K->p00 * o_powf(pi0[k] + A3(pp3d, i, j, k), K->cpdrd);
         }
         A3(qv3d, i, j, k) = qvnew;
         A3(qc3d, i, j, k) = qcnew;
@@ -124,7 +125,7 @@ int o_satadj(cm1o_model *M, int nrk, float dt, float *rho, float *pp3d, float *p
     for (j = 1; j <= nj; j++)
       for (i = 1; i <= ni; i++) {
         float tnew = (th0[k] + A3(th3d, i, j, k)) * (pi0[k] + A3(pp3d, i, j, k));
-        float esl = 611.2f * expf(17.67f * (tnew - 273.15f) / (tnew - 29.65f));
+        float esl = 611.2f * o_expf(17.67f * (tnew - 273.15f) / (tnew - 29.65f));
         esl = fminf(esl, A3(prs, i, j, k) * 0.5f);
         float qvs = K->eps * esl / (A3(prs, i, j, k) - esl);
         if (A3(qc, i, j, k) > 1.0e-12f || A3(qv, i, j, k) > qvs) {
@@ -154,7 +155,7 @@ int o_satadj(cm1o_model *M, int nrk, float dt, float *rho, float *pp3d, float *p
             doit = 0;
             if (fabsf(tnew - tlast) > converge) {
               tlast = tnew;
-              esl = 611.2f * expf(17.67f * (tnew - 273.15f) / (tnew - 29.65f));
+              esl = 611.2f * o_expf(17.67f * (tnew - 273.15f) / (tnew - 29.65f));
               esl = fminf(esl, A3(prs, i, j, k) * 0.5f);
               qvs = K->eps * esl / (pnew - esl);
               doit = 1;
@@ -162,7 +163,7 @@ int o_satadj(cm1o_model *M, int nrk, float dt, float *rho, float *pp3d, float *p
             if (n == 100) { iflag = 1; doit = 0; }
           }
           A3(prs, i, j, k) = pnew;
-          A3(pp3d, i, j, k) = powf(pnew * K->rp00, K->rovcp) - pi0[k];
+          A3(pp3d, i, j, k) = o_powf(pnew * K->rp00, K->rovcp) - pi0[k];
           A3(th3d, i, j, k) = tnew / (pi0[k] + A3(pp3d, i, j, k)) - th0[k];
           A3(qc, i, j, k) = qcnew;
           A3(qv, i, j, k) = qvnew;

@@ -5,6 +5,9 @@
  * Constants: src/constants.F:69-170; metrics: src/param.F:7790-8215,
  * :8590-8622, :8980-9060, :9182-9218; base state: src/base.F:2085-2198. */
 #include "cm1o_internal.h"
+
+int cm1o_libm_mode = 0;
+void cm1o_set_libm(int mode) { cm1o_libm_mode = mode; }
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -206,7 +209,7 @@ void cm1o_getset(cm1o_model *M, int fresh_start) {
       for (j = 1; j <= nj; j++)
         for (i = 1; i <= ni; i++) {
           float pp = A3(M->f[F_PP3D], i, j, k), th = A3(M->f[F_TH3D], i, j, k);
-          A3(prs, i, j, k) = K->p00 * powf(pi0[k] + pp, K->cpdrd);
+          A3(prs, i, j, k) = K->p00 * o_powf(pi0[k] + pp, K->cpdrd);
           if (M->c.imoist == 1)
             A3(rho, i, j, k) = A3(prs, i, j, k) /
                 ((th0[k] + th) * (pi0[k] + pp) * (K->rd + fmaxf(0.0f, A3(M->f[F_Q3D0], i, j, k)) * K->rv));
@@ -281,13 +284,13 @@ void o_solve1(cm1o_model *M) {
             if (fabsf(dt * A3(thten1, i, j, k)) > tem || fabsf(dt * qvt) > K->qsmall) {
               float thnew = A3(tha, i, j, k) + dt * A3(thten1, i, j, k);
               float qvnew = A3(M->f[F_QA0], i, j, k) + dt * qvt;
-              float pinew = powf(A3(rho, i, j, k) * (th0[k] + thnew) * (K->rd + K->rv * qvnew) * K->rp00, K->rddcv) - pi0[k];
+              float pinew = o_powf(A3(rho, i, j, k) * (th0[k] + thnew) * (K->rd + K->rv * qvnew) * K->rp00, K->rddcv) - pi0[k];
               A3(ppten, i, j, k) = (pinew - A3(ppi, i, j, k)) * rdt;
             } else A3(ppten, i, j, k) = 0.0f;
           } else {
             if (fabsf(dt * A3(thten1, i, j, k)) > tem) {
               float thnew = A3(tha, i, j, k) + dt * A3(thten1, i, j, k);
-              float pinew = powf(A3(rho, i, j, k) * (th0[k] + thnew) * K->rd * K->rp00, K->rddcv) - pi0[k];
+              float pinew = o_powf(A3(rho, i, j, k) * (th0[k] + thnew) * K->rd * K->rp00, K->rddcv) - pi0[k];
               A3(ppten, i, j, k) = (pinew - A3(ppi, i, j, k)) * rdt;
             } else A3(ppten, i, j, k) = 0.0f;
           }
@@ -486,7 +489,7 @@ void o_solve2(cm1o_model *M) {
       for (k = 1; k <= nk; k++)
         for (j = 1; j <= nj; j++)
           for (i = 1; i <= ni; i++) {
-            A3(prs, i, j, k) = K->p00 * powf(pi0[k] + A3(pp3d, i, j, k), K->cpdrd);
+            A3(prs, i, j, k) = K->p00 * o_powf(pi0[k] + A3(pp3d, i, j, k), K->cpdrd);
             if (M->c.imoist == 1)
               A3(rho, i, j, k) = A3(prs, i, j, k) / ((th0[k] + A3(th3d, i, j, k)) * (pi0[k] + A3(pp3d, i, j, k)) *
                                                       (K->rd + fmaxf(0.0f, A3(M->f[F_Q3D0], i, j, k)) * K->rv));
@@ -542,7 +545,7 @@ void o_solve2(cm1o_model *M) {
         for (j = 1; j <= nj; j++)
           for (i = 1; i <= ni; i++) {
             A3(pp3d, i, j, k) = A3(pp3d, i, j, k) + tem;
-            A3(prs, i, j, k) = K->p00 * powf(pi0[k] + A3(pp3d, i, j, k), K->cpdrd);
+            A3(prs, i, j, k) = K->p00 * o_powf(pi0[k] + A3(pp3d, i, j, k), K->cpdrd);
             if (M->c.imoist == 1)
               A3(rho, i, j, k) = A3(prs, i, j, k) / ((th0[k] + A3(th3d, i, j, k)) * (pi0[k] + A3(pp3d, i, j, k)) *
                                                       (K->rd + fmaxf(0.0f, A3(M->f[F_Q3D0], i, j, k)) * K->rv));

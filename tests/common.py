@@ -30,9 +30,11 @@ def case(ni=40, nj=36, nk=30, dx=1000.0, dy=1000.0, dz=500.0, dt=6.0, bc=2, imoi
     return cfgd, cols, fields
 
 
-def make_oracle(cfgd, cols, fields):
+def make_oracle(cfgd, cols, fields, libm=0):
+    """libm=0: exp/pow rounded as the reference (host C library); libm=1:
+    as the HIP kernels (double precision, rounded once)."""
     cfg = pyoracle.make_config(**cfgd)
-    o = pyoracle.Oracle(cfg)
+    o = pyoracle.Oracle(cfg, libm=libm)
     for n in ("zf", "zh", "mh", "mf"):
         o.kf(n)[:cfgd["nk"] + 3] = cols[n]
     o.L.cm1o_set_extrap(o.h, None)
@@ -110,31 +112,3 @@ def nsound_for(cfgd):
     if cfgd["dt"] / float(np.float32(ns)) > float(dtsm):
         ns += 2
     return ns
-
-
-def ulp_floor(cfgd, cols, fields, nsteps, make=None):
-    """The FP32 noise floor of a case: per-field relative L2 between two
-    oracle runs of nsteps that differ only by a one-ulp change of pi' in
-    every cell after the first step (the size of change one misrounded
-    pow/exp makes through the dry-mass pressure adjustment)."""
-    ni, nj, nk = cfgd["ni"], cfgd["nj"], cfgd["nk"]
-    a = make_oracle(cfgd, cols, fields)
-    b = make_oracle(cfgd, cols, fields)
-    a.step(nsteps)
-    b.step(1)
-    pp = b.f("pp3d")
-    pp[:] = np.nextafter(pp, np.float32(1.0))
-    b.f("ppi")[:] = pp
-    b.step(nsteps - 1)
-    out = {}
-    for name, stag in PROG:
-        x = interior(a.f(name), nk, nj, ni, stag)
-        y = interior(b.f(name), nk, nj, ni, stag)
-        out[name] = rel_l2(y, x)
-    return out
-
-
-def parity_bound(floor, name, tol=1.0e-5):
-    """Per-field bound: the north-star 1e-5, or three times the case's FP32
-    noise floor where that floor is itself above 1e-5."""
-    return max(tol, 3.0 * floor.get(name, 0.0))

@@ -2,20 +2,18 @@
 
 The oracle reproduces the reference CM1 FP32 build bit for bit
 (test_oracle_reference.py).  The HIP path evaluates every stencil, solve
-and reduction with the reference's operation order and no FMA contraction,
-so everything that does not go through a libm transcendental is bitwise
-identical.  exp/pow are evaluated in double precision and rounded once; the
-host C library's expf/powf (what the reference uses) differ from that in
-~0.06% of calls by one ulp, which the dry-mass pressure fix spreads to every
-cell.  Full-step tolerance: per-field relative L2 <= 1e-5 after 10 large steps
-(north star), except where the case's own FP32 noise floor (two oracle runs
-one ulp of pi' apart, tests/common.py:ulp_floor) is larger: then 3x that
-floor.  Domain budgets to 1e-6.
+and reduction with the reference's operation order and no FMA contraction;
+it evaluates exp/pow in double precision and rounds once, where the
+reference calls the host C library's expf/powf.  Run with the same exp/pow
+rounding (oracle libm=1), the oracle and the GPU agree bit for bit on every
+prognostic field, after one step and after ten; that is the check here.
+How far the libm rounding alone moves a case from the reference is
+measured in test_gpu_reference.py.  Domain budgets to 1e-6.
 """
 import numpy as np
 import pytest
 
-from common import PROG, case, interior, make_gpu, make_oracle, parity_bound, rel_l2, ulp_floor
+from common import PROG, case, interior, make_gpu, make_oracle, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -33,7 +31,7 @@ def config0(**kw):
 
 def run_pair(nsteps, **kw):
     cfgd, cols, fields = case(**kw)
-    o = make_oracle(cfgd, cols, fields)
+    o = make_oracle(cfgd, cols, fields, libm=1)
     g = make_gpu(cfgd, cols, fields)
     o.step(nsteps)
     g.step(nsteps)
@@ -55,23 +53,18 @@ def run_pair(nsteps, **kw):
     (2, dict(psolver=2, advwenos=2, advwenov=2, nsound=10)),
 ], ids=["open", "periodic", "open-weno", "periodic-sounde-weno", "open-sounde-weno-last"])
 def test_one_step_dynamics_bitwise(require_gpu, bc, opts):
-    """One large step: winds, theta and moisture are bit-identical; pi',
-    rho, prs differ only through libm pow (ulp level)."""
+    """One large step: every prognostic field bit-identical."""
     res, _, _, _ = run_pair(1, bc=bc, ni=40, nj=36, nk=30, amp=3.0, **opts)
     print(res)
-    for name in ("ua", "va", "wa", "tha", "qa0", "qa1", "qa2"):
-        assert res[name][1] == 0, (name, res[name])
-    for name in ("ppi", "rho", "prs"):
-        assert res[name][0] <= 1.0e-6, (name, res[name])
+    for name, r in res.items():
+        assert r[1] == 0, (name, r)
 
 
 def test_ten_steps_config0(require_gpu):
     res, o, g, cfgd = run_pair(10, **config0())
-    floor = ulp_floor(*case(**config0()), 10)
     print(res)
-    print("fp32 floor", floor)
     for name, (err, nbad, n) in res.items():
-        assert err <= parity_bound(floor, name), (name, err, floor[name])
+        assert nbad == 0, (name, err, nbad)
     # domain budgets: dry-air mass and total water (double precision)
     tm_g, tw_g = g.budgets()
     ni, nj, nk = cfgd["ni"], cfgd["nj"], cfgd["nk"]
@@ -87,11 +80,9 @@ def test_ten_steps_config0(require_gpu):
 def test_ten_steps_dry_periodic(require_gpu):
     kw = dict(bc=1, imoist=0, ptype=0, ni=48, nj=40, nk=32, amp=2.0)
     res, _, _, _ = run_pair(10, **kw)
-    floor = ulp_floor(*case(**kw), 10)
     print(res)
-    print("fp32 floor", floor)
     for name, (err, nbad, n) in res.items():
-        assert err <= parity_bound(floor, name), (name, err, floor[name])
+        assert nbad == 0, (name, err, nbad)
 
 
 @pytest.mark.parametrize("opts", [dict(advwenos=1, advwenov=1), dict(psolver=2, advwenos=1, nsound=10)],
@@ -101,8 +92,6 @@ def test_ten_steps_weno_sounde(require_gpu, opts):
     64x64x40 supercell."""
     kw = dict(config0(), **opts)
     res, _, _, _ = run_pair(10, **kw)
-    floor = ulp_floor(*case(**kw), 10)
     print(res)
-    print("fp32 floor", floor)
     for name, (err, nbad, n) in res.items():
-        assert err <= parity_bound(floor, name), (name, err, floor[name])
+        assert nbad == 0, (name, err, nbad)
