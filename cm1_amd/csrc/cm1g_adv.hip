@@ -6,265 +6,7 @@
 //
 // Every flux is evaluated with the reference's operation order in FP32
 // without contraction, so results are bitwise those of the reference.
-#include "cm1g_internal.h"
-
-__device__ __forceinline__ float flx3(float s1, float s2, float s3) {
-  return (-1.0f / 6.0f) * s1 + (5.0f / 6.0f) * s2 + (2.0f / 6.0f) * s3;
-}
-__device__ __forceinline__ float flx5(float s1, float s2, float s3, float s4, float s5) {
-  return (2.0f / 60.0f) * s1 + (-13.0f / 60.0f) * s2 + (47.0f / 60.0f) * s3 +
-         (27.0f / 60.0f) * s4 + (-3.0f / 60.0f) * s5;
-}
-
-// weno5 with the Borges et al. indicators (siform=2),
-// src/adv_routines.F:6166-6199: the indicators are formed in single
-// precision and widened, the weights and the combination are double
-// precision, the result is rounded to single once.
-__device__ __forceinline__ float weno5(float s1, float s2, float s3, float s4, float s5, double weps) {
-  const float a1 = s1 - 2.0f * s2 + s3, e1 = s1 - 4.0f * s2 + 3.0f * s3;
-  const float a2 = s2 - 2.0f * s3 + s4, e2 = s2 - s4;
-  const float a3 = s3 - 2.0f * s4 + s5, e3 = 3.0f * s3 - 4.0f * s4 + s5;
-  const double b1 = (double)((13.0f / 12.0f) * (a1 * a1) + 0.25f * (e1 * e1));
-  const double b2 = (double)((13.0f / 12.0f) * (a2 * a2) + 0.25f * (e2 * e2));
-  const double b3 = (double)((13.0f / 12.0f) * (a3 * a3) + 0.25f * (e3 * e3));
-  const double d13 = fabs(b1 - b3);
-  const double q1 = fmin(1.0e30, d13 / (b1 + weps));
-  const double q2 = fmin(1.0e30, d13 / (b2 + weps));
-  const double q3 = fmin(1.0e30, d13 / (b3 + weps));
-  const double w1 = (double)0.1f * ((double)1.0f + q1 * q1);
-  const double w2 = (double)0.6f * ((double)1.0f + q2 * q2);
-  const double w3 = (double)0.3f * ((double)1.0f + q3 * q3);
-  const float p1 = (2.0f / 6.0f) * s1 + (-7.0f / 6.0f) * s2 + (11.0f / 6.0f) * s3;
-  const float p2 = (-1.0f / 6.0f) * s2 + (5.0f / 6.0f) * s3 + (2.0f / 6.0f) * s4;
-  const float p3 = (2.0f / 6.0f) * s3 + (5.0f / 6.0f) * s4 + (-1.0f / 6.0f) * s5;
-  return (float)((w1 * p1 + w2 * p2 + w3 * p3) / (w1 + w2 + w3));
-}
-
-// positive-definite upstream value (Beets & Koren), src/adv_routines.F:6331-6348
-__device__ __forceinline__ float upstrpd(float s1, float s2, float s3, double weps) {
-  const float dd = s2 - s1;
-  const float rr = (s3 - s2) / (copysignf((float)weps, dd) + dd);
-  const float phi = fmaxf(0.0f, fminf(rr, fminf((1.0f / 6.0f) + (2.0f / 6.0f) * rr, 1.0f)));
-  return s2 + phi * (s2 - s1);
-}
-
-// the pdef magnitude test of hadv_weno5/vadv_weno5: false when all five
-// stencil values are within weps of zero (the flux is then zero)
-__device__ __forceinline__ bool weno_doit(float s1, float s2, float s3, float s4, float s5, double weps) {
-  const float bmax = fmaxf(fmaxf(fmaxf(fmaxf(fabsf(s1), fabsf(s2)), fabsf(s3)), fabsf(s4)), fabsf(s5));
-  return bmax > (float)weps;
-}
-
-// upwind face value with 5th-order fluxes or WENO5
-template <bool WENO>
-__device__ __forceinline__ float face5(float s1, float s2, float s3, float s4, float s5, double weps) {
-  if (WENO) return weno5(s1, s2, s3, s4, s5, weps);
-  return flx5(s1, s2, s3, s4, s5);
-}
-
-// upwind 5th-order face value between a(m) (left) and a(m+1) given the
-// five-point stencils: positive flow uses a(m-2..m+2), negative a(m+3..m-1)
-#define FLXP(A, o1, o2, o3, o4, o5) flx5(A(o1), A(o2), A(o3), A(o4), A(o5))
-
-// ------------------------------------------------------------ horizontal
-// stag 1: scalar; 2: u; 3: v; 4: w.  Writes dumx/dumy over the reference
-// loop ranges.
-template <bool WENO>
-static void hadv5t(cm1g_model *m, int stag, const float *rru, const float *rrv, float *dumx,
-                   float *dumy, const float *a, double weps) {
-  const KP P = m->P;
-  const int ni = P.ni, nj = P.nj, nk = P.nk;
-  if (stag == 1) {
-    for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
-      auto S = [&](int ii, int jj) { return a[IXD(P, ii, jj, k)]; };
-      float ru = rru[IXD(P, i, j, k)], rv = rrv[IXD(P, i, j, k)];
-      dumx[IXD(P, i, j, k)] = (ru >= 0.0f)
-          ? ru * face5<WENO>(S(i - 3, j), S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j), weps)
-          : ru * face5<WENO>(S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j), S(i - 2, j), weps);
-      dumy[IXD(P, i, j, k)] = (rv >= 0.0f)
-          ? rv * face5<WENO>(S(i, j - 3), S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1), weps)
-          : rv * face5<WENO>(S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1), S(i, j - 2), weps);
-    });
-  } else if (stag == 2) {
-    int i1 = P.ibw == 1 ? 2 : 1, i2 = P.ibe == 1 ? ni : ni + 1;
-    for3(m->stream, i1 - 1, i2, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
-      auto S = [&](int ii, int jj) { return a[IXD(P, ii, jj, k)]; };
-      float ubar = 0.5f * (rru[IXD(P, i, j, k)] + rru[IXD(P, i + 1, j, k)]);
-      dumx[IXD(P, i, j, k)] = (ubar >= 0.0f)
-          ? ubar * face5<WENO>(S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j), S(i + 2, j), weps)
-          : ubar * face5<WENO>(S(i + 3, j), S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j), weps);
-      float vbar = 0.5f * (rrv[IXD(P, i, j, k)] + rrv[IXD(P, i - 1, j, k)]);
-      dumy[IXD(P, i, j, k)] = (vbar >= 0.0f)
-          ? vbar * face5<WENO>(S(i, j - 3), S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1), weps)
-          : vbar * face5<WENO>(S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1), S(i, j - 2), weps);
-    });
-  } else if (stag == 3) {
-    int j1 = P.ibs == 1 ? 2 : 1, j2 = P.ibn == 1 ? nj : nj + 1;
-    for3(m->stream, 1, ni + 1, j1 - 1, j2, 1, nk, [=] __device__(int i, int j, int k) {
-      auto S = [&](int ii, int jj) { return a[IXD(P, ii, jj, k)]; };
-      float ubar = 0.5f * (rru[IXD(P, i, j, k)] + rru[IXD(P, i, j - 1, k)]);
-      dumx[IXD(P, i, j, k)] = (ubar >= 0.0f)
-          ? ubar * face5<WENO>(S(i - 3, j), S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j), weps)
-          : ubar * face5<WENO>(S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j), S(i - 2, j), weps);
-      float vbar = 0.5f * (rrv[IXD(P, i, j, k)] + rrv[IXD(P, i, j + 1, k)]);
-      dumy[IXD(P, i, j, k)] = (vbar >= 0.0f)
-          ? vbar * face5<WENO>(S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1), S(i, j + 2), weps)
-          : vbar * face5<WENO>(S(i, j + 3), S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1), weps);
-    });
-  } else {
-    for3(m->stream, 1, ni + 1, 1, nj + 1, 2, nk, [=] __device__(int i, int j, int k) {
-      auto S = [&](int ii, int jj) { return a[IXD(P, ii, jj, k)]; };
-      float cc2 = 0.5f * (P.c2[k] + P.c2[k]);
-      float cc1 = 1.0f - cc2;
-      float ubar = cc2 * rru[IXD(P, i, j, k)] + cc1 * rru[IXD(P, i, j, k - 1)];
-      dumx[IXD(P, i, j, k)] = (ubar >= 0.0f)
-          ? ubar * face5<WENO>(S(i - 3, j), S(i - 2, j), S(i - 1, j), S(i, j), S(i + 1, j), weps)
-          : ubar * face5<WENO>(S(i + 2, j), S(i + 1, j), S(i, j), S(i - 1, j), S(i - 2, j), weps);
-      float vbar = cc2 * rrv[IXD(P, i, j, k)] + cc1 * rrv[IXD(P, i, j, k - 1)];
-      dumy[IXD(P, i, j, k)] = (vbar >= 0.0f)
-          ? vbar * face5<WENO>(S(i, j - 3), S(i, j - 2), S(i, j - 1), S(i, j), S(i, j + 1), weps)
-          : vbar * face5<WENO>(S(i, j + 2), S(i, j + 1), S(i, j), S(i, j - 1), S(i, j - 2), weps);
-    });
-  }
-}
-
-// hadv_flx5 (src/adv_routines.F:1335-1476) or hadv_weno5 (:204-430, no
-// pdef test: the momentum equations)
-static void hadv5(cm1g_model *m, int stag, const float *rru, const float *rrv, float *dumx,
-                  float *dumy, const float *a, bool weno = false, double weps = 0.0) {
-  if (weno) hadv5t<true>(m, stag, rru, rrv, dumx, dumy, a, weps);
-  else hadv5t<false>(m, stag, rru, rrv, dumx, dumy, a, weps);
-}
-
-__device__ __forceinline__ float wface(const KP &P, int stag, const float *rrw, int i, int j, int k) {
-  if (stag == 1) return rrw[IXD(P, i, j, k)];
-  if (stag == 2) return 0.5f * (rrw[IXD(P, i, j, k)] + rrw[IXD(P, i - 1, j, k)]);
-  if (stag == 3) return 0.5f * (rrw[IXD(P, i, j, k)] + rrw[IXD(P, i, j - 1, k)]);
-  return 0.5f * (rrw[IXD(P, i, j, k)] + rrw[IXD(P, i, j, k + 1)]);
-}
-
-// vertical face flux at level k for staggering stag (interior 5th order and
-// the lower-order boundary fluxes of vadv_lwr_flx_23/35); S(kk) gives the
-// advected value at level kk of the column, wb the face mass flux.
-template <class SF>
-__device__ __forceinline__ float vflux5_of(const KP &P, int stag, float wb, SF S, int k) {
-  const int nk = P.nk;
-  if (stag != 4) {
-    if (k >= 4 && k <= nk - 2)
-      return (wb >= 0.0f) ? wb * flx5(S(k - 3), S(k - 2), S(k - 1), S(k), S(k + 1))
-                          : wb * flx5(S(k + 2), S(k + 1), S(k), S(k - 1), S(k - 2));
-    if (k == 3)
-      return (wb >= 0.0f) ? wb * flx3(S(k - 2), S(k - 1), S(k))
-                          : wb * flx5(S(k + 2), S(k + 1), S(k), S(k - 1), S(k - 2));
-    if (k == nk - 1)
-      return (wb >= 0.0f) ? wb * flx5(S(k - 3), S(k - 2), S(k - 1), S(k), S(k + 1))
-                          : wb * flx3(S(k + 1), S(k), S(k - 1));
-    // k == 2 or k == nk
-    float lin;
-    if (stag == 1) lin = P.c1[k] * S(k - 1) + P.c2[k] * S(k);
-    else {
-      float cc2 = 0.5f * (P.c2[k] + P.c2[k]);
-      float cc1 = 1.0f - cc2;
-      lin = cc1 * S(k - 1) + cc2 * S(k);
-    }
-    if (k == 2) return (wb >= 0.0f) ? wb * lin : wb * flx3(S(k + 1), S(k), S(k - 1));
-    return (wb >= 0.0f) ? wb * flx3(S(k - 2), S(k - 1), S(k)) : wb * lin;
-  }
-  if (k >= 3 && k <= nk - 2)
-    return (wb >= 0.0f) ? wb * flx5(S(k - 2), S(k - 1), S(k), S(k + 1), S(k + 2))
-                        : wb * flx5(S(k + 3), S(k + 2), S(k + 1), S(k), S(k - 1));
-  if (k == 1)
-    return (wb >= 0.0f) ? wb * 0.5f * (S(k) + S(k + 1)) : wb * flx3(S(k + 2), S(k + 1), S(k));
-  if (k == nk)
-    return (wb >= 0.0f) ? wb * flx3(S(k - 1), S(k), S(k + 1)) : wb * 0.5f * (S(k) + S(k + 1));
-  if (k == 2)
-    return (wb >= 0.0f) ? wb * flx3(S(k - 1), S(k), S(k + 1))
-                        : wb * flx5(S(k + 3), S(k + 2), S(k + 1), S(k), S(k - 1));
-  // k == nk-1
-  return (wb >= 0.0f) ? wb * flx5(S(k - 2), S(k - 1), S(k), S(k + 1), S(k + 2))
-                      : wb * flx3(S(k + 2), S(k + 1), S(k));
-}
-
-// vertical face flux with WENO5 (vadv_weno5, src/adv_routines.F:2359-2557)
-// and its lower-order boundary fluxes (vadv_lwr_weno_23 :4038-4208,
-// vadv_lwr_weno_35 :4214-4406, vadv_lwr_weno_35_b :4412-4447).  pdefweno
-// (scalars only) adds the magnitude test and the one-sided upstream values
-// at k=3 and k=nk-1.
-template <class SF>
-__device__ __forceinline__ float vfluxw_of(const KP &P, int stag, float wb, SF S, int k, int pdefweno,
-                                           double weps) {
-  const int nk = P.nk;
-  if (stag == 4) {
-    if (k == 1)
-      return (wb >= 0.0f) ? wb * 0.5f * (S(k) + S(k + 1)) : wb * upstrpd(S(k + 2), S(k + 1), S(k), weps);
-    if (k == nk)
-      return (wb > 0.0f) ? wb * upstrpd(S(k - 1), S(k), S(k + 1), weps) : wb * 0.5f * (S(k) + S(k + 1));
-    if (k == 2)
-      return (wb >= 0.0f) ? wb * upstrpd(S(k - 1), S(k), S(k + 1), weps)
-                          : wb * weno5(S(k + 3), S(k + 2), S(k + 1), S(k), S(k - 1), weps);
-    if (k == nk - 1)
-      return (wb > 0.0f) ? wb * weno5(S(k - 2), S(k - 1), S(k), S(k + 1), S(k + 2), weps)
-                         : wb * upstrpd(S(k + 2), S(k + 1), S(k), weps);
-    return (wb >= 0.0f) ? wb * weno5(S(k - 2), S(k - 1), S(k), S(k + 1), S(k + 2), weps)
-                        : wb * weno5(S(k + 3), S(k + 2), S(k + 1), S(k), S(k - 1), weps);
-  }
-  if (k == 2 || k == nk) {
-    float lin;
-    if (stag == 1) lin = P.c1[k] * S(k - 1) + P.c2[k] * S(k);
-    else {
-      float cc2 = 0.5f * (P.c2[k] + P.c2[k]);
-      float cc1 = 1.0f - cc2;
-      lin = cc1 * S(k - 1) + cc2 * S(k);
-    }
-    if (k == 2) return (wb >= 0.0f) ? wb * lin : wb * upstrpd(S(k + 1), S(k), S(k - 1), weps);
-    return (wb > 0.0f) ? wb * upstrpd(S(k - 2), S(k - 1), S(k), weps) : wb * lin;
-  }
-  if (stag == 1 && pdefweno == 1) {
-    if (k == 3 && wb > 0.0f) return wb * upstrpd(S(k - 2), S(k - 1), S(k), weps);
-    if (k == nk - 1 && wb < 0.0f) return wb * upstrpd(S(k + 1), S(k), S(k - 1), weps);
-    float s1, s2, s3, s4, s5;
-    if (wb >= 0.0f) { s1 = S(k - 3); s2 = S(k - 2); s3 = S(k - 1); s4 = S(k); s5 = S(k + 1); }
-    else { s1 = S(k + 2); s2 = S(k + 1); s3 = S(k); s4 = S(k - 1); s5 = S(k - 2); }
-    return weno_doit(s1, s2, s3, s4, s5, weps) ? wb * weno5(s1, s2, s3, s4, s5, weps) : 0.0f;
-  }
-  if (k == 3)
-    return (wb >= 0.0f) ? wb * upstrpd(S(k - 2), S(k - 1), S(k), weps)
-                        : wb * weno5(S(k + 2), S(k + 1), S(k), S(k - 1), S(k - 2), weps);
-  if (k == nk - 1) {
-    const bool up = (stag == 1) ? (wb >= 0.0f) : (wb > 0.0f);
-    return up ? wb * weno5(S(k - 3), S(k - 2), S(k - 1), S(k), S(k + 1), weps)
-              : wb * upstrpd(S(k + 1), S(k), S(k - 1), weps);
-  }
-  return (wb >= 0.0f) ? wb * weno5(S(k - 3), S(k - 2), S(k - 1), S(k), S(k + 1), weps)
-                      : wb * weno5(S(k + 2), S(k + 1), S(k), S(k - 1), S(k - 2), weps);
-}
-
-__device__ __forceinline__ float vflux5(const KP &P, int stag, const float *rrw, const float *a,
-                                       int i, int j, int k) {
-  auto S = [&](int kk) { return a[IXD(P, i, j, kk)]; };
-  return vflux5_of(P, stag, wface(P, stag, rrw, i, j, k), S, k);
-}
-
-// vadv_flx5 (src/adv_routines.F:3269-3388) or vadv_weno5 (:2359-2557) for
-// the momentum equations (no pdef variant)
-static void vadv5(cm1g_model *m, int stag, const float *rrw, float *dumz, const float *a,
-                  bool weno = false, double weps = 0.0) {
-  const KP P = m->P;
-  const int ni = P.ni, nj = P.nj, nk = P.nk;
-  int ilo = 1, ihi = ni, jlo = 1, jhi = nj, klo = 2, khi = nk;
-  if (stag == 2) { ilo = P.ibw == 1 ? 2 : 1; ihi = P.ibe == 1 ? ni : ni + 1; }
-  if (stag == 3) { jlo = P.ibs == 1 ? 2 : 1; jhi = P.ibn == 1 ? nj : nj + 1; }
-  if (stag == 4) { klo = 1; khi = nk; }
-  if (weno)
-    for3(m->stream, ilo, ihi, jlo, jhi, klo, khi, [=] __device__(int i, int j, int k) {
-      auto S = [&](int kk) { return a[IXD(P, i, j, kk)]; };
-      dumz[IXD(P, i, j, k)] = vfluxw_of(P, stag, wface(P, stag, rrw, i, j, k), S, k, 0, weps);
-    });
-  else
-    for3(m->stream, ilo, ihi, jlo, jhi, klo, khi, [=] __device__(int i, int j, int k) {
-      dumz[IXD(P, i, j, k)] = vflux5(P, stag, rrw, a, i, j, k);
-    });
-}
+#include "cm1g_adv_common.h"
 
 // ------------------------------------------------------------------ pdef
 // One direction of the positive-definite correction (pdefx1+pdefx2,
@@ -364,19 +106,6 @@ static void pdef_dir(cm1g_model *m, int dir, const float *vel, const float *adv,
 }
 
 // ------------------------------------------------------ fused advs column
-// Upwind 5th-order face value for the face between cells m-1 and m along one
-// horizontal line, from the seven values S(m-3..m+3) (only six are used).
-template <bool WENO>
-__device__ __forceinline__ float hface5(float r, float sm3, float sm2, float sm1, float s0, float sp1,
-                                        float sp2, int pdef, double weps) {
-  if (!WENO) return (r >= 0.0f) ? r * flx5(sm3, sm2, sm1, s0, sp1) : r * flx5(sp2, sp1, s0, sm1, sm2);
-  float s1, s2, s3, s4, s5;
-  if (r >= 0.0f) { s1 = sm3; s2 = sm2; s3 = sm1; s4 = s0; s5 = sp1; }
-  else { s1 = sp2; s2 = sp1; s3 = s0; s4 = sm1; s5 = sm2; }
-  if (pdef == 1 && !weno_doit(s1, s2, s3, s4, s5, weps)) return 0.0f;
-  return r * weno5(s1, s2, s3, s4, s5, weps);
-}
-
 struct AdvsArgs {
   const float *s;                 // advected scalar
   float sadd;                     // nonzero: advect (th0(k)-th0r) + s
@@ -389,52 +118,86 @@ struct AdvsArgs {
   double weps;
 };
 
-// One column (i,j) of advs (src/adv.F:14-658 with hadv_flx5/vadv_flx5, or
-// hadv_weno5/vadv_weno5 when WENO):
-// face fluxes at i, i+1, j, j+1 with the open-boundary inflow fixes
-// (src/adv.F:211-340), the vertical fluxes marched up the column with a
-// six-level register window, and the tendency assembly.  Each flux is the
-// same FP32 expression as the reference's, so results are bitwise equal
-// to separate flux / divergence passes.
+// Operands of one level of a k_advs_col column (raw loads; the theta
+// offset th0-th0r is applied when they are used).
+struct AdvLevel {
+  float xm3, xm2, xm1, xp1, xp2, xp3;  // s(i-3..i+3), centre excluded
+  float ym3, ym2, ym1, yp1, yp2, yp3;  // s(j-3..j+3)
+  float ru0, ru1, rv0, rv1, rw1;       // face mass fluxes (rw1: face k+1)
+  float dv, st0, sv0, t11v, thtv;      // divx, initial tendency, s0, t11, thterm
+  float wnew;                          // s(k+4), entering the vertical window
+};
+
 template <bool WENO>
-__global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a) {
-  const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+__device__ __forceinline__ void advs_load(const KP &P, const AdvsArgs &a, int i, int j, int k,
+                                          AdvLevel &L) {
+  const int px = P.px, pxy = P.pxy, nk = P.nk;
+  const int c = IXD(P, i, j, k);
+  const float *s = a.s;
+  L.xm3 = s[c - 3]; L.xm2 = s[c - 2]; L.xm1 = s[c - 1];
+  L.xp1 = s[c + 1]; L.xp2 = s[c + 2]; L.xp3 = s[c + 3];
+  L.ym3 = s[c - 3 * px]; L.ym2 = s[c - 2 * px]; L.ym1 = s[c - px];
+  L.yp1 = s[c + px]; L.yp2 = s[c + 2 * px]; L.yp3 = s[c + 3 * px];
+  L.ru0 = a.rru[c]; L.ru1 = a.rru[c + 1];
+  L.rv0 = a.rrv[c]; L.rv1 = a.rrv[c + px];
+  L.rw1 = (k < nk) ? a.rrw[c + pxy] : 0.0f;
+  L.wnew = (k + 4 <= nk) ? s[c + 4 * pxy] : 0.0f;
+  L.dv = a.divx[c];
+  L.st0 = a.sten0[c];
+  L.sv0 = a.s0[c];
+  const bool th = (a.t11 != nullptr);
+  L.t11v = th ? a.t11[c] : 0.0f;
+  L.thtv = th ? a.thterm[c] : 0.0f;
+}
+
+// One column (i,j) of advs (src/adv.F:14-658 with hadv_flx5/vadv_flx5, or
+// hadv_weno5/vadv_weno5 when WENO): face fluxes at i, i+1, j, j+1 with the
+// open-boundary inflow fixes (src/adv.F:211-340), the vertical fluxes
+// marched up the column with a six-level register window, and the tendency
+// assembly.  Each flux is the same FP32 expression as the reference's, so
+// results are bitwise equal to separate flux / divergence passes.  The
+// operands of level k+1 are loaded before level k is evaluated and stored.
+template <bool WENO>
+__global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a, ColGrid cg) {
+  int i, j;
+  if (!col_tile(cg, i, j)) return;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   if (i > ni || j > nj) return;
   const bool doitw = (P.wbc == 2 && P.ibw == 1), doite = (P.ebc == 2 && P.ibe == 1);
   const bool doits = (P.sbc == 2 && P.ibs == 1), doitn = (P.nbc == 2 && P.ibn == 1);
   const float *s = a.s;
   const bool sadd = (a.sadd != 0.0f);
-  const int px = P.px, pxy = P.pxy;
+  const float th0r = P.K.th0r;
   auto val = [&](int kk) -> float {  // column value at level kk (1..nk)
     if (kk < 1 || kk > nk) return 0.0f;
     const float v = s[IXD(P, i, j, kk)];
-    return sadd ? (P.th0[kk] - P.K.th0r) + v : v;
+    return sadd ? (P.th0[kk] - th0r) + v : v;
   };
   // window: wv[0..5] = S(k-2 .. k+3)
   float wv0 = val(-1), wv1 = val(0), wv2 = val(1), wv3 = val(2), wv4 = val(3), wv5 = val(4);
   float fz_lo = 0.0f;
+  AdvLevel nx;
+  advs_load<WENO>(P, a, i, j, 1, nx);
   for (int k = 1; k <= nk; k++) {
+    const AdvLevel L = nx;
+    if (k < nk) advs_load<WENO>(P, a, i, j, k + 1, nx);
     const int c = IXD(P, i, j, k);
-    const float tem = sadd ? (P.th0[k] - P.K.th0r) : 0.0f;
-    auto H = [&](int off) -> float {
-      const float v = s[c + off];
-      return sadd ? tem + v : v;
-    };
+    const float tem = sadd ? (P.th0[k] - th0r) : 0.0f;
+    auto H = [=](float v) -> float { return sadd ? tem + v : v; };
     const float sc = wv2;
     // x faces i and i+1
-    const float xm3 = H(-3), xm2 = H(-2), xm1 = H(-1), xp1 = H(1), xp2 = H(2), xp3 = H(3);
-    const float ru0 = a.rru[c], ru1 = a.rru[c + 1];
+    const float xm3 = H(L.xm3), xm2 = H(L.xm2), xm1 = H(L.xm1), xp1 = H(L.xp1), xp2 = H(L.xp2),
+                xp3 = H(L.xp3);
+    const float ru0 = L.ru0, ru1 = L.ru1;
     float fx0 = hface5<WENO>(ru0, xm3, xm2, xm1, sc, xp1, xp2, a.pdef, a.weps);
     float fx1 = hface5<WENO>(ru1, xm2, xm1, sc, xp1, xp2, xp3, a.pdef, a.weps);
     const float fx0_raw = fx0;
     if (doitw && i == 1 && ru0 >= 0.0f) fx0 = fx1;
     if (doite && i == ni && ru1 <= 0.0f) fx1 = fx0_raw;
     // y faces j and j+1
-    const float ym3 = H(-3 * px), ym2 = H(-2 * px), ym1 = H(-px), yp1 = H(px), yp2 = H(2 * px),
-                yp3 = H(3 * px);
-    const float rv0 = a.rrv[c], rv1 = a.rrv[c + px];
+    const float ym3 = H(L.ym3), ym2 = H(L.ym2), ym1 = H(L.ym1), yp1 = H(L.yp1), yp2 = H(L.yp2),
+                yp3 = H(L.yp3);
+    const float rv0 = L.rv0, rv1 = L.rv1;
     float fy0 = hface5<WENO>(rv0, ym3, ym2, ym1, sc, yp1, yp2, a.pdef, a.weps);
     float fy1 = hface5<WENO>(rv1, ym2, ym1, sc, yp1, yp2, yp3, a.pdef, a.weps);
     const float fy0_raw = fy0;
@@ -454,8 +217,8 @@ __global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a) {
         const int o = kk - kf;  // -3..2
         return o == -3 ? wv0 : o == -2 ? wv1 : o == -1 ? wv2 : o == 0 ? wv3 : o == 1 ? wv4 : wv5;
       };
-      fz_hi = WENO ? vfluxw_of(P, 1, a.rrw[c + pxy], S, kf, a.pdefweno, a.weps)
-                   : vflux5_of(P, 1, a.rrw[c + pxy], S, kf);
+      fz_hi = WENO ? vfluxw_of(P, 1, L.rw1, S, kf, a.pdefweno, a.weps)
+                   : vflux5_of(P, 1, L.rw1, S, kf);
     }
     float az;
     if (k == 1) az = -fz_hi * P.rdz * P.mh[1];
@@ -466,12 +229,12 @@ __global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a) {
       a.advy[c] = ay;
       a.advz[c] = az;
     } else {
-      float st = a.sten0[c];
-      if (a.t11) st = st - a.t11[c] * a.thterm[c];
-      st = st + ((ax + ay) + az + sc * a.divx[c]) * P.rr0[k];
+      float st = L.st0;
+      if (a.t11) st = st - L.t11v * L.thtv;
+      st = st + ((ax + ay) + az + sc * L.dv) * P.rr0[k];
       if (a.out_sten) a.out_sten[c] = st;
       if (a.out_new) {
-        float v = a.s0[c] + a.dt * st;
+        float v = L.sv0 + a.dt * st;
         if (a.smeps_flush && fabsf(v) < P.K.smeps) v = 0.0f;
         a.out_new[c] = v;
       }
@@ -482,7 +245,7 @@ __global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a) {
     wv2 = wv3;
     wv3 = wv4;
     wv4 = wv5;
-    wv5 = val(k + 4);
+    wv5 = (k + 4 <= nk) ? (sadd ? (P.th0[k + 4] - th0r) + L.wnew : L.wnew) : 0.0f;
   }
 }
 
@@ -523,17 +286,18 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
   g.pdefweno = pdefweno;
   g.weps = weps;
   auto *kern = weno ? k_advs_col<true> : k_advs_col<false>;
-  const dim3 blk(64, 4, 1), grd((ni + 63) / 64, (nj + 3) / 4, 1);
+  const ColGrid cg = col_grid(ni, nj);
+  const dim3 blk(256, 1, 1), grd(cg.per * NXCD, 1, 1);
   // an in-place update (out_new aliasing the advected field) goes through
   // a partner array, since neighbouring columns still read the old values
   const bool inplace = (out_new != nullptr && out_new == s_in);
   if (inplace) g.out_new = m->thb;
   if (pdef != 1) {
     g.mode = 0;
-    hipLaunchKernelGGL(kern, grd, blk, 0, m->stream, P, g);
+    hipLaunchKernelGGL(kern, grd, blk, 0, m->stream, P, g, cg);
   } else {
     g.mode = 1;
-    hipLaunchKernelGGL(kern, grd, blk, 0, m->stream, P, g);
+    hipLaunchKernelGGL(kern, grd, blk, 0, m->stream, P, g, cg);
     float *advx = m->tx, *advy = m->ty, *advz = m->tz;
     float *dumx = m->sx, *dumy = m->sy, *dumz = m->sz;
     const float *rru = m->rrus, *rrv = m->rrvs, *rrw = m->rrws;
@@ -578,123 +342,6 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
 
 // ----------------------------------------------------------- advu/v/w
 // The momentum tendencies use the pre-acoustic mass fluxes rru/rrv/rrw
-// (model fields) and divx; uten/vten/wten receive uten1 + advection.
-void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy) {
-  const KP P = m->P;
-  const int ni = P.ni, nj = P.nj, nk = P.nk;
-  const float *rru = m->f[CM1G_RRU], *rrv = m->f[CM1G_RRV], *rrw = m->f[CM1G_RRW];
-  const float *divx = m->f[CM1G_DIVX];
-  float *dumx = m->sx, *dumy = m->sy, *dumz = m->sz;
-  // WENO5 on every stage (advwenov=1) or the last one (2), src/adv.F:958-960
-  const bool weno = (m->c.advwenov == 1) || (m->c.advwenov == 2 && nrk == 3);
-  const double weps = (double)(100.0f * P.K.epsilon);
-  const bool doitw = (P.wbc == 2 && P.ibw == 1), doite = (P.ebc == 2 && P.ibe == 1);
-  const bool doits = (P.sbc == 2 && P.ibs == 1), doitn = (P.nbc == 2 && P.ibn == 1);
-  // ---- u
-  if (mask & 1) {
-    const float *u3d = m->f[CM1G_U3D];
-    float *uten = m->f[CM1G_UTEN];
-    int i1 = P.ibw == 1 ? 2 : 1, i2 = P.ibe == 1 ? ni : ni + 1;
-    hadv5(m, 2, rru, rrv, dumx, dumy, u3d, weno, weps);
-    if (doits || doitn)
-      for2(m->stream, i1, i2, 1, nk, [=] __device__(int i, int k) {
-        if (doits && (rrv[IXD(P, i, 1, k)] + rrv[IXD(P, i - 1, 1, k)]) >= 0.0f)
-          dumy[IXD(P, i, 1, k)] = dumy[IXD(P, i, 2, k)];
-        if (doitn && (rrv[IXD(P, i, nj + 1, k)] + rrv[IXD(P, i - 1, nj + 1, k)]) <= 0.0f)
-          dumy[IXD(P, i, nj + 1, k)] = dumy[IXD(P, i, nj, k)];
-      });
-    vadv5(m, 2, rrw, dumz, u3d, weno, weps);
-    for3(m->stream, i1, i2, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-      int c = IXD(P, i, j, k);
-      float ax = -(dumx[c] - dumx[IXD(P, i - 1, j, k)]) * P.rdx * P.uf[i + 2];
-      float ay = -(dumy[IXD(P, i, j + 1, k)] - dumy[c]) * P.rdy * P.vh[j + 2];
-      if (doits && j == 1 && (rrv[IXD(P, i, 1, k)] + rrv[IXD(P, i - 1, 1, k)]) >= 0.0f)
-        ay = ay - u3d[c] * 0.5f * ((rrv[IXD(P, i - 1, 2, k)] - rrv[IXD(P, i - 1, 1, k)]) +
-                                  (rrv[IXD(P, i, 2, k)] - rrv[IXD(P, i, 1, k)])) * P.rdy * P.vh[1 + 2];
-      if (doitn && j == nj && (rrv[IXD(P, i, nj + 1, k)] + rrv[IXD(P, i - 1, nj + 1, k)]) <= 0.0f)
-        ay = ay - u3d[c] * 0.5f * ((rrv[IXD(P, i - 1, nj + 1, k)] - rrv[IXD(P, i - 1, nj, k)]) +
-                                  (rrv[IXD(P, i, nj + 1, k)] - rrv[IXD(P, i, nj, k)])) * P.rdy * P.vh[nj + 2];
-      float az;
-      if (k == 1) az = -dumz[IXD(P, i, j, 2)] * P.rdz * P.mh[1];
-      else if (k == nk) az = +dumz[c] * P.rdz * P.mh[nk];
-      else az = -(dumz[IXD(P, i, j, k + 1)] - dumz[c]) * P.rdz * P.mh[k];
-      uten[c] = uten[c] + ((ax + ay) + az + u3d[c] * 0.5f * (divx[c] + divx[IXD(P, i - 1, j, k)])) * P.rr0[k];
-    });
-  }
-  // ---- v
-  if (mask & 2) {
-    const float *v3d = m->f[CM1G_V3D];
-    float *vten = m->f[CM1G_VTEN];
-    int j1 = P.ibs == 1 ? 2 : 1, j2 = P.ibn == 1 ? nj : nj + 1;
-    hadv5(m, 3, rru, rrv, dumx, dumy, v3d, weno, weps);
-    if (doitw || doite)
-      for2(m->stream, j1, j2, 1, nk, [=] __device__(int j, int k) {
-        if (doitw && (rru[IXD(P, 1, j, k)] + rru[IXD(P, 1, j - 1, k)]) >= 0.0f)
-          dumx[IXD(P, 1, j, k)] = dumx[IXD(P, 2, j, k)];
-        if (doite && (rru[IXD(P, ni + 1, j, k)] + rru[IXD(P, ni + 1, j - 1, k)]) <= 0.0f)
-          dumx[IXD(P, ni + 1, j, k)] = dumx[IXD(P, ni, j, k)];
-      });
-    vadv5(m, 3, rrw, dumz, v3d, weno, weps);
-    for3(m->stream, 1, ni, j1, j2, 1, nk, [=] __device__(int i, int j, int k) {
-      int c = IXD(P, i, j, k);
-      float ax = -(dumx[IXD(P, i + 1, j, k)] - dumx[c]) * P.rdx * P.uh[i + 2];
-      float ay = -(dumy[c] - dumy[IXD(P, i, j - 1, k)]) * P.rdy * P.vf[j + 2];
-      if (doitw && i == 1 && (rru[IXD(P, 1, j, k)] + rru[IXD(P, 1, j - 1, k)]) >= 0.0f)
-        ax = ax - v3d[c] * 0.5f * ((rru[IXD(P, 2, j - 1, k)] - rru[IXD(P, 1, j - 1, k)]) +
-                                  (rru[IXD(P, 2, j, k)] - rru[IXD(P, 1, j, k)])) * P.rdx * P.uh[1 + 2];
-      if (doite && i == ni && (rru[IXD(P, ni + 1, j, k)] + rru[IXD(P, ni + 1, j - 1, k)]) <= 0.0f)
-        ax = ax - v3d[c] * 0.5f * ((rru[IXD(P, ni + 1, j - 1, k)] - rru[IXD(P, ni, j - 1, k)]) +
-                                  (rru[IXD(P, ni + 1, j, k)] - rru[IXD(P, ni, j, k)])) * P.rdx * P.uh[ni + 2];
-      float az;
-      if (k == 1) az = -dumz[IXD(P, i, j, 2)] * P.rdz * P.mh[1];
-      else if (k == nk) az = +dumz[c] * P.rdz * P.mh[nk];
-      else az = -(dumz[IXD(P, i, j, k + 1)] - dumz[c]) * P.rdz * P.mh[k];
-      vten[c] = vten[c] + ((ax + ay) + az + v3d[c] * 0.5f * (divx[c] + divx[IXD(P, i, j - 1, k)])) * P.rr0[k];
-    });
-  }
-  // ---- w (+ buoyancy, src/solve2.F:899-907: dum8 holds buoyancy)
-  if (mask & 4) {
-    const float *w3d = m->f[CM1G_W3D];
-    float *wten = m->f[CM1G_WTEN];
-    const float *dum8 = m->f[CM1G_DUM8];
-    hadv5(m, 4, rru, rrv, dumx, dumy, w3d, weno, weps);
-    if (doitw || doite)
-      for2(m->stream, 1, nj, 2, nk, [=] __device__(int j, int k) {
-        float cc2 = 0.5f * (P.c2[k] + P.c2[k]), cc1 = 1.0f - cc2;
-        if (doitw && (cc2 * rru[IXD(P, 1, j, k)] + cc1 * rru[IXD(P, 1, j, k - 1)]) >= 0.0f)
-          dumx[IXD(P, 1, j, k)] = dumx[IXD(P, 2, j, k)];
-        if (doite && (cc2 * rru[IXD(P, ni + 1, j, k)] + cc1 * rru[IXD(P, ni + 1, j, k - 1)]) <= 0.0f)
-          dumx[IXD(P, ni + 1, j, k)] = dumx[IXD(P, ni, j, k)];
-      });
-    if (doits || doitn)
-      for2(m->stream, 1, ni, 2, nk, [=] __device__(int i, int k) {
-        float cc2 = 0.5f * (P.c2[k] + P.c2[k]), cc1 = 1.0f - cc2;
-        if (doits && (cc2 * rrv[IXD(P, i, 1, k)] + cc1 * rrv[IXD(P, i, 1, k - 1)]) >= 0.0f)
-          dumy[IXD(P, i, 1, k)] = dumy[IXD(P, i, 2, k)];
-        if (doitn && (cc2 * rrv[IXD(P, i, nj + 1, k)] + cc1 * rrv[IXD(P, i, nj + 1, k - 1)]) <= 0.0f)
-          dumy[IXD(P, i, nj + 1, k)] = dumy[IXD(P, i, nj, k)];
-      });
-    vadv5(m, 4, rrw, dumz, w3d, weno, weps);
-    for3(m->stream, 1, ni, 1, nj, 2, nk, [=] __device__(int i, int j, int k) {
-      int c = IXD(P, i, j, k);
-      float cc2 = 0.5f * (P.c2[k] + P.c2[k]), cc1 = 1.0f - cc2;
-      float ax = -(dumx[IXD(P, i + 1, j, k)] - dumx[c]) * P.rdx * P.uh[i + 2];
-      float ay = -(dumy[IXD(P, i, j + 1, k)] - dumy[c]) * P.rdy * P.vh[j + 2];
-      if (doitw && i == 1 && (cc2 * rru[IXD(P, 1, j, k)] + cc1 * rru[IXD(P, 1, j, k - 1)]) >= 0.0f)
-        ax = ax - w3d[c] * (P.c1[k] * (rru[IXD(P, 2, j, k - 1)] - rru[IXD(P, 1, j, k - 1)]) +
-                            P.c2[k] * (rru[IXD(P, 2, j, k)] - rru[IXD(P, 1, j, k)])) * P.rdx * P.uh[1 + 2];
-      if (doite && i == ni && (cc2 * rru[IXD(P, ni + 1, j, k)] + cc1 * rru[IXD(P, ni + 1, j, k - 1)]) <= 0.0f)
-        ax = ax - w3d[c] * (P.c1[k] * (rru[IXD(P, ni + 1, j, k - 1)] - rru[IXD(P, ni, j, k - 1)]) +
-                            P.c2[k] * (rru[IXD(P, ni + 1, j, k)] - rru[IXD(P, ni, j, k)])) * P.rdx * P.uh[ni + 2];
-      if (doits && j == 1 && (cc2 * rrv[IXD(P, i, 1, k)] + cc1 * rrv[IXD(P, i, 1, k - 1)]) >= 0.0f)
-        ay = ay - w3d[c] * (P.c1[k] * (rrv[IXD(P, i, 2, k - 1)] - rrv[IXD(P, i, 1, k - 1)]) +
-                            P.c2[k] * (rrv[IXD(P, i, 2, k)] - rrv[IXD(P, i, 1, k)])) * P.rdy * P.vh[1 + 2];
-      if (doitn && j == nj && (cc2 * rrv[IXD(P, i, nj + 1, k)] + cc1 * rrv[IXD(P, i, nj + 1, k - 1)]) <= 0.0f)
-        ay = ay - w3d[c] * (P.c1[k] * (rrv[IXD(P, i, nj + 1, k - 1)] - rrv[IXD(P, i, nj, k - 1)]) +
-                            P.c2[k] * (rrv[IXD(P, i, nj + 1, k)] - rrv[IXD(P, i, nj, k)])) * P.rdy * P.vh[nj + 2];
-      float az = -(dumz[c] - dumz[IXD(P, i, j, k - 1)]) * P.rdz * P.mf[k];
-      float t = wten[c] + ((ax + ay) + az + w3d[c] * (P.c2[k] * divx[c] + P.c1[k] * divx[IXD(P, i, j, k - 1)])) * P.rrf0[k];
-      wten[c] = buoy ? t + (P.c1[k] * dum8[IXD(P, i, j, k - 1)] + P.c2[k] * dum8[c]) : t;
-    });
-  }
-}
+// (model fields) and divx; uten/vten/wten are updated in place
+// (column kernels in cm1g_advm.hip).
+void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy) { advm_run(m, nrk, mask, buoy); }

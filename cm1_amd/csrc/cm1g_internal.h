@@ -31,11 +31,14 @@ struct KP {
   const float *uh, *ruh, *uf, *vh, *rvh, *vf;
 };
 
+// Element (i,j,k) of a 3-D array, i = -2..ni+4, j = -2..nj+4, k = 0..nk+2.
+// i = 1 sits at a multiple of 32 floats (CM1G_IOFF), so a wavefront's 64
+// consecutive interior i fill exactly two 128-B cache lines.
 __device__ __host__ __forceinline__ int IXD(const KP &P, int i, int j, int k) {
-  return (i + 2) + (j + 2) * P.px + k * P.pxy;
+  return (i + CM1G_IOFF) + (j + 2) * P.px + k * P.pxy;
 }
 __device__ __host__ __forceinline__ int IX2D(const KP &P, int i, int j) {
-  return (i + 2) + (j + 2) * P.px;
+  return (i + CM1G_IOFF) + (j + 2) * P.px;
 }
 
 // bc kinds
@@ -120,6 +123,34 @@ inline void for2(hipStream_t s, int ilo, int ihi, int jlo, int jhi, F f) {
   hipLaunchKernelGGL(k_for2<F>, g, b, 0, s, f, ilo, ni, jlo, nj);
 }
 
+// Column kernels run on a 1-D grid of 64x4-column tiles.  The hardware
+// hands consecutive workgroups to the 8 XCDs in turn (each with its own L2);
+// this maps the workgroup id to a tile so that every XCD owns one
+// contiguous band of tile rows, walked in order, and the tiles that share
+// halo rows run on the same XCD at about the same time.  The grid is padded
+// to a multiple of 8 workgroups; tiles past the end return at once.
+constexpr int NXCD = 8;
+struct ColGrid {
+  int gx, gy, nb, per;  // tiles in i and j, tiles, tiles per XCD
+};
+inline ColGrid col_grid(int ni, int nj) {
+  ColGrid g;
+  g.gx = (ni + 63) / 64;
+  g.gy = (nj + 3) / 4;
+  g.nb = g.gx * g.gy;
+  g.per = (g.nb + NXCD - 1) / NXCD;
+  return g;
+}
+// returns false for padding workgroups
+__device__ __forceinline__ bool col_tile(const ColGrid &g, int &i, int &j) {
+  const int b = (int)blockIdx.x;
+  const int t = (b % NXCD) * g.per + b / NXCD;
+  if (t >= g.nb) return false;
+  i = 1 + (t % g.gx) * 64 + (int)(threadIdx.x & 63);
+  j = 1 + (t / g.gx) * 4 + (int)(threadIdx.x >> 6);
+  return true;
+}
+
 // ---------------------------------------------------------------- modules
 // bc fills (cm1g_bc.hip)
 void bc_fill(cm1g_model *m, int kind, float *a);
@@ -128,6 +159,7 @@ void radbc4(cm1g_model *m, const float *ua, const float *u3d, const float *va,
             const float *v3d, float dt);
 // advection (cm1g_adv.hip)
 void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy);
+void advm_run(cm1g_model *m, int nrk, int mask, int buoy);  // cm1g_advm.hip
 void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float *s0,
               const float *sten0, const float *thterm_div, float *out_sten, float *out_new,
               int pdef, int pdefweno, float dt, double weps, int smeps_flush);

@@ -111,7 +111,7 @@ extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
   if (m->c.csmax == 0.0f) m->c.csmax = 350.0f;
   KP &P = m->P;
   P.ni = cfg->ni; P.nj = cfg->nj; P.nk = cfg->nk;
-  P.px = ((cfg->ni + 7) + 63) / 64 * 64;
+  P.px = ((cfg->ni + CM1G_IOFF + 5) + 63) / 64 * 64;
   P.py = cfg->nj + 7;
   P.pxy = P.px * P.py;
   P.wbc = cfg->wbc; P.ebc = cfg->ebc; P.sbc = cfg->sbc; P.nbc = cfg->nbc;
@@ -322,16 +322,24 @@ static bool field_ok(const cm1g_model *m, int id) {
   return m && id >= 0 && id < CM1G_NFIELDS && m->f[id] != nullptr;
 }
 
+// the logical box (i = -2..ni+4) as rows of the padded device box
+static float *box_origin(const cm1g_model *m, int id) { return m->f[id] + (CM1G_IOFF - 2); }
+
 extern "C" int cm1g_put_field(cm1g_model *m, int id, const float *host) {
   ARGCHK(field_ok(m, id) && host, "cm1g_put_field: bad field id or null pointer");
-  HIPCHK(hipMemcpy(m->f[id], host, m->ntot * sizeof(float), hipMemcpyHostToDevice));
+  const size_t w = (size_t)m->P.ni + 7, rows = (size_t)m->P.py * (size_t)(m->P.nk + 3);
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy2D(box_origin(m, id), (size_t)m->P.px * sizeof(float), host, w * sizeof(float),
+                     w * sizeof(float), rows, hipMemcpyHostToDevice));
   return 0;
 }
 
 extern "C" int cm1g_get_field(cm1g_model *m, int id, float *host) {
   ARGCHK(field_ok(m, id) && host, "cm1g_get_field: bad field id or null pointer");
+  const size_t w = (size_t)m->P.ni + 7, rows = (size_t)m->P.py * (size_t)(m->P.nk + 3);
   HIPCHK(hipStreamSynchronize(m->stream));
-  HIPCHK(hipMemcpy(host, m->f[id], m->ntot * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy2D(host, w * sizeof(float), box_origin(m, id), (size_t)m->P.px * sizeof(float),
+                     w * sizeof(float), rows, hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -354,7 +362,7 @@ extern "C" int cm1g_put_array(cm1g_model *m, int id, int stag, const float *host
   HIPCHK(hipStreamSynchronize(m->stream));
   const size_t dpitch = (size_t)m->P.px * sizeof(float), plane = (size_t)m->P.pxy;
   for (size_t k = 0; k < nz; k++)
-    HIPCHK(hipMemcpy2D(m->f[id] + k * plane, dpitch, host + k * nx * ny, nx * sizeof(float),
+    HIPCHK(hipMemcpy2D(box_origin(m, id) + k * plane, dpitch, host + k * nx * ny, nx * sizeof(float),
                        nx * sizeof(float), ny, hipMemcpyHostToDevice));
   return 0;
 }
@@ -366,7 +374,7 @@ extern "C" int cm1g_get_array(cm1g_model *m, int id, int stag, float *host) {
   HIPCHK(hipStreamSynchronize(m->stream));
   const size_t spitch = (size_t)m->P.px * sizeof(float), plane = (size_t)m->P.pxy;
   for (size_t k = 0; k < nz; k++)
-    HIPCHK(hipMemcpy2D(host + k * nx * ny, nx * sizeof(float), m->f[id] + k * plane, spitch,
+    HIPCHK(hipMemcpy2D(host + k * nx * ny, nx * sizeof(float), box_origin(m, id) + k * plane, spitch,
                        nx * sizeof(float), ny, hipMemcpyDeviceToHost));
   return 0;
 }

@@ -124,9 +124,9 @@ __global__ void __launch_bounds__(256) k_sound_ring(KP P, SubArgs a) {
 // Each level issues all of its loads before any store, so a level costs one
 // memory round trip; the downward sweep also prefetches the next level's
 // operands before storing the current one.
-__global__ void __launch_bounds__(256) k_sound_sub(KP P, SubArgs a) {
-  const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+__global__ void __launch_bounds__(256) k_sound_sub(KP P, SubArgs a, ColGrid cg) {
+  int i, j;
+  if (!col_tile(cg, i, j)) return;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   if (i > ni || j > nj) return;
   const float dts = a.dts, rdz = P.rdz, cp = P.K.cp, smeps = P.K.smeps;
@@ -364,7 +364,8 @@ void sound_run(cm1g_model *m, int nrk) {
   a.rrw = rrw;
   // u, v and ppd alternate between the state arrays and scratch arrays
   float *ub[2] = {u3d, m->scr1}, *vb[2] = {v3d, m->scr2}, *pb[2] = {ppd, m->tx};
-  const dim3 blk(64, 4, 1), grd((ni + 63) / 64, (nj + 3) / 4, 1);
+  const ColGrid cg = col_grid(ni, nj);
+  const dim3 blk(256, 1, 1), grd(cg.per * NXCD, 1, 1);
   const dim3 ring_grd((ni + nj + 1 + 255) / 256, nk, 1);
   for (int n = 1; n <= nloop; n++) {
     const int src = (n - 1) & 1;
@@ -377,7 +378,7 @@ void sound_run(cm1g_model *m, int nrk) {
     a.last = (n == nloop);
     ktimer_mark(m, CM1G_KT_SOUND_SUBSTEP);
     hipLaunchKernelGGL(k_sound_ring, ring_grd, dim3(256), 0, m->stream, P, a);
-    hipLaunchKernelGGL(k_sound_sub, grd, blk, 0, m->stream, P, a);
+    hipLaunchKernelGGL(k_sound_sub, grd, blk, 0, m->stream, P, a, cg);
     ktimer_mark(m, CM1G_KT_SOUND_SUBSTEP);
     if (n < nloop) bc_fill(m, BC_P, pb[src ^ 1]);
   }

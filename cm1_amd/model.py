@@ -74,14 +74,14 @@ class Model:
 
     # ---- 3-D fields in the logical (nk+3, nj+7, ni+7) box
     def put(self, name, arr):
-        full = np.zeros((self.pz, self.py, self.px), np.float32)
-        full[:, :, :self.ni + 7] = arr
-        N.check(self.L.cm1g_put_field(self.h, N.FID[name], full.ctypes.data), "put " + name)
+        a = np.ascontiguousarray(arr, np.float32)
+        assert a.shape == (self.pz, self.py, self.ni + 7), a.shape
+        N.check(self.L.cm1g_put_field(self.h, N.FID[name], a.ctypes.data), "put " + name)
 
     def get(self, name):
-        full = np.empty((self.pz, self.py, self.px), np.float32)
-        N.check(self.L.cm1g_get_field(self.h, N.FID[name], full.ctypes.data), "get " + name)
-        return full[:, :, :self.ni + 7].copy()
+        a = np.empty((self.pz, self.py, self.ni + 7), np.float32)
+        N.check(self.L.cm1g_get_field(self.h, N.FID[name], a.ctypes.data), "get " + name)
+        return a
 
     def get_2d(self, name):
         n = self.px * self.py + (self.px + self.py) * self.pz

@@ -90,6 +90,11 @@ enum { CM1G_D_RAIN, CM1G_D_PRATE, CM1G_D_RADBCW, CM1G_D_RADBCE, CM1G_D_RADBCS,
 
 typedef struct cm1g_model cm1g_model;
 
+/* Device layout of a 3-D field (cm1g_field_ptr): element (i,j,k) at
+ * (i + CM1G_IOFF) + (j+2)*px + k*px*py floats, i = -2..ni+4,
+ * j = -2..nj+4, k = 0..nk+2 (px, py from cm1g_dims). */
+#define CM1G_IOFF 31
+
 /* --- set-up, residency ------------------------------------------------ */
 
 /* Replaces the module-scalar hand-over of param()/set_constants()
@@ -112,7 +117,8 @@ int cm1g_metrics(cm1g_model *m);
  * (src/base.F:2085-2198). */
 int cm1g_base_derive(cm1g_model *m);
 
-/* Copy a whole padded host box (px*py*pz floats, device layout) in/out. */
+/* Copy a whole host box in/out: (nk+3)*(nj+7)*(ni+7) floats, i = -2..ni+4
+ * fastest, then j = -2..nj+4, then k = 0..nk+2. */
 int cm1g_put_field(cm1g_model *m, int id, const float *host);
 int cm1g_get_field(cm1g_model *m, int id, float *host);
 /* The same for a host array in the reference's own shape (what a Fortran
