@@ -25,6 +25,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernel, measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE counters (calibrated on a plain copy) and recorded
+# in this summary
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_supercell_1024x1024x96.md")
 
 STRETCH = dict(dz_bot=50.0, dz_top=500.0, str_bot=500.0, str_top=17000.0, ztop=30000.0)
 
@@ -77,6 +81,18 @@ def cpu_baseline(threads):
                         f"(dx=250 m, stretched dz, dt=1.5 s, Kessler, open BCs), {nsteps} steps, "
                         f"median per-step time {t:.3f} s excluding step 1; nsound chosen by the "
                         f"reference (4, vs 8 on the GPU)"))
+
+
+def measured_traffic(kernel):
+    """Bytes per launch of `kernel` from the committed counter summary."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                if line.startswith(f"traffic {kernel} bytes_per_launch="):
+                    return float(line.split("=", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -144,7 +160,7 @@ def main():
             avg_ms = tot.value / nl.value
             gbs = nbytes.value / (avg_ms * 1e-3) / 1e9
             roof = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
+                        frac=round(gbs / HBM_PEAK_GBS, 4), traffic=measured_traffic(name.value.decode()),
                         kernel=name.value.decode(), avg_ms=round(avg_ms, 4),
                         launches=nl.value, bytes_per_launch=nbytes.value)
     # whole-step algorithmic rate (frozen byte table)

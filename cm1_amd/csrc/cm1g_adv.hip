@@ -9,100 +9,197 @@
 #include "cm1g_adv_common.h"
 
 // ------------------------------------------------------------------ pdef
-// One direction of the positive-definite correction (pdefx1+pdefx2,
-// pdefy1+pdefy2 or pdefz).  The reference scatters mass to i-1,i,i+1
-// inside a sequential loop; here each cell gathers the same terms in the
-// same order (left neighbour, itself, right neighbour), which reproduces
-// the sequential sums bit for bit.
-static void pdef_dir(cm1g_model *m, int dir, const float *vel, const float *adv, float *advout,
-                     float *dum, const float *s0, const float *s, float dt) {
-  const KP P = m->P;
-  const int ni = P.ni, nj = P.nj, nk = P.nk;
-  // pdefx1 / pdefy1 / first part of pdefz: dum
-  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-    int c = IXD(P, i, j, k);
-    float div;
-    if (dir == 0) div = (vel[IXD(P, i + 1, j, k)] - vel[c]) * P.rdx * P.uh[i + 2];
-    else if (dir == 1) div = (vel[IXD(P, i, j + 1, k)] - vel[c]) * P.rdy * P.vh[j + 2];
-    else div = (vel[IXD(P, i, j, k + 1)] - vel[c]) * P.rdz * P.mh[k];
-    float d = P.rho0[k] * s0[c] + dt * (adv[c] + s[c] * div);
-    if (dir == 0) {
-      if (P.wbc == 2 && P.ibw == 1 && i == 1 && vel[IXD(P, 1, j, k)] >= 0.0f) d = P.rho0[k] * s0[c];
-      if (P.ebc == 2 && P.ibe == 1 && i == ni && vel[IXD(P, ni + 1, j, k)] <= 0.0f) d = P.rho0[k] * s0[c];
-    } else if (dir == 1) {
-      if (P.sbc == 2 && P.ibs == 1 && j == 1 && vel[IXD(P, i, 1, k)] >= 0.0f) d = P.rho0[k] * s0[c];
-      if (P.nbc == 2 && P.ibn == 1 && j == nj && vel[IXD(P, i, nj + 1, k)] <= 0.0f) d = P.rho0[k] * s0[c];
-    }
-    dum[c] = d;
-  });
-  if (dir != 2) {
-    bc_fill(m, BC_S, dum);
+// The positive-definite correction (pdscheme=1, src/pdef.F:15-581), per
+// direction: dum = rho0*s0 + dt*(adv + s*div) (pdefx1/pdefy1/pdefz), its
+// halo fill (bcs, or zero below and above the column), and the mass each
+// cell takes from or gives to its two neighbours (pdefx2/pdefy2/pdefz).
+// The reference scatters inside a sequential loop; here each cell gathers
+// the same terms in the same order (left neighbour, itself, right
+// neighbour), which reproduces the sequential sums bit for bit.  dum is not
+// stored: each cell evaluates it at the five points it needs, mapping halo
+// points to the interior point the fill copies.
+
+// mass flux of direction dir at the lower face of (c, level k) and at the
+// upper one (c + stride): with scale, the getdivx scaling rho0(k) for u,v
+// and rf0 for w (zero at k=1 and nk+1)
+__device__ __forceinline__ void pdef_faces(const KP &P, int dir, const float *vel, int scale, int c,
+                                           int k, float &lo, float &hi) {
+  if (dir == 0) {
+    lo = vel[c];
+    hi = vel[c + 1];
+    if (scale) { lo = lo * P.rho0[k]; hi = hi * P.rho0[k]; }
+  } else if (dir == 1) {
+    lo = vel[c];
+    hi = vel[c + P.px];
+    if (scale) { lo = lo * P.rho0[k]; hi = hi * P.rho0[k]; }
   } else {
-    for2(m->stream, 1, ni, 1, nj, [=] __device__(int i, int j) {
-      dum[IXD(P, i, j, 0)] = 0.0f;
-      dum[IXD(P, i, j, nk + 1)] = 0.0f;
-    });
+    lo = vel[c];
+    hi = vel[c + P.pxy];
+    if (scale) {
+      lo = (k == 1) ? 0.0f : lo * P.rf0[k];
+      hi = (k + 1 == P.nk + 1) ? 0.0f : hi * P.rf0[k + 1];
+    }
   }
-  // second part: gather the mass exchanged with the two neighbours
-  const float rdt = 1.0f / dt;
-  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-    const float smeps = P.K.smeps;
-    auto D = [&](int o) -> float {
-      if (dir == 0) return dum[IXD(P, i + o, j, k)];
-      if (dir == 1) return dum[IXD(P, i, j + o, k)];
-      return dum[IXD(P, i, j, k + o)];
-    };
-    // the reference visits cells 0..ni+1 (x, y) or 1..nk (z)
-    auto visited = [&](int o) -> bool {
-      if (dir == 0) { int ii = i + o; return ii >= 0 && ii <= ni + 1; }
-      if (dir == 1) { int jj = j + o; return jj >= 0 && jj <= nj + 1; }
-      int kk = k + o; return kk >= 1 && kk <= nk;
-    };
-    float mass = 0.0f;
-    bool flag = false;
-    float d0 = D(0);
-    // contribution from the left neighbour (cell -1 scatters foo2*foo3 to us)
-    if (visited(-1)) {
-      float dm = D(-1);
-      if (dm < 0.0f) {
-        float foo1 = fmaxf(0.0f, D(-2)), foo2 = fmaxf(0.0f, d0);
-        if (foo1 + foo2 > smeps) {
-          float foo3 = fmaxf(dm, -(foo1 + foo2)) / (foo1 + foo2);
-          mass = mass + foo2 * foo3;
-          if (d0 > smeps) flag = true;
-        }
-      }
+}
+
+// dum at (i,j,k) for direction dir (0 x, 1 y, 2 z), interior point
+__device__ __forceinline__ float pdef_dum(const KP &P, int dir, const float *vel, int scale,
+                                          const float *adv, const float *s0, const float *s, float dt,
+                                          int i, int j, int k) {
+  const int c = IXD(P, i, j, k);
+  float lo, hi;
+  pdef_faces(P, dir, vel, scale, c, k, lo, hi);
+  float div;
+  if (dir == 0) div = (hi - lo) * P.rdx * P.uh[i + 2];
+  else if (dir == 1) div = (hi - lo) * P.rdy * P.vh[j + 2];
+  else div = (hi - lo) * P.rdz * P.mh[k];
+  float d = P.rho0[k] * s0[c] + dt * (adv[c] + s[c] * div);
+  if (dir == 0) {
+    if (P.wbc == 2 && P.ibw == 1 && i == 1 && lo >= 0.0f) d = P.rho0[k] * s0[c];
+    if (P.ebc == 2 && P.ibe == 1 && i == P.ni && hi <= 0.0f) d = P.rho0[k] * s0[c];
+  } else if (dir == 1) {
+    if (P.sbc == 2 && P.ibs == 1 && j == 1 && lo >= 0.0f) d = P.rho0[k] * s0[c];
+    if (P.nbc == 2 && P.ibn == 1 && j == P.nj && hi <= 0.0f) d = P.rho0[k] * s0[c];
+  }
+  return d;
+}
+
+// halo point of the horizontal fill (bcs, src/bc.F:180-364) -> the
+// interior point whose value it holds
+__device__ __forceinline__ int pdef_wrap(int m, int n, bool periodic) {
+  if (m < 1) return periodic ? m + n : 1;
+  if (m > n) return periodic ? m - n : n;
+  return m;
+}
+
+// the corrected tendency of one cell from dum at offsets -2..2 along the
+// direction; vm/vp: the neighbours -1/+1 are cells the reference visits
+__device__ __forceinline__ float pdef_gather(float dm2, float dm1, float d0, float dp1, float dp2,
+                                             bool vm, bool vp, float smeps, bool &flag) {
+  float mass = 0.0f;
+  flag = false;
+  if (vm && dm1 < 0.0f) {
+    const float foo1 = fmaxf(0.0f, dm2), foo2 = fmaxf(0.0f, d0);
+    if (foo1 + foo2 > smeps) {
+      const float foo3 = fmaxf(dm1, -(foo1 + foo2)) / (foo1 + foo2);
+      mass = mass + foo2 * foo3;
+      if (d0 > smeps) flag = true;
     }
-    if (d0 < 0.0f) {
-      float foo1 = fmaxf(0.0f, D(-1)), foo2 = fmaxf(0.0f, D(1));
-      if (foo1 + foo2 > smeps) {
-        float foo3 = fmaxf(d0, -(foo1 + foo2)) / (foo1 + foo2);
-        mass = mass - (foo1 + foo2) * foo3;
-        flag = true;
-      }
+  }
+  if (d0 < 0.0f) {
+    const float foo1 = fmaxf(0.0f, dm1), foo2 = fmaxf(0.0f, dp1);
+    if (foo1 + foo2 > smeps) {
+      const float foo3 = fmaxf(d0, -(foo1 + foo2)) / (foo1 + foo2);
+      mass = mass - (foo1 + foo2) * foo3;
+      flag = true;
     }
-    if (visited(1)) {
-      float dp = D(1);
-      if (dp < 0.0f) {
-        float foo1 = fmaxf(0.0f, d0), foo2 = fmaxf(0.0f, D(2));
-        if (foo1 + foo2 > smeps) {
-          float foo3 = fmaxf(dp, -(foo1 + foo2)) / (foo1 + foo2);
-          mass = mass + foo1 * foo3;
-          if (d0 > smeps) flag = true;
-        }
-      }
+  }
+  if (vp && dp1 < 0.0f) {
+    const float foo1 = fmaxf(0.0f, d0), foo2 = fmaxf(0.0f, dp2);
+    if (foo1 + foo2 > smeps) {
+      const float foo3 = fmaxf(dp1, -(foo1 + foo2)) / (foo1 + foo2);
+      mass = mass + foo1 * foo3;
+      if (d0 > smeps) flag = true;
     }
-    int c = IXD(P, i, j, k);
-    float out = adv[c];
+  }
+  return mass;
+}
+
+// x and y directions (pdefx1/x2, pdefy1/y2): corrected advx, advy
+struct PdefArgs {
+  const float *rru, *rrv, *rrw, *advx, *advy, *advz, *s0, *s;
+  float *ax_out, *ay_out;
+  const float *sten0, *t11, *thterm, *divx;
+  float *out_sten, *out_new;
+  float dt;
+  int smeps_flush, scale;
+};
+
+__global__ void __launch_bounds__(256) k_pdef_xy(KP P, PdefArgs a) {
+  const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int k = 1 + (int)blockIdx.z;
+  const int ni = P.ni, nj = P.nj;
+  if (i > ni || j > nj) return;
+  const int c = IXD(P, i, j, k);
+  const float smeps = P.K.smeps, rdt = 1.0f / a.dt;
+  const float r0s0 = P.rho0[k] * a.s0[c];
+  {
+    const bool per = (P.wbc == 1);
+    float d[5];
+#pragma unroll
+    for (int o = -2; o <= 2; o++)
+      d[o + 2] = pdef_dum(P, 0, a.rru, a.scale, a.advx, a.s0, a.s, a.dt, pdef_wrap(i + o, ni, per), j, k);
+    bool flag;
+    const float mass = pdef_gather(d[0], d[1], d[2], d[3], d[4], i - 1 >= 0, i + 1 <= ni + 1, smeps, flag);
+    float out = a.advx[c];
     if (flag) {
-      float div;
-      if (dir == 0) div = (vel[IXD(P, i + 1, j, k)] - vel[c]) * P.rdx * P.uh[i + 2];
-      else if (dir == 1) div = (vel[IXD(P, i, j + 1, k)] - vel[c]) * P.rdy * P.vh[j + 2];
-      else div = (vel[IXD(P, i, j, k + 1)] - vel[c]) * P.rdz * P.mh[k];
-      out = (d0 + mass - P.rho0[k] * s0[c]) * rdt - s[c] * div;
+      float lo, hi;
+      pdef_faces(P, 0, a.rru, a.scale, c, k, lo, hi);
+      const float div = (hi - lo) * P.rdx * P.uh[i + 2];
+      out = (d[2] + mass - r0s0) * rdt - a.s[c] * div;
     }
-    advout[c] = out;
-  });
+    a.ax_out[c] = out;
+  }
+  {
+    const bool per = (P.sbc == 1);
+    float d[5];
+#pragma unroll
+    for (int o = -2; o <= 2; o++)
+      d[o + 2] = pdef_dum(P, 1, a.rrv, a.scale, a.advy, a.s0, a.s, a.dt, i, pdef_wrap(j + o, nj, per), k);
+    bool flag;
+    const float mass = pdef_gather(d[0], d[1], d[2], d[3], d[4], j - 1 >= 0, j + 1 <= nj + 1, smeps, flag);
+    float out = a.advy[c];
+    if (flag) {
+      float lo, hi;
+      pdef_faces(P, 1, a.rrv, a.scale, c, k, lo, hi);
+      const float div = (hi - lo) * P.rdy * P.vh[j + 2];
+      out = (d[2] + mass - r0s0) * rdt - a.s[c] * div;
+    }
+    a.ay_out[c] = out;
+  }
+}
+
+// z direction (pdefz) marched up each column with a window of dum, then the
+// tendency with the corrected advx, advy, advz and the s0 + dt*sten update
+__global__ void __launch_bounds__(256) k_pdef_z(KP P, PdefArgs a, ColGrid cg) {
+  int i, j;
+  if (!col_tile(cg, i, j)) return;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  if (i > ni || j > nj) return;
+  const float smeps = P.K.smeps, rdt = 1.0f / a.dt;
+  auto dz = [&](int kk) -> float {  // dum of pdefz; zero outside 1..nk
+    if (kk < 1 || kk > nk) return 0.0f;
+    return pdef_dum(P, 2, a.rrw, a.scale, a.advz, a.s0, a.s, a.dt, i, j, kk);
+  };
+  float dm2 = 0.0f, dm1 = 0.0f, d0 = dz(1), dp1 = dz(2), dp2 = dz(3);
+  for (int k = 1; k <= nk; k++) {
+    const int c = IXD(P, i, j, k);
+    bool flag;
+    const float mass = pdef_gather(dm2, dm1, d0, dp1, dp2, k - 1 >= 1, k + 1 <= nk, smeps, flag);
+    const float sc = a.s[c];
+    float az = a.advz[c];
+    if (flag) {
+      float lo, hi;
+      pdef_faces(P, 2, a.rrw, a.scale, c, k, lo, hi);
+      const float div = (hi - lo) * P.rdz * P.mh[k];
+      az = (d0 + mass - P.rho0[k] * a.s0[c]) * rdt - sc * div;
+    }
+    float st = a.sten0[c];
+    if (a.t11) st = st - a.t11[c] * a.thterm[c];
+    st = st + ((a.ax_out[c] + a.ay_out[c]) + az + sc * a.divx[c]) * P.rr0[k];
+    if (a.out_sten) a.out_sten[c] = st;
+    if (a.out_new) {
+      float v = a.s0[c] + a.dt * st;
+      if (a.smeps_flush && fabsf(v) < P.K.smeps) v = 0.0f;
+      a.out_new[c] = v;
+    }
+    dm2 = dm1;
+    dm1 = d0;
+    d0 = dp1;
+    dp1 = dp2;
+    dp2 = dz(k + 3);
+  }
 }
 
 // ------------------------------------------------------ fused advs column
@@ -116,6 +213,8 @@ struct AdvsArgs {
   int smeps_flush, mode;          // mode 0: final tendency; 1: advx/advy/advz only
   int pdef, pdefweno;             // WENO5 magnitude tests (horizontal / vertical)
   double weps;
+  int scale;                      // rru/rrv/rrw are the averaged velocities: scale by
+                                  // rho0 / rf0 on the fly (getdivx, src/misclibs.F:222-370)
 };
 
 // Operands of one level of a k_advs_col column (raw loads; the theta
@@ -141,6 +240,12 @@ __device__ __forceinline__ void advs_load(const KP &P, const AdvsArgs &a, int i,
   L.ru0 = a.rru[c]; L.ru1 = a.rru[c + 1];
   L.rv0 = a.rrv[c]; L.rv1 = a.rrv[c + px];
   L.rw1 = (k < nk) ? a.rrw[c + pxy] : 0.0f;
+  if (a.scale) {
+    const float r0 = P.rho0[k];
+    L.ru0 = L.ru0 * r0; L.ru1 = L.ru1 * r0;
+    L.rv0 = L.rv0 * r0; L.rv1 = L.rv1 * r0;
+    if (k < nk) L.rw1 = L.rw1 * P.rf0[k + 1];
+  }
   L.wnew = (k + 4 <= nk) ? s[c + 4 * pxy] : 0.0f;
   L.dv = a.divx[c];
   L.st0 = a.sten0[c];
@@ -271,9 +376,10 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
   g.sten0 = sten0;
   g.t11 = thterm_div;
   g.thterm = thterm_div ? m->f[CM1G_THTERM] : nullptr;
-  g.rru = m->rrus;
-  g.rrv = m->rrvs;
-  g.rrw = m->rrws;
+  g.rru = m->f[CM1G_RRU];
+  g.rrv = m->f[CM1G_RRV];
+  g.rrw = m->f[CM1G_RRW];
+  g.scale = m->advs_unscaled ? 0 : 1;
   g.divx = m->f[CM1G_DIVX];
   g.out_sten = out_sten;
   g.out_new = out_new;
@@ -287,9 +393,10 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
   g.weps = weps;
   auto *kern = weno ? k_advs_col<true> : k_advs_col<false>;
   const ColGrid cg = col_grid(ni, nj);
-  const dim3 blk(256, 1, 1), grd(cg.per * NXCD, 1, 1);
-  // an in-place update (out_new aliasing the advected field) goes through
-  // a partner array, since neighbouring columns still read the old values
+  const dim3 blk = col_threads(cg), grd = col_blocks(cg);
+  // an in-place update (out_new aliasing the advected field) writes a
+  // partner array, since neighbouring columns still read the old values;
+  // the field and its partner then exchange buffers
   const bool inplace = (out_new != nullptr && out_new == s_in);
   if (inplace) g.out_new = m->thb;
   if (pdef != 1) {
@@ -298,50 +405,52 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
   } else {
     g.mode = 1;
     hipLaunchKernelGGL(kern, grd, blk, 0, m->stream, P, g, cg);
-    float *advx = m->tx, *advy = m->ty, *advz = m->tz;
-    float *dumx = m->sx, *dumy = m->sy, *dumz = m->sz;
-    const float *rru = m->rrus, *rrv = m->rrvs, *rrw = m->rrws;
-    const float *a = s_in;
+    // the positive-definite pass reads the advected values themselves
+    // (never the theta offset form: pdef is for moisture and tracers)
     if (sadd != 0.0f) {
-      // the positive-definite pass needs the advected values themselves
-      float *sv = m->f[CM1G_SADV];
-      for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-        float tem = P.th0[k] - P.K.th0r;
-        sv[IXD(P, i, j, k)] = tem + s_in[IXD(P, i, j, k)];
-      });
-      a = sv;
+      cm1g_set_error("advs: pdef with an offset field is not supported");
+      return;
     }
-    // each direction writes its corrected tendency into dumx/dumy/dumz
-    pdef_dir(m, 0, rru, advx, dumx, m->scr1, s0, a, dt);
-    pdef_dir(m, 2, rrw, advz, dumz, m->scr1, s0, a, dt);
-    pdef_dir(m, 1, rrv, advy, dumy, m->scr1, s0, a, dt);
-    const float *ax = dumx, *ay = dumy, *az = dumz;
-    const float *t11 = thterm_div, *thterm = g.thterm, *divx = g.divx;
-    float *onew = g.out_new;
-    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-      int c = IXD(P, i, j, k);
-      float st = sten0[c];
-      if (t11) st = st - t11[c] * thterm[c];
-      st = st + ((ax[c] + ay[c]) + az[c] + a[c] * divx[c]) * P.rr0[k];
-      if (out_sten) out_sten[c] = st;
-      if (onew) {
-        float v = s0[c] + dt * st;
-        if (smeps_flush && fabsf(v) < P.K.smeps) v = 0.0f;
-        onew[c] = v;
-      }
-    });
+    PdefArgs pa;
+    pa.rru = g.rru;
+    pa.rrv = g.rrv;
+    pa.rrw = g.rrw;
+    pa.scale = g.scale;
+    pa.advx = m->tx;
+    pa.advy = m->ty;
+    pa.advz = m->tz;
+    pa.s0 = s0;
+    pa.s = s_in;
+    pa.ax_out = m->sx;
+    pa.ay_out = m->sy;
+    pa.sten0 = sten0;
+    pa.t11 = thterm_div;
+    pa.thterm = g.thterm;
+    pa.divx = g.divx;
+    pa.out_sten = out_sten;
+    pa.out_new = g.out_new;
+    pa.dt = dt;
+    pa.smeps_flush = smeps_flush;
+    hipLaunchKernelGGL(k_pdef_xy, dim3((ni + 63) / 64, (nj + 3) / 4, nk), dim3(64, 4, 1), 0, m->stream,
+                       P, pa);
+    hipLaunchKernelGGL(k_pdef_z, grd, blk, 0, m->stream, P, pa, cg);
   }
   if (inplace) {
-    const float *src = m->thb;
-    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-      int c = IXD(P, i, j, k);
-      out_new[c] = src[c];
-    });
+    // the partner now holds the new field: exchange the two buffers
+    for (int id = 0; id < CM1G_NFIELDS; id++)
+      if (m->f[id] == out_new) {
+        m->f[id] = m->thb;
+        m->thb = out_new;
+        break;
+      }
   }
 }
 
 // ----------------------------------------------------------- advu/v/w
 // The momentum tendencies use the pre-acoustic mass fluxes rru/rrv/rrw
-// (model fields) and divx; uten/vten/wten are updated in place
-// (column kernels in cm1g_advm.hip).
-void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy) { advm_run(m, nrk, mask, buoy); }
+// (model fields) and divx; uten/vten/wten are updated in place, or, with
+// from_ten1, set to uten1/vten1/wten1 plus advection (column kernels in
+// cm1g_advm.hip).
+void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1) {
+  advm_run(m, nrk, mask, buoy, from_ten1);
+}

@@ -15,7 +15,8 @@
 struct AdvmArgs {
   const float *a;                    // advected velocity component
   const float *rru, *rrv, *rrw, *divx, *dum8;
-  float *ten;                        // uten / vten / wten (updated in place)
+  const float *ten_in;               // tendency the advection is added to
+  float *ten;                        // uten / vten / wten
   double weps;
   int buoy;                          // w: add the buoyancy term (src/solve2.F:899-907)
   int i1, i2, j1, j2;                // point range of the staggered grid
@@ -41,7 +42,7 @@ __device__ __forceinline__ void advm_load(const KP &P, const AdvmArgs &g, int i,
   L.xp1 = a[c + 1]; L.xp2 = a[c + 2]; L.xp3 = a[c + 3];
   L.ym3 = a[c - 3 * px]; L.ym2 = a[c - 2 * px]; L.ym1 = a[c - px];
   L.yp1 = a[c + px]; L.yp2 = a[c + 2 * px]; L.yp3 = a[c + 3 * px];
-  L.ten = g.ten[c];
+  L.ten = g.ten_in[c];
   if (STAG == 2) {
     // rru(i-1), rru(i), rru(i+1); rrv(i,j), rrv(i-1,j), rrv(i,j+1), rrv(i-1,j+1)
     L.r0 = g.rru[c - 1]; L.r1 = g.rru[c]; L.r2 = g.rru[c + 1];
@@ -231,9 +232,32 @@ __global__ void __launch_bounds__(256) k_advm_col(KP P, AdvmArgs g, ColGrid cg) 
   }
 }
 
+// With from_ten1 the tendencies start from uten1/vten1/wten1 (the copy of
+// src/solve2.F:459-471): the advection kernels read them, and this kernel
+// copies the points they do not write (the faces i=ni+1 / j=nj+1, the
+// open-boundary faces outside the advected range, and w at k=1).
+__global__ void __launch_bounds__(256) k_ten_edges(KP P, const float *u1, const float *v1,
+                                                   const float *w1, float *u, float *v, float *w,
+                                                   int ui1, int ui2, int vj1, int vj2) {
+  const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  if (i > ni + 1 || j > nj + 1) return;
+  const bool ucol = (j == nj + 1) || i < ui1 || i > ui2;
+  const bool vcol = (i == ni + 1) || j < vj1 || j > vj2;
+  const bool wcol = (i == ni + 1) || (j == nj + 1);
+  for (int k = 1; k <= nk; k++) {
+    const int c = IXD(P, i, j, k);
+    if (ucol) u[c] = u1[c];
+    if (vcol) v[c] = v1[c];
+    if (wcol || k == 1) w[c] = w1[c];
+    if (!(ucol || vcol || wcol)) break;
+  }
+}
+
 // uten/vten/wten += advection (the momentum part of adv_uvw); mask bit 1 u,
 // 2 v, 4 w; buoy adds the buoyancy term to wten.
-void advm_run(cm1g_model *m, int nrk, int mask, int buoy) {
+void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj;
   const bool weno = (m->c.advwenov == 1) || (m->c.advwenov == 2 && nrk == 3);
@@ -245,38 +269,48 @@ void advm_run(cm1g_model *m, int nrk, int mask, int buoy) {
   g.dum8 = m->f[CM1G_DUM8];
   g.weps = (double)(100.0f * P.K.epsilon);
   g.buoy = buoy;
-  const dim3 blk(256, 1, 1);
+  const int ui1 = P.ibw == 1 ? 2 : 1, ui2 = P.ibe == 1 ? ni : ni + 1;
+  const int vj1 = P.ibs == 1 ? 2 : 1, vj2 = P.ibn == 1 ? nj : nj + 1;
+  if (from_ten1) {
+    // the launch covers all three components, so every mask bit must be set
+    hipLaunchKernelGGL(k_ten_edges, dim3((ni + 1 + 63) / 64, (nj + 1 + 3) / 4, 1), dim3(64, 4, 1), 0,
+                       m->stream, P, m->f[CM1G_UTEN1], m->f[CM1G_VTEN1], m->f[CM1G_WTEN1],
+                       m->f[CM1G_UTEN], m->f[CM1G_VTEN], m->f[CM1G_WTEN], ui1, ui2, vj1, vj2);
+  }
   if (mask & 1) {
     g.a = m->f[CM1G_U3D];
     g.ten = m->f[CM1G_UTEN];
-    g.i1 = P.ibw == 1 ? 2 : 1;
-    g.i2 = P.ibe == 1 ? ni : ni + 1;
+    g.ten_in = from_ten1 ? m->f[CM1G_UTEN1] : g.ten;
+    g.i1 = ui1;
+    g.i2 = ui2;
     g.j1 = 1;
     g.j2 = nj;
     const ColGrid cg = col_grid(ni + 1, nj);
     auto *kern = weno ? k_advm_col<2, true> : k_advm_col<2, false>;
-    hipLaunchKernelGGL(kern, dim3(cg.per * NXCD), blk, 0, m->stream, P, g, cg);
+    hipLaunchKernelGGL(kern, col_blocks(cg), col_threads(cg), 0, m->stream, P, g, cg);
   }
   if (mask & 2) {
     g.a = m->f[CM1G_V3D];
     g.ten = m->f[CM1G_VTEN];
+    g.ten_in = from_ten1 ? m->f[CM1G_VTEN1] : g.ten;
     g.i1 = 1;
     g.i2 = ni;
-    g.j1 = P.ibs == 1 ? 2 : 1;
-    g.j2 = P.ibn == 1 ? nj : nj + 1;
+    g.j1 = vj1;
+    g.j2 = vj2;
     const ColGrid cg = col_grid(ni, nj + 1);
     auto *kern = weno ? k_advm_col<3, true> : k_advm_col<3, false>;
-    hipLaunchKernelGGL(kern, dim3(cg.per * NXCD), blk, 0, m->stream, P, g, cg);
+    hipLaunchKernelGGL(kern, col_blocks(cg), col_threads(cg), 0, m->stream, P, g, cg);
   }
   if (mask & 4) {
     g.a = m->f[CM1G_W3D];
     g.ten = m->f[CM1G_WTEN];
+    g.ten_in = from_ten1 ? m->f[CM1G_WTEN1] : g.ten;
     g.i1 = 1;
     g.i2 = ni;
     g.j1 = 1;
     g.j2 = nj;
     const ColGrid cg = col_grid(ni, nj);
     auto *kern = weno ? k_advm_col<4, true> : k_advm_col<4, false>;
-    hipLaunchKernelGGL(kern, dim3(cg.per * NXCD), blk, 0, m->stream, P, g, cg);
+    hipLaunchKernelGGL(kern, col_blocks(cg), col_threads(cg), 0, m->stream, P, g, cg);
   }
 }

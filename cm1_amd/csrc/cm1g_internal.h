@@ -55,7 +55,6 @@ struct cm1g_model {
   float *dtu0, *dtv0;           // 2-D open-boundary masks (src/param.F:9182-9218)
   // scratch (device)
   float *sx, *sy, *sz, *tx, *ty, *tz;   // advection fluxes / pdef work
-  float *rrus, *rrvs, *rrws;    // time-averaged mass fluxes scaled by rho0 (getdivx)
   float *thb;                   // ping-pong partner for th3d / q3d updates
   float *scr1, *scr2;           // Thomas coefficients
   double *dred;                 // reduction workspace (per-k partial sums etc.)
@@ -66,7 +65,7 @@ struct cm1g_model {
   size_t ntot;
   double mass1, train;
   float cflmax;
-  int nth_buf;                  // unused
+  int advs_unscaled;            // cm1g_advs: use rru/rrv/rrw as given (no rho0/rf0 scaling)
   // kernel timer: selected kernel id and the event pairs recorded so far
   int kt_which;
   std::vector<hipEvent_t> kt_ev;
@@ -123,31 +122,35 @@ inline void for2(hipStream_t s, int ilo, int ihi, int jlo, int jhi, F f) {
   hipLaunchKernelGGL(k_for2<F>, g, b, 0, s, f, ilo, ni, jlo, nj);
 }
 
-// Column kernels run on a 1-D grid of 64x4-column tiles.  The hardware
-// hands consecutive workgroups to the 8 XCDs in turn (each with its own L2);
-// this maps the workgroup id to a tile so that every XCD owns one
-// contiguous band of tile rows, walked in order, and the tiles that share
-// halo rows run on the same XCD at about the same time.  The grid is padded
-// to a multiple of 8 workgroups; tiles past the end return at once.
+// Column kernels run on a 1-D grid of 64 x ty column tiles (one wavefront
+// per tile row).  The hardware hands consecutive workgroups to the 8 XCDs in
+// turn (each with its own L2); this maps the workgroup id to a tile so that
+// every XCD owns one contiguous band of tile rows, walked in order, and the
+// tiles that share halo rows run on the same XCD at about the same time.
+// The grid is padded to a multiple of 8 workgroups; tiles past the end
+// return at once.
 constexpr int NXCD = 8;
 struct ColGrid {
-  int gx, gy, nb, per;  // tiles in i and j, tiles, tiles per XCD
+  int gx, gy, ty, nb, per;  // tiles in i and j, tile rows, tiles, tiles per XCD
 };
-inline ColGrid col_grid(int ni, int nj) {
+inline ColGrid col_grid(int ni, int nj, int ty = 4) {
   ColGrid g;
+  g.ty = ty;
   g.gx = (ni + 63) / 64;
-  g.gy = (nj + 3) / 4;
+  g.gy = (nj + ty - 1) / ty;
   g.nb = g.gx * g.gy;
   g.per = (g.nb + NXCD - 1) / NXCD;
   return g;
 }
+inline dim3 col_blocks(const ColGrid &g) { return dim3(g.per * NXCD, 1, 1); }
+inline dim3 col_threads(const ColGrid &g) { return dim3(64 * g.ty, 1, 1); }
 // returns false for padding workgroups
 __device__ __forceinline__ bool col_tile(const ColGrid &g, int &i, int &j) {
   const int b = (int)blockIdx.x;
   const int t = (b % NXCD) * g.per + b / NXCD;
   if (t >= g.nb) return false;
   i = 1 + (t % g.gx) * 64 + (int)(threadIdx.x & 63);
-  j = 1 + (t / g.gx) * 4 + (int)(threadIdx.x >> 6);
+  j = 1 + (t / g.gx) * g.ty + (int)(threadIdx.x >> 6);
   return true;
 }
 
@@ -158,8 +161,8 @@ void bc_fill_n(cm1g_model *m, int kind, float *const *arrs, int n);
 void radbc4(cm1g_model *m, const float *ua, const float *u3d, const float *va,
             const float *v3d, float dt);
 // advection (cm1g_adv.hip)
-void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy);
-void advm_run(cm1g_model *m, int nrk, int mask, int buoy);  // cm1g_advm.hip
+void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1 = 0);
+void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1);  // cm1g_advm.hip
 void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float *s0,
               const float *sten0, const float *thterm_div, float *out_sten, float *out_new,
               int pdef, int pdefweno, float dt, double weps, int smeps_flush);

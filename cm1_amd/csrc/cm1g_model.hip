@@ -105,8 +105,7 @@ extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
   ARGCHK(cfg->nsound >= 1 && cfg->nsound <= 192, "cm1g_create: nsound out of range");
   ARGCHK(cfg->npt >= 0 && cfg->npt <= 8, "cm1g_create: npt must be 0..8");
   ARGCHK(cfg->dx > 0 && cfg->dy > 0 && cfg->dz > 0 && cfg->dt > 0, "cm1g_create: dx,dy,dz,dt must be > 0");
-  cm1g_model *m = new cm1g_model();
-  memset((void *)m, 0, sizeof(cm1g_model));
+  cm1g_model *m = new cm1g_model();  // value-initialized: pointers null, counters zero
   m->c = *cfg;
   if (m->c.csmax == 0.0f) m->c.csmax = 350.0f;
   KP &P = m->P;
@@ -143,7 +142,7 @@ extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
   for (int i = 0; i < CM1G_D_NFIELDS && !rc; i++) rc = dalloc(&m->d2[i], (size_t)P.pxy + (size_t)(P.px + P.py) * pz);
   if (!rc) rc = dalloc(&m->dtu0, P.pxy);
   if (!rc) rc = dalloc(&m->dtv0, P.pxy);
-  float **scr[] = {&m->sx, &m->sy, &m->sz, &m->tx, &m->ty, &m->tz, &m->rrus, &m->rrvs, &m->rrws,
+  float **scr[] = {&m->sx, &m->sy, &m->sz, &m->tx, &m->ty, &m->tz,
                    &m->scr1, &m->scr2, &m->thb};
   for (auto p : scr)
     if (!rc) rc = dalloc(p, m->ntot);
@@ -213,8 +212,8 @@ extern "C" int cm1g_destroy(cm1g_model *m) {
   for (int i = 0; i < CM1G_K_NFIELDS; i++) { if (m->kf[i]) hipFree(m->kf[i]); free(m->hk[i]); }
   for (int i = 0; i < 6; i++) if (m->h[i]) hipFree(m->h[i]);
   for (int i = 0; i < CM1G_D_NFIELDS; i++) if (m->d2[i]) hipFree(m->d2[i]);
-  float *scr[] = {m->dtu0, m->dtv0, m->sx, m->sy, m->sz, m->tx, m->ty, m->tz, m->rrus, m->rrvs,
-                  m->rrws, m->scr1, m->scr2, m->thb, m->fred};
+  float *scr[] = {m->dtu0, m->dtv0, m->sx, m->sy, m->sz, m->tx, m->ty, m->tz,
+                  m->scr1, m->scr2, m->thb, m->fred};
   for (auto p : scr) if (p) hipFree(p);
   if (m->dred) hipFree(m->dred);
   if (m->dscal) hipFree(m->dscal);
@@ -643,11 +642,14 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
   float *uten = f[CM1G_UTEN], *vten = f[CM1G_VTEN], *wten = f[CM1G_WTEN];
   const float *uten1 = f[CM1G_UTEN1], *vten1 = f[CM1G_VTEN1], *wten1 = f[CM1G_WTEN1];
   float *rru = f[CM1G_RRU], *rrv = f[CM1G_RRV], *rrw = f[CM1G_RRW], *divx = f[CM1G_DIVX];
-  float *th3d = f[CM1G_TH3D], *pp3d = f[CM1G_PP3D], *sadv = f[CM1G_SADV];
+  float *pp3d = f[CM1G_PP3D], *sadv = f[CM1G_SADV];
   float *thterm = f[CM1G_THTERM], *dum8 = f[CM1G_DUM8], *t11 = f[CM1G_T11], *t22 = f[CM1G_T22];
   float *rho = f[CM1G_RHO], *prs = f[CM1G_PRS];
-  const float *qv = f[CM1G_Q3D0], *qc = f[CM1G_Q3D1], *qr = f[CM1G_Q3D2];
   for (int nrk = 1; nrk <= 3; nrk++) {
+    // the scalar updates swap th3d / q3d with a partner array: fetch the
+    // current buffers at the top of the stage and again after the updates
+    float *th3d = f[CM1G_TH3D];
+    const float *qv = f[CM1G_Q3D0], *qc = f[CM1G_Q3D1], *qr = f[CM1G_Q3D2];
     const float dttmp = dt / (float)(3 + 1 - nrk);
     // thermodynamic prep (src/solve2.F:302-450)
     for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
@@ -673,13 +675,6 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
       }
     });
     bc_fill(m, BC_P, t11);
-    // tendencies start from the pre-RK values (src/solve2.F:459-471)
-    for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
-      int c = IXD(P, i, j, k);
-      uten[c] = uten1[c];
-      vten[c] = vten1[c];
-      wten[c] = wten1[c];
-    });
     // rru, rrv, rrw (src/solve2.F:490-545)
     for3(m->stream, 0, ni + 2, 0, nj + 2, 1, nk, [=] __device__(int i, int j, int k) {
       int c = IXD(P, i, j, k);
@@ -703,7 +698,8 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
       if (fabsf(d) < P.K.smeps) d = 0.0f;
       divx[c] = d;
     });
-    adv_uvw(m, nrk, 7, 1);
+    // tendencies: the pre-RK values (src/solve2.F:459-471) plus advection
+    adv_uvw(m, nrk, 7, 1, 1);
     // sadv = pi0 + pi' (src/solve2.F:1027-1050)
     for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
       int c = IXD(P, i, j, k);
@@ -725,6 +721,7 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
                  (double)(0.01f * P.K.epsilon), 1);
       }
     }
+    th3d = f[CM1G_TH3D];
     // pressure and density (src/solve2.F:1536-1656)
     if (nrk == 3) {
       float *qvw = f[CM1G_Q3D0], *qcw = f[CM1G_Q3D1];
@@ -923,13 +920,11 @@ extern "C" int cm1g_advs(cm1g_model *m, int nrk, int s_id, int s0_id, int sten_i
   ARGCHK(field_ok(m, s_id) && field_ok(m, s0_id) && field_ok(m, sten_id), "cm1g_advs: bad field id");
   ARGCHK(nrk >= 1 && nrk <= 3, "cm1g_advs: nrk must be 1..3");
   // the model's rru/rrv/rrw are used as given (no rho0 scaling here)
-  const size_t bytes = m->ntot * sizeof(float);
-  HIPCHK(hipMemcpyAsync(m->rrus, m->f[CM1G_RRU], bytes, hipMemcpyDeviceToDevice, m->stream));
-  HIPCHK(hipMemcpyAsync(m->rrvs, m->f[CM1G_RRV], bytes, hipMemcpyDeviceToDevice, m->stream));
-  HIPCHK(hipMemcpyAsync(m->rrws, m->f[CM1G_RRW], bytes, hipMemcpyDeviceToDevice, m->stream));
   float *sten = m->f[sten_id];
+  m->advs_unscaled = 1;
   advs_run(m, nrk, m->f[s_id], 0.0f, m->f[s0_id], sten, nullptr, sten, nullptr, pdef, pdefweno, dt,
            weps, 0);
+  m->advs_unscaled = 0;
   return check_launch();
 }
 extern "C" int cm1g_advu(cm1g_model *m, int nrk) {

@@ -6,6 +6,11 @@
 // (src/misclibs.F:2726-2814).
 #include "cm1g_internal.h"
 
+// rows of columns per acoustic-substep workgroup (64 columns each)
+#ifndef CM1G_SOUND_TY
+#define CM1G_SOUND_TY 4
+#endif
+
 static void sound_nloop(int nrk, int nsound, float dt, int *nloop_out, float *dts_out) {
   int nloop;
   float dts;
@@ -39,6 +44,12 @@ struct SubArgs {
   const float *uten, *vten, *wten, *thv, *piadv, *ppterm, *ppten, *dtu0, *dtv0;
   const float *radbcw, *radbce, *radbcs, *radbcn;
   float *w3d, *pp3d, *fpk, *dum1, *dum2, *rru, *rrv, *rrw;
+  // first substep: w and pi' are read from wR/ppR (wa/ppi after the first
+  // RK stage) and the averages start from 0 (u, v) and beta*wa (w), which
+  // is what the reference's copies before the substep loop set
+  // (src/sound.F:191-250)
+  const float *wR, *ppR, *wa;
+  int first;
 };
 
 // new u at (i,j,k): radiative open-boundary update (ssopenbcw/e,
@@ -78,11 +89,17 @@ __device__ __forceinline__ float sub_v(const KP &P, const SubArgs &a, int i, int
 // store a new u (v) and fold it into the substep average (src/sound.F:806-835)
 __device__ __forceinline__ void put_u(const SubArgs &a, int c, float u) {
   a.uY[c] = u;
-  a.rru[c] = a.last ? (a.rru[c] + u) * a.tavg : a.rru[c] + u;
+  const float r = a.first ? 0.0f : a.rru[c];
+  a.rru[c] = a.last ? (r + u) * a.tavg : r + u;
 }
 __device__ __forceinline__ void put_v(const SubArgs &a, int c, float v) {
   a.vY[c] = v;
-  a.rrv[c] = a.last ? (a.rrv[c] + v) * a.tavg : a.rrv[c] + v;
+  const float r = a.first ? 0.0f : a.rrv[c];
+  a.rrv[c] = a.last ? (r + v) * a.tavg : r + v;
+}
+__device__ __forceinline__ void put_w_avg(const SubArgs &a, int c, float w) {
+  const float r = a.first ? a.wa[c] * a.beta : a.rrw[c];
+  a.rrw[c] = a.last ? (r + w * a.alph) * a.tavg : r + w;
 }
 
 // The averaging ring of one substep: u and v on the faces i = ni+1 and
@@ -108,7 +125,7 @@ __global__ void __launch_bounds__(256) k_sound_ring(KP P, SubArgs a) {
   const int c = IXD(P, i, j, k);
   put_u(a, c, sub_u(P, a, i, j, k));
   put_v(a, c, sub_v(P, a, i, j, k));
-  a.rrw[c] = a.last ? (a.rrw[c] + a.w3d[c] * a.alph) * a.tavg : a.rrw[c] + a.w3d[c];
+  put_w_avg(a, c, a.wR[c]);
 }
 
 // One acoustic substep for a column (i,j): u/v forward step (also the
@@ -124,7 +141,7 @@ __global__ void __launch_bounds__(256) k_sound_ring(KP P, SubArgs a) {
 // Each level issues all of its loads before any store, so a level costs one
 // memory round trip; the downward sweep also prefetches the next level's
 // operands before storing the current one.
-__global__ void __launch_bounds__(256) k_sound_sub(KP P, SubArgs a, ColGrid cg) {
+__global__ void __launch_bounds__(64 * CM1G_SOUND_TY) k_sound_sub(KP P, SubArgs a, ColGrid cg) {
   int i, j;
   if (!col_tile(cg, i, j)) return;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
@@ -147,7 +164,7 @@ __global__ void __launch_bounds__(256) k_sound_sub(KP P, SubArgs a, ColGrid cg) 
   int c = IXD(P, i, j, 1);
   float pa_m = a.piadv[c - pxy], pa_c = a.piadv[c];
   float th_m = a.thv[c - pxy];
-  float w_c = a.w3d[c];
+  float w_c = a.wR[c];
   for (int k = 1; k <= nk; k++, c += pxy) {
     // ---- loads
     const float uXc = a.uX[c], uXe = a.uX[c + 1], utc = a.uten[c], ute = a.uten[c + 1];
@@ -156,11 +173,11 @@ __global__ void __launch_bounds__(256) k_sound_sub(KP P, SubArgs a, ColGrid cg) 
                 pdn = a.ppdX[c + px];
     const float thc = a.thv[c], thw = a.thv[c - 1], the = a.thv[c + 1], ths = a.thv[c - px],
                 thn = a.thv[c + px];
-    const float rruc = a.rru[c], rrvc = a.rrv[c];
+    const float rruc = a.first ? 0.0f : a.rru[c], rrvc = a.first ? 0.0f : a.rrv[c];
     const float paw = a.piadv[c - 1], pae = a.piadv[c + 1], pas = a.piadv[c - px], pan = a.piadv[c + px];
     const float pa_p = a.piadv[c + pxy];
-    const float ppterm = a.ppterm[c], ppten = a.ppten[c], wten = a.wten[c], pp_c = a.pp3d[c];
-    const float w_p = a.w3d[c + pxy];
+    const float ppterm = a.ppterm[c], ppten = a.ppten[c], wten = a.wten[c], pp_c = a.ppR[c];
+    const float w_p = (k == nk) ? a.w3d[c + pxy] : a.wR[c + pxy];
     float rw = 0.0f, re = 0.0f, rs = 0.0f, rn = 0.0f;
     if (bw) rw = a.radbcw[(j + 2) + k * P.py];
     if (be) re = a.radbce[(j + 2) + k * P.py];
@@ -242,7 +259,8 @@ __global__ void __launch_bounds__(256) k_sound_sub(KP P, SubArgs a, ColGrid cg) 
   float pa_c2 = a.piadv[c];
   // operands of the current level
   float q_d1 = 0.0f, q_d2 = 0.0f;
-  float q_pp = a.pp3d[c], q_fpk = a.fpk[c], q_pm = a.piadv[c - pxy], q_term = a.ppterm[c], q_rrw = a.rrw[c];
+  float q_pp = a.ppR[c], q_fpk = a.fpk[c], q_pm = a.piadv[c - pxy], q_term = a.ppterm[c];
+  float q_rrw = a.first ? a.wa[c] * a.beta : a.rrw[c];
   for (int k = nk; k >= 1; k--, c -= pxy) {
     const float p_old = q_pp, fk = q_fpk, pa_m2 = q_pm, pterm = q_term, rrwc = q_rrw;
     float wk;
@@ -260,11 +278,11 @@ __global__ void __launch_bounds__(256) k_sound_sub(KP P, SubArgs a, ColGrid cg) 
         q_d1 = a.dum1[cb];
         q_d2 = a.dum2[cb];
       }
-      q_pp = a.pp3d[cb];
+      q_pp = a.ppR[cb];
       q_fpk = a.fpk[cb];
       q_pm = a.piadv[cb - pxy];
       q_term = a.ppterm[cb];
-      q_rrw = a.rrw[cb];
+      q_rrw = a.first ? a.wa[cb] * a.beta : a.rrw[cb];
     }
     const float tem = dts * rdz * P.mh[k];
     const float r2 = dts * rdz * P.mh[k] * P.rr0[k] * P.rf0[k + 1];
@@ -276,7 +294,7 @@ __global__ void __launch_bounds__(256) k_sound_sub(KP P, SubArgs a, ColGrid cg) 
     float pn = p_old + fk + alph * (wk * (pk1 + qk) + w_above * (pk2 - qk));
     if (fabsf(pn) < smeps) pn = 0.0f;
     const float d = a.kdiv * (pn - p_old);
-    if (k >= 2) a.w3d[c] = wk;
+    if (k >= 2 || a.wR != a.w3d) a.w3d[c] = wk;
     a.pp3d[c] = pn;
     if (a.last) a.dum1[c] = d;
     a.ppdY[c] = pn + d;
@@ -302,29 +320,6 @@ void sound_run(cm1g_model *m, int nrk) {
   float *rru = m->f[CM1G_RRU], *rrv = m->f[CM1G_RRV], *rrw = m->f[CM1G_RRW];
   const float *ppi = m->f[CM1G_PPI];
   float *pp3d = m->f[CM1G_PP3D], *ppx = m->f[CM1G_PPX];
-
-  // prep copies (src/sound.F:191-250)
-  if (nrk == 1) {
-    for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
-      int c = IXD(P, i, j, k);
-      ppd[c] = ppx[c];
-      rru[c] = 0.0f;
-      rrv[c] = 0.0f;
-      rrw[c] = wa[c] * beta;
-    });
-  } else {
-    for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
-      int c = IXD(P, i, j, k);
-      u3d[c] = ua[c];
-      rru[c] = 0.0f;
-      v3d[c] = va[c];
-      rrv[c] = 0.0f;
-      w3d[c] = wa[c];
-      rrw[c] = wa[c] * beta;
-      pp3d[c] = ppi[c];
-      ppd[c] = ppx[c];
-    });
-  }
 
   SubArgs a;
   a.dts = dts;
@@ -362,27 +357,50 @@ void sound_run(cm1g_model *m, int nrk) {
   a.rru = rru;
   a.rrv = rrv;
   a.rrw = rrw;
-  // u, v and ppd alternate between the state arrays and scratch arrays
-  float *ub[2] = {u3d, m->scr1}, *vb[2] = {v3d, m->scr2}, *pb[2] = {ppd, m->tx};
-  const ColGrid cg = col_grid(ni, nj);
-  const dim3 blk(256, 1, 1), grd(cg.per * NXCD, 1, 1);
+  // The copies the reference makes before the substeps (u3d=ua, ..., ppd=ppx,
+  // src/sound.F:191-250) are folded into the first substep, which reads ua,
+  // va, wa, ppi, ppx directly (after the first RK stage) and starts the
+  // averages itself.  u, v and ppd alternate between the state arrays and
+  // scratch arrays so that neighbouring columns read the old values; the
+  // last substep writes u3d/v3d.
+  const bool fold = (nrk >= 2);
+  a.wa = wa;
+  const ColGrid cg = col_grid(ni, nj, CM1G_SOUND_TY);
+  const dim3 blk = col_threads(cg), grd = col_blocks(cg);
   const dim3 ring_grd((ni + nj + 1 + 255) / 256, nk, 1);
+  float *ppdb[2] = {ppd, m->tx};
+  float *uprev = nullptr, *vprev = nullptr;
   for (int n = 1; n <= nloop; n++) {
-    const int src = (n - 1) & 1;
-    a.uX = ub[src];
-    a.vX = vb[src];
-    a.uY = ub[src ^ 1];
-    a.vY = vb[src ^ 1];
-    a.ppdX = pb[src];
-    a.ppdY = pb[src ^ 1];
+    // destination of this substep: u3d on the last one, alternating before
+    float *udst, *vdst;
+    if (fold) {
+      const bool scratch = ((nloop - n) & 1) != 0;
+      udst = scratch ? m->scr1 : u3d;
+      vdst = scratch ? m->scr2 : v3d;
+    } else {
+      const bool scratch = (n & 1) != 0;
+      udst = scratch ? m->scr1 : u3d;
+      vdst = scratch ? m->scr2 : v3d;
+    }
+    a.uX = (n == 1) ? (fold ? ua : u3d) : uprev;
+    a.vX = (n == 1) ? (fold ? va : v3d) : vprev;
+    a.uY = udst;
+    a.vY = vdst;
+    a.ppdX = (n == 1) ? ppx : ppdb[(n - 2) & 1];
+    a.ppdY = ppdb[(n - 1) & 1];
+    a.wR = (n == 1 && fold) ? wa : w3d;
+    a.ppR = (n == 1 && fold) ? ppi : pp3d;
+    a.first = (n == 1);
     a.last = (n == nloop);
-    ktimer_mark(m, CM1G_KT_SOUND_SUBSTEP);
     hipLaunchKernelGGL(k_sound_ring, ring_grd, dim3(256), 0, m->stream, P, a);
+    ktimer_mark(m, CM1G_KT_SOUND_SUBSTEP);
     hipLaunchKernelGGL(k_sound_sub, grd, blk, 0, m->stream, P, a, cg);
     ktimer_mark(m, CM1G_KT_SOUND_SUBSTEP);
-    if (n < nloop) bc_fill(m, BC_P, pb[src ^ 1]);
+    if (n < nloop) bc_fill(m, BC_P, a.ppdY);
+    uprev = udst;
+    vprev = vdst;
   }
-  if (nloop & 1) {
+  if (!fold && (nloop & 1)) {
     // the last substep wrote the scratch pair: copy the updated range back
     const float *us = m->scr1, *vs = m->scr2;
     for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
@@ -544,23 +562,15 @@ void sounde_run(cm1g_model *m, int nrk) {
     });
 }
 
-// getdiv (t11 from the time-averaged velocities) and getdivx (mass fluxes
-// scaled by rho0/rf0 into rrus/rrvs/rrws, and divx from them).
+// getdiv (t11 from the time-averaged velocities) and getdivx (divx from the
+// mass fluxes scaled by rho0/rf0; the advection kernels apply the same
+// scaling on the fly instead of storing scaled copies).
 void getdiv_both(cm1g_model *m, int want_t11) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   const float *rru = m->f[CM1G_RRU], *rrv = m->f[CM1G_RRV], *rrw = m->f[CM1G_RRW];
-  float *rrus = m->rrus, *rrvs = m->rrvs, *rrws = m->rrws;
   float *t11 = m->f[CM1G_T11], *divx = m->f[CM1G_DIVX];
   const float smeps = P.K.smeps;
-  for3(m->stream, 0, ni + 2, 0, nj + 2, 1, nk + 1, [=] __device__(int i, int j, int k) {
-    int c = IXD(P, i, j, k);
-    if (k <= nk) {
-      rrus[c] = rru[c] * P.rho0[k];
-      rrvs[c] = rrv[c] * P.rho0[k];
-    }
-    rrws[c] = (k == 1 || k == nk + 1) ? 0.0f : rrw[c] * P.rf0[k];
-  });
   for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
     int c = IXD(P, i, j, k);
     if (want_t11) {
