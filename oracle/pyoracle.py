@@ -10,6 +10,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcm1oracle.so")
+# the same restatement carried out in double precision (REAL -> REAL(8), the
+# analogue of the reference's -DDP build)
+LIB64_PATH = os.path.join(HERE, "libcm1oracle64.so")
 
 FIELDS = ["ua", "u3d", "uten", "uten1", "va", "v3d", "vten", "vten1",
           "wa", "w3d", "wten", "wten1", "rru", "rrv", "rrw",
@@ -32,20 +35,28 @@ KID = {n: i for i, n in enumerate(KFIELDS)}
 D2 = {"rain": 0, "prate": 1, "radbcw": 2, "radbce": 3, "radbcs": 4, "radbcn": 5}
 
 
+_INTS = ("ni", "nj", "nk", "wbc", "ebc", "sbc", "nbc", "imoist", "ptype", "psolver",
+         "advwenos", "advwenov", "apmasscon", "irbc", "nsound", "npt", "pdtra")
+_REALS = ("dx", "dy", "dz", "dt", "alph", "kdiv", "csmax")
+
+
 class Config(C.Structure):
-    _fields_ = [(n, C.c_int32) for n in (
-        "ni", "nj", "nk", "wbc", "ebc", "sbc", "nbc", "imoist", "ptype", "psolver",
-        "advwenos", "advwenov", "apmasscon", "irbc", "nsound", "npt", "pdtra")] + \
-        [(n, C.c_float) for n in ("dx", "dy", "dz", "dt", "alph", "kdiv", "csmax")]
+    _fields_ = [(n, C.c_int32) for n in _INTS] + [(n, C.c_float) for n in _REALS]
 
 
-def _load():
-    if not os.path.exists(LIB_PATH):
-        raise FileNotFoundError(f"{LIB_PATH} not built (run make -C oracle)")
-    lib = C.CDLL(LIB_PATH)
-    vp, fp = C.c_void_p, C.POINTER(C.c_float)
+class Config64(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in _INTS] + [(n, C.c_double) for n in _REALS]
+
+
+def _load(fp64=False):
+    path = LIB64_PATH if fp64 else LIB_PATH
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} not built (run make -C oracle)")
+    lib = C.CDLL(path)
+    real = C.c_double if fp64 else C.c_float
+    vp, fp = C.c_void_p, C.POINTER(real)
     lib.cm1o_create.restype = vp
-    lib.cm1o_create.argtypes = [C.POINTER(Config)]
+    lib.cm1o_create.argtypes = [C.POINTER(Config64 if fp64 else Config)]
     lib.cm1o_destroy.argtypes = [vp]
     for name in ("cm1o_field", "cm1o_kfield", "cm1o_2dfield", "cm1o_hfield"):
         getattr(lib, name).restype = fp
@@ -61,20 +72,19 @@ def _load():
     lib.cm1o_advs.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
     for name in ("cm1o_advu", "cm1o_advv", "cm1o_advw"):
         getattr(lib, name).argtypes = [vp, C.c_int]
-    lib.cm1o_sound.argtypes = [vp, C.c_int, C.c_float]
+    lib.cm1o_sound.argtypes = [vp, C.c_int, real]
     lib.cm1o_kessler_branch.argtypes = [vp]
     lib.cm1o_set_libm.argtypes = [C.c_int]
     return lib
 
 
-_LIB = None
+_LIB = {}
 
 
-def lib():
-    global _LIB
-    if _LIB is None:
-        _LIB = _load()
-    return _LIB
+def lib(fp64=False):
+    if fp64 not in _LIB:
+        _LIB[fp64] = _load(fp64)
+    return _LIB[fp64]
 
 
 def make_config(**kw):
@@ -90,11 +100,15 @@ class Oracle:
     """One oracle model; arrays are numpy views shaped (k, j, i) with
     index offsets i0=2, j0=2 (i=-2 at index 0), k0=0."""
 
-    def __init__(self, cfg, libm=0):
+    def __init__(self, cfg, libm=0, fp64=False):
         """libm: 0 evaluates exp/pow with the host C library (the
         reference's rounding), 1 in double precision rounded once (the HIP
-        kernels' rounding)."""
-        self.L = lib()
+        kernels' rounding).  fp64: run the double-precision build; cfg is
+        then converted to a Config64 and the array views are float64."""
+        self.L = lib(fp64)
+        self.fp64 = fp64
+        if fp64 and not isinstance(cfg, Config64):
+            cfg = Config64(**{n: getattr(cfg, n) for n, _ in Config64._fields_})
         self.cfg = cfg
         self.libm = libm
         self.h = self.L.cm1o_create(C.byref(cfg))

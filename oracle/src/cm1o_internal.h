@@ -1,6 +1,37 @@
 /* Internal declarations of the CPU oracle (test infrastructure only). */
 #ifndef CM1O_INTERNAL_H
 #define CM1O_INTERNAL_H
+
+/* CM1O_FP64: the same restatement carried out in double precision, the
+ * analogue of the reference's -DDP build (every REAL becomes REAL(8),
+ * src/Makefile:73-76: -r8 with -DDP).  The Makefile builds it from
+ * copies of the sources with the 'f' suffix taken off every literal, so
+ * constants are double too.  System headers are read before 'float' is
+ * redefined. */
+#ifdef CM1O_FP64
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+static inline double o64_fabs(double x) { return fabs(x); }
+static inline double o64_fmax(double x, double y) { return fmax(x, y); }
+static inline double o64_fmin(double x, double y) { return fmin(x, y); }
+static inline double o64_sqrt(double x) { return sqrt(x); }
+static inline double o64_exp(double x) { return exp(x); }
+static inline double o64_pow(double x, double y) { return pow(x, y); }
+static inline double o64_copysign(double x, double y) { return copysign(x, y); }
+static inline long o64_lround(double x) { return lround(x); }
+#define float double
+#define fabsf o64_fabs
+#define fmaxf o64_fmax
+#define fminf o64_fmin
+#define sqrtf o64_sqrt
+#define expf o64_exp
+#define powf o64_pow
+#define copysignf o64_copysign
+#define lroundf o64_lround
+#endif
+
 #include "cm1o.h"
 #include <math.h>
 

@@ -30,11 +30,12 @@ def case(ni=40, nj=36, nk=30, dx=1000.0, dy=1000.0, dz=500.0, dt=6.0, bc=2, imoi
     return cfgd, cols, fields
 
 
-def make_oracle(cfgd, cols, fields, libm=0):
+def make_oracle(cfgd, cols, fields, libm=0, fp64=False):
     """libm=0: exp/pow rounded as the reference (host C library); libm=1:
-    as the HIP kernels (double precision, rounded once)."""
+    as the HIP kernels (double precision, rounded once).  fp64: the
+    double-precision build, started from the same (single-precision) state."""
     cfg = pyoracle.make_config(**cfgd)
-    o = pyoracle.Oracle(cfg, libm=libm)
+    o = pyoracle.Oracle(cfg, libm=libm, fp64=fp64)
     for n in ("zf", "zh", "mh", "mf"):
         o.kf(n)[:cfgd["nk"] + 3] = cols[n]
     o.L.cm1o_set_extrap(o.h, None)

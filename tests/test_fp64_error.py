@@ -1,0 +1,78 @@
+"""Error of both single-precision builds against a double-precision one.
+
+The reference's FP64 build (-DDP with -r8, src/Makefile:73-76) does not
+compile with this image's flang (module_ra_rrtmg_lw.f90 passes REAL(8)
+actuals to REAL(4) dummies), so the double-precision solution comes from
+the oracle's own FP64 build (libcm1oracle64.so: every REAL carried as
+double, constants included), started from the reference's t=0 state.
+
+Two single-precision solutions are measured against it after the
+fixture's 10 large steps:
+  - the reference FP32 build's own output (the golden fixture), and
+  - the oracle with the HIP kernels' exp/pow rounding, which the GPU
+    reproduces bit for bit (tests/test_gpu_reference.py asserts that).
+The GPU's arithmetic must be no less accurate than the reference's: its
+relative L2 distance from FP64 is bounded by twice the reference's (both
+are FP32 solutions whose rounding differs only in exp/pow; the factor
+covers the chaotic growth of that difference in convective cases).
+The FP64 build is pinned against the FP32 one on every fixture through
+the fields FP32 carries to ~1e-7 (rho, prs, ua): a type mistake in the
+double build (a value truncated to float, a float-only constant) leaves
+them far from the FP32 values or no closer than FP32 rounding.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from common import PROG, interior, load_fixture, make_oracle, nsound_for, rel_l2
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "ref_*.npz")))
+STAG = dict(PROG)
+
+
+def fp64_solution(cfgd, cols, fields, nsteps):
+    o = make_oracle(cfgd, cols, {n: a.astype(np.float64) for n, a in fields.items()}, fp64=True)
+    o.step(nsteps)
+    return o
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
+def test_fp32_builds_vs_fp64(path):
+    cfgd, cols, fields, nsteps, target, z = load_fixture(path)
+    cfgd["nsound"] = nsound_for(cfgd)
+    ni, nj, nk = cfgd["ni"], cfgd["nj"], cfgd["nk"]
+    o64 = fp64_solution(cfgd, cols, fields, nsteps)
+    assert o64.f("ua").dtype == np.float64
+    gpu_arith = make_oracle(cfgd, cols, fields, libm=1)
+    gpu_arith.step(nsteps)
+    for name, ref in target.items():
+        truth = interior(o64.f(name), nk, nj, ni, STAG[name])
+        g = interior(gpu_arith.f(name), nk, nj, ni, STAG[name])
+        e_ref, e_gpu = rel_l2(ref, truth), rel_l2(g, truth)
+        print(f"{name:4s} reference-FP32 vs FP64 {e_ref:.2e}   GPU-FP32 vs FP64 {e_gpu:.2e}")
+        assert e_gpu <= 2.0 * e_ref + 1.0e-7, (name, e_gpu, e_ref)
+        if name in ("rho", "prs", "ua"):
+            assert 0.0 < e_ref < 1.0e-5, (name, e_ref)
+
+
+def test_fp64_build_is_double():
+    """The double build rounds differently from the single one from the first
+    operation: one step of a small case differs from FP32 in every field it
+    changes, and by no more than FP32 rounding accumulates over a step."""
+    from common import case
+    cfgd, cols, fields = case(ni=16, nj=14, nk=12, nsound=6, seed=3)
+    o32 = make_oracle(cfgd, cols, fields)
+    o64 = make_oracle(cfgd, cols, {n: a.astype(np.float64) for n, a in fields.items()}, fp64=True)
+    o32.step(1)
+    o64.step(1)
+    nk, nj, ni = cfgd["nk"], cfgd["nj"], cfgd["ni"]
+    for name in ("tha", "rho", "prs"):
+        a = interior(o32.f(name), nk, nj, ni).astype(np.float64)
+        b = interior(o64.f(name), nk, nj, ni)
+        d = np.abs(a - b).max() / np.abs(b).max()
+        assert 0.0 < d < 1.0e-5, (name, d)
+        # the FP64 values are not FP32-representable
+        assert np.count_nonzero(b != b.astype(np.float32).astype(np.float64)) > b.size // 2
