@@ -76,7 +76,17 @@ def cpu_baseline(threads):
     if len(ts) > 1:
         ts = ts[1:]
     t = float(np.median(ts)) if ts else float("nan")
+    cpu_model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    cpu_model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return dict(value=ni * nj * nk / t, unit="cell-updates/s", cores=threads, kind="reference",
+                cpu=cpu_model,
                 sample=(f"reference CM1 FP32 OpenMP build, {ni}x{nj}x{nk} subdomain of the same case "
                         f"(dx=250 m, stretched dz, dt=1.5 s, Kessler, open BCs), {nsteps} steps, "
                         f"median per-step time {t:.3f} s excluding step 1; nsound chosen by the "

@@ -22,7 +22,10 @@ struct AdvmArgs {
   int i1, i2, j1, j2;                // point range of the staggered grid
 };
 
-// Operands of one level (raw loads).
+// Operands of one level (raw loads).  With OWN the advected component's
+// own mass flux (rru for u, rrv for v, rrw for w) is not loaded: inside
+// the time step it is rho0*u, rho0*v, rf0*w of the same array
+// (src/solve2.F:490-545), formed from the values already in registers.
 struct AdvmLevel {
   float xm3, xm2, xm1, xp1, xp2, xp3;  // a(i-3..i+3), centre excluded
   float ym3, ym2, ym1, yp1, yp2, yp3;  // a(j-3..j+3)
@@ -32,7 +35,7 @@ struct AdvmLevel {
   float wnew;                          // a(k+4), entering the vertical window
 };
 
-template <int STAG>
+template <int STAG, bool OWN>
 __device__ __forceinline__ void advm_load(const KP &P, const AdvmArgs &g, int i, int j, int k,
                                           AdvmLevel &L) {
   const int px = P.px, pxy = P.pxy, nk = P.nk;
@@ -45,7 +48,9 @@ __device__ __forceinline__ void advm_load(const KP &P, const AdvmArgs &g, int i,
   L.ten = g.ten_in[c];
   if (STAG == 2) {
     // rru(i-1), rru(i), rru(i+1); rrv(i,j), rrv(i-1,j), rrv(i,j+1), rrv(i-1,j+1)
-    L.r0 = g.rru[c - 1]; L.r1 = g.rru[c]; L.r2 = g.rru[c + 1];
+    if (!OWN) {
+      L.r0 = g.rru[c - 1]; L.r1 = g.rru[c]; L.r2 = g.rru[c + 1];
+    }
     L.r3 = g.rrv[c]; L.q0 = g.rrv[c - 1]; L.q1 = g.rrv[c + px];
     L.b8 = g.rrv[c + px - 1];
     // vertical face k+1: rrw(i,k+1), rrw(i-1,k+1)
@@ -55,7 +60,9 @@ __device__ __forceinline__ void advm_load(const KP &P, const AdvmArgs &g, int i,
     L.wnew = (k + 4 <= nk) ? a[c + 4 * pxy] : 0.0f;
   } else if (STAG == 3) {
     // rrv(j-1), rrv(j), rrv(j+1); rru(i,j), rru(i,j-1), rru(i+1,j), rru(i+1,j-1)
-    L.r0 = g.rrv[c - px]; L.r1 = g.rrv[c]; L.r2 = g.rrv[c + px];
+    if (!OWN) {
+      L.r0 = g.rrv[c - px]; L.r1 = g.rrv[c]; L.r2 = g.rrv[c + px];
+    }
     L.r3 = g.rru[c]; L.q0 = g.rru[c - px]; L.q1 = g.rru[c + 1];
     L.b8 = g.rru[c + 1 - px];
     L.wz0 = (k < nk) ? g.rrw[c + pxy] : 0.0f;
@@ -67,15 +74,17 @@ __device__ __forceinline__ void advm_load(const KP &P, const AdvmArgs &g, int i,
     L.r0 = g.rru[c]; L.r1 = g.rru[c + 1]; L.r2 = g.rrv[c]; L.r3 = g.rrv[c + px];
     L.q0 = L.q1 = 0.0f;
     // vertical face k (between w(k) and w(k+1)): rrw(k), rrw(k+1)
-    L.wz0 = g.rrw[c];
-    L.wz1 = g.rrw[c + pxy];
+    if (!OWN) {
+      L.wz0 = g.rrw[c];
+      L.wz1 = g.rrw[c + pxy];
+    }
     L.dv0 = g.divx[c]; L.dv1 = 0.0f;
     L.b8 = g.buoy ? g.dum8[c] : 0.0f;
     L.wnew = (k + 4 <= nk + 1) ? a[c + 4 * pxy] : 0.0f;
   }
 }
 
-template <int STAG, bool WENO>
+template <int STAG, bool WENO, bool OWN>
 __global__ void __launch_bounds__(256) k_advm_col(KP P, AdvmArgs g, ColGrid cg) {
   int i, j;
   if (!col_tile(cg, i, j)) return;
@@ -107,15 +116,31 @@ __global__ void __launch_bounds__(256) k_advm_col(KP P, AdvmArgs g, ColGrid cg) 
       const int o = kk - 2;
       return o == -2 ? w0 : o == -1 ? w1 : o == 0 ? w2 : o == 1 ? w3 : o == 2 ? w4 : w5;
     };
-    const float wb = 0.5f * (g.rrw[c1] + g.rrw[c1 + P.pxy]);
+    const float rw1 = OWN ? 0.0f : g.rrw[c1];
+    // the window holds a(0..5) here: w(2) = w2
+    const float rw2 = OWN ? ((nk >= 2) ? P.rf0[2] * w2 : 0.0f) : g.rrw[c1 + P.pxy];
+    const float wb = 0.5f * (rw1 + rw2);
     fz_lo = WENO ? vfluxw_of(P, 4, wb, S1, 1, 0, weps) : vflux5_of(P, 4, wb, S1, 1);
   }
   AdvmLevel nx;
-  advm_load<STAG>(P, g, i, j, k0, nx);
+  advm_load<STAG, OWN>(P, g, i, j, k0, nx);
   for (int k = k0; k <= nk; k++) {
-    const AdvmLevel L = nx;
-    if (k < nk) advm_load<STAG>(P, g, i, j, k + 1, nx);
+    AdvmLevel L = nx;
+    if (k < nk) advm_load<STAG, OWN>(P, g, i, j, k + 1, nx);
     const int c = IXD(P, i, j, k);
+    if (OWN) {
+      if (STAG == 2) {
+        const float r = P.rho0[k];
+        L.r0 = r * L.xm1; L.r1 = r * w2; L.r2 = r * L.xp1;
+      } else if (STAG == 3) {
+        const float r = P.rho0[k];
+        L.r0 = r * L.ym1; L.r1 = r * w2; L.r2 = r * L.yp1;
+      } else {
+        // the window holds a(k-2..k+3): w(k) = w2, w(k+1) = w3
+        L.wz0 = (k == 1) ? 0.0f : P.rf0[k] * w2;
+        L.wz1 = (k == nk) ? 0.0f : P.rf0[k + 1] * w3;
+      }
+    }
     float ax, ay, az, t;
     if (STAG == 2) {
       const float sc = w2;
@@ -256,8 +281,10 @@ __global__ void __launch_bounds__(256) k_ten_edges(KP P, const float *u1, const 
 }
 
 // uten/vten/wten += advection (the momentum part of adv_uvw); mask bit 1 u,
-// 2 v, 4 w; buoy adds the buoyancy term to wten.
-void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1) {
+// 2 v, 4 w; buoy adds the buoyancy term to wten; own forms each
+// component's own mass flux from the velocity (valid while rru/rrv/rrw
+// hold rho0*u, rho0*v, rf0*w, as between src/solve2.F:490-545 and sound).
+void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj;
   const bool weno = (m->c.advwenov == 1) || (m->c.advwenov == 2 && nrk == 3);
@@ -286,7 +313,8 @@ void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1) {
     g.j1 = 1;
     g.j2 = nj;
     const ColGrid cg = col_grid(ni + 1, nj);
-    auto *kern = weno ? k_advm_col<2, true> : k_advm_col<2, false>;
+    auto *kern = own ? (weno ? k_advm_col<2, true, true> : k_advm_col<2, false, true>)
+                     : (weno ? k_advm_col<2, true, false> : k_advm_col<2, false, false>);
     hipLaunchKernelGGL(kern, col_blocks(cg), col_threads(cg), 0, m->stream, P, g, cg);
   }
   if (mask & 2) {
@@ -298,7 +326,8 @@ void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1) {
     g.j1 = vj1;
     g.j2 = vj2;
     const ColGrid cg = col_grid(ni, nj + 1);
-    auto *kern = weno ? k_advm_col<3, true> : k_advm_col<3, false>;
+    auto *kern = own ? (weno ? k_advm_col<3, true, true> : k_advm_col<3, false, true>)
+                     : (weno ? k_advm_col<3, true, false> : k_advm_col<3, false, false>);
     hipLaunchKernelGGL(kern, col_blocks(cg), col_threads(cg), 0, m->stream, P, g, cg);
   }
   if (mask & 4) {
@@ -310,7 +339,8 @@ void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1) {
     g.j1 = 1;
     g.j2 = nj;
     const ColGrid cg = col_grid(ni, nj);
-    auto *kern = weno ? k_advm_col<4, true> : k_advm_col<4, false>;
+    auto *kern = own ? (weno ? k_advm_col<4, true, true> : k_advm_col<4, false, true>)
+                     : (weno ? k_advm_col<4, true, false> : k_advm_col<4, false, false>);
     hipLaunchKernelGGL(kern, col_blocks(cg), col_threads(cg), 0, m->stream, P, g, cg);
   }
 }

@@ -161,15 +161,16 @@ void bc_fill_n(cm1g_model *m, int kind, float *const *arrs, int n);
 void radbc4(cm1g_model *m, const float *ua, const float *u3d, const float *va,
             const float *v3d, float dt);
 // advection (cm1g_adv.hip)
-void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1 = 0);
-void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1);  // cm1g_advm.hip
+void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1 = 0, int own = 0);
+void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own);  // cm1g_advm.hip
 void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float *s0,
               const float *sten0, const float *thterm_div, float *out_sten, float *out_new,
               int pdef, int pdefweno, float dt, double weps, int smeps_flush);
 // acoustic (cm1g_sound.hip)
 void sound_run(cm1g_model *m, int nrk);
 void sounde_run(cm1g_model *m, int nrk);
-void getdiv_both(cm1g_model *m, int want_t11);
+void getdiv_both(cm1g_model *m, int want_t11, int want_divx = 1, float *div = nullptr);
+void scale_mass_fluxes(cm1g_model *m);
 void cfl_run(cm1g_model *m, float dt, const float *u, const float *v, const float *w);
 // microphysics (cm1g_micro.hip)
 void pdefq_run(cm1g_model *m, float rmax, const float *rho, float *q);

@@ -497,6 +497,15 @@ __global__ void k_train(int nj, double tem, const double *rows, double *dscal) {
   for (int j = 0; j < nj; j++) tr = tr + rows[j] * tem;
   dscal[3] = tr;
 }
+// train += the fallout's column budgets of this call (the end of
+// src/kessler.F:fallout)
+static void train_accumulate(cm1g_model *m) {
+  double *rows = m->dred + (size_t)m->P.ni * m->P.nj;
+  hipLaunchKernelGGL(k_train_rows, dim3((m->P.nj + 63) / 64), dim3(64), 0, m->stream, m->P.ni,
+                     m->P.nj, m->dred, rows);
+  hipLaunchKernelGGL(k_train, dim3(1), dim3(1), 0, m->stream, m->P.nj,
+                     (double)(m->c.dx * m->c.dy), rows, m->dscal);
+}
 
 // ------------------------------------------------------------------ getset
 extern "C" int cm1g_getset(cm1g_model *m, int fresh_start) {
@@ -575,13 +584,6 @@ extern "C" int cm1g_solve1(cm1g_model *m) {
   const float dt = m->c.dt;
   float **f = m->f;
   float *uten1 = f[CM1G_UTEN1], *vten1 = f[CM1G_VTEN1], *wten1 = f[CM1G_WTEN1], *thten1 = f[CM1G_THTEN1];
-  for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
-    int c = IXD(P, i, j, k);
-    uten1[c] = 0.0f;
-    vten1[c] = 0.0f;
-    wten1[c] = 0.0f;
-    thten1[c] = 0.0f;
-  });
   for (int n = 0; n < m->c.npt; n++) {
     float *pt = f[CM1G_PTTEN0 + n];
     for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) { pt[IXD(P, i, j, k)] = 0.0f; });
@@ -593,10 +595,19 @@ extern "C" int cm1g_solve1(cm1g_model *m) {
   const float *qa0 = f[CM1G_QA0];
   const float *qpten = f[CM1G_QPTEN], *qtten = f[CM1G_QTTEN], *qvten = f[CM1G_QVTEN], *qcten = f[CM1G_QCTEN];
   const float rdt = 1.0f / dt;
-  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+  // ten1 = 0 (no Rayleigh damping, diffusion or turbulence) over the
+  // staggered ranges, then ppten and the diabatic ten1 on the interior
+  for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
     const Consts &K = P.K;
     const float tem = 0.0001f * K.tsmall;
     int c = IXD(P, i, j, k);
+    uten1[c] = 0.0f;
+    vten1[c] = 0.0f;
+    wten1[c] = 0.0f;
+    if (i > ni || j > nj) {
+      thten1[c] = 0.0f;
+      return;
+    }
     float th1 = 0.0f;       // thten1 was just zeroed (irdamp = 0)
     float pt;
     if (moist) {
@@ -675,31 +686,30 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
       }
     });
     bc_fill(m, BC_P, t11);
-    // rru, rrv, rrw (src/solve2.F:490-545)
+    // rru, rrv, rrw (src/solve2.F:490-545) and divx (:587-625) in one pass:
+    // the fluxes divx needs at i+1, j+1, k+1 are formed from the
+    // velocities with the same products
     for3(m->stream, 0, ni + 2, 0, nj + 2, 1, nk, [=] __device__(int i, int j, int k) {
-      int c = IXD(P, i, j, k);
-      if (j <= nj + 1) rru[c] = P.rho0[k] * u3d[c];
-      if (i <= ni + 1) rrv[c] = P.rho0[k] * v3d[c];
+      const int c = IXD(P, i, j, k);
+      const float r0 = P.rho0[k];
+      const float ru0 = r0 * u3d[c], rv0 = r0 * v3d[c];
+      if (j <= nj + 1) rru[c] = ru0;
+      if (i <= ni + 1) rrv[c] = rv0;
       if (i <= ni + 1 && j <= nj + 1) {
-        if (k == 1) {
-          rrw[c] = 0.0f;
-          rrw[IXD(P, i, j, nk + 1)] = 0.0f;
-        } else {
-          rrw[c] = P.rf0[k] * w3d[c];
-        }
+        const float rw0 = (k == 1) ? 0.0f : P.rf0[k] * w3d[c];
+        const float rw1 = (k == nk) ? 0.0f : P.rf0[k + 1] * w3d[c + P.pxy];
+        rrw[c] = rw0;
+        if (k == nk) rrw[IXD(P, i, j, nk + 1)] = 0.0f;
+        const float ru1 = r0 * u3d[c + 1], rv1 = r0 * v3d[c + P.px];
+        float d = ((ru1 - ru0) * P.rdx * P.uh[i + 2] + (rv1 - rv0) * P.rdy * P.vh[j + 2]) +
+                  (rw1 - rw0) * P.rdz * P.mh[k];
+        if (fabsf(d) < P.K.smeps) d = 0.0f;
+        divx[c] = d;
       }
     });
-    // divx (src/solve2.F:587-625)
-    for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
-      int c = IXD(P, i, j, k);
-      float d = ((rru[IXD(P, i + 1, j, k)] - rru[c]) * P.rdx * P.uh[i + 2] +
-                 (rrv[IXD(P, i, j + 1, k)] - rrv[c]) * P.rdy * P.vh[j + 2]) +
-                (rrw[IXD(P, i, j, k + 1)] - rrw[c]) * P.rdz * P.mh[k];
-      if (fabsf(d) < P.K.smeps) d = 0.0f;
-      divx[c] = d;
-    });
-    // tendencies: the pre-RK values (src/solve2.F:459-471) plus advection
-    adv_uvw(m, nrk, 7, 1, 1);
+    // tendencies: the pre-RK values (src/solve2.F:459-471) plus advection,
+    // each component's own mass flux formed from its velocity
+    adv_uvw(m, nrk, 7, 1, 1, 1);
     // sadv = pi0 + pi' (src/solve2.F:1027-1050)
     for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
       int c = IXD(P, i, j, k);
@@ -805,11 +815,7 @@ extern "C" int cm1g_mp_driver(cm1g_model *m) {
     return check_launch();
   }
   kessler_branch_run(m);
-  double *rows = m->dred + (size_t)m->P.ni * m->P.nj;
-  hipLaunchKernelGGL(k_train_rows, dim3((m->P.nj + 63) / 64), dim3(64), 0, m->stream, m->P.ni,
-                     m->P.nj, m->dred, rows);
-  hipLaunchKernelGGL(k_train, dim3(1), dim3(1), 0, m->stream, m->P.nj,
-                     (double)(m->c.dx * m->c.dy), rows, m->dscal);
+  train_accumulate(m);
   return check_launch();
 }
 
@@ -828,36 +834,57 @@ extern "C" int cm1g_solve3(cm1g_model *m) {
     float *sarr[4] = {th3d, f[CM1G_Q3D0], f[CM1G_Q3D1], f[CM1G_Q3D2]};
     bc_fill_n(m, BC_S, sarr, moist ? 4 : 1);
   }
-  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-    int c = IXD(P, i, j, k);
-    ppx[c] = pp3d[c] + ppx[c];
-  });
+  // rho's halo first (rf below reads it); ppx's halo after the update
+  bc_fill(m, BC_P, rho);
   {
-    float *arr[2] = {ppx, rho};
-    bc_fill_n(m, BC_P, arr, 2);
-  }
-  if (moist && m->c.ptype >= 1) {
+    // one pass over the step's final copies (src/solve3.F:246-705):
+    // ppx = pp3d + ppx and the diabatic tendencies on the interior,
+    // ua/va/wa = u3d/v3d/w3d over i,j = 0..n+2, ppi/tha/qa = pp3d/th3d/q3d,
+    // rr = 1/rho and rf (extrapolated at k = 1 and nk+1) over 0..n+1
+    const int diab = (moist && m->c.ptype >= 1);
     const float rdt = 1.0f / dt;
     float *qpten = f[CM1G_QPTEN], *qtten = f[CM1G_QTTEN], *qvten = f[CM1G_QVTEN], *qcten = f[CM1G_QCTEN];
-    const float *qv = f[CM1G_Q3D0], *qc = f[CM1G_Q3D1];
-    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-      int c = IXD(P, i, j, k);
-      qpten[c] = (pp3d[c] - qpten[c]) * rdt;
-      qtten[c] = (th3d[c] - qtten[c]) * rdt;
-      qvten[c] = (qv[c] - qvten[c]) * rdt;
-      qcten[c] = (qc[c] - qcten[c]) * rdt;
-    });
-  }
-  {
     float *ua = f[CM1G_UA], *va = f[CM1G_VA], *wa = f[CM1G_WA];
     const float *u3d = f[CM1G_U3D], *v3d = f[CM1G_V3D], *w3d = f[CM1G_W3D];
+    float *ppi = f[CM1G_PPI], *tha = f[CM1G_THA];
+    float *qa0 = f[CM1G_QA0], *qa1 = f[CM1G_QA1], *qa2 = f[CM1G_QA2];
+    const float *q0 = f[CM1G_Q3D0], *q1 = f[CM1G_Q3D1], *q2 = f[CM1G_Q3D2];
     for3(m->stream, 0, ni + 2, 0, nj + 2, 1, nk, [=] __device__(int i, int j, int k) {
-      int c = IXD(P, i, j, k);
+      const int c = IXD(P, i, j, k);
       ua[c] = u3d[c];
       va[c] = v3d[c];
       wa[c] = w3d[c];
+      if (i > ni + 1 || j > nj + 1) return;
+      const float pp = pp3d[c], th = th3d[c];
+      ppi[c] = pp;
+      tha[c] = th;
+      float qv = 0.0f, qc = 0.0f;
+      if (moist) {
+        qv = q0[c];
+        qc = q1[c];
+        qa0[c] = qv;
+        qa1[c] = qc;
+        qa2[c] = q2[c];
+      }
+      const float rh = rho[c];
+      rr[c] = 1.0f / rh;
+      if (k >= 2) rf[c] = P.c1[k] * rho[IXD(P, i, j, k - 1)] + P.c2[k] * rh;
+      else {
+        rf[c] = P.cgs1 * rh + P.cgs2 * rho[IXD(P, i, j, 2)] + P.cgs3 * rho[IXD(P, i, j, 3)];
+        rf[IXD(P, i, j, nk + 1)] = P.cgt1 * rho[IXD(P, i, j, nk)] + P.cgt2 * rho[IXD(P, i, j, nk - 1)] +
+                                   P.cgt3 * rho[IXD(P, i, j, nk - 2)];
+      }
+      if (i < 1 || i > ni || j < 1 || j > nj) return;
+      ppx[c] = pp + ppx[c];
+      if (diab) {
+        qpten[c] = (pp - qpten[c]) * rdt;
+        qtten[c] = (th - qtten[c]) * rdt;
+        qvten[c] = (qv - qvten[c]) * rdt;
+        qcten[c] = (qc - qcten[c]) * rdt;
+      }
     });
   }
+  bc_fill(m, BC_P, ppx);
   for (int n = 0; n < m->c.npt; n++) {
     float *pta = f[CM1G_PTA0 + n];
     const float *pt3d = f[CM1G_PT3D0 + n];
@@ -865,31 +892,6 @@ extern "C" int cm1g_solve3(cm1g_model *m) {
       pta[IXD(P, i, j, k)] = pt3d[IXD(P, i, j, k)];
     });
   }
-  {
-    float *ppi = f[CM1G_PPI], *tha = f[CM1G_THA];
-    float *qa0 = f[CM1G_QA0], *qa1 = f[CM1G_QA1], *qa2 = f[CM1G_QA2];
-    const float *q0 = f[CM1G_Q3D0], *q1 = f[CM1G_Q3D1], *q2 = f[CM1G_Q3D2];
-    for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
-      int c = IXD(P, i, j, k);
-      ppi[c] = pp3d[c];
-      tha[c] = th3d[c];
-      if (moist) {
-        qa0[c] = q0[c];
-        qa1[c] = q1[c];
-        qa2[c] = q2[c];
-      }
-    });
-  }
-  for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
-    int c = IXD(P, i, j, k);
-    rr[c] = 1.0f / rho[c];
-    if (k >= 2) rf[c] = P.c1[k] * rho[IXD(P, i, j, k - 1)] + P.c2[k] * rho[c];
-    else {
-      rf[c] = P.cgs1 * rho[IXD(P, i, j, 1)] + P.cgs2 * rho[IXD(P, i, j, 2)] + P.cgs3 * rho[IXD(P, i, j, 3)];
-      rf[IXD(P, i, j, nk + 1)] = P.cgt1 * rho[IXD(P, i, j, nk)] + P.cgt2 * rho[IXD(P, i, j, nk - 1)] +
-                                 P.cgt3 * rho[IXD(P, i, j, nk - 2)];
-    }
-  });
   return check_launch();
 }
 
@@ -952,6 +954,7 @@ extern "C" int cm1g_sound(cm1g_model *m, int nrk, float dttmp) {
 extern "C" int cm1g_kessler_branch(cm1g_model *m) {
   ARGCHK(m, "null model");
   kessler_branch_run(m);
+  train_accumulate(m);
   return check_launch();
 }
 extern "C" int cm1g_pdefq(cm1g_model *m, float rmax, int q_id) {
@@ -972,15 +975,15 @@ extern "C" int cm1g_calccflquick(cm1g_model *m, float dt, int u_id, int v_id, in
   return check_launch();
 }
 extern "C" int cm1g_getdiv(cm1g_model *m, int div_id) {
-  ARGCHK(field_ok(m, div_id), "cm1g_getdiv: bad field id");
-  getdiv_both(m, 1);
-  if (div_id != CM1G_T11)
-    HIPCHK(hipMemcpyAsync(m->f[div_id], m->f[CM1G_T11], m->ntot * sizeof(float), hipMemcpyDeviceToDevice, m->stream));
+  ARGCHK(field_ok(m, div_id) && div_id != CM1G_RRU && div_id != CM1G_RRV && div_id != CM1G_RRW,
+         "cm1g_getdiv: bad field id");
+  getdiv_both(m, 1, 0, m->f[div_id]);
   return check_launch();
 }
 extern "C" int cm1g_getdivx(cm1g_model *m) {
   ARGCHK(m, "null model");
-  getdiv_both(m, 0);
+  getdiv_both(m, 0, 1);
+  scale_mass_fluxes(m);
   return check_launch();
 }
 extern "C" int cm1g_scalars(cm1g_model *m, double *out4) {

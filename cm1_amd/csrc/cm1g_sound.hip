@@ -563,13 +563,14 @@ void sounde_run(cm1g_model *m, int nrk) {
 }
 
 // getdiv (t11 from the time-averaged velocities) and getdivx (divx from the
-// mass fluxes scaled by rho0/rf0; the advection kernels apply the same
-// scaling on the fly instead of storing scaled copies).
-void getdiv_both(cm1g_model *m, int want_t11) {
+// mass fluxes scaled by rho0/rf0; inside the time step the advection
+// kernels apply the same scaling on the fly instead of storing scaled
+// copies).  want_divx = 0 computes only the divergence into `div`.
+void getdiv_both(cm1g_model *m, int want_t11, int want_divx, float *div) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   const float *rru = m->f[CM1G_RRU], *rrv = m->f[CM1G_RRV], *rrw = m->f[CM1G_RRW];
-  float *t11 = m->f[CM1G_T11], *divx = m->f[CM1G_DIVX];
+  float *t11 = div ? div : m->f[CM1G_T11], *divx = m->f[CM1G_DIVX];
   const float smeps = P.K.smeps;
   for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
     int c = IXD(P, i, j, k);
@@ -580,6 +581,7 @@ void getdiv_both(cm1g_model *m, int want_t11) {
       if (fabsf(d) < smeps) d = 0.0f;
       t11[c] = d;
     }
+    if (!want_divx) return;
     float ru1 = rru[IXD(P, i + 1, j, k)] * P.rho0[k], ru0 = rru[c] * P.rho0[k];
     float rv1 = rrv[IXD(P, i, j + 1, k)] * P.rho0[k], rv0 = rrv[c] * P.rho0[k];
     float rw1 = (k == nk) ? 0.0f : rrw[IXD(P, i, j, k + 1)] * P.rf0[k + 1];
@@ -588,6 +590,28 @@ void getdiv_both(cm1g_model *m, int want_t11) {
               (rw1 - rw0) * P.rdz * P.mh[k];
     if (fabsf(d) < smeps) d = 0.0f;
     divx[c] = d;
+  });
+}
+
+// The in-place scaling of getdivx (src/misclibs.F:246-273): rru, rrv by
+// rho0, rrw by rf0 with rrw = 0 at k = 1 and nk+1, over the reference's
+// ranges.  Only the routine-level entry stores the scaled fluxes.
+void scale_mass_fluxes(cm1g_model *m) {
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  float *rru = m->f[CM1G_RRU], *rrv = m->f[CM1G_RRV], *rrw = m->f[CM1G_RRW];
+  for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+    const int c = IXD(P, i, j, k);
+    if (j <= nj) rru[c] = rru[c] * P.rho0[k];
+    if (i <= ni) rrv[c] = rrv[c] * P.rho0[k];
+    if (i <= ni && j <= nj) {
+      if (k == 1) {
+        rrw[c] = 0.0f;
+        rrw[IXD(P, i, j, nk + 1)] = 0.0f;
+      } else {
+        rrw[c] = rrw[c] * P.rf0[k];
+      }
+    }
   });
 }
 

@@ -174,7 +174,10 @@ int cm1g_pdefq(cm1g_model *m, float rmax, int q_id);
 int cm1g_bc(cm1g_model *m, int id, int kind);
 /* calccflquick(dt,uh,vh,mh,ua,va,wa,...), src/misclibs.F:2726-2814 */
 int cm1g_calccflquick(cm1g_model *m, float dt, int u_id, int v_id, int w_id, float *cflmax);
-/* getdiv / getdivx on the model's rru/rrv/rrw, src/misclibs.F:108-370 */
+/* getdiv (src/misclibs.F:108-214): divergence of the model's rru/rrv/rrw
+ * into field div_id (t11 at src/solve2.F:1280).  getdivx
+ * (src/misclibs.F:222-370): scales rru/rrv by rho0 and rrw by rf0 in place
+ * (rrw = 0 at k = 1 and nk+1) and computes divx from them. */
 int cm1g_getdiv(cm1g_model *m, int div_id);
 int cm1g_getdivx(cm1g_model *m);
 /* dtsmall: nsound for dt (src/misclibs.F:818-915) */

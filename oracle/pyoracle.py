@@ -74,6 +74,10 @@ def _load(fp64=False):
         getattr(lib, name).argtypes = [vp, C.c_int]
     lib.cm1o_sound.argtypes = [vp, C.c_int, real]
     lib.cm1o_kessler_branch.argtypes = [vp]
+    lib.cm1o_getdiv.argtypes = [vp, C.c_int]
+    lib.cm1o_getdivx.argtypes = [vp]
+    lib.cm1o_calccflquick.argtypes = [vp, real, C.c_int, C.c_int, C.c_int]
+    lib.cm1o_pdefq.argtypes = [vp, real, C.c_int]
     lib.cm1o_set_libm.argtypes = [C.c_int]
     return lib
 

@@ -103,6 +103,10 @@ void cm1o_sound(cm1o_model *m, int nrk, float dttmp);
  * precision rounded once (the HIP kernels' evaluation). Process-wide. */
 void cm1o_set_libm(int mode);
 void cm1o_kessler_branch(cm1o_model *m);
+void cm1o_getdiv(cm1o_model *m, int div_id);      /* getdiv(rru,rrv,rrw -> div) */
+void cm1o_getdivx(cm1o_model *m);                 /* scales rru/rrv/rrw in place, divx */
+void cm1o_calccflquick(cm1o_model *m, float dt, int u_id, int v_id, int w_id); /* -> scalars[2] */
+void cm1o_pdefq(cm1o_model *m, float rmax, int q_id);
 
 #ifdef __cplusplus
 }

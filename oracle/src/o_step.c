@@ -658,3 +658,11 @@ void cm1o_sound(cm1o_model *M, int nrk, float dttmp) {
   else o_sound(M, nrk, dttmp);
 }
 void cm1o_kessler_branch(cm1o_model *M) { o_mp_driver(M); }
+void cm1o_getdiv(cm1o_model *M, int div_id) {
+  o_getdiv(M, M->f[F_RRU], M->f[F_RRV], M->f[F_RRW], M->f[div_id]);
+}
+void cm1o_getdivx(cm1o_model *M) { o_getdivx(M, M->f[F_RRU], M->f[F_RRV], M->f[F_RRW], M->f[F_DIVX]); }
+void cm1o_calccflquick(cm1o_model *M, float dt, int u_id, int v_id, int w_id) {
+  o_calccflquick(M, dt, M->f[u_id], M->f[v_id], M->f[w_id]);
+}
+void cm1o_pdefq(cm1o_model *M, float rmax, int q_id) { o_pdefq(M, rmax, M->f[F_RHO], M->f[q_id]); }
