@@ -87,39 +87,49 @@ void ktimer_mark(cm1g_model *m, int which);
 void cm1g_set_error(const std::string &s);
 
 // ---------------------------------------------------------------- launching
+// Element-wise launches.  Point i = 1 sits at the start of a 128-byte line
+// (CM1G_IOFF), so the launch origin in i is rounded down to 1 - 64m: every
+// wavefront then covers whole lines of one row, also for ranges that start
+// in the halo (i = 0, -1, -2), whose points go to one mostly idle wavefront
+// instead of shifting every wavefront across a line boundary (which costs
+// up to twice the time of the aligned launch).
+inline int i_origin(int ilo) { return ilo >= 1 ? 1 + ((ilo - 1) / 64) * 64 : 1 - ((64 - ilo) / 64) * 64; }
+
 template <class F>
-__global__ void __launch_bounds__(256) k_for3(F f, int i0, int ni, int j0, int nj, int k0) {
+__global__ void __launch_bounds__(256) k_for3(F f, int i0, int ilo, int ihi, int j0, int nj, int k0) {
   int i = i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   int j = j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
   int k = k0 + (int)blockIdx.z;
-  if (i < i0 + ni && j < j0 + nj) f(i, j, k);
+  if (i >= ilo && i <= ihi && j < j0 + nj) f(i, j, k);
 }
 
 // run f(i,j,k) over the inclusive box [ilo,ihi]x[jlo,jhi]x[klo,khi]
 template <class F>
 inline void for3(hipStream_t s, int ilo, int ihi, int jlo, int jhi, int klo, int khi, F f) {
-  int ni = ihi - ilo + 1, nj = jhi - jlo + 1, nk = khi - klo + 1;
-  if (ni <= 0 || nj <= 0 || nk <= 0) return;
+  int nj = jhi - jlo + 1, nk = khi - klo + 1;
+  if (ihi < ilo || nj <= 0 || nk <= 0) return;
+  const int i0 = i_origin(ilo), ni = ihi - i0 + 1;
   dim3 b(64, 4, 1);
   dim3 g((ni + 63) / 64, (nj + 3) / 4, nk);
-  hipLaunchKernelGGL(k_for3<F>, g, b, 0, s, f, ilo, ni, jlo, nj, klo);
+  hipLaunchKernelGGL(k_for3<F>, g, b, 0, s, f, i0, ilo, ihi, jlo, nj, klo);
 }
 
 template <class F>
-__global__ void __launch_bounds__(256) k_for2(F f, int i0, int ni, int j0, int nj) {
+__global__ void __launch_bounds__(256) k_for2(F f, int i0, int ilo, int ihi, int j0, int nj) {
   int i = i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   int j = j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  if (i < i0 + ni && j < j0 + nj) f(i, j);
+  if (i >= ilo && i <= ihi && j < j0 + nj) f(i, j);
 }
 
 // run f(i,j) over [ilo,ihi]x[jlo,jhi] (column kernels loop k inside)
 template <class F>
 inline void for2(hipStream_t s, int ilo, int ihi, int jlo, int jhi, F f) {
-  int ni = ihi - ilo + 1, nj = jhi - jlo + 1;
-  if (ni <= 0 || nj <= 0) return;
+  int nj = jhi - jlo + 1;
+  if (ihi < ilo || nj <= 0) return;
+  const int i0 = i_origin(ilo), ni = ihi - i0 + 1;
   dim3 b(64, 4, 1);
   dim3 g((ni + 63) / 64, (nj + 3) / 4, 1);
-  hipLaunchKernelGGL(k_for2<F>, g, b, 0, s, f, ilo, ni, jlo, nj);
+  hipLaunchKernelGGL(k_for2<F>, g, b, 0, s, f, i0, ilo, ihi, jlo, nj);
 }
 
 // Column kernels run on a 1-D grid of 64 x ty column tiles (one wavefront
@@ -132,7 +142,9 @@ inline void for2(hipStream_t s, int ilo, int ihi, int jlo, int jhi, F f) {
 constexpr int NXCD = 8;
 struct ColGrid {
   int gx, gy, ty, nb, per;  // tiles in i and j, tile rows, tiles, tiles per XCD
+  int i0, j0;               // first column of tile 0 (i0 on a line start)
 };
+// columns i = 1..ni, j = 1..nj
 inline ColGrid col_grid(int ni, int nj, int ty = 4) {
   ColGrid g;
   g.ty = ty;
@@ -140,18 +152,32 @@ inline ColGrid col_grid(int ni, int nj, int ty = 4) {
   g.gy = (nj + ty - 1) / ty;
   g.nb = g.gx * g.gy;
   g.per = (g.nb + NXCD - 1) / NXCD;
+  g.i0 = 1;
+  g.j0 = 1;
+  return g;
+}
+// columns covering [ilo,ihi] x [jlo,jhi], tiles aligned as for3 aligns them
+// (the kernel checks the range itself)
+inline ColGrid col_grid_box(int ilo, int ihi, int jlo, int jhi, int ty = 4) {
+  const int i0 = i_origin(ilo);
+  ColGrid g = col_grid(ihi - i0 + 1, jhi - jlo + 1, ty);
+  g.i0 = i0;
+  g.j0 = jlo;
   return g;
 }
 inline dim3 col_blocks(const ColGrid &g) { return dim3(g.per * NXCD, 1, 1); }
 inline dim3 col_threads(const ColGrid &g) { return dim3(64 * g.ty, 1, 1); }
-// returns false for padding workgroups
-__device__ __forceinline__ bool col_tile(const ColGrid &g, int &i, int &j) {
-  const int b = (int)blockIdx.x;
+// returns false for padding workgroups; b is the workgroup's index in the
+// column grid
+__device__ __forceinline__ bool col_tile_b(const ColGrid &g, int b, int &i, int &j) {
   const int t = (b % NXCD) * g.per + b / NXCD;
   if (t >= g.nb) return false;
-  i = 1 + (t % g.gx) * 64 + (int)(threadIdx.x & 63);
-  j = 1 + (t / g.gx) * g.ty + (int)(threadIdx.x >> 6);
+  i = g.i0 + (t % g.gx) * 64 + (int)(threadIdx.x & 63);
+  j = g.j0 + (t / g.gx) * g.ty + (int)(threadIdx.x >> 6);
   return true;
+}
+__device__ __forceinline__ bool col_tile(const ColGrid &g, int &i, int &j) {
+  return col_tile_b(g, (int)blockIdx.x, i, j);
 }
 
 // ---------------------------------------------------------------- modules

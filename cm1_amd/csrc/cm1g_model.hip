@@ -577,6 +577,57 @@ extern "C" int cm1g_getset(cm1g_model *m, int fresh_start) {
 }
 
 // ------------------------------------------------------------------ solve1
+// rru = rho0*u, rrv = rho0*v, rrw = rf0*w (0 at k = 1 and nk+1) over
+// i,j = 0..n+2 (src/solve2.F:490-545) and divx over 0..n+1 (:587-625),
+// marching up each column: the fluxes divx needs at i+1, j+1 are formed
+// from the velocities with the same products, w(k+1) is carried to the next
+// level, and each level's loads are issued before the previous level's
+// stores.
+__global__ void __launch_bounds__(256) k_massflux_div(KP P, const float *u3d, const float *v3d,
+                                                      const float *w3d, float *rru, float *rrv,
+                                                      float *rrw, float *divx, ColGrid cg) {
+  int i, j;
+  if (!col_tile(cg, i, j)) return;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  if (i < 0 || i > ni + 2 || j > nj + 2) return;
+  const bool du = (j <= nj + 1), dv = (i <= ni + 1), dd = du && dv;
+  const int px = P.px, pxy = P.pxy;
+  int c = IXD(P, i, j, 1);
+  float u0 = u3d[c], u1 = dd ? u3d[c + 1] : 0.0f, v0 = v3d[c], v1 = dd ? v3d[c + px] : 0.0f;
+  float wk = 0.0f, wk1 = (dd && nk >= 2) ? w3d[c + pxy] : 0.0f;
+  for (int k = 1; k <= nk; k++, c += pxy) {
+    const float cu0 = u0, cu1 = u1, cv0 = v0, cv1 = v1, cw0 = wk, cw1 = wk1;
+    if (k < nk) {
+      const int cn = c + pxy;
+      u0 = u3d[cn];
+      v0 = v3d[cn];
+      if (dd) {
+        u1 = u3d[cn + 1];
+        v1 = v3d[cn + px];
+        wk = cw1;
+        wk1 = (k + 1 < nk) ? w3d[cn + pxy] : 0.0f;
+      }
+    }
+    const float r0 = P.rho0[k];
+    const float ru0 = r0 * cu0, rv0 = r0 * cv0;
+    if (du) rru[c] = ru0;
+    if (dv) rrv[c] = rv0;
+    if (dd) {
+      const float rw0 = (k == 1) ? 0.0f : P.rf0[k] * cw0;
+      const float rw1 = (k == nk) ? 0.0f : P.rf0[k + 1] * cw1;
+      rrw[c] = rw0;
+      if (k == nk) rrw[c + pxy] = 0.0f;
+      if (i <= ni + 1 && j <= nj + 1) {
+        const float ru1 = r0 * cu1, rv1 = r0 * cv1;
+        float d = ((ru1 - ru0) * P.rdx * P.uh[i + 2] + (rv1 - rv0) * P.rdy * P.vh[j + 2]) +
+                  (rw1 - rw0) * P.rdz * P.mh[k];
+        if (fabsf(d) < P.K.smeps) d = 0.0f;
+        divx[c] = d;
+      }
+    }
+  }
+}
+
 extern "C" int cm1g_solve1(cm1g_model *m) {
   ARGCHK(m, "cm1g_solve1: null model");
   const KP P = m->P;
@@ -686,27 +737,12 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
       }
     });
     bc_fill(m, BC_P, t11);
-    // rru, rrv, rrw (src/solve2.F:490-545) and divx (:587-625) in one pass:
-    // the fluxes divx needs at i+1, j+1, k+1 are formed from the
-    // velocities with the same products
-    for3(m->stream, 0, ni + 2, 0, nj + 2, 1, nk, [=] __device__(int i, int j, int k) {
-      const int c = IXD(P, i, j, k);
-      const float r0 = P.rho0[k];
-      const float ru0 = r0 * u3d[c], rv0 = r0 * v3d[c];
-      if (j <= nj + 1) rru[c] = ru0;
-      if (i <= ni + 1) rrv[c] = rv0;
-      if (i <= ni + 1 && j <= nj + 1) {
-        const float rw0 = (k == 1) ? 0.0f : P.rf0[k] * w3d[c];
-        const float rw1 = (k == nk) ? 0.0f : P.rf0[k + 1] * w3d[c + P.pxy];
-        rrw[c] = rw0;
-        if (k == nk) rrw[IXD(P, i, j, nk + 1)] = 0.0f;
-        const float ru1 = r0 * u3d[c + 1], rv1 = r0 * v3d[c + P.px];
-        float d = ((ru1 - ru0) * P.rdx * P.uh[i + 2] + (rv1 - rv0) * P.rdy * P.vh[j + 2]) +
-                  (rw1 - rw0) * P.rdz * P.mh[k];
-        if (fabsf(d) < P.K.smeps) d = 0.0f;
-        divx[c] = d;
-      }
-    });
+    // rru, rrv, rrw (src/solve2.F:490-545) and divx (:587-625) in one pass
+    {
+      const ColGrid cg = col_grid_box(0, ni + 2, 0, nj + 2);
+      hipLaunchKernelGGL(k_massflux_div, col_blocks(cg), col_threads(cg), 0, m->stream, P, u3d, v3d,
+                         w3d, rru, rrv, rrw, divx, cg);
+    }
     // tendencies: the pre-RK values (src/solve2.F:459-471) plus advection,
     // each component's own mass flux formed from its velocity
     adv_uvw(m, nrk, 7, 1, 1, 1);

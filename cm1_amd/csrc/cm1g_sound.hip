@@ -50,6 +50,7 @@ struct SubArgs {
   // (src/sound.F:191-250)
   const float *wR, *ppR, *wa;
   int first;
+  int nring;                      // trailing workgroups that do the averaging ring
 };
 
 // new u at (i,j,k): radiative open-boundary update (ssopenbcw/e,
@@ -105,12 +106,11 @@ __device__ __forceinline__ void put_w_avg(const SubArgs &a, int c, float w) {
 // The averaging ring of one substep: u and v on the faces i = ni+1 and
 // j = nj+1 (unchanged by the step where the open-boundary masks are zero,
 // advanced otherwise) and the w average there (w is not advanced at those
-// points).  Thread t walks the ring (i = 1..ni+1 at j = nj+1, then
-// j = 1..nj at i = ni+1); blockIdx.y is the level.  Reads uX/vX/ppdX only,
-// so it may run before or after k_sound_sub.
-__global__ void __launch_bounds__(256) k_sound_ring(KP P, SubArgs a) {
-  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int k = 1 + (int)blockIdx.y;
+// points).  Point t walks the ring (i = 1..ni+1 at j = nj+1, then
+// j = 1..nj at i = ni+1) at level k.  It reads uX/vX/ppdX and writes only
+// ring points, which the column tiles neither read nor write, so the ring
+// runs alongside them.
+__device__ __forceinline__ void sound_ring_point(const KP &P, const SubArgs &a, int t, int k) {
   const int ni = P.ni, nj = P.nj;
   int i, j;
   if (t <= ni) {
@@ -141,7 +141,18 @@ __global__ void __launch_bounds__(256) k_sound_ring(KP P, SubArgs a) {
 // Each level issues all of its loads before any store, so a level costs one
 // memory round trip; the downward sweep also prefetches the next level's
 // operands before storing the current one.
+//
+// The a.nring workgroups after the column tiles do the averaging ring (one
+// thread per ring point and level); they fill the CUs the last tiles leave
+// idle.
 __global__ void __launch_bounds__(64 * CM1G_SOUND_TY) k_sound_sub(KP P, SubArgs a, ColGrid cg) {
+  const int nmain = cg.per * NXCD;
+  if ((int)blockIdx.x >= nmain) {
+    const int npts = P.ni + P.nj + 1;
+    const int t = ((int)blockIdx.x - nmain) * (int)blockDim.x + (int)threadIdx.x;
+    if (t < npts * P.nk) sound_ring_point(P, a, t % npts, 1 + t / npts);
+    return;
+  }
   int i, j;
   if (!col_tile(cg, i, j)) return;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
@@ -237,8 +248,10 @@ __global__ void __launch_bounds__(64 * CM1G_SOUND_TY) k_sound_sub(KP P, SubArgs 
     a.vY[c] = vn0;
     a.rrv[c] = a.last ? (rrvc + vn0) * a.tavg : rrvc + vn0;
     a.fpk[c] = fpk;
+    // d1 depends only on the stage's fixed coefficients (thv, piadv,
+    // ppterm, dts): stored on the first substep, read on the others
     if (k >= 2 && k < nk) {
-      a.dum1[c] = d1p;
+      if (a.first) a.dum1[c] = d1p;
       a.dum2[c] = d2p;
     }
     if (k == nk) w_nk1 = w_p;
@@ -367,7 +380,10 @@ void sound_run(cm1g_model *m, int nrk) {
   a.wa = wa;
   const ColGrid cg = col_grid(ni, nj, CM1G_SOUND_TY);
   const dim3 blk = col_threads(cg), grd = col_blocks(cg);
-  const dim3 ring_grd((ni + nj + 1 + 255) / 256, nk, 1);
+  {
+    const int nthr = (int)blk.x, npts = (ni + nj + 1) * nk;
+    a.nring = (npts + nthr - 1) / nthr;
+  }
   float *ppdb[2] = {ppd, m->tx};
   float *uprev = nullptr, *vprev = nullptr;
   for (int n = 1; n <= nloop; n++) {
@@ -392,9 +408,8 @@ void sound_run(cm1g_model *m, int nrk) {
     a.ppR = (n == 1 && fold) ? ppi : pp3d;
     a.first = (n == 1);
     a.last = (n == nloop);
-    hipLaunchKernelGGL(k_sound_ring, ring_grd, dim3(256), 0, m->stream, P, a);
     ktimer_mark(m, CM1G_KT_SOUND_SUBSTEP);
-    hipLaunchKernelGGL(k_sound_sub, grd, blk, 0, m->stream, P, a, cg);
+    hipLaunchKernelGGL(k_sound_sub, dim3(grd.x + a.nring), blk, 0, m->stream, P, a, cg);
     ktimer_mark(m, CM1G_KT_SOUND_SUBSTEP);
     if (n < nloop) bc_fill(m, BC_P, a.ppdY);
     uprev = udst;
