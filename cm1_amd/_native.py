@@ -26,7 +26,8 @@ for _n in range(8):
     FID[f"pt3d{_n}"] = _PTA0 + 8 + _n
     FID[f"ptten{_n}"] = _PTA0 + 16 + _n
 KFIELDS = ["mh", "rmh", "mf", "rmf", "c1", "c2", "zh", "zf", "pi0", "th0", "prs0",
-           "qv0", "rho0", "rr0", "rf0", "rrf0", "thv0", "rth0", "qc0", "qi0", "u0", "v0"]
+           "qv0", "rho0", "rr0", "rf0", "rrf0", "thv0", "rth0", "qc0", "qi0", "u0", "v0",
+           "tauh", "tauf"]
 KID = {n: i for i, n in enumerate(KFIELDS)}
 D2 = {"rain": 0, "prate": 1, "radbcw": 2, "radbce": 3, "radbcs": 4, "radbcn": 5}
 KT_SOUND_SUBSTEP = 1
@@ -41,7 +42,8 @@ class Config(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "ni", "nj", "nk", "wbc", "ebc", "sbc", "nbc", "imoist", "ptype", "psolver",
         "advwenos", "advwenov", "apmasscon", "irbc", "nsound", "npt", "pdtra")] + \
-        [(n, C.c_float) for n in ("dx", "dy", "dz", "dt", "alph", "kdiv", "csmax")]
+        [(n, C.c_float) for n in ("dx", "dy", "dz", "dt", "alph", "kdiv", "csmax")] + \
+        [("irdamp", C.c_int32), ("rdalpha", C.c_float), ("zd", C.c_float)]
 
 
 class CM1GError(RuntimeError):

@@ -27,7 +27,9 @@ struct KP {
   float cgs1, cgs2, cgs3, cgt1, cgt2, cgt3;
   Consts K;
   const float *mh, *rmh, *mf, *c1, *c2, *pi0, *th0, *qv0, *rho0, *rr0, *rf0, *rrf0,
-      *thv0, *rth0, *qc0, *qi0;
+      *thv0, *rth0, *qc0, *qi0, *u0, *v0, *tauh, *tauf;
+  int irdamp;
+  float rdalpha;
   const float *uh, *ruh, *uf, *vh, *rvh, *vf;
 };
 
@@ -56,7 +58,7 @@ struct cm1g_model {
   // scratch (device)
   float *sx, *sy, *sz, *tx, *ty, *tz;   // advection fluxes / pdef work
   float *thb;                   // ping-pong partner for th3d / q3d updates
-  float *scr1, *scr2;           // Thomas coefficients
+  float *scr1, *scr2;           // u/v partner arrays of the acoustic substeps
   double *dred;                 // reduction workspace (per-k partial sums etc.)
   float *fred;                  // float reduction workspace
   double *dscal;                // device scalars: [0] mass1 [1] mass2 [2] tem(pp shift) [3] train

@@ -105,6 +105,8 @@ extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
   ARGCHK(cfg->nsound >= 1 && cfg->nsound <= 192, "cm1g_create: nsound out of range");
   ARGCHK(cfg->npt >= 0 && cfg->npt <= 8, "cm1g_create: npt must be 0..8");
   ARGCHK(cfg->dx > 0 && cfg->dy > 0 && cfg->dz > 0 && cfg->dt > 0, "cm1g_create: dx,dy,dz,dt must be > 0");
+  ARGCHK(cfg->irdamp == 0 || cfg->irdamp == 1,
+         "cm1g_create: irdamp must be 0 or 1 (damping toward the base state)");
   cm1g_model *m = new cm1g_model();  // value-initialized: pointers null, counters zero
   m->c = *cfg;
   if (m->c.csmax == 0.0f) m->c.csmax = 350.0f;
@@ -117,6 +119,9 @@ extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
   P.ibw = cfg->wbc != 1; P.ibe = cfg->ebc != 1; P.ibs = cfg->sbc != 1; P.ibn = cfg->nbc != 1;
   P.dx = cfg->dx; P.dy = cfg->dy; P.dz = cfg->dz;
   P.rdx = 1.0f / cfg->dx; P.rdy = 1.0f / cfg->dy; P.rdz = 1.0f / cfg->dz;
+  // rdalpha = 0 without damping (src/param.F:724)
+  P.irdamp = cfg->irdamp;
+  P.rdalpha = cfg->irdamp ? cfg->rdalpha : 0.0f;
   set_constants(P.K);
   const int pz = cfg->nk + 3;
   m->ntot = (size_t)P.pxy * pz;
@@ -194,7 +199,8 @@ extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
   P.th0 = m->kf[CM1G_K_TH0]; P.qv0 = m->kf[CM1G_K_QV0]; P.rho0 = m->kf[CM1G_K_RHO0];
   P.rr0 = m->kf[CM1G_K_RR0]; P.rf0 = m->kf[CM1G_K_RF0]; P.rrf0 = m->kf[CM1G_K_RRF0];
   P.thv0 = m->kf[CM1G_K_THV0]; P.rth0 = m->kf[CM1G_K_RTH0]; P.qc0 = m->kf[CM1G_K_QC0];
-  P.qi0 = m->kf[CM1G_K_QI0];
+  P.qi0 = m->kf[CM1G_K_QI0]; P.u0 = m->kf[CM1G_K_U0]; P.v0 = m->kf[CM1G_K_V0];
+  P.tauh = m->kf[CM1G_K_TAUH]; P.tauf = m->kf[CM1G_K_TAUF];
   P.uh = m->h[0]; P.ruh = m->h[1]; P.uf = m->h[2]; P.vh = m->h[3]; P.rvh = m->h[4]; P.vf = m->h[5];
   rc = cm1g_metrics(m);
   if (rc) {
@@ -270,7 +276,24 @@ extern "C" int cm1g_metrics(cm1g_model *m) {
   }
   m->P.cgs1 = w6[0]; m->P.cgs2 = w6[1]; m->P.cgs3 = w6[2];
   m->P.cgt1 = w6[3]; m->P.cgt2 = w6[4]; m->P.cgt3 = w6[5];
-  int ids[] = {CM1G_K_ZF, CM1G_K_ZH, CM1G_K_MH, CM1G_K_MF, CM1G_K_RMH, CM1G_K_RMF, CM1G_K_C1, CM1G_K_C2};
+  // Rayleigh-damping coefficients (src/param.F:8633-8668): zero below zd,
+  // 0.5*(1-cos(pi*(z-zd)/(zf(nk+1)-zd))) above, with the host's
+  // single-precision cos as the reference evaluates it; taus = tauh
+  {
+    float *tauh = m->hk[CM1G_K_TAUH], *tauf = m->hk[CM1G_K_TAUF];
+    const float pi = 3.1415926535897932384626433f, zd = m->c.zd, maxz = zf[nk + 1];
+    for (int k = 0; k <= nk + 2; k++) tauh[k] = tauf[k] = 0.0f;
+    if (m->c.irdamp >= 1 && zd < maxz) {
+      ARGCHK(zd >= 0.5f * maxz,
+             "cm1g_metrics: Rayleigh damping would cover more than half the domain (zd < maxz/2)");
+      for (int k = 1; k <= nk; k++)
+        if (zh[k] > zd) tauh[k] = 0.5f * (1.0f - cosf(pi * (zh[k] - zd) / (zf[nk + 1] - zd)));
+      for (int k = 1; k <= nk + 1; k++)
+        if (zf[k] > zd) tauf[k] = 0.5f * (1.0f - cosf(pi * (zf[k] - zd) / (zf[nk + 1] - zd)));
+    }
+  }
+  int ids[] = {CM1G_K_ZF, CM1G_K_ZH, CM1G_K_MH, CM1G_K_MF, CM1G_K_RMH, CM1G_K_RMF, CM1G_K_C1, CM1G_K_C2,
+               CM1G_K_TAUH, CM1G_K_TAUF};
   for (int id : ids) {
     int rc = upload_column(m, id);
     if (rc) return rc;
@@ -646,20 +669,31 @@ extern "C" int cm1g_solve1(cm1g_model *m) {
   const float *qa0 = f[CM1G_QA0];
   const float *qpten = f[CM1G_QPTEN], *qtten = f[CM1G_QTTEN], *qvten = f[CM1G_QVTEN], *qcten = f[CM1G_QCTEN];
   const float rdt = 1.0f / dt;
-  // ten1 = 0 (no Rayleigh damping, diffusion or turbulence) over the
-  // staggered ranges, then ppten and the diabatic ten1 on the interior
+  // ten1 over the staggered ranges: Rayleigh damping toward the base state
+  // (irdamp = 1, src/solve1.F:414-426; no terrain, so tauh(i-1) = tauh(i)
+  // and taus = tauh) or zero (:442-453); then ppten and the diabatic ten1
+  // on the interior (:1716-1756)
+  const float *ua = f[CM1G_UA], *va = f[CM1G_VA], *wa = f[CM1G_WA];
+  const int rdamp = P.irdamp;
   for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
     const Consts &K = P.K;
     const float tem = 0.0001f * K.tsmall;
     int c = IXD(P, i, j, k);
-    uten1[c] = 0.0f;
-    vten1[c] = 0.0f;
-    wten1[c] = 0.0f;
+    float u1 = 0.0f, v1 = 0.0f, w1 = 0.0f, th1 = 0.0f;
+    if (rdamp) {
+      const float ra = P.rdalpha, th = P.tauh[k];
+      u1 = -ra * 0.5f * (th + th) * (ua[c] - P.u0[k]);
+      v1 = -ra * 0.5f * (th + th) * (va[c] - P.v0[k]);
+      w1 = -ra * P.tauf[k] * wa[c];
+      th1 = -ra * th * tha[c];
+    }
+    uten1[c] = u1;
+    vten1[c] = v1;
+    wten1[c] = w1;
     if (i > ni || j > nj) {
-      thten1[c] = 0.0f;
+      thten1[c] = th1;
       return;
     }
-    float th1 = 0.0f;       // thten1 was just zeroed (irdamp = 0)
     float pt;
     if (moist) {
       float qvt = 0.0f;     // qten(nqv) was just zeroed

@@ -5,8 +5,8 @@
 !
 ! usage: cm1g_host <state-in> <state-out>
 !
-! state-in:  nsteps (int32), cm1g_config (96 bytes), the columns zf, zh,
-!            mh, mf, pi0, th0, qv0 (nk+3 floats each), then ua, va, wa,
+! state-in:  nsteps (int32), cm1g_config (108 bytes), the columns zf, zh,
+!            mh, mf, pi0, th0, qv0, u0, v0 (nk+3 floats each), then ua, va, wa,
 !            tha, ppi, qa0, qa1, qa2 in their reference shapes (i fastest)
 ! state-out: cflmax of the last step, dry-air mass and total water (3 x
 !            float64), then ua, va, wa, tha, ppi, qa0, qa1, qa2, rho, prs
@@ -24,8 +24,9 @@ program cm1g_host
   real(c_float) :: cflmax
   real(c_double) :: tmass, tmois
   character(len=1024) :: fin, fout
-  integer(c_int), parameter :: col_ids(7) = [CM1G_K_ZF, CM1G_K_ZH, CM1G_K_MH, CM1G_K_MF, &
-                                            CM1G_K_PI0, CM1G_K_TH0, CM1G_K_QV0]
+  integer(c_int), parameter :: col_ids(9) = [CM1G_K_ZF, CM1G_K_ZH, CM1G_K_MH, CM1G_K_MF, &
+                                            CM1G_K_PI0, CM1G_K_TH0, CM1G_K_QV0, CM1G_K_U0, &
+                                            CM1G_K_V0]
   integer(c_int), parameter :: in_ids(8) = [CM1G_UA, CM1G_VA, CM1G_WA, CM1G_THA, CM1G_PPI, &
                                            CM1G_QA0, CM1G_QA1, CM1G_QA2]
   integer(c_int), parameter :: in_stag(8) = [CM1G_STAG_U, CM1G_STAG_V, CM1G_STAG_W, CM1G_STAG_S, &
@@ -51,7 +52,7 @@ program cm1g_host
 
   call check(cm1g_create(cfg, m), 'cm1g_create')
   ! vertical grid, then metrics; base-state sounding, then derived columns
-  do n = 1, 7
+  do n = 1, 9
     read(u) col
     call check(cm1g_put_column(m, col_ids(n), col), 'cm1g_put_column')
     if (n == 4) call check(cm1g_metrics(m), 'cm1g_metrics')

@@ -18,6 +18,8 @@ module cm1gpu
     integer(c_int32_t) :: imoist, ptype, psolver, advwenos, advwenov
     integer(c_int32_t) :: apmasscon, irbc, nsound, npt, pdtra
     real(c_float) :: dx, dy, dz, dt, alph, kdiv, csmax
+    integer(c_int32_t) :: irdamp
+    real(c_float) :: rdalpha, zd
   end type cm1g_config
 
   ! return codes
@@ -136,6 +138,8 @@ module cm1gpu
   integer(c_int), parameter :: CM1G_K_QI0   = 19
   integer(c_int), parameter :: CM1G_K_U0    = 20
   integer(c_int), parameter :: CM1G_K_V0    = 21
+  integer(c_int), parameter :: CM1G_K_TAUH  = 22
+  integer(c_int), parameter :: CM1G_K_TAUF  = 23
 
   ! 2-D field ids
   integer(c_int), parameter :: CM1G_D_RAIN    = 0

@@ -15,7 +15,7 @@ def make_config(**kw):
     d = dict(ni=64, nj=64, nk=40, wbc=2, ebc=2, sbc=2, nbc=2, imoist=1, ptype=1,
              psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4, nsound=0, npt=0,
              pdtra=1, dx=1000.0, dy=1000.0, dz=500.0, dt=6.0, alph=0.6, kdiv=0.1,
-             csmax=350.0)
+             csmax=350.0, irdamp=0, rdalpha=3.3333333333e-3, zd=15000.0)
     d.update(kw)
     cfg = N.Config(**d)
     if cfg.nsound <= 0:

@@ -12,8 +12,9 @@
  * Array layout on the device: every 3-D array (scalar, u, v and w
  * staggerings alike) is one padded box i = -2..ni+4, j = -2..nj+4,
  * k = 0..nk+2, i fastest, element (i,j,k) at
- *     (i + 2) + (j + 2) * px + k * px * py,
- * with px >= ni + 7 (rounded up for 256-byte rows) and py = nj + 7.  This is
+ *     (i + CM1G_IOFF) + (j + 2) * px + k * px * py,
+ * with px = ni + 36 rounded up to a multiple of 64 (i = 1 starts a 128-byte
+ * line) and py = nj + 7.  This is
  * a superset of the reference bounds ib:ie(+1), jb:je(+1), kb:ke(+1) with
  * ngxy = 3 and ngz = 1 (src/cm1.F:747-777); cm1g_put_field/cm1g_get_field
  * convert from/to the caller's own Fortran-ordered box.
@@ -58,6 +59,9 @@ typedef struct cm1g_config {
   float dx, dy, dz, dt;
   float alph, kdiv;               /* param3 */
   float csmax;                    /* 350 (src/constants.F) */
+  int32_t irdamp;                 /* 0 none; 1 Rayleigh damping above zd toward the base
+                                     state (src/solve1.F:414-426, src/param.F:8633-8668) */
+  float rdalpha, zd;              /* damping rate (1/s) and height (m), param3 */
 } cm1g_config;
 
 /* 3-D field ids (names follow the reference's arrays). */
@@ -81,6 +85,7 @@ enum {
   CM1G_K_MH, CM1G_K_RMH, CM1G_K_MF, CM1G_K_RMF, CM1G_K_C1, CM1G_K_C2, CM1G_K_ZH, CM1G_K_ZF,
   CM1G_K_PI0, CM1G_K_TH0, CM1G_K_PRS0, CM1G_K_QV0, CM1G_K_RHO0, CM1G_K_RR0, CM1G_K_RF0,
   CM1G_K_RRF0, CM1G_K_THV0, CM1G_K_RTH0, CM1G_K_QC0, CM1G_K_QI0, CM1G_K_U0, CM1G_K_V0,
+  CM1G_K_TAUH, CM1G_K_TAUF,       /* Rayleigh-damping coefficients (set by cm1g_metrics) */
   CM1G_K_NFIELDS
 };
 

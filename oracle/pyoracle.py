@@ -30,7 +30,8 @@ for _n in range(8):
     FID[f"pt3d{_n}"] = F_PTA0 + 8 + _n
     FID[f"ptten{_n}"] = F_PTA0 + 16 + _n
 KFIELDS = ["mh", "rmh", "mf", "rmf", "c1", "c2", "zh", "zf", "pi0", "th0", "prs0",
-           "qv0", "rho0", "rr0", "rf0", "rrf0", "thv0", "rth0", "qc0", "qi0", "u0", "v0"]
+           "qv0", "rho0", "rr0", "rf0", "rrf0", "thv0", "rth0", "qc0", "qi0", "u0", "v0",
+           "tauh", "tauf"]
 KID = {n: i for i, n in enumerate(KFIELDS)}
 D2 = {"rain": 0, "prate": 1, "radbcw": 2, "radbce": 3, "radbcs": 4, "radbcn": 5}
 
@@ -40,12 +41,17 @@ _INTS = ("ni", "nj", "nk", "wbc", "ebc", "sbc", "nbc", "imoist", "ptype", "psolv
 _REALS = ("dx", "dy", "dz", "dt", "alph", "kdiv", "csmax")
 
 
+_TAIL = (("irdamp", "i"), ("rdalpha", "r"), ("zd", "r"))
+
+
 class Config(C.Structure):
-    _fields_ = [(n, C.c_int32) for n in _INTS] + [(n, C.c_float) for n in _REALS]
+    _fields_ = [(n, C.c_int32) for n in _INTS] + [(n, C.c_float) for n in _REALS] + \
+        [(n, C.c_int32 if t == "i" else C.c_float) for n, t in _TAIL]
 
 
 class Config64(C.Structure):
-    _fields_ = [(n, C.c_int32) for n in _INTS] + [(n, C.c_double) for n in _REALS]
+    _fields_ = [(n, C.c_int32) for n in _INTS] + [(n, C.c_double) for n in _REALS] + \
+        [(n, C.c_int32 if t == "i" else C.c_double) for n, t in _TAIL]
 
 
 def _load(fp64=False):
@@ -95,7 +101,7 @@ def make_config(**kw):
     d = dict(ni=64, nj=64, nk=40, wbc=2, ebc=2, sbc=2, nbc=2, imoist=1, ptype=1,
              psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4, nsound=4, npt=0,
              pdtra=1, dx=1000.0, dy=1000.0, dz=500.0, dt=6.0, alph=0.6, kdiv=0.1,
-             csmax=350.0)
+             csmax=350.0, irdamp=0, rdalpha=3.3333333333e-3, zd=15000.0)
     d.update(kw)
     return Config(**d)
 

@@ -43,6 +43,8 @@ typedef struct cm1o_config {
   float dx, dy, dz, dt;
   float alph, kdiv;
   float csmax;                    /* 350 (src/constants.F) */
+  int32_t irdamp;                 /* 1 = Rayleigh damping above zd (src/solve1.F:414-426) */
+  float rdalpha, zd;              /* damping rate (1/s) and height (m) */
 } cm1o_config;
 
 /* Field identifiers for cm1o_field(). */
@@ -64,7 +66,7 @@ enum {
 enum {
   K_MH, K_RMH, K_MF, K_RMF, K_C1, K_C2, K_ZH, K_ZF,
   K_PI0, K_TH0, K_PRS0, K_QV0, K_RHO0, K_RR0, K_RF0, K_RRF0, K_THV0, K_RTH0,
-  K_QC0, K_QI0, K_U0, K_V0, K_NKFIELDS
+  K_QC0, K_QI0, K_U0, K_V0, K_TAUH, K_TAUF, K_NKFIELDS
 };
 
 /* 2-D fields */

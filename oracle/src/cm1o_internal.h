@@ -21,6 +21,7 @@ static inline double o64_exp(double x) { return exp(x); }
 static inline double o64_pow(double x, double y) { return pow(x, y); }
 static inline double o64_copysign(double x, double y) { return copysign(x, y); }
 static inline long o64_lround(double x) { return lround(x); }
+static inline double o64_cos(double x) { return cos(x); }
 #define float double
 #define fabsf o64_fabs
 #define fmaxf o64_fmax
@@ -30,6 +31,7 @@ static inline long o64_lround(double x) { return lround(x); }
 #define powf o64_pow
 #define copysignf o64_copysign
 #define lroundf o64_lround
+#define cosf o64_cos
 #endif
 
 #include "cm1o.h"

@@ -98,6 +98,20 @@ void cm1o_set_extrap(cm1o_model *M, const float *cgs6) {
     c2[k] = (zf[k] - zh[k - 1]) / (zh[k] - zh[k - 1]);
     c1[k] = 1.0f - c2[k];
   }
+  /* Rayleigh-damping coefficients (src/param.F:8633-8668): zero below
+   * zd, 0.5*(1-cos(pi*(z-zd)/(zf(nk+1)-zd))) above, with the host's
+   * single-precision cos as the reference evaluates it. */
+  {
+    float *tauh = M->kf[K_TAUH], *tauf = M->kf[K_TAUF];
+    const float pi = 3.1415926535897932384626433f, zd = M->c.zd, maxz = zf[nk + 1];
+    for (k = 0; k <= nk + 2; k++) { tauh[k] = 0.0f; tauf[k] = 0.0f; }
+    if (M->c.irdamp >= 1 && zd < maxz) {
+      for (k = 1; k <= nk; k++)
+        if (zh[k] > zd) tauh[k] = 0.5f * (1.0f - cosf(pi * (zh[k] - zd) / (zf[nk + 1] - zd)));
+      for (k = 1; k <= nk + 1; k++)
+        if (zf[k] > zd) tauf[k] = 0.5f * (1.0f - cosf(pi * (zf[k] - zd) / (zf[nk + 1] - zd)));
+    }
+  }
   if (cgs6) {
     M->cgs1 = cgs6[0]; M->cgs2 = cgs6[1]; M->cgs3 = cgs6[2];
     M->cgt1 = cgs6[3]; M->cgt2 = cgs6[4]; M->cgt3 = cgs6[5];
@@ -255,14 +269,29 @@ void o_solve1(cm1o_model *M) {
   const float *pi0 = M->kf[K_PI0], *th0 = M->kf[K_TH0];
   float *ppten = M->f[F_PPTEN], *thten1 = M->f[F_THTEN1];
   int i, j, k, n;
-  for (k = 1; k <= nk; k++)
-    for (j = 1; j <= nj + 1; j++)
-      for (i = 1; i <= ni + 1; i++) {
-        A3(M->f[F_UTEN1], i, j, k) = 0.0f;
-        A3(M->f[F_VTEN1], i, j, k) = 0.0f;
-        A3(M->f[F_WTEN1], i, j, k) = 0.0f;
-        A3(thten1, i, j, k) = 0.0f;
-      }
+  if (M->c.irdamp == 1) {
+    /* Rayleigh damping toward the base state, src/solve1.F:414-426
+     * (no terrain: tauh, tauf, u0, v0 are columns; taus = tauh) */
+    const float ra = M->c.rdalpha;
+    const float *tauh = M->kf[K_TAUH], *tauf = M->kf[K_TAUF], *u0 = M->kf[K_U0], *v0 = M->kf[K_V0];
+    for (k = 1; k <= nk; k++)
+      for (j = 1; j <= nj + 1; j++)
+        for (i = 1; i <= ni + 1; i++) {
+          A3(M->f[F_UTEN1], i, j, k) = -ra * 0.5f * (tauh[k] + tauh[k]) * (A3(M->f[F_UA], i, j, k) - u0[k]);
+          A3(M->f[F_VTEN1], i, j, k) = -ra * 0.5f * (tauh[k] + tauh[k]) * (A3(M->f[F_VA], i, j, k) - v0[k]);
+          A3(M->f[F_WTEN1], i, j, k) = -ra * tauf[k] * A3(M->f[F_WA], i, j, k);
+          A3(thten1, i, j, k) = -ra * tauh[k] * A3(M->f[F_THA], i, j, k);
+        }
+  } else {
+    for (k = 1; k <= nk; k++)
+      for (j = 1; j <= nj + 1; j++)
+        for (i = 1; i <= ni + 1; i++) {
+          A3(M->f[F_UTEN1], i, j, k) = 0.0f;
+          A3(M->f[F_VTEN1], i, j, k) = 0.0f;
+          A3(M->f[F_WTEN1], i, j, k) = 0.0f;
+          A3(thten1, i, j, k) = 0.0f;
+        }
+  }
   if (M->c.imoist == 1)
     for (n = 0; n < 3; n++)
       for (k = 1; k <= nk; k++)

@@ -14,12 +14,13 @@ PROG = [("ua", "u"), ("va", "v"), ("wa", "w"), ("tha", None), ("ppi", None),
 
 def case(ni=40, nj=36, nk=30, dx=1000.0, dy=1000.0, dz=500.0, dt=6.0, bc=2, imoist=1,
          ptype=1, nsound=6, amp=3.0, umax=25.0, stretch=None, noise=0.1, seed=7, npt=0,
-         bfx=0.45, bfy=0.55, brh=None, psolver=3, advwenos=0, advwenov=0):
+         bfx=0.45, bfy=0.55, brh=None, psolver=3, advwenos=0, advwenov=0, irdamp=0, zd=None):
     cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=bc, ebc=bc, sbc=bc, nbc=bc, imoist=imoist,
                 ptype=ptype, psolver=psolver, advwenos=advwenos, advwenov=advwenov,
                 apmasscon=1, irbc=4,
                 nsound=nsound, npt=npt, pdtra=1, dx=dx, dy=dy, dz=dz, dt=dt, alph=0.6,
-                kdiv=0.1, csmax=350.0)
+                kdiv=0.1, csmax=350.0, irdamp=irdamp, rdalpha=3.3333333333e-3,
+                zd=(0.7 * nk * dz if zd is None else zd))
     bub = [dict(x=bfx * ni * dx, y=bfy * nj * dy, z=1400.0, amp=amp,
                 rh=(8.0 * dx if brh is None else brh), rv=1400.0)]
     cols, fields, dzn = hinit.initial_state(ni, nj, nk, dx, dy, dz, stretch=stretch,
@@ -39,8 +40,9 @@ def make_oracle(cfgd, cols, fields, libm=0, fp64=False):
     for n in ("zf", "zh", "mh", "mf"):
         o.kf(n)[:cfgd["nk"] + 3] = cols[n]
     o.L.cm1o_set_extrap(o.h, None)
-    for n in ("pi0", "th0", "qv0"):
-        o.kf(n)[:cfgd["nk"] + 3] = cols[n]
+    for n in ("pi0", "th0", "qv0", "u0", "v0"):
+        if n in cols:
+            o.kf(n)[:cfgd["nk"] + 3] = cols[n]
     o.base_derive()
     for name, arr in fields.items():
         o.f(name)[:, :, :] = arr
@@ -75,15 +77,24 @@ def load_fixture(path):
     z = np.load(path)
     ni, nj, nk, nsteps, bc = [int(v) for v in z["meta"]]
     dt = float(z["dt"][0])
-    psolver, advwenos, advwenov = [int(v) for v in z["opts"]] if "opts" in z else (3, 0, 0)
+    opts = [int(v) for v in z["opts"]] if "opts" in z else [3, 0, 0]
+    psolver, advwenos, advwenov = opts[:3]
+    irdamp = opts[3] if len(opts) > 3 else 0
+    rdalpha, zd = [float(v) for v in z["rdamp"]] if "rdamp" in z else (3.3333333333e-3, 15000.0)
     cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=bc, ebc=bc, sbc=bc, nbc=bc, imoist=1, ptype=1,
                 psolver=psolver, advwenos=advwenos, advwenov=advwenov, apmasscon=1, irbc=4,
                 nsound=0, npt=0, pdtra=1, dx=1000.0, dy=1000.0, dz=500.0, dt=dt, alph=0.6,
-                kdiv=0.1, csmax=350.0)
+                kdiv=0.1, csmax=350.0, irdamp=irdamp, rdalpha=rdalpha, zd=zd)
     cols, _ = hinit.vertical_grid(nk, 500.0)
     for n in ("pi0", "th0", "qv0"):
         c = np.zeros(nk + 3, np.float32)
         c[1:nk + 1] = z[n]
+        cols[n] = c
+    # base-state winds: the reference starts from ua = u0, va = v0
+    # (horizontally uniform columns)
+    for n, key in (("u0", "t0_u"), ("v0", "t0_v")):
+        c = np.zeros(nk + 3, np.float32)
+        c[1:nk + 1] = z[key][:, 0, 0]
         cols[n] = c
     f = {n: hinit.blank(ni, nj, nk) for n in ("ua", "va", "wa", "tha", "ppi", "qa0", "qa1", "qa2")}
     f["ua"][1:nk + 1, 3:nj + 3, 3:ni + 4] = z["t0_u"]

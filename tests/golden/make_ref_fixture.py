@@ -9,10 +9,10 @@ stores its GrADS output at t=0 and after N large steps as float32 arrays:
 * ``tN_*``: the same prognostics plus rho and prs after N steps.
 
 * ``opts``: the dynamical-core switches the run used (psolver, advwenos,
-  advwenov).
+  advwenov, irdamp), and ``rdamp``: (rdalpha, zd) of the Rayleigh damper.
 
 Usage: python tests/golden/make_ref_fixture.py [name ni nj nk nsteps dt wbc
-[psolver advwenos advwenov]]
+[psolver advwenos advwenov [irdamp zd]]]
 """
 import os
 import sys
@@ -25,25 +25,28 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import refrun  # noqa: E402
 
 
-OPT_KEYS = ("psolver", "advwenos", "advwenov")
-OPT_DEFAULTS = dict(psolver=3, advwenos=0, advwenov=0)
+OPT_KEYS = ("psolver", "advwenos", "advwenov", "irdamp")
+OPT_DEFAULTS = dict(psolver=3, advwenos=0, advwenov=0, irdamp=0)
+RDALPHA = 3.3333333333e-3
 
 
 def make(name, ni, nj, nk, nsteps, dt, bc, workdir="/tmp/cm1_fixture", opts=None, threads=1,
-         outdir=HERE):
+         outdir=HERE, zd=15000.0):
     """Run the reference and write <outdir>/<name>.npz.  opts: namelist
-    switches of the dynamical core (psolver, advwenos, advwenov)."""
+    switches of the dynamical core (psolver, advwenos, advwenov, irdamp);
+    zd: the damping height when irdamp = 1."""
     o = dict(OPT_DEFAULTS)
     o.update(opts or {})
     ov = dict(nx=ni, ny=nj, nz=nk, dtl=dt, timax=nsteps * dt, tapfrq=nsteps * dt,
-              wbc=bc, ebc=bc, sbc=bc, nbc=bc, **o)
+              wbc=bc, ebc=bc, sbc=bc, nbc=bc, rdalpha=RDALPHA, zd=zd, **o)
     refrun.run_reference(workdir, ov, threads=threads)
     S = refrun.read_grads(workdir, "cm1out_s.ctl")
     U = refrun.read_grads(workdir, "cm1out_u.ctl")
     V = refrun.read_grads(workdir, "cm1out_v.ctl")
     W = refrun.read_grads(workdir, "cm1out_w.ctl")
     out = dict(meta=np.array([ni, nj, nk, nsteps, bc], np.int32), dt=np.array([dt], np.float32),
-               opts=np.array([o[k] for k in OPT_KEYS], np.int32))
+               opts=np.array([o[k] for k in OPT_KEYS], np.int32),
+               rdamp=np.array([RDALPHA, zd], np.float32))
     for t, tag in ((0, "t0"), (1, "tN")):
         out[f"{tag}_u"] = U["u"][t]
         out[f"{tag}_v"] = V["v"][t]
@@ -65,9 +68,10 @@ def make(name, ni, nj, nk, nsteps, dt, bc, workdir="/tmp/cm1_fixture", opts=None
 if __name__ == "__main__":
     if len(sys.argv) > 1:
         a = sys.argv[1:]
-        opts = dict(zip(OPT_KEYS, (int(v) for v in a[7:10])))
+        opts = dict(zip(OPT_KEYS, (int(v) for v in a[7:11])))
+        zd = float(a[11]) if len(a) > 11 else 15000.0
         print(make(a[0], int(a[1]), int(a[2]), int(a[3]), int(a[4]), float(a[5]), int(a[6]),
-                   opts=opts))
+                   opts=opts, zd=zd))
     else:
         print(make("ref_open_24x20x18_10", 24, 20, 18, 10, 6.0, 2))
         print(make("ref_periodic_20x24x18_10", 20, 24, 18, 10, 6.0, 1))

@@ -38,9 +38,9 @@ def test_fortran_host_matches_python_host(require_gpu, tmp_path):
     with open(fin, "wb") as f:
         f.write(np.int32(nsteps).tobytes())
         f.write(bytes(cfg))
-        for n in ("zf", "zh", "mh", "mf", "pi0", "th0", "qv0"):
+        for n in ("zf", "zh", "mh", "mf", "pi0", "th0", "qv0", "u0", "v0"):
             c = np.zeros(nk + 3, np.float32)
-            c[:] = np.asarray(cols[n], np.float32)[:nk + 3]
+            c[:] = np.asarray(cols.get(n, np.zeros(nk + 3)), np.float32)[:nk + 3]
             f.write(c.tobytes())
         for n, stag in IN_FIELDS:
             f.write(np.ascontiguousarray(ref_shape(fields[n], ni, nj, nk, stag)).tobytes())

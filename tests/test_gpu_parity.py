@@ -51,7 +51,9 @@ def run_pair(nsteps, **kw):
     (2, dict(advwenos=1, advwenov=1)),
     (1, dict(psolver=2, advwenos=1, nsound=10)),
     (2, dict(psolver=2, advwenos=2, advwenov=2, nsound=10)),
-], ids=["open", "periodic", "open-weno", "periodic-sounde-weno", "open-sounde-weno-last"])
+    (2, dict(irdamp=1, zd=10000.0)),
+], ids=["open", "periodic", "open-weno", "periodic-sounde-weno", "open-sounde-weno-last",
+        "open-rdamp"])
 def test_one_step_dynamics_bitwise(require_gpu, bc, opts):
     """One large step: every prognostic field bit-identical."""
     res, _, _, _ = run_pair(1, bc=bc, ni=40, nj=36, nk=30, amp=3.0, **opts)
@@ -93,5 +95,16 @@ def test_ten_steps_weno_sounde(require_gpu, opts):
     kw = dict(config0(), **opts)
     res, _, _, _ = run_pair(10, **kw)
     print(res)
+    for name, (err, nbad, n) in res.items():
+        assert nbad == 0, (name, err, nbad)
+
+
+def test_ten_steps_rayleigh_damping(require_gpu):
+    """The shipped default irdamp=1 (src/solve1.F:414-426): the 64x64x40
+    supercell with the damping layer over the top 6 km (zd = 14 km)."""
+    kw = dict(config0(), irdamp=1, zd=14000.0)
+    res, o, _, cfgd = run_pair(10, **kw)
+    print(res)
+    assert o.kf("tauh")[cfgd["nk"]] > 0.9
     for name, (err, nbad, n) in res.items():
         assert nbad == 0, (name, err, nbad)
