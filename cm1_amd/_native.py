@@ -92,6 +92,9 @@ SIGNATURES = {
                                     C.POINTER(C.c_float)]),
     "cm1g_getdiv": (C.c_int, [C.c_void_p, C.c_int]),
     "cm1g_getdivx": (C.c_int, [C.c_void_p]),
+    "cm1g_state_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "cm1g_state_save": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "cm1g_state_load": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "cm1g_dtsmall": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_int32)]),
     "cm1g_budgets": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "cm1g_scalars": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),

@@ -416,6 +416,107 @@ extern "C" int cm1g_get_2d(cm1g_model *m, int id, float *host) {
   return 0;
 }
 
+// ------------------------------------------------------------ checkpoint
+// Arrays a step carries into the next: the time-level "a" arrays and their
+// "3d" copies (halos included), ppx, the previous step's diabatic
+// tendencies, rho/prs/rr/rf, the tracers, and the 2-D fields (rain, prate,
+// and the radiative-boundary speeds of the last stage, which the next
+// step's first acoustic substeps use).
+struct StateHeader {
+  char magic[8];
+  int32_t ni, nj, nk, px, py, npt, imoist, n3d, n2d, reserved;
+  double dscal[4];
+  float cflmax, pad;
+};
+static_assert(sizeof(StateHeader) == 88, "checkpoint header layout");
+
+static int state_ids(const cm1g_model *m, int *ids) {
+  static const int base[] = {CM1G_UA, CM1G_VA, CM1G_WA, CM1G_THA, CM1G_PPI, CM1G_U3D, CM1G_V3D,
+                             CM1G_W3D, CM1G_TH3D, CM1G_PP3D, CM1G_PPX, CM1G_RHO, CM1G_PRS,
+                             CM1G_RR, CM1G_RF};
+  int n = 0;
+  for (int id : base) ids[n++] = id;
+  if (m->c.imoist == 1) {
+    const int moist[] = {CM1G_QA0, CM1G_QA1, CM1G_QA2, CM1G_Q3D0, CM1G_Q3D1, CM1G_Q3D2,
+                         CM1G_QPTEN, CM1G_QTTEN, CM1G_QVTEN, CM1G_QCTEN};
+    for (int id : moist) ids[n++] = id;
+  }
+  for (int t = 0; t < m->c.npt; t++) {
+    ids[n++] = CM1G_PTA0 + t;
+    ids[n++] = CM1G_PT3D0 + t;
+  }
+  return n;
+}
+
+static size_t state_box(const cm1g_model *m) {
+  return ((size_t)m->P.ni + 7) * (size_t)m->P.py * (size_t)(m->P.nk + 3) * sizeof(float);
+}
+static size_t state_2d(const cm1g_model *m) {
+  return ((size_t)m->P.pxy + (size_t)(m->P.px + m->P.py) * (m->c.nk + 3)) * sizeof(float);
+}
+
+extern "C" int cm1g_state_bytes(cm1g_model *m, int64_t *nbytes) {
+  ARGCHK(m && nbytes, "cm1g_state_bytes: null argument");
+  int ids[64];
+  const int n3 = state_ids(m, ids);
+  *nbytes = (int64_t)(sizeof(StateHeader) + n3 * state_box(m) + CM1G_D_NFIELDS * state_2d(m));
+  return 0;
+}
+
+extern "C" int cm1g_state_save(cm1g_model *m, void *host, int64_t nbytes) {
+  ARGCHK(m && host, "cm1g_state_save: null argument");
+  int64_t need = 0;
+  cm1g_state_bytes(m, &need);
+  ARGCHK(nbytes >= need, "cm1g_state_save: buffer too small");
+  int ids[64];
+  const int n3 = state_ids(m, ids);
+  StateHeader h;
+  memset(&h, 0, sizeof(h));
+  memcpy(h.magic, "CM1GST01", 8);
+  h.ni = m->P.ni; h.nj = m->P.nj; h.nk = m->P.nk; h.px = m->P.px; h.py = m->P.py;
+  h.npt = m->c.npt; h.imoist = m->c.imoist; h.n3d = n3; h.n2d = CM1G_D_NFIELDS;
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(h.dscal, m->dscal, 4 * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&h.cflmax, m->fred, sizeof(float), hipMemcpyDeviceToHost));
+  char *p = (char *)host;
+  memcpy(p, &h, sizeof(h));
+  p += sizeof(h);
+  for (int n = 0; n < n3; n++, p += state_box(m)) {
+    int rc = cm1g_get_field(m, ids[n], (float *)p);
+    if (rc) return rc;
+  }
+  for (int n = 0; n < CM1G_D_NFIELDS; n++, p += state_2d(m))
+    HIPCHK(hipMemcpy(p, m->d2[n], state_2d(m), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int cm1g_state_load(cm1g_model *m, const void *host, int64_t nbytes) {
+  ARGCHK(m && host, "cm1g_state_load: null argument");
+  int64_t need = 0;
+  cm1g_state_bytes(m, &need);
+  ARGCHK(nbytes >= need, "cm1g_state_load: buffer too small");
+  StateHeader h;
+  memcpy(&h, host, sizeof(h));
+  int ids[64];
+  const int n3 = state_ids(m, ids);
+  ARGCHK(memcmp(h.magic, "CM1GST01", 8) == 0, "cm1g_state_load: not a cm1gpu state");
+  ARGCHK(h.ni == m->P.ni && h.nj == m->P.nj && h.nk == m->P.nk && h.px == m->P.px &&
+             h.py == m->P.py && h.npt == m->c.npt && h.imoist == m->c.imoist && h.n3d == n3 &&
+             h.n2d == CM1G_D_NFIELDS,
+         "cm1g_state_load: the state was saved from a model of another configuration");
+  const char *p = (const char *)host + sizeof(h);
+  for (int n = 0; n < n3; n++, p += state_box(m)) {
+    int rc = cm1g_put_field(m, ids[n], (const float *)p);
+    if (rc) return rc;
+  }
+  for (int n = 0; n < CM1G_D_NFIELDS; n++, p += state_2d(m))
+    HIPCHK(hipMemcpy(m->d2[n], p, state_2d(m), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(m->dscal, h.dscal, 4 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(m->fred, &h.cflmax, sizeof(float), hipMemcpyHostToDevice));
+  m->mass1 = h.dscal[0];
+  return 0;
+}
+
 extern "C" int cm1g_field_ptr(cm1g_model *m, int id, float **dptr) {
   ARGCHK(field_ok(m, id) && dptr, "cm1g_field_ptr: bad field id");
   *dptr = m->f[id];

@@ -234,6 +234,142 @@ module cm1gpu
       type(c_ptr), value :: m
       real(c_double), intent(out) :: tmass, tmois
     end function
+    integer(c_int) function cm1g_dims(m, px, py, pz) bind(C, name="cm1g_dims")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: m
+      integer(c_int64_t), intent(out) :: px, py, pz
+    end function
+    integer(c_int) function cm1g_put_field(m, id, a) bind(C, name="cm1g_put_field")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      integer(c_int), value :: id
+      real(c_float), intent(in) :: a(*)
+    end function
+    integer(c_int) function cm1g_get_field(m, id, a) bind(C, name="cm1g_get_field")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      integer(c_int), value :: id
+      real(c_float), intent(out) :: a(*)
+    end function
+    integer(c_int) function cm1g_put_2d(m, id, a) bind(C, name="cm1g_put_2d")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      integer(c_int), value :: id
+      real(c_float), intent(in) :: a(*)
+    end function
+    integer(c_int) function cm1g_get_2d(m, id, a) bind(C, name="cm1g_get_2d")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      integer(c_int), value :: id
+      real(c_float), intent(out) :: a(*)
+    end function
+    integer(c_int) function cm1g_field_ptr(m, id, dptr) bind(C, name="cm1g_field_ptr")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: id
+      type(c_ptr), intent(out) :: dptr
+    end function
+    ! routine-level entries (the reference routine each replaces is named
+    ! in include/cm1gpu.h)
+    integer(c_int) function cm1g_advs(m, nrk, s_id, s0_id, sten_id, pdef, pdefweno, dt, weps) &
+        bind(C, name="cm1g_advs")
+      import :: c_int, c_ptr, c_float, c_double
+      type(c_ptr), value :: m
+      integer(c_int), value :: nrk, s_id, s0_id, sten_id, pdef, pdefweno
+      real(c_float), value :: dt
+      real(c_double), value :: weps
+    end function
+    integer(c_int) function cm1g_advu(m, nrk) bind(C, name="cm1g_advu")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: nrk
+    end function
+    integer(c_int) function cm1g_advv(m, nrk) bind(C, name="cm1g_advv")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: nrk
+    end function
+    integer(c_int) function cm1g_advw(m, nrk) bind(C, name="cm1g_advw")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: nrk
+    end function
+    integer(c_int) function cm1g_sound(m, nrk, dttmp) bind(C, name="cm1g_sound")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      integer(c_int), value :: nrk
+      real(c_float), value :: dttmp
+    end function
+    integer(c_int) function cm1g_kessler_branch(m) bind(C, name="cm1g_kessler_branch")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_pdefq(m, rmax, q_id) bind(C, name="cm1g_pdefq")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      real(c_float), value :: rmax
+      integer(c_int), value :: q_id
+    end function
+    integer(c_int) function cm1g_bc(m, id, kind) bind(C, name="cm1g_bc")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: id, kind
+    end function
+    integer(c_int) function cm1g_calccflquick(m, dt, u_id, v_id, w_id, cflmax) &
+        bind(C, name="cm1g_calccflquick")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      real(c_float), value :: dt
+      integer(c_int), value :: u_id, v_id, w_id
+      real(c_float), intent(out) :: cflmax
+    end function
+    integer(c_int) function cm1g_getdiv(m, div_id) bind(C, name="cm1g_getdiv")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: div_id
+    end function
+    integer(c_int) function cm1g_getdivx(m) bind(C, name="cm1g_getdivx")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_scalars(m, out4) bind(C, name="cm1g_scalars")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: m
+      real(c_double), intent(out) :: out4(4)
+    end function
+    ! device-state checkpoint
+    integer(c_int) function cm1g_state_bytes(m, nbytes) bind(C, name="cm1g_state_bytes")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: m
+      integer(c_int64_t), intent(out) :: nbytes
+    end function
+    integer(c_int) function cm1g_state_save(m, buf, nbytes) bind(C, name="cm1g_state_save")
+      import :: c_int, c_ptr, c_int64_t, c_int8_t
+      type(c_ptr), value :: m
+      integer(c_int8_t), intent(out) :: buf(*)
+      integer(c_int64_t), value :: nbytes
+    end function
+    integer(c_int) function cm1g_state_load(m, buf, nbytes) bind(C, name="cm1g_state_load")
+      import :: c_int, c_ptr, c_int64_t, c_int8_t
+      type(c_ptr), value :: m
+      integer(c_int8_t), intent(in) :: buf(*)
+      integer(c_int64_t), value :: nbytes
+    end function
+    ! kernel timer (measurement hook)
+    integer(c_int) function cm1g_ktimer_start(m, which) bind(C, name="cm1g_ktimer_start")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: which
+    end function
+    integer(c_int) function cm1g_ktimer_stop(m, total_ms, launches, bytes_per_launch, name, &
+                                             name_len) bind(C, name="cm1g_ktimer_stop")
+      import :: c_int, c_ptr, c_double, c_int64_t, c_char
+      type(c_ptr), value :: m
+      real(c_double), intent(out) :: total_ms, bytes_per_launch
+      integer(c_int64_t), intent(out) :: launches
+      character(kind=c_char), intent(out) :: name(*)
+      integer(c_int), value :: name_len
+    end function
   end interface
 
 contains

@@ -108,6 +108,18 @@ class Model:
     def sync(self):
         N.check(self.L.cm1g_sync(self.h), "cm1g_sync")
 
+    def save_state(self):
+        """Device-state checkpoint (cm1g_state_save) as a uint8 array."""
+        n = C.c_int64()
+        N.check(self.L.cm1g_state_bytes(self.h, C.byref(n)), "cm1g_state_bytes")
+        buf = np.empty(n.value, np.uint8)
+        N.check(self.L.cm1g_state_save(self.h, buf.ctypes.data, n.value), "cm1g_state_save")
+        return buf
+
+    def load_state(self, buf):
+        buf = np.ascontiguousarray(buf, np.uint8)
+        N.check(self.L.cm1g_state_load(self.h, buf.ctypes.data, buf.size), "cm1g_state_load")
+
     def budgets(self):
         a, b = C.c_double(), C.c_double()
         N.check(self.L.cm1g_budgets(self.h, C.byref(a), C.byref(b)), "cm1g_budgets")

@@ -191,6 +191,19 @@ int cm1g_dtsmall(const cm1g_config *cfg, int32_t *nsound);
  * src/misclibs.F:3134-3191) and total water qv+qc+qr (totmois,
  * src/misclibs.F:3199-3264). */
 int cm1g_budgets(cm1g_model *m, double *tmass, double *tmois);
+/* Device-state checkpoint (the restart write/read of
+ * src/restart_write.F:173-306 and src/restart_read.F, in this library's own
+ * layout): every array and scalar a step carries into the next, taken
+ * between steps.  cm1g_state_bytes gives the size; cm1g_state_save copies
+ * the state into a host buffer of that size; cm1g_state_load restores it
+ * into a model created with the same configuration (checked), after which
+ * stepping continues bit for bit as if uninterrupted.  Layout: an 88-byte
+ * header, then 3-D fields as logical boxes (cm1g_put_field order), then the
+ * 2-D fields in device order. */
+int cm1g_state_bytes(cm1g_model *m, int64_t *nbytes);
+int cm1g_state_save(cm1g_model *m, void *host, int64_t nbytes);
+int cm1g_state_load(cm1g_model *m, const void *host, int64_t nbytes);
+
 /* Scalars carried between steps: [0] mass1, [1] train (domain rain
  * accumulation, src/kessler.F:472), [2] cflmax, [3] mass2. */
 int cm1g_scalars(cm1g_model *m, double *out4);
