@@ -49,11 +49,14 @@ __device__ __forceinline__ float pdef_dum(const KP &P, int dir, const float *vel
   const int c = IXD(P, i, j, k);
   float lo, hi;
   pdef_faces(P, dir, vel, scale, c, k, lo, hi);
-  float div;
-  if (dir == 0) div = (hi - lo) * P.rdx * P.uh[i + 2];
-  else if (dir == 1) div = (hi - lo) * P.rdy * P.vh[j + 2];
-  else div = (hi - lo) * P.rdz * P.mh[k];
-  float d = P.rho0[k] * s0[c] + dt * (adv[c] + s[c] * div);
+  // s*(flux difference)*metric, associated left to right as the reference
+  // writes it (src/pdef.F:44-45, :256-257, :482-483)
+  const float sc = s[c];
+  float sdiv;
+  if (dir == 0) sdiv = sc * (hi - lo) * P.rdx * P.uh[i + 2];
+  else if (dir == 1) sdiv = sc * (hi - lo) * P.rdy * P.vh[j + 2];
+  else sdiv = sc * (hi - lo) * P.rdz * P.mh[k];
+  float d = P.rho0[k] * s0[c] + dt * (adv[c] + sdiv);
   if (dir == 0) {
     if (P.wbc == 2 && P.ibw == 1 && i == 1 && lo >= 0.0f) d = P.rho0[k] * s0[c];
     if (P.ebc == 2 && P.ibe == 1 && i == P.ni && hi <= 0.0f) d = P.rho0[k] * s0[c];
@@ -136,8 +139,7 @@ __global__ void __launch_bounds__(256) k_pdef_xy(KP P, PdefArgs a) {
     if (flag) {
       float lo, hi;
       pdef_faces(P, 0, a.rru, a.scale, c, k, lo, hi);
-      const float div = (hi - lo) * P.rdx * P.uh[i + 2];
-      out = (d[2] + mass - r0s0) * rdt - a.s[c] * div;
+      out = (d[2] + mass - r0s0) * rdt - a.s[c] * (hi - lo) * P.rdx * P.uh[i + 2];
     }
     a.ax_out[c] = out;
   }
@@ -153,8 +155,7 @@ __global__ void __launch_bounds__(256) k_pdef_xy(KP P, PdefArgs a) {
     if (flag) {
       float lo, hi;
       pdef_faces(P, 1, a.rrv, a.scale, c, k, lo, hi);
-      const float div = (hi - lo) * P.rdy * P.vh[j + 2];
-      out = (d[2] + mass - r0s0) * rdt - a.s[c] * div;
+      out = (d[2] + mass - r0s0) * rdt - a.s[c] * (hi - lo) * P.rdy * P.vh[j + 2];
     }
     a.ay_out[c] = out;
   }
@@ -182,8 +183,7 @@ __global__ void __launch_bounds__(256) k_pdef_z(KP P, PdefArgs a, ColGrid cg) {
     if (flag) {
       float lo, hi;
       pdef_faces(P, 2, a.rrw, a.scale, c, k, lo, hi);
-      const float div = (hi - lo) * P.rdz * P.mh[k];
-      az = (d0 + mass - P.rho0[k] * a.s0[c]) * rdt - sc * div;
+      az = (d0 + mass - P.rho0[k] * a.s0[c]) * rdt - sc * (hi - lo) * P.rdz * P.mh[k];
     }
     float st = a.sten0[c];
     if (a.t11) st = st - a.t11[c] * a.thterm[c];

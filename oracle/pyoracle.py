@@ -74,6 +74,8 @@ def _load(fp64=False):
     lib.cm1o_base_derive.argtypes = [vp]
     lib.cm1o_getset.argtypes = [vp, C.c_int]
     lib.cm1o_step.argtypes = [vp]
+    for name in ("cm1o_solve1", "cm1o_solve2", "cm1o_mp_driver", "cm1o_solve3"):
+        getattr(lib, name).argtypes = [vp]
     lib.cm1o_bc.argtypes = [vp, C.c_int, C.c_int]
     lib.cm1o_advs.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
     for name in ("cm1o_advu", "cm1o_advv", "cm1o_advw"):

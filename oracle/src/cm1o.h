@@ -92,6 +92,11 @@ void cm1o_base_derive(cm1o_model *m);
 void cm1o_getset(cm1o_model *m, int fresh_start);
 /* One full large step: solve1, solve2, mp_driver, solve3 (src/cm1.F:3313-3476). */
 void cm1o_step(cm1o_model *m);
+/* Its four parts (for phase-by-phase comparisons). */
+void cm1o_solve1(cm1o_model *m);
+void cm1o_solve2(cm1o_model *m);
+void cm1o_mp_driver(cm1o_model *m);
+void cm1o_solve3(cm1o_model *m);
 
 /* Individual routines on the model's own arrays (for unit parity tests). */
 void cm1o_bc(cm1o_model *m, int id, int kind);  /* kind 0 s,1 p,2 u,3 v,4 w(flag1),5 w(flag0) */

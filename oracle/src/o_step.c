@@ -663,6 +663,10 @@ void cm1o_step(cm1o_model *M) {
 }
 
 /* ----------------------------------------------------- unit entry points */
+void cm1o_solve1(cm1o_model *M) { o_solve1(M); }
+void cm1o_solve2(cm1o_model *M) { o_solve2(M); }
+void cm1o_mp_driver(cm1o_model *M) { if (M->c.imoist == 1) o_mp_driver(M); }
+void cm1o_solve3(cm1o_model *M) { o_solve3(M); }
 void cm1o_bc(cm1o_model *M, int id, int kind) {
   float *a = M->f[id];
   switch (kind) {

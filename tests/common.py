@@ -80,10 +80,11 @@ def load_fixture(path):
     opts = [int(v) for v in z["opts"]] if "opts" in z else [3, 0, 0]
     psolver, advwenos, advwenov = opts[:3]
     irdamp = opts[3] if len(opts) > 3 else 0
+    npt = opts[4] if len(opts) > 4 else 0
     rdalpha, zd = [float(v) for v in z["rdamp"]] if "rdamp" in z else (3.3333333333e-3, 15000.0)
     cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=bc, ebc=bc, sbc=bc, nbc=bc, imoist=1, ptype=1,
                 psolver=psolver, advwenos=advwenos, advwenov=advwenov, apmasscon=1, irbc=4,
-                nsound=0, npt=0, pdtra=1, dx=1000.0, dy=1000.0, dz=500.0, dt=dt, alph=0.6,
+                nsound=0, npt=npt, pdtra=1, dx=1000.0, dy=1000.0, dz=500.0, dt=dt, alph=0.6,
                 kdiv=0.1, csmax=350.0, irdamp=irdamp, rdalpha=rdalpha, zd=zd)
     cols, _ = hinit.vertical_grid(nk, 500.0)
     for n in ("pi0", "th0", "qv0"):
@@ -108,6 +109,11 @@ def load_fixture(path):
     target = {"ua": z["tN_u"], "va": z["tN_v"], "wa": z["tN_w"], "tha": z["tN_th"],
               "ppi": z["tN_pi"], "qa0": z["tN_qv"], "qa1": z["tN_qc"], "qa2": z["tN_qr"],
               "rho": z["tN_rho"], "prs": z["tN_prs"]}
+    # passive tracers (iptra = 1): pta(n) is the reference's pt(n+1)
+    for n in range(npt):
+        f[f"pta{n}"] = hinit.blank(ni, nj, nk)
+        f[f"pta{n}"][1:nk + 1, 3:nj + 3, 3:ni + 3] = z[f"t0_pt{n + 1}"]
+        target[f"pta{n}"] = z[f"tN_pt{n + 1}"]
     return cfgd, cols, f, nsteps, target, z
 
 

@@ -9,10 +9,13 @@ stores its GrADS output at t=0 and after N large steps as float32 arrays:
 * ``tN_*``: the same prognostics plus rho and prs after N steps.
 
 * ``opts``: the dynamical-core switches the run used (psolver, advwenos,
-  advwenov, irdamp), and ``rdamp``: (rdalpha, zd) of the Rayleigh damper.
+  advwenov, irdamp, npt), and ``rdamp``: (rdalpha, zd) of the Rayleigh
+  damper;
+* with npt > 0 (iptra = 1): ``t0_ptN``/``tN_ptN``, the passive tracers the
+  reference initialises (src/init3d.F:328-349) and advects.
 
 Usage: python tests/golden/make_ref_fixture.py [name ni nj nk nsteps dt wbc
-[psolver advwenos advwenov [irdamp zd]]]
+[psolver advwenos advwenov [irdamp zd [npt]]]]
 """
 import os
 import sys
@@ -25,8 +28,8 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import refrun  # noqa: E402
 
 
-OPT_KEYS = ("psolver", "advwenos", "advwenov", "irdamp")
-OPT_DEFAULTS = dict(psolver=3, advwenos=0, advwenov=0, irdamp=0)
+OPT_KEYS = ("psolver", "advwenos", "advwenov", "irdamp", "npt")
+OPT_DEFAULTS = dict(psolver=3, advwenos=0, advwenov=0, irdamp=0, npt=0)
 RDALPHA = 3.3333333333e-3
 
 
@@ -37,8 +40,12 @@ def make(name, ni, nj, nk, nsteps, dt, bc, workdir="/tmp/cm1_fixture", opts=None
     zd: the damping height when irdamp = 1."""
     o = dict(OPT_DEFAULTS)
     o.update(opts or {})
+    npt = o.pop("npt")
     ov = dict(nx=ni, ny=nj, nz=nk, dtl=dt, timax=nsteps * dt, tapfrq=nsteps * dt,
               wbc=bc, ebc=bc, sbc=bc, nbc=bc, rdalpha=RDALPHA, zd=zd, **o)
+    if npt > 0:
+        ov.update(iptra=1, npt=npt)
+    o["npt"] = npt
     refrun.run_reference(workdir, ov, threads=threads)
     S = refrun.read_grads(workdir, "cm1out_s.ctl")
     U = refrun.read_grads(workdir, "cm1out_u.ctl")
@@ -58,6 +65,8 @@ def make(name, ni, nj, nk, nsteps, dt, bc, workdir="/tmp/cm1_fixture", opts=None
         out[f"{tag}_qr"] = S["qr"][t]
         out[f"{tag}_rho"] = S["rho"][t]
         out[f"{tag}_prs"] = S["prs"][t]
+        for n in range(1, npt + 1):
+            out[f"{tag}_pt{n}"] = S[f"pt{n}"][t]
     for n in ("pi0", "th0", "qv0", "prs0"):
         out[n] = S[n][0, :, 0, 0]
     path = os.path.join(outdir, f"{name}.npz")
@@ -68,8 +77,10 @@ def make(name, ni, nj, nk, nsteps, dt, bc, workdir="/tmp/cm1_fixture", opts=None
 if __name__ == "__main__":
     if len(sys.argv) > 1:
         a = sys.argv[1:]
-        opts = dict(zip(OPT_KEYS, (int(v) for v in a[7:11])))
+        opts = dict(zip(OPT_KEYS[:4], (int(v) for v in a[7:11])))
         zd = float(a[11]) if len(a) > 11 else 15000.0
+        if len(a) > 12:
+            opts["npt"] = int(a[12])
         print(make(a[0], int(a[1]), int(a[2]), int(a[3]), int(a[4]), float(a[5]), int(a[6]),
                    opts=opts, zd=zd))
     else:

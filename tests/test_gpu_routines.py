@@ -84,6 +84,14 @@ def test_advs(require_gpu, pdef, weno):
     # through the plain path (src/solve2.F:1394, :1490)
     cases = (("q3d2", "qa2", "qten2"), ("q3d0", "qa0", "qten0")) if pdef else \
         (("q3d2", "qa2", "qten2"), ("th3d", "tha", "thten"))
+    if pdef:
+        # a block of 1 g/kg with sharp edges in all three directions, so the
+        # 5th-order fluxes undershoot and the mass borrowing acts in x, y, z
+        blk = np.zeros((o.pz, o.py, o.px), np.float32)
+        blk[4:12, 8:20, 10:25] = 1.0e-3
+        for name in ("q3d2", "qa2"):
+            o.f(name)[:] = blk
+            g.put(name, blk)
     for s, s0, sten in cases:
         gcall(g, "cm1g_advs", nrk, N.FID[s], N.FID[s0], N.FID[sten], pdef, pdef,
               C.c_float(dttmp), C.c_double(weps))
