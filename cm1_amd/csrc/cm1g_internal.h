@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 #include "../../include/cm1gpu.h"
+#include "cm1g_libm.h"
 
 // Physical constants, single precision as set_constants() computes them
 // (src/constants.F:69-170, testcase=0).
@@ -207,10 +208,7 @@ void kessler_branch_run(cm1g_model *m);
 void mass_sums(cm1g_model *m);           // dumk3/dumk4 -> dscal
 void budgets_run(cm1g_model *m, double *tmass, double *tmois);
 
-// FP32 exp and pow evaluated in double precision and rounded once: the
-// result is the correctly rounded FP32 value except in vanishingly rare
-// near-midpoint cases, which keeps agreement with the host C library's
-// expf/powf (what the reference's REAL exp() and ** lower to) at the
-// level of isolated 1-ulp differences.
-__device__ __forceinline__ float exp_f(float x) { return (float)exp((double)x); }
-__device__ __forceinline__ float pow_f(float x, float y) { return (float)pow((double)x, (double)y); }
+// FP32 exp and pow with the host C library's rounding (what the
+// reference's REAL exp() and ** lower to), restated in cm1g_libm.h.
+__device__ __forceinline__ float exp_f(float x) { return cm1libm::expf_host(x); }
+__device__ __forceinline__ float pow_f(float x, float y) { return cm1libm::powf_host(x, y); }

@@ -2,13 +2,10 @@
 
 The oracle reproduces the reference CM1 FP32 build bit for bit
 (test_oracle_reference.py).  The HIP path evaluates every stencil, solve
-and reduction with the reference's operation order and no FMA contraction;
-it evaluates exp/pow in double precision and rounds once, where the
-reference calls the host C library's expf/powf.  Run with the same exp/pow
-rounding (oracle libm=1), the oracle and the GPU agree bit for bit on every
-prognostic field, after one step and after ten; that is the check here.
-How far the libm rounding alone moves a case from the reference is
-measured in test_gpu_reference.py.  Domain budgets to 1e-6.
+and reduction with the reference's operation order and no FMA contraction,
+and exp/pow with the host C library's method (cm1g_libm.h), so the oracle
+and the GPU agree bit for bit on every prognostic field, after one step and
+after ten; that is the check here.  Domain budgets to 1e-6.
 """
 import numpy as np
 import pytest
@@ -31,7 +28,7 @@ def config0(**kw):
 
 def run_pair(nsteps, **kw):
     cfgd, cols, fields = case(**kw)
-    o = make_oracle(cfgd, cols, fields, libm=1)
+    o = make_oracle(cfgd, cols, fields)
     g = make_gpu(cfgd, cols, fields)
     o.step(nsteps)
     g.step(nsteps)

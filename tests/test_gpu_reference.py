@@ -1,16 +1,13 @@
 """HIP path against the reference CM1's own output (golden vectors written
 by the reference executable, tests/golden/make_ref_fixture.py), from the
-reference's t=0 state over 10 large steps.
+reference's t=0 state over 10 (and 60) large steps.
 
-The only difference between the two builds' arithmetic is the rounding of
-exp/pow: the reference calls the host C library's expf/powf, the GPU
-evaluates them in double precision and rounds once (they differ by one ulp
-in ~0.06% of calls).  The oracle, which reproduces the reference bit for
-bit, is run a second time with the GPU's exp/pow rounding; the GPU must
-equal that run bit for bit, and its distance from the reference is then
-exactly the effect of libm rounding on the case.  Per field the bound is
-the north-star relative L2 1e-5, or that measured libm effect where the
-case amplifies it beyond 1e-5 (convective cases do within 10 steps).
+The HIP kernels follow the reference's operation order without
+contraction and evaluate exp/pow with the host C library's method
+(cm1g_libm.h), so every prognostic field (u, v, w, theta', pi', qv, qc, qr,
+rho, prs, and the passive tracers where present) must equal the reference
+build's output bit for bit; the relative L2 distance (0) is printed beside
+the north-star bound of 1e-5.
 """
 import glob
 import os
@@ -18,7 +15,7 @@ import os
 import numpy as np
 import pytest
 
-from common import PROG, interior, load_fixture, make_gpu, make_oracle, nsound_for, rel_l2
+from common import PROG, load_fixture, make_gpu, nsound_for, rel_l2
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -33,15 +30,10 @@ def test_gpu_matches_reference_fixture(require_gpu, path):
     cfgd["nsound"] = nsound_for(cfgd)
     g = make_gpu(cfgd, cols, fields)
     g.step(nsteps)
-    o = make_oracle(cfgd, cols, fields, libm=1)
-    o.step(nsteps)
-    ni, nj, nk = cfgd["ni"], cfgd["nj"], cfgd["nk"]
     for name, ref in target.items():
         got = g.interior(name, STAG.get(name))
-        same_rounding = interior(o.f(name), nk, nj, ni, STAG.get(name))
+        nbad = int(np.count_nonzero(got != ref))
         err = rel_l2(got, ref)
-        libm_effect = rel_l2(same_rounding, ref)
-        print(name, "gpu-ref", err, "libm effect", libm_effect,
-              "gpu!=oracle(libm=1)", int(np.count_nonzero(got != same_rounding)))
-        assert np.array_equal(got, same_rounding), name
-        assert err <= max(TOL, libm_effect), (name, err, libm_effect)
+        print(name, "gpu vs reference rel L2", err, "differing values", nbad)
+        assert err <= TOL, (name, err)
+        assert nbad == 0, (name, nbad)

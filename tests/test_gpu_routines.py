@@ -9,7 +9,7 @@ step.  Both models are first stepped twice from the same seeded state so
 the inputs are a realistic mid-run state, then every 3-D array of the
 oracle is copied into the GPU model, the routine is called on both, and
 the arrays the routine writes are compared bit for bit over the whole
-padded box (exp/pow evaluated as the GPU does, oracle libm mode 1).
+padded box.
 """
 import ctypes as C
 
@@ -26,7 +26,7 @@ pytestmark = pytest.mark.gpu
 def pair(**kw):
     cfgd, cols, fields = case(**kw)
     g = make_gpu(cfgd, cols, fields)
-    o = make_oracle(cfgd, cols, fields, libm=1)
+    o = make_oracle(cfgd, cols, fields)
     g.step(2)
     o.step(2)
     for n in pyoracle.FIELDS:
