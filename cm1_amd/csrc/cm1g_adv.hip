@@ -227,7 +227,7 @@ struct AdvLevel {
   float wnew;                          // s(k+4), entering the vertical window
 };
 
-template <bool WENO>
+template <bool WENO, bool TH>
 __device__ __forceinline__ void advs_load(const KP &P, const AdvsArgs &a, int i, int j, int k,
                                           AdvLevel &L) {
   const int px = P.px, pxy = P.pxy, nk = P.nk;
@@ -250,9 +250,8 @@ __device__ __forceinline__ void advs_load(const KP &P, const AdvsArgs &a, int i,
   L.dv = a.divx[c];
   L.st0 = a.sten0[c];
   L.sv0 = a.s0[c];
-  const bool th = (a.t11 != nullptr);
-  L.t11v = th ? a.t11[c] : 0.0f;
-  L.thtv = th ? a.thterm[c] : 0.0f;
+  L.t11v = TH ? a.t11[c] : 0.0f;
+  L.thtv = TH ? a.thterm[c] : 0.0f;
 }
 
 // One column (i,j) of advs (src/adv.F:14-658 with hadv_flx5/vadv_flx5, or
@@ -262,7 +261,9 @@ __device__ __forceinline__ void advs_load(const KP &P, const AdvsArgs &a, int i,
 // assembly.  Each flux is the same FP32 expression as the reference's, so
 // results are bitwise equal to separate flux / divergence passes.  The
 // operands of level k+1 are loaded before level k is evaluated and stored.
-template <bool WENO>
+// The uniform switches are compile-time: SADD (theta's offset form), TH
+// (theta's t11*thterm term), MODE1 (advx/advy/advz for the pdef pass).
+template <bool WENO, bool SADD, bool TH, bool MODE1>
 __global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a, ColGrid cg) {
   int i, j;
   if (!col_tile(cg, i, j)) return;
@@ -271,7 +272,7 @@ __global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a, ColGrid cg) 
   const bool doitw = (P.wbc == 2 && P.ibw == 1), doite = (P.ebc == 2 && P.ibe == 1);
   const bool doits = (P.sbc == 2 && P.ibs == 1), doitn = (P.nbc == 2 && P.ibn == 1);
   const float *s = a.s;
-  const bool sadd = (a.sadd != 0.0f);
+  constexpr bool sadd = SADD;
   const float th0r = P.K.th0r;
   auto val = [&](int kk) -> float {  // column value at level kk (1..nk)
     if (kk < 1 || kk > nk) return 0.0f;
@@ -282,10 +283,10 @@ __global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a, ColGrid cg) 
   float wv0 = val(-1), wv1 = val(0), wv2 = val(1), wv3 = val(2), wv4 = val(3), wv5 = val(4);
   float fz_lo = 0.0f;
   AdvLevel nx;
-  advs_load<WENO>(P, a, i, j, 1, nx);
+  advs_load<WENO, TH>(P, a, i, j, 1, nx);
   for (int k = 1; k <= nk; k++) {
     const AdvLevel L = nx;
-    if (k < nk) advs_load<WENO>(P, a, i, j, k + 1, nx);
+    if (k < nk) advs_load<WENO, TH>(P, a, i, j, k + 1, nx);
     const int c = IXD(P, i, j, k);
     const float tem = sadd ? (P.th0[k] - th0r) : 0.0f;
     auto H = [=](float v) -> float { return sadd ? tem + v : v; };
@@ -329,13 +330,13 @@ __global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a, ColGrid cg) 
     if (k == 1) az = -fz_hi * P.rdz * P.mh[1];
     else if (k == nk) az = +fz_lo * P.rdz * P.mh[nk];
     else az = -(fz_hi - fz_lo) * P.rdz * P.mh[k];
-    if (a.mode == 1) {
+    if (MODE1) {
       a.advx[c] = ax;
       a.advy[c] = ay;
       a.advz[c] = az;
     } else {
       float st = L.st0;
-      if (a.t11) st = st - L.t11v * L.thtv;
+      if (TH) st = st - L.t11v * L.thtv;
       st = st + ((ax + ay) + az + sc * L.dv) * P.rr0[k];
       if (a.out_sten) a.out_sten[c] = st;
       if (a.out_new) {
@@ -391,7 +392,17 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
   g.pdef = pdef;
   g.pdefweno = pdefweno;
   g.weps = weps;
-  auto *kern = weno ? k_advs_col<true> : k_advs_col<false>;
+  // compile-time variants: theta (offset form, with or without the t11
+  // term), any other scalar in the plain or the pdef (advx/advy/advz) pass
+  const bool sadd_on = (sadd != 0.0f), th_on = (thterm_div != nullptr);
+  auto pick = [&](bool mode1) -> void (*)(KP, AdvsArgs, ColGrid) {
+    if (sadd_on && th_on && !mode1) return weno ? k_advs_col<true, true, true, false> : k_advs_col<false, true, true, false>;
+    if (sadd_on && !th_on && !mode1) return weno ? k_advs_col<true, true, false, false> : k_advs_col<false, true, false, false>;
+    if (!sadd_on && !th_on && !mode1) return weno ? k_advs_col<true, false, false, false> : k_advs_col<false, false, false, false>;
+    if (!sadd_on && th_on && !mode1) return weno ? k_advs_col<true, false, true, false> : k_advs_col<false, false, true, false>;
+    // the pdef pass: advx/advy/advz only (the t11 term is added by k_pdef_z)
+    return weno ? k_advs_col<true, false, false, true> : k_advs_col<false, false, false, true>;
+  };
   const ColGrid cg = col_grid(ni, nj);
   const dim3 blk = col_threads(cg), grd = col_blocks(cg);
   // an in-place update (out_new aliasing the advected field) writes a
@@ -401,16 +412,16 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
   if (inplace) g.out_new = m->thb;
   if (pdef != 1) {
     g.mode = 0;
-    hipLaunchKernelGGL(kern, grd, blk, 0, m->stream, P, g, cg);
+    hipLaunchKernelGGL(pick(false), grd, blk, 0, m->stream, P, g, cg);
   } else {
-    g.mode = 1;
-    hipLaunchKernelGGL(kern, grd, blk, 0, m->stream, P, g, cg);
     // the positive-definite pass reads the advected values themselves
     // (never the theta offset form: pdef is for moisture and tracers)
     if (sadd != 0.0f) {
       cm1g_set_error("advs: pdef with an offset field is not supported");
       return;
     }
+    g.mode = 1;
+    hipLaunchKernelGGL(pick(true), grd, blk, 0, m->stream, P, g, cg);
     PdefArgs pa;
     pa.rru = g.rru;
     pa.rrv = g.rrv;
