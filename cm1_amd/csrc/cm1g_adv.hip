@@ -118,21 +118,44 @@ struct PdefArgs {
   int smeps_flush, scale;
 };
 
+// One 64 x 4 block per tile of one level.  Each cell's dum in x and in y
+// is evaluated once and shared through LDS: a cell needs dum at i-2..i+2
+// and at j-2..j+2, so the block evaluates its own 64 x 4 points in each
+// direction plus the two-point halos (4 x 4 in x, 64 x 4 in y), at the
+// indices the halo fill maps them to.
 __global__ void __launch_bounds__(256) k_pdef_xy(KP P, PdefArgs a) {
-  const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  __shared__ float dxs[4][68];
+  __shared__ float dys[8][64];
+  const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+  const int i0 = 1 + (int)blockIdx.x * 64, j0 = 1 + (int)blockIdx.y * 4;
+  const int i = i0 + tx, j = j0 + ty;
   const int k = 1 + (int)blockIdx.z;
   const int ni = P.ni, nj = P.nj;
-  if (i > ni || j > nj) return;
+  const bool perx = (P.wbc == 1), pery = (P.sbc == 1);
+  const bool in = (i <= ni && j <= nj);
+  {
+    // x: own point of this row, and the row's halo points i0-2, i0-1, i0+64, i0+65
+    const int jj = pdef_wrap(j, nj, pery);
+    dxs[ty][tx + 2] = pdef_dum(P, 0, a.rru, a.scale, a.advx, a.s0, a.s, a.dt, pdef_wrap(i, ni, perx), jj, k);
+    if (tx < 4) {
+      const int o = (tx < 2) ? tx - 2 : 62 + tx;
+      dxs[ty][(tx < 2) ? tx : 64 + tx] =
+          pdef_dum(P, 0, a.rru, a.scale, a.advx, a.s0, a.s, a.dt, pdef_wrap(i0 + o, ni, perx), jj, k);
+    }
+    // y: own point of this column, and one halo row (j0-2, j0-1, j0+4, j0+5)
+    const int ii = (i <= ni) ? i : ni;
+    dys[ty + 2][tx] = pdef_dum(P, 1, a.rrv, a.scale, a.advy, a.s0, a.s, a.dt, ii, pdef_wrap(j, nj, pery), k);
+    const int hr = (ty < 2) ? ty : ty + 4;
+    dys[hr][tx] = pdef_dum(P, 1, a.rrv, a.scale, a.advy, a.s0, a.s, a.dt, ii,
+                           pdef_wrap(j0 - 2 + hr, nj, pery), k);
+  }
+  __syncthreads();
+  if (!in) return;
   const int c = IXD(P, i, j, k);
   const float smeps = P.K.smeps, rdt = 1.0f / a.dt;
   const float r0s0 = P.rho0[k] * a.s0[c];
   {
-    const bool per = (P.wbc == 1);
-    float d[5];
-#pragma unroll
-    for (int o = -2; o <= 2; o++)
-      d[o + 2] = pdef_dum(P, 0, a.rru, a.scale, a.advx, a.s0, a.s, a.dt, pdef_wrap(i + o, ni, per), j, k);
+    const float *d = &dxs[ty][tx];
     bool flag;
     const float mass = pdef_gather(d[0], d[1], d[2], d[3], d[4], i - 1 >= 0, i + 1 <= ni + 1, smeps, flag);
     float out = a.advx[c];
@@ -144,18 +167,14 @@ __global__ void __launch_bounds__(256) k_pdef_xy(KP P, PdefArgs a) {
     a.ax_out[c] = out;
   }
   {
-    const bool per = (P.sbc == 1);
-    float d[5];
-#pragma unroll
-    for (int o = -2; o <= 2; o++)
-      d[o + 2] = pdef_dum(P, 1, a.rrv, a.scale, a.advy, a.s0, a.s, a.dt, i, pdef_wrap(j + o, nj, per), k);
     bool flag;
-    const float mass = pdef_gather(d[0], d[1], d[2], d[3], d[4], j - 1 >= 0, j + 1 <= nj + 1, smeps, flag);
+    const float mass = pdef_gather(dys[ty][tx], dys[ty + 1][tx], dys[ty + 2][tx], dys[ty + 3][tx],
+                                   dys[ty + 4][tx], j - 1 >= 0, j + 1 <= nj + 1, smeps, flag);
     float out = a.advy[c];
     if (flag) {
       float lo, hi;
       pdef_faces(P, 1, a.rrv, a.scale, c, k, lo, hi);
-      out = (d[2] + mass - r0s0) * rdt - a.s[c] * (hi - lo) * P.rdy * P.vh[j + 2];
+      out = (dys[ty + 2][tx] + mass - r0s0) * rdt - a.s[c] * (hi - lo) * P.rdy * P.vh[j + 2];
     }
     a.ay_out[c] = out;
   }
