@@ -37,24 +37,19 @@ void kessler_branch_run(cm1g_model *m) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   const float dt = m->c.dt;
-  float *dum1 = m->f[CM1G_DUM1], *dum3 = m->f[CM1G_DUM3], *thten = m->f[CM1G_THTEN];
   float *th3d = m->f[CM1G_TH3D], *pp3d = m->f[CM1G_PP3D], *rho = m->f[CM1G_RHO];
   float *prs = m->f[CM1G_PRS];
   float *qv = m->f[CM1G_Q3D0], *qc = m->f[CM1G_Q3D1], *qr = m->f[CM1G_Q3D2];
-  float *qtenv = m->f[CM1G_QTEN0], *vr = m->f[CM1G_QTEN2];
+  float *vr = m->f[CM1G_QTEN2];
   float *rain = m->d2[CM1G_D_RAIN], *prate = m->d2[CM1G_D_PRATE];
   double *colbud = m->dred;
   int *iflag = m->iflag;
 
-  // prelude (src/mp_driver.F:100-148): T, copy of T, copy of qv, rho copy
-  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-    int c = IXD(P, i, j, k);
-    float t = (P.th0[k] + th3d[c]) * (P.pi0[k] + pp3d[c]);
-    dum1[c] = t;
-    thten[c] = t;
-    qtenv[c] = qv[c];
-    dum3[c] = rho[c];
-  });
+  // The reference's prelude (src/mp_driver.F:100-148) stores T, a copy of
+  // T in thten, a copy of qv in qten(nqv) and a copy of rho in dum3; in the
+  // Kessler branch these feed only budget diagnostics (geterain, the rr
+  // argument of kessler/satadj) and the other schemes' tendency checks, and
+  // kessler recomputes T itself, so nothing on this path reads them.
   pdefq_run(m, 0.0f, rho, qv);
   pdefq_run(m, P.K.qsmall, rho, qc);
   pdefq_run(m, P.K.qsmall, rho, qr);
@@ -107,7 +102,6 @@ void kessler_branch_run(cm1g_model *m) {
     int c = IXD(P, i, j, k);
     float th = th3d[c], pp = pp3d[c];
     float t = (P.th0[k] + th) * (P.pi0[k] + pp);
-    dum1[c] = t;
     float qvnew = qv[c], qcnew = qc[c], qrnew = qr[c];
     float ar = 0.0f, cr = 0.0f, er = 0.0f;
     if (qcnew > 0.0f) {
