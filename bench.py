@@ -50,6 +50,110 @@ def workload_config1(ni=1024, nj=1024, nk=96, seed=2024):
     return cfgd, cols, fields
 
 
+def workload_ensemble(n=48, seed=2025):
+    """Batched supercell ensemble: n members with (nx, ny) drawn from
+    {96,128,192,256}, nz from {40,56,64}, dx=dy=1 km, a 20 km model top, dt=6 s,
+    nsound=6; per-member Weisman-Klemp sounding with qv scaled by U(0.9,1.1),
+    shear u=U(10,30) tanh(z/3 km), and a centred U(1,3) K bubble."""
+    from cm1_amd import init as hinit
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        ni, nj = (int(rng.choice([96, 128, 192, 256])) for _ in range(2))
+        nk = int(rng.choice([40, 56, 64]))
+        qvs, umax, amp = float(rng.uniform(0.9, 1.1)), float(rng.uniform(10, 30)), float(rng.uniform(1, 3))
+        dx = 1000.0
+        dz = 20000.0 / nk
+        bub = [dict(x=0.5 * ni * dx, y=0.5 * nj * dx, z=1400.0, amp=amp, rh=10000.0, rv=1400.0)]
+        cols, fields, dzn = hinit.initial_state(ni, nj, nk, dx, dx, dz, umax=umax, bubbles=bub,
+                                                qv_scale=qvs)
+        cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=2, ebc=2, sbc=2, nbc=2, imoist=1, ptype=1,
+                    psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4, nsound=6, npt=0,
+                    pdtra=1, dx=dx, dy=dx, dz=dzn, dt=6.0, alph=0.6, kdiv=0.1, csmax=350.0)
+        out.append((cfgd, cols, fields))
+    return out
+
+
+def bench_ensemble(args):
+    """Ensemble throughput: the members stepped one after another (each step
+    one captured graph) against the batched ensemble step (one member-indexed
+    launch per kernel, one graph per step) at the fixed tile height and at
+    the autotuned one.  Prints one JSON line."""
+    import torch
+    from cm1_amd import model
+    import cm1_amd._native as N
+    import ctypes as C
+    L = N.lib()
+    work = workload_ensemble(args.ensemble)
+    members = []
+    for cfgd, cols, fields in work:
+        members.append(model.setup(model.make_config(**cfgd), cols, fields))
+    del work
+    cells = sum(m.ni * m.nj * m.nk for m in members)
+    print(f"ensemble: {len(members)} members, {cells} cells", file=sys.stderr, flush=True)
+
+    def seq_step():
+        for m in members:
+            N.check(L.cm1g_step(m.h, None), "cm1g_step")
+
+    for _ in range(args.warmup):
+        seq_step()
+    for m in members:
+        m.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        seq_step()
+    for m in members:
+        m.sync()
+    ms_seq = (time.perf_counter() - t0) * 1000.0 / args.steps
+    print(f"ensemble: one by one {ms_seq:.2f} ms/step", file=sys.stderr, flush=True)
+
+    e = model.Ensemble(members)
+
+    def ens_timed(nsteps):
+        e.sync()
+        t = time.perf_counter()
+        e.step(nsteps)
+        e.sync()
+        return (time.perf_counter() - t) * 1000.0 / nsteps
+
+    e.set_tiles(4)
+    e.step(max(1, args.warmup))
+    ms_fixed = ens_timed(args.steps)
+    print(f"ensemble: batched, 4-row tiles {ms_fixed:.2f} ms/step", file=sys.stderr, flush=True)
+    best, tune_ms = e.tune(max(2, args.steps // 4))
+    ms_tuned = ens_timed(args.steps)
+    print(f"ensemble: batched, tuned ({best}-row tiles) {ms_tuned:.2f} ms/step", file=sys.stderr, flush=True)
+    e.close()
+    shapes = sorted({(m.ni, m.nj, m.nk) for m in members})
+    line = {
+        "metric": "ensemble member cell-updates/s per RK3 step (FP32)",
+        "value": cells / (ms_tuned / 1000.0),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_tuned,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"supercell ensemble, {len(members)} members, (nx,ny) in "
+                               "{96,128,192,256}, nz in {40,56,64}, dx=1km dt=6s nsound=6 Kessler",
+                   "members": len(members), "cells": cells, "shapes": len(shapes),
+                   "ms_per_step_one_by_one": round(ms_seq, 3),
+                   "ms_per_step_batched_fixed_tiles": round(ms_fixed, 3),
+                   "ms_per_step_batched_tuned": round(ms_tuned, 3),
+                   "tuned_tile_rows": best,
+                   "tuner_ms_per_step": {str(k): round(v, 3) for k, v in tune_ms.items()},
+                   "gain_batched_tuned_vs_one_by_one": round(ms_seq / ms_tuned, 3)},
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def cpu_baseline(threads):
     """Reference CM1 (FP32 OpenMP build) on a 256x256x96 sample of the case."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -116,11 +220,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (reference kernel for HBM counters)")
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="benchmark an ensemble of this many members instead (batched steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
               "multi-GPU horizontal decomposition is not implemented", file=sys.stderr)
         return 2
+
+    if args.ensemble > 0:
+        return bench_ensemble(args)
 
     import torch
     from cm1_amd import model
@@ -144,20 +253,25 @@ def main():
         m.step(1)
     m.sync()
     L = N.lib()
-    # HIP events around every launch of the acoustic substep kernel on the
-    # model's stream during the timed steps
-    kt_ok = L.cm1g_ktimer_start(m.h, N.KT_SOUND_SUBSTEP) == 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    cfl = 0.0
-    for _ in range(args.steps):
-        cfl = m.step(1)
-    m.sync()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1000.0 / args.steps
+
+    def timed(nsteps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        c = 0.0
+        for _ in range(nsteps):
+            c = m.step(1)
+        m.sync()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1000.0 / nsteps, c
+
+    # the production path: each step replayed as one captured HIP graph
+    ms, cfl = timed(args.steps)
     cells = cfgd["ni"] * cfgd["nj"] * cfgd["nk"]
     value = cells / (ms / 1000.0)
+    # the same steps launched kernel by kernel, with HIP events around every
+    # launch of the acoustic substep kernel on the model's stream
+    kt_ok = L.cm1g_ktimer_start(m.h, N.KT_SOUND_SUBSTEP) == 0
+    ms_eager, _ = timed(args.steps)
 
     roof = None
     if kt_ok:
@@ -199,6 +313,7 @@ def main():
                                f"dt=1.5s nsound=8 Kessler open-BC 8 bubbles",
                    "nx": cfgd["ni"], "ny": cfgd["nj"], "nz": cfgd["nk"], "nsound": cfgd["nsound"],
                    "cflmax": cfl,
+                   "ms_per_step_kernel_by_kernel": round(ms_eager, 3),
                    "step_algorithmic_GBps": round(step_gbs, 1),
                    "step_bytes_per_cell": bytes_model.bytes_per_cell_step(cfgd["nsound"])},
         "roofline": roof,

@@ -72,6 +72,14 @@ SIGNATURES = {
     "cm1g_sync": (C.c_int, [C.c_void_p]),
     "cm1g_getset": (C.c_int, [C.c_void_p, C.c_int]),
     "cm1g_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "cm1g_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+    "cm1g_set_tiles": (C.c_int, [C.c_void_p, C.c_int]),
+    "cm1g_ensemble_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+    "cm1g_ensemble_destroy": (C.c_int, [C.c_void_p]),
+    "cm1g_ensemble_step": (C.c_int, [C.c_void_p]),
+    "cm1g_ensemble_sync": (C.c_int, [C.c_void_p]),
+    "cm1g_ensemble_set_tiles": (C.c_int, [C.c_void_p, C.c_int]),
+    "cm1g_ensemble_tune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "cm1g_solve1": (C.c_int, [C.c_void_p]),
     "cm1g_solve2": (C.c_int, [C.c_void_p]),
     "cm1g_mp_driver": (C.c_int, [C.c_void_p]),

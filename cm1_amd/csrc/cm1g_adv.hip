@@ -123,13 +123,13 @@ struct PdefArgs {
 // and at j-2..j+2, so the block evaluates its own 64 x 4 points in each
 // direction plus the two-point halos (4 x 4 in x, 64 x 4 in y), at the
 // indices the halo fill maps them to.
-__global__ void __launch_bounds__(256) k_pdef_xy(KP P, PdefArgs a) {
+__device__ __forceinline__ void pdef_xy(const KP &P, const PdefArgs &a, int bx, int by, int bz) {
   __shared__ float dxs[4][68];
   __shared__ float dys[8][64];
   const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
-  const int i0 = 1 + (int)blockIdx.x * 64, j0 = 1 + (int)blockIdx.y * 4;
+  const int i0 = 1 + bx * 64, j0 = 1 + by * 4;
   const int i = i0 + tx, j = j0 + ty;
-  const int k = 1 + (int)blockIdx.z;
+  const int k = 1 + bz;
   const int ni = P.ni, nj = P.nj;
   const bool perx = (P.wbc == 1), pery = (P.sbc == 1);
   const bool in = (i <= ni && j <= nj);
@@ -182,9 +182,9 @@ __global__ void __launch_bounds__(256) k_pdef_xy(KP P, PdefArgs a) {
 
 // z direction (pdefz) marched up each column with a window of dum, then the
 // tendency with the corrected advx, advy, advz and the s0 + dt*sten update
-__global__ void __launch_bounds__(256) k_pdef_z(KP P, PdefArgs a, ColGrid cg) {
+__device__ __forceinline__ void pdef_z(const KP &P, const PdefArgs &a, const ColGrid &cg, int bx) {
   int i, j;
-  if (!col_tile(cg, i, j)) return;
+  if (!col_tile_b(cg, bx, i, j)) return;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   if (i > ni || j > nj) return;
   const float smeps = P.K.smeps, rdt = 1.0f / a.dt;
@@ -236,7 +236,7 @@ struct AdvsArgs {
                                   // rho0 / rf0 on the fly (getdivx, src/misclibs.F:222-370)
 };
 
-// Operands of one level of a k_advs_col column (raw loads; the theta
+// Operands of one level of a advs_col column (raw loads; the theta
 // offset th0-th0r is applied when they are used).
 struct AdvLevel {
   float xm3, xm2, xm1, xp1, xp2, xp3;  // s(i-3..i+3), centre excluded
@@ -283,9 +283,9 @@ __device__ __forceinline__ void advs_load(const KP &P, const AdvsArgs &a, int i,
 // The uniform switches are compile-time: SADD (theta's offset form), TH
 // (theta's t11*thterm term), MODE1 (advx/advy/advz for the pdef pass).
 template <bool WENO, bool SADD, bool TH, bool MODE1>
-__global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a, ColGrid cg) {
+__device__ __forceinline__ void advs_col(const KP &P, const AdvsArgs &a, const ColGrid &cg, int bx) {
   int i, j;
-  if (!col_tile(cg, i, j)) return;
+  if (!col_tile_b(cg, bx, i, j)) return;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   if (i > ni || j > nj) return;
   const bool doitw = (P.wbc == 2 && P.ibw == 1), doite = (P.ebc == 2 && P.ibe == 1);
@@ -382,6 +382,12 @@ __global__ void __launch_bounds__(256) k_advs_col(KP P, AdvsArgs a, ColGrid cg) 
 // sten0 = thten1, thterm_div[0] = t11 and thterm_div[1] = thterm.
 // out_new, if given, receives s0 + dt*tendency with the smeps flush
 // (src/solve2.F:1416-1426, :1518-1528).
+template <bool WENO, bool SADD, bool TH, bool MODE1>
+static void advs_launch(cm1g_model *m, const KP &P, const AdvsArgs &a, const ColGrid &cg) {
+  launch<256>(m, col_blocks(cg), col_threads(cg),
+              [=] __device__(int bx, int, int) { advs_col<WENO, SADD, TH, MODE1>(P, a, cg, bx); });
+}
+
 void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float *s0,
               const float *sten0, const float *thterm_div, float *out_sten, float *out_new,
               int pdef, int pdefweno, float dt, double weps, int smeps_flush) {
@@ -414,16 +420,16 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
   // compile-time variants: theta (offset form, with or without the t11
   // term), any other scalar in the plain or the pdef (advx/advy/advz) pass
   const bool sadd_on = (sadd != 0.0f), th_on = (thterm_div != nullptr);
-  auto pick = [&](bool mode1) -> void (*)(KP, AdvsArgs, ColGrid) {
-    if (sadd_on && th_on && !mode1) return weno ? k_advs_col<true, true, true, false> : k_advs_col<false, true, true, false>;
-    if (sadd_on && !th_on && !mode1) return weno ? k_advs_col<true, true, false, false> : k_advs_col<false, true, false, false>;
-    if (!sadd_on && !th_on && !mode1) return weno ? k_advs_col<true, false, false, false> : k_advs_col<false, false, false, false>;
-    if (!sadd_on && th_on && !mode1) return weno ? k_advs_col<true, false, true, false> : k_advs_col<false, false, true, false>;
-    // the pdef pass: advx/advy/advz only (the t11 term is added by k_pdef_z)
-    return weno ? k_advs_col<true, false, false, true> : k_advs_col<false, false, false, true>;
+  using Launcher = void (*)(cm1g_model *, const KP &, const AdvsArgs &, const ColGrid &);
+  auto pick = [&](bool mode1) -> Launcher {
+    if (sadd_on && th_on && !mode1) return weno ? advs_launch<true, true, true, false> : advs_launch<false, true, true, false>;
+    if (sadd_on && !th_on && !mode1) return weno ? advs_launch<true, true, false, false> : advs_launch<false, true, false, false>;
+    if (!sadd_on && !th_on && !mode1) return weno ? advs_launch<true, false, false, false> : advs_launch<false, false, false, false>;
+    if (!sadd_on && th_on && !mode1) return weno ? advs_launch<true, false, true, false> : advs_launch<false, false, true, false>;
+    // the pdef pass: advx/advy/advz only (the t11 term is added by pdef_z)
+    return weno ? advs_launch<true, false, false, true> : advs_launch<false, false, false, true>;
   };
-  const ColGrid cg = col_grid(ni, nj);
-  const dim3 blk = col_threads(cg), grd = col_blocks(cg);
+  const ColGrid cg = col_grid(ni, nj, m->tile_rows);
   // an in-place update (out_new aliasing the advected field) writes a
   // partner array, since neighbouring columns still read the old values;
   // the field and its partner then exchange buffers
@@ -431,7 +437,7 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
   if (inplace) g.out_new = m->thb;
   if (pdef != 1) {
     g.mode = 0;
-    hipLaunchKernelGGL(pick(false), grd, blk, 0, m->stream, P, g, cg);
+    pick(false)(m, P, g, cg);
   } else {
     // the positive-definite pass reads the advected values themselves
     // (never the theta offset form: pdef is for moisture and tracers)
@@ -440,7 +446,7 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
       return;
     }
     g.mode = 1;
-    hipLaunchKernelGGL(pick(true), grd, blk, 0, m->stream, P, g, cg);
+    pick(true)(m, P, g, cg);
     PdefArgs pa;
     pa.rru = g.rru;
     pa.rrv = g.rrv;
@@ -461,9 +467,9 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
     pa.out_new = g.out_new;
     pa.dt = dt;
     pa.smeps_flush = smeps_flush;
-    hipLaunchKernelGGL(k_pdef_xy, dim3((ni + 63) / 64, (nj + 3) / 4, nk), dim3(64, 4, 1), 0, m->stream,
-                       P, pa);
-    hipLaunchKernelGGL(k_pdef_z, grd, blk, 0, m->stream, P, pa, cg);
+    launch<256>(m, dim3((ni + 63) / 64, (nj + 3) / 4, nk), dim3(64, 4, 1),
+                [=] __device__(int bx, int by, int bz) { pdef_xy(P, pa, bx, by, bz); });
+    launch<256>(m, col_blocks(cg), col_threads(cg), [=] __device__(int bx, int, int) { pdef_z(P, pa, cg, bx); });
   }
   if (inplace) {
     // the partner now holds the new field: exchange the two buffers

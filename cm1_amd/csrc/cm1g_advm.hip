@@ -85,9 +85,9 @@ __device__ __forceinline__ void advm_load(const KP &P, const AdvmArgs &g, int i,
 }
 
 template <int STAG, bool WENO, bool OWN>
-__global__ void __launch_bounds__(256) k_advm_col(KP P, AdvmArgs g, ColGrid cg) {
+__device__ __forceinline__ void advm_col(const KP &P, const AdvmArgs &g, const ColGrid &cg, int bx) {
   int i, j;
-  if (!col_tile(cg, i, j)) return;
+  if (!col_tile_b(cg, bx, i, j)) return;
   if (i < g.i1 || i > g.i2 || j < g.j1 || j > g.j2) return;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   const bool doitw = (P.wbc == 2 && P.ibw == 1), doite = (P.ebc == 2 && P.ibe == 1);
@@ -261,11 +261,11 @@ __global__ void __launch_bounds__(256) k_advm_col(KP P, AdvmArgs g, ColGrid cg) 
 // src/solve2.F:459-471): the advection kernels read them, and this kernel
 // copies the points they do not write (the faces i=ni+1 / j=nj+1, the
 // open-boundary faces outside the advected range, and w at k=1).
-__global__ void __launch_bounds__(256) k_ten_edges(KP P, const float *u1, const float *v1,
-                                                   const float *w1, float *u, float *v, float *w,
-                                                   int ui1, int ui2, int vj1, int vj2) {
-  const int i = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+__device__ __forceinline__ void ten_edges(const KP &P, const float *u1, const float *v1, const float *w1,
+                                          float *u, float *v, float *w, int ui1, int ui2, int vj1,
+                                          int vj2, int bx, int by) {
+  const int i = 1 + bx * (int)blockDim.x + (int)threadIdx.x;
+  const int j = 1 + by * (int)blockDim.y + (int)threadIdx.y;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   if (i > ni + 1 || j > nj + 1) return;
   const bool ucol = (j == nj + 1) || i < ui1 || i > ui2;
@@ -284,6 +284,12 @@ __global__ void __launch_bounds__(256) k_ten_edges(KP P, const float *u1, const 
 // 2 v, 4 w; buoy adds the buoyancy term to wten; own forms each
 // component's own mass flux from the velocity (valid while rru/rrv/rrw
 // hold rho0*u, rho0*v, rf0*w, as between src/solve2.F:490-545 and sound).
+template <int STAG, bool WENO, bool OWN>
+static void advm_launch(cm1g_model *m, const KP &P, const AdvmArgs &g, const ColGrid &cg) {
+  launch<256>(m, col_blocks(cg), col_threads(cg),
+              [=] __device__(int bx, int, int) { advm_col<STAG, WENO, OWN>(P, g, cg, bx); });
+}
+
 void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj;
@@ -300,9 +306,12 @@ void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own
   const int vj1 = P.ibs == 1 ? 2 : 1, vj2 = P.ibn == 1 ? nj : nj + 1;
   if (from_ten1) {
     // the launch covers all three components, so every mask bit must be set
-    hipLaunchKernelGGL(k_ten_edges, dim3((ni + 1 + 63) / 64, (nj + 1 + 3) / 4, 1), dim3(64, 4, 1), 0,
-                       m->stream, P, m->f[CM1G_UTEN1], m->f[CM1G_VTEN1], m->f[CM1G_WTEN1],
-                       m->f[CM1G_UTEN], m->f[CM1G_VTEN], m->f[CM1G_WTEN], ui1, ui2, vj1, vj2);
+    const float *u1 = m->f[CM1G_UTEN1], *v1 = m->f[CM1G_VTEN1], *w1 = m->f[CM1G_WTEN1];
+    float *u = m->f[CM1G_UTEN], *v = m->f[CM1G_VTEN], *w = m->f[CM1G_WTEN];
+    launch<256>(m, dim3((ni + 1 + 63) / 64, (nj + 1 + 3) / 4, 1), dim3(64, 4, 1),
+                [=] __device__(int bx, int by, int) {
+                  ten_edges(P, u1, v1, w1, u, v, w, ui1, ui2, vj1, vj2, bx, by);
+                });
   }
   if (mask & 1) {
     g.a = m->f[CM1G_U3D];
@@ -312,10 +321,10 @@ void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own
     g.i2 = ui2;
     g.j1 = 1;
     g.j2 = nj;
-    const ColGrid cg = col_grid(ni + 1, nj);
-    auto *kern = own ? (weno ? k_advm_col<2, true, true> : k_advm_col<2, false, true>)
-                     : (weno ? k_advm_col<2, true, false> : k_advm_col<2, false, false>);
-    hipLaunchKernelGGL(kern, col_blocks(cg), col_threads(cg), 0, m->stream, P, g, cg);
+    const ColGrid cg = col_grid(ni + 1, nj, m->tile_rows);
+    auto *kern = own ? (weno ? advm_launch<2, true, true> : advm_launch<2, false, true>)
+                     : (weno ? advm_launch<2, true, false> : advm_launch<2, false, false>);
+    kern(m, P, g, cg);
   }
   if (mask & 2) {
     g.a = m->f[CM1G_V3D];
@@ -325,10 +334,10 @@ void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own
     g.i2 = ni;
     g.j1 = vj1;
     g.j2 = vj2;
-    const ColGrid cg = col_grid(ni, nj + 1);
-    auto *kern = own ? (weno ? k_advm_col<3, true, true> : k_advm_col<3, false, true>)
-                     : (weno ? k_advm_col<3, true, false> : k_advm_col<3, false, false>);
-    hipLaunchKernelGGL(kern, col_blocks(cg), col_threads(cg), 0, m->stream, P, g, cg);
+    const ColGrid cg = col_grid(ni, nj + 1, m->tile_rows);
+    auto *kern = own ? (weno ? advm_launch<3, true, true> : advm_launch<3, false, true>)
+                     : (weno ? advm_launch<3, true, false> : advm_launch<3, false, false>);
+    kern(m, P, g, cg);
   }
   if (mask & 4) {
     g.a = m->f[CM1G_W3D];
@@ -338,9 +347,9 @@ void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own
     g.i2 = ni;
     g.j1 = 1;
     g.j2 = nj;
-    const ColGrid cg = col_grid(ni, nj);
-    auto *kern = own ? (weno ? k_advm_col<4, true, true> : k_advm_col<4, false, true>)
-                     : (weno ? k_advm_col<4, true, false> : k_advm_col<4, false, false>);
-    hipLaunchKernelGGL(kern, col_blocks(cg), col_threads(cg), 0, m->stream, P, g, cg);
+    const ColGrid cg = col_grid(ni, nj, m->tile_rows);
+    auto *kern = own ? (weno ? advm_launch<4, true, true> : advm_launch<4, false, true>)
+                     : (weno ? advm_launch<4, true, false> : advm_launch<4, false, false>);
+    kern(m, P, g, cg);
   }
 }

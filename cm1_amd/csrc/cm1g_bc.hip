@@ -160,27 +160,28 @@ __device__ __forceinline__ void bc_cell(const KP &P, int kind, int phase, float 
   }
 }
 
-__global__ void __launch_bounds__(256) k_bc(KP P, int kind, int phase, BcArrs A, int tmin, int nt) {
-  int t = tmin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  int k = (int)blockIdx.y;
+__device__ __forceinline__ void bc_run(const KP &P, int kind, int phase, const BcArrs &A, int tmin, int nt,
+                                       int bx, int by) {
+  const int t = tmin + bx * (int)blockDim.x + (int)threadIdx.x;
+  const int k = by;
   if (t >= tmin + nt) return;
   for (int n = 0; n < A.n; n++) bc_cell(P, kind, phase, A.a[n], t, k);
 }
 
 void bc_fill_n(cm1g_model *m, int kind, float *const *arrs, int n) {
-  const KP &P = m->P;
+  const KP P = m->P;
   BcArrs A;
   A.n = n;
   for (int q = 0; q < n && q < 8; q++) A.a[q] = arrs[q];
   const int tmin = -2;
   const int nt = (P.ni > P.nj ? P.ni : P.nj) + 7;
   dim3 b(256), g((nt + 255) / 256, P.nk + 2);
-  hipLaunchKernelGGL(k_bc, g, b, 0, m->stream, P, kind, 0, A, tmin, nt);
-  hipLaunchKernelGGL(k_bc, g, b, 0, m->stream, P, kind, 1, A, tmin, nt);
+  for (int phase = 0; phase < 2; phase++)
+    launch<256>(m, g, b, [=] __device__(int bx, int by, int) { bc_run(P, kind, phase, A, tmin, nt, bx, by); });
   if (kind == BC_W1) {
-    int ntt = (P.ni + 2) * (P.nj + 2);
-    dim3 g2((ntt + 255) / 256, 1);
-    hipLaunchKernelGGL(k_bc, g2, b, 0, m->stream, P, kind, 2, A, 0, ntt);
+    const int ntt = (P.ni + 2) * (P.nj + 2);
+    launch<256>(m, dim3((ntt + 255) / 256, 1), b,
+                [=] __device__(int bx, int by, int) { bc_run(P, kind, 2, A, 0, ntt, bx, by); });
   }
 }
 
@@ -189,12 +190,12 @@ void bc_fill(cm1g_model *m, int kind, float *a) { bc_fill_n(m, kind, &a, 1); }
 // radbcew4 / radbcns4 (src/bc.F:2054-2172): per boundary column, the
 // vertical mean of the clipped phase speed.  One thread per column; the k
 // sum runs sequentially in the reference's order.
-__global__ void k_radbc(KP P, const float *u1, const float *u2, const float *v1, const float *v2,
-                        float dt, float csmax, float *radbcw, float *radbce, float *radbcs,
-                        float *radbcn) {
+__device__ __forceinline__ void radbc_cols(const KP &P, const float *u1, const float *u2, const float *v1,
+                                           const float *v2, float dt, float csmax, float *radbcw,
+                                           float *radbce, float *radbcs, float *radbcn, int bx, int by) {
   const int ni = P.ni, nj = P.nj, nk = P.nk;
-  int t = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  int side = (int)blockIdx.y;
+  const int t = 1 + bx * (int)blockDim.x + (int)threadIdx.x;
+  const int side = by;
   const float umax = csmax;
   if (side == 0 && P.ibw == 1 && P.wbc == 2 && t <= nj) {
     int j = t;
@@ -252,10 +253,12 @@ __global__ void k_radbc(KP P, const float *u1, const float *u2, const float *v1,
 
 void radbc4(cm1g_model *m, const float *ua, const float *u3d, const float *va, const float *v3d,
             float dt) {
-  const KP &P = m->P;
-  int n = (P.ni > P.nj ? P.ni : P.nj);
-  dim3 b(64), g((n + 63) / 64, 4);
-  hipLaunchKernelGGL(k_radbc, g, b, 0, m->stream, P, ua, u3d, va, v3d, dt, m->c.csmax,
-                     m->d2[CM1G_D_RADBCW], m->d2[CM1G_D_RADBCE], m->d2[CM1G_D_RADBCS],
-                     m->d2[CM1G_D_RADBCN]);
+  const KP P = m->P;
+  const int n = (P.ni > P.nj ? P.ni : P.nj);
+  const float csmax = m->c.csmax;
+  float *rw = m->d2[CM1G_D_RADBCW], *re = m->d2[CM1G_D_RADBCE], *rs = m->d2[CM1G_D_RADBCS],
+        *rn = m->d2[CM1G_D_RADBCN];
+  launch<64>(m, dim3((n + 63) / 64, 4), dim3(64), [=] __device__(int bx, int by, int) {
+    radbc_cols(P, ua, u3d, va, v3d, dt, csmax, rw, re, rs, rn, bx, by);
+  });
 }

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
+#include <type_traits>
 #include <string>
 #include <vector>
 #include "../../include/cm1gpu.h"
@@ -73,7 +75,30 @@ struct cm1g_model {
   int kt_which;
   std::vector<hipEvent_t> kt_ev;
   size_t kt_used;
+  // Whole large steps captured as HIP graphs.  The scalar updates exchange
+  // field buffers with the partner array, so the buffers a step reads depend
+  // on the arrangement it starts from: one graph per arrangement (the
+  // exchanges cycle through a few), keyed by the field pointers before the
+  // step, with the arrangement the step leaves behind.
+  struct StepGraph {
+    std::vector<float *> before, after;
+    hipGraphExec_t exec;
+  };
+  std::vector<StepGraph> graphs;
+  int no_graphs;                // 1: cm1g_step launches kernel by kernel
+  int tile_rows;                // rows of 64 columns per workgroup of the column kernels (1, 2 or 4)
+  // non-null while an ensemble records this member's step: launches are
+  // appended here instead of run
+  std::vector<struct LaunchRec> *rec;
 };
+
+// drop the captured steps (after anything that changes launch arguments)
+void graphs_clear(cm1g_model *m);
+// the host side of one large step (solve1, solve2, mp_driver, solve3)
+int step_body(cm1g_model *m);
+// every field pointer and the scalar updates' partner, and back
+std::vector<float *> field_map(const cm1g_model *m);
+void set_field_map(cm1g_model *m, const std::vector<float *> &v);
 
 // bracket a launch of kernel `which` with timer events (no-op unless selected)
 void ktimer_mark(cm1g_model *m, int which);
@@ -90,6 +115,75 @@ void ktimer_mark(cm1g_model *m, int which);
 void cm1g_set_error(const std::string &s);
 
 // ---------------------------------------------------------------- launching
+// Every kernel of the time step is a device functor f(bx, by, bz), run by
+// each workgroup (bx, by, bz) of its grid and launched through launch<NT>()
+// (NT = the workgroup size it is compiled for).  A model runs it as a kernel
+// of its own (k_one).  An ensemble instead records the launches of every
+// member's step and runs the matching launches of all members as one
+// kernel (k_many): the members' grids are laid end to end, each starting on
+// a multiple of 8 workgroups, and each workgroup finds its member in a
+// table of start offsets and runs that member's functor.
+struct BatchSlot {
+  int start, n, gx, gy;  // first workgroup, workgroups, grid x and y of one member
+};
+
+template <int NT, class F>
+__global__ void __launch_bounds__(NT) k_one(F f) {
+  f((int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+}
+
+template <int NT, class F>
+__global__ void __launch_bounds__(NT) k_many(const F *__restrict__ fs, const BatchSlot *__restrict__ slots,
+                                             int nmem) {
+  // the search is uniform across the workgroup: keep it in scalar registers
+  // so the member's functor is read with scalar loads
+  const int b = (int)blockIdx.x;
+  int lo = 0, hi = nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (__builtin_amdgcn_readfirstlane(slots[mid].start) <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  const int start = __builtin_amdgcn_readfirstlane(slots[lo].start);
+  const int n = __builtin_amdgcn_readfirstlane(slots[lo].n);
+  const int gx = __builtin_amdgcn_readfirstlane(slots[lo].gx);
+  const int gy = __builtin_amdgcn_readfirstlane(slots[lo].gy);
+  const int l = b - start;
+  if (l >= n) return;  // padding up to the next member's start
+  const F f = fs[lo];    // read before the functor stores anything
+  f(l % gx, (l / gx) % gy, l / (gx * gy));
+}
+
+// One recorded launch: the functor's bytes, its grid and workgroup, and the
+// launcher of the k_many instance for its type (which also identifies it).
+struct LaunchRec {
+  void (*many)(hipStream_t, const void *fs, const BatchSlot *slots, int nmem, int nblocks, dim3 block);
+  dim3 grid, block;
+  std::vector<unsigned char> fn;
+};
+
+template <int NT, class F>
+void launch_many(hipStream_t s, const void *fs, const BatchSlot *slots, int nmem, int nblocks, dim3 block) {
+  hipLaunchKernelGGL((k_many<NT, F>), dim3(nblocks), block, 0, s, (const F *)fs, slots, nmem);
+}
+
+template <int NT, class F>
+inline void launch(cm1g_model *m, dim3 g, dim3 b, const F &f) {
+  static_assert(std::is_trivially_copyable<F>::value, "launch: functor must be trivially copyable");
+  if (m->rec) {
+    LaunchRec r;
+    r.many = &launch_many<NT, F>;
+    r.grid = g;
+    r.block = b;
+    r.fn.resize(sizeof(F));
+    std::memcpy(r.fn.data(), (const void *)&f, sizeof(F));
+    m->rec->push_back(std::move(r));
+    return;
+  }
+  hipLaunchKernelGGL((k_one<NT, F>), g, b, 0, m->stream, f);
+}
+
 // Element-wise launches.  Point i = 1 sits at the start of a 128-byte line
 // (CM1G_IOFF), so the launch origin in i is rounded down to 1 - 64m: every
 // wavefront then covers whole lines of one row, also for ranges that start
@@ -98,41 +192,33 @@ void cm1g_set_error(const std::string &s);
 // up to twice the time of the aligned launch).
 inline int i_origin(int ilo) { return ilo >= 1 ? 1 + ((ilo - 1) / 64) * 64 : 1 - ((64 - ilo) / 64) * 64; }
 
-template <class F>
-__global__ void __launch_bounds__(256) k_for3(F f, int i0, int ilo, int ihi, int j0, int nj, int k0) {
-  int i = i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  int j = j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  int k = k0 + (int)blockIdx.z;
-  if (i >= ilo && i <= ihi && j < j0 + nj) f(i, j, k);
-}
-
 // run f(i,j,k) over the inclusive box [ilo,ihi]x[jlo,jhi]x[klo,khi]
 template <class F>
-inline void for3(hipStream_t s, int ilo, int ihi, int jlo, int jhi, int klo, int khi, F f) {
-  int nj = jhi - jlo + 1, nk = khi - klo + 1;
+inline void for3(cm1g_model *m, int ilo, int ihi, int jlo, int jhi, int klo, int khi, F f) {
+  const int nj = jhi - jlo + 1, nk = khi - klo + 1;
   if (ihi < ilo || nj <= 0 || nk <= 0) return;
   const int i0 = i_origin(ilo), ni = ihi - i0 + 1;
-  dim3 b(64, 4, 1);
-  dim3 g((ni + 63) / 64, (nj + 3) / 4, nk);
-  hipLaunchKernelGGL(k_for3<F>, g, b, 0, s, f, i0, ilo, ihi, jlo, nj, klo);
-}
-
-template <class F>
-__global__ void __launch_bounds__(256) k_for2(F f, int i0, int ilo, int ihi, int j0, int nj) {
-  int i = i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  int j = j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  if (i >= ilo && i <= ihi && j < j0 + nj) f(i, j);
+  launch<256>(m, dim3((ni + 63) / 64, (nj + 3) / 4, nk), dim3(64, 4, 1),
+              [=] __device__(int bx, int by, int bz) {
+                const int i = i0 + bx * 64 + (int)threadIdx.x;
+                const int j = jlo + by * 4 + (int)threadIdx.y;
+                const int k = klo + bz;
+                if (i >= ilo && i <= ihi && j <= jhi) f(i, j, k);
+              });
 }
 
 // run f(i,j) over [ilo,ihi]x[jlo,jhi] (column kernels loop k inside)
 template <class F>
-inline void for2(hipStream_t s, int ilo, int ihi, int jlo, int jhi, F f) {
-  int nj = jhi - jlo + 1;
+inline void for2(cm1g_model *m, int ilo, int ihi, int jlo, int jhi, F f) {
+  const int nj = jhi - jlo + 1;
   if (ihi < ilo || nj <= 0) return;
   const int i0 = i_origin(ilo), ni = ihi - i0 + 1;
-  dim3 b(64, 4, 1);
-  dim3 g((ni + 63) / 64, (nj + 3) / 4, 1);
-  hipLaunchKernelGGL(k_for2<F>, g, b, 0, s, f, i0, ilo, ihi, jlo, nj);
+  launch<256>(m, dim3((ni + 63) / 64, (nj + 3) / 4, 1), dim3(64, 4, 1),
+              [=] __device__(int bx, int by, int) {
+                const int i = i0 + bx * 64 + (int)threadIdx.x;
+                const int j = jlo + by * 4 + (int)threadIdx.y;
+                if (i >= ilo && i <= ihi && j <= jhi) f(i, j);
+              });
 }
 
 // Column kernels run on a 1-D grid of 64 x ty column tiles (one wavefront
@@ -178,9 +264,6 @@ __device__ __forceinline__ bool col_tile_b(const ColGrid &g, int b, int &i, int 
   i = g.i0 + (t % g.gx) * 64 + (int)(threadIdx.x & 63);
   j = g.j0 + (t / g.gx) * g.ty + (int)(threadIdx.x >> 6);
   return true;
-}
-__device__ __forceinline__ bool col_tile(const ColGrid &g, int &i, int &j) {
-  return col_tile_b(g, (int)blockIdx.x, i, j);
 }
 
 // ---------------------------------------------------------------- modules

@@ -9,7 +9,7 @@
 void pdefq_run(cm1g_model *m, float rmax, const float *rho, float *q) {
   const KP P = m->P;
   const int nk = P.nk;
-  for2(m->stream, 1, P.ni, 1, P.nj, [=] __device__(int i, int j) {
+  for2(m, 1, P.ni, 1, P.nj, [=] __device__(int i, int j) {
     double t1 = 0.0, t2 = 0.0;
     for (int k = 1; k <= nk; k++) {
       int c = IXD(P, i, j, k);
@@ -54,14 +54,14 @@ void kessler_branch_run(cm1g_model *m) {
   pdefq_run(m, P.K.qsmall, rho, qc);
   pdefq_run(m, P.K.qsmall, rho, qr);
   // k_fallout
-  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+  for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
     int c = IXD(P, i, j, k);
     float v = 0.0f;
     if (qr[c] > 1.0e-12f) v = 14.34f * pow_f(rho[c] * qr[c], 0.1346f) * sqrtf(1.15f / rho[c]);
     vr[c] = v;
   });
   // fallout: per column, variable number of sedimentation substeps
-  for2(m->stream, 1, ni, 1, nj, [=] __device__(int i, int j) {
+  for2(m, 1, ni, 1, nj, [=] __device__(int i, int j) {
     const float rdz = P.rdz;
     float crmax = 0.0f;
     for (int k = 1; k <= nk; k++) crmax = fmaxf(crmax, vr[IXD(P, i, j, k)] * dt * rdz * P.mh[k]);
@@ -97,7 +97,7 @@ void kessler_branch_run(cm1g_model *m) {
     colbud[(i - 1) + (j - 1) * ni] = bud;
   });
   // kessler (pointwise)
-  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+  for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
     const Consts &K = P.K;
     int c = IXD(P, i, j, k);
     float th = th3d[c], pp = pp3d[c];
@@ -149,7 +149,7 @@ void kessler_branch_run(cm1g_model *m) {
     qr[c] = qrnew;
   });
   // satadj (nrk=4), iterative saturation adjustment
-  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+  for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
     const Consts &K = P.K;
     const float converge = 2.0f * K.tsmall;
     int c = IXD(P, i, j, k);

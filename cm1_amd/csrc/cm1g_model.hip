@@ -110,6 +110,7 @@ extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
   cm1g_model *m = new cm1g_model();  // value-initialized: pointers null, counters zero
   m->c = *cfg;
   if (m->c.csmax == 0.0f) m->c.csmax = 350.0f;
+  m->tile_rows = 4;
   KP &P = m->P;
   P.ni = cfg->ni; P.nj = cfg->nj; P.nk = cfg->nk;
   P.px = ((cfg->ni + CM1G_IOFF + 5) + 63) / 64 * 64;
@@ -214,6 +215,7 @@ extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
 extern "C" int cm1g_destroy(cm1g_model *m) {
   if (!m) return 0;
   if (m->stream) hipStreamSynchronize(m->stream);
+  graphs_clear(m);
   for (int i = 0; i < CM1G_NFIELDS; i++) if (m->f[i]) hipFree(m->f[i]);
   for (int i = 0; i < CM1G_K_NFIELDS; i++) { if (m->kf[i]) hipFree(m->kf[i]); free(m->hk[i]); }
   for (int i = 0; i < 6; i++) if (m->h[i]) hipFree(m->h[i]);
@@ -250,6 +252,8 @@ extern "C" int cm1g_get_column(cm1g_model *m, int id, float *host) {
 
 extern "C" int cm1g_metrics(cm1g_model *m) {
   ARGCHK(m, "cm1g_metrics: null model");
+  // the metric scalars are launch arguments of the captured steps
+  graphs_clear(m);
   const int nk = m->c.nk;
   float *zf = m->hk[CM1G_K_ZF], *zh = m->hk[CM1G_K_ZH], *mh = m->hk[CM1G_K_MH], *mf = m->hk[CM1G_K_MF];
   float *c1 = m->hk[CM1G_K_C1], *c2 = m->hk[CM1G_K_C2];
@@ -303,6 +307,7 @@ extern "C" int cm1g_metrics(cm1g_model *m) {
 
 extern "C" int cm1g_base_derive(cm1g_model *m) {
   ARGCHK(m, "cm1g_base_derive: null model");
+  graphs_clear(m);
   const int nk = m->c.nk;
   const Consts &K = m->P.K;
   float *pi0 = m->hk[CM1G_K_PI0], *th0 = m->hk[CM1G_K_TH0], *qv0 = m->hk[CM1G_K_QV0];
@@ -542,9 +547,10 @@ static int check_launch() {
 // Per-level partial sums of the dry-air mass and of pi for the apmasscon
 // adjustment (src/solve2.F:1617-1656), in double precision.
 #define MASS_NB 32
-__global__ void __launch_bounds__(256) k_mass_levels(KP P, const float *rho, const float *pp3d, double *out) {
+__device__ __forceinline__ void mass_level_sums(const KP &P, const float *rho, const float *pp3d, double *out,
+                                                int bx, int by) {
   __shared__ double r3[256], r4[256];
-  const int k = 1 + (int)blockIdx.x, b = (int)blockIdx.y;
+  const int k = 1 + bx, b = by;
   double a3 = 0.0, a4 = 0.0;
   for (int j = 1 + b; j <= P.nj; j += MASS_NB)
     for (int i = 1 + (int)threadIdx.x; i <= P.ni; i += blockDim.x) {
@@ -578,7 +584,8 @@ __device__ inline void dd_add(double &hi, double &lo, double x) {
   lo += err;
 }
 
-__global__ void k_mass_final(int nk, int mode, float rddcv, double ntot, const double *lev, double *dscal) {
+__device__ __forceinline__ void mass_final(int nk, int mode, float rddcv, double ntot, const double *lev,
+                                           double *dscal) {
   double h3 = 0.0, l3 = 0.0, h4 = 0.0, l4 = 0.0;
   for (int k = 1; k <= nk; k++) {
     double s3 = 0.0, s4 = 0.0;
@@ -600,23 +607,28 @@ __global__ void k_mass_final(int nk, int mode, float rddcv, double ntot, const d
 }
 
 static void mass_levels(cm1g_model *m, const float *rho, int mode) {
-  hipLaunchKernelGGL(k_mass_levels, dim3(m->P.nk, MASS_NB), dim3(256), 0, m->stream, m->P, rho,
-                     m->f[CM1G_PP3D], m->dred);
-  double ntot = (double)((long long)m->P.ni * m->P.nj * m->P.nk);
-  hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(1), 0, m->stream, m->P.nk, mode, m->P.K.rddcv,
-                     ntot, m->dred, m->dscal);
+  const KP P = m->P;
+  const float *pp3d = m->f[CM1G_PP3D];
+  double *lev = m->dred, *dscal = m->dscal;
+  launch<256>(m, dim3(P.nk, MASS_NB), dim3(256),
+              [=] __device__(int bx, int by, int) { mass_level_sums(P, rho, pp3d, lev, bx, by); });
+  const double ntot = (double)((long long)P.ni * P.nj * P.nk);
+  const int nk = P.nk;
+  const float rddcv = P.K.rddcv;
+  launch<64>(m, dim3(1), dim3(1),
+             [=] __device__(int, int, int) { mass_final(nk, mode, rddcv, ntot, lev, dscal); });
 }
 
 // domain rain-accumulation budget (src/kessler.F:470-476): per-row sums in
 // i order, then rows in j order, as the reference does.
-__global__ void k_train_rows(int ni, int nj, const double *colbud, double *rows) {
-  int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void train_rows(int ni, int nj, const double *colbud, double *rows, int bx) {
+  const int j = bx * (int)blockDim.x + (int)threadIdx.x;
   if (j >= nj) return;
   double b = 0.0;
   for (int i = 0; i < ni; i++) b = b + colbud[i + j * ni];
   rows[j] = b;
 }
-__global__ void k_train(int nj, double tem, const double *rows, double *dscal) {
+__device__ __forceinline__ void train_sum(int nj, double tem, const double *rows, double *dscal) {
   double tr = dscal[3];
   for (int j = 0; j < nj; j++) tr = tr + rows[j] * tem;
   dscal[3] = tr;
@@ -624,11 +636,13 @@ __global__ void k_train(int nj, double tem, const double *rows, double *dscal) {
 // train += the fallout's column budgets of this call (the end of
 // src/kessler.F:fallout)
 static void train_accumulate(cm1g_model *m) {
-  double *rows = m->dred + (size_t)m->P.ni * m->P.nj;
-  hipLaunchKernelGGL(k_train_rows, dim3((m->P.nj + 63) / 64), dim3(64), 0, m->stream, m->P.ni,
-                     m->P.nj, m->dred, rows);
-  hipLaunchKernelGGL(k_train, dim3(1), dim3(1), 0, m->stream, m->P.nj,
-                     (double)(m->c.dx * m->c.dy), rows, m->dscal);
+  const int ni = m->P.ni, nj = m->P.nj;
+  const double *colbud = m->dred;
+  double *rows = m->dred + (size_t)ni * nj, *dscal = m->dscal;
+  const double tem = (double)(m->c.dx * m->c.dy);
+  launch<64>(m, dim3((nj + 63) / 64), dim3(64),
+             [=] __device__(int bx, int, int) { train_rows(ni, nj, colbud, rows, bx); });
+  launch<64>(m, dim3(1), dim3(1), [=] __device__(int, int, int) { train_sum(nj, tem, rows, dscal); });
 }
 
 // ------------------------------------------------------------------ getset
@@ -660,7 +674,7 @@ extern "C" int cm1g_getset(cm1g_model *m, int fresh_start) {
   const float *pp3d = f[CM1G_PP3D], *th3d = f[CM1G_TH3D], *qv = f[CM1G_Q3D0];
   const int moist = m->c.imoist;
   if (fresh_start) {
-    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
       const Consts &K = P.K;
       int c = IXD(P, i, j, k);
       float p = K.p00 * pow_f(P.pi0[k] + pp3d[c], K.cpdrd);
@@ -672,14 +686,14 @@ extern "C" int cm1g_getset(cm1g_model *m, int fresh_start) {
     });
   }
   bc_fill(m, BC_S, rho);
-  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+  for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
     int c = IXD(P, i, j, k);
     rr[c] = 1.0f / rho[c];
     rf[c] = P.c1[k] * rho[IXD(P, i, j, k - 1)] + P.c2[k] * rho[c];
   });
   bc_fill(m, BC_S, rr);
   bc_fill(m, BC_S, rf);
-  for2(m->stream, 0, ni + 1, 0, nj + 1, [=] __device__(int i, int j) {
+  for2(m, 0, ni + 1, 0, nj + 1, [=] __device__(int i, int j) {
     rf[IXD(P, i, j, 1)] = P.cgs1 * rho[IXD(P, i, j, 1)] + P.cgs2 * rho[IXD(P, i, j, 2)] + P.cgs3 * rho[IXD(P, i, j, 3)];
     rf[IXD(P, i, j, nk + 1)] = P.cgt1 * rho[IXD(P, i, j, nk)] + P.cgt2 * rho[IXD(P, i, j, nk - 1)] +
                                P.cgt3 * rho[IXD(P, i, j, nk - 2)];
@@ -688,7 +702,7 @@ extern "C" int cm1g_getset(cm1g_model *m, int fresh_start) {
     mass_levels(m, rho, 0);
     float *ppx = f[CM1G_PPX];
     const float *ppi = f[CM1G_PPI];
-    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
       ppx[IXD(P, i, j, k)] = ppi[IXD(P, i, j, k)];
     });
   }
@@ -707,11 +721,11 @@ extern "C" int cm1g_getset(cm1g_model *m, int fresh_start) {
 // from the velocities with the same products, w(k+1) is carried to the next
 // level, and each level's loads are issued before the previous level's
 // stores.
-__global__ void __launch_bounds__(256) k_massflux_div(KP P, const float *u3d, const float *v3d,
-                                                      const float *w3d, float *rru, float *rrv,
-                                                      float *rrw, float *divx, ColGrid cg) {
+__device__ __forceinline__ void massflux_div(const KP &P, const float *u3d, const float *v3d, const float *w3d,
+                                             float *rru, float *rrv, float *rrw, float *divx,
+                                             const ColGrid &cg, int bx) {
   int i, j;
-  if (!col_tile(cg, i, j)) return;
+  if (!col_tile_b(cg, bx, i, j)) return;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   if (i < 0 || i > ni + 2 || j > nj + 2) return;
   const bool du = (j <= nj + 1), dv = (i <= ni + 1), dd = du && dv;
@@ -761,7 +775,7 @@ extern "C" int cm1g_solve1(cm1g_model *m) {
   float *uten1 = f[CM1G_UTEN1], *vten1 = f[CM1G_VTEN1], *wten1 = f[CM1G_WTEN1], *thten1 = f[CM1G_THTEN1];
   for (int n = 0; n < m->c.npt; n++) {
     float *pt = f[CM1G_PTTEN0 + n];
-    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) { pt[IXD(P, i, j, k)] = 0.0f; });
+    for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) { pt[IXD(P, i, j, k)] = 0.0f; });
   }
   float *ppten = f[CM1G_PPTEN];
   const float *rho = f[CM1G_RHO], *tha = f[CM1G_THA], *ppi = f[CM1G_PPI];
@@ -776,7 +790,7 @@ extern "C" int cm1g_solve1(cm1g_model *m) {
   // on the interior (:1716-1756)
   const float *ua = f[CM1G_UA], *va = f[CM1G_VA], *wa = f[CM1G_WA];
   const int rdamp = P.irdamp;
-  for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+  for3(m, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
     const Consts &K = P.K;
     const float tem = 0.0001f * K.tsmall;
     int c = IXD(P, i, j, k);
@@ -849,7 +863,7 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
     const float *qv = f[CM1G_Q3D0], *qc = f[CM1G_Q3D1], *qr = f[CM1G_Q3D2];
     const float dttmp = dt / (float)(3 + 1 - nrk);
     // thermodynamic prep (src/solve2.F:302-450)
-    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
       const Consts &K = P.K;
       int c = IXD(P, i, j, k);
       float th = th3d[c];
@@ -874,15 +888,16 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
     bc_fill(m, BC_P, t11);
     // rru, rrv, rrw (src/solve2.F:490-545) and divx (:587-625) in one pass
     {
-      const ColGrid cg = col_grid_box(0, ni + 2, 0, nj + 2);
-      hipLaunchKernelGGL(k_massflux_div, col_blocks(cg), col_threads(cg), 0, m->stream, P, u3d, v3d,
-                         w3d, rru, rrv, rrw, divx, cg);
+      const ColGrid cg = col_grid_box(0, ni + 2, 0, nj + 2, m->tile_rows);
+      launch<256>(m, col_blocks(cg), col_threads(cg), [=] __device__(int bx, int, int) {
+        massflux_div(P, u3d, v3d, w3d, rru, rrv, rrw, divx, cg, bx);
+      });
     }
     // tendencies: the pre-RK values (src/solve2.F:459-471) plus advection,
     // each component's own mass flux formed from its velocity
     adv_uvw(m, nrk, 7, 1, 1, 1);
     // sadv = pi0 + pi' (src/solve2.F:1027-1050)
-    for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+    for3(m, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
       int c = IXD(P, i, j, k);
       sadv[c] = P.pi0[k] + pp3d[c];
     });
@@ -908,7 +923,7 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
       float *qvw = f[CM1G_Q3D0], *qcw = f[CM1G_Q3D1];
       float *qpten = f[CM1G_QPTEN], *qtten = f[CM1G_QTTEN], *qvten = f[CM1G_QVTEN], *qcten = f[CM1G_QCTEN];
       const int diab = (moist && m->c.ptype >= 1);
-      for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
         const Consts &K = P.K;
         int c = IXD(P, i, j, k);
         float pp = pp3d[c], th = th3d[c];
@@ -955,7 +970,7 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
     if (nrk == 3 && m->c.apmasscon == 1) {
       const double *dscal = m->dscal;
       float *qvw = f[CM1G_Q3D0];
-      for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
         const Consts &K = P.K;
         int c = IXD(P, i, j, k);
         const float tem = (float)dscal[2];
@@ -1020,7 +1035,7 @@ extern "C" int cm1g_solve3(cm1g_model *m) {
     float *ppi = f[CM1G_PPI], *tha = f[CM1G_THA];
     float *qa0 = f[CM1G_QA0], *qa1 = f[CM1G_QA1], *qa2 = f[CM1G_QA2];
     const float *q0 = f[CM1G_Q3D0], *q1 = f[CM1G_Q3D1], *q2 = f[CM1G_Q3D2];
-    for3(m->stream, 0, ni + 2, 0, nj + 2, 1, nk, [=] __device__(int i, int j, int k) {
+    for3(m, 0, ni + 2, 0, nj + 2, 1, nk, [=] __device__(int i, int j, int k) {
       const int c = IXD(P, i, j, k);
       ua[c] = u3d[c];
       va[c] = v3d[c];
@@ -1059,19 +1074,89 @@ extern "C" int cm1g_solve3(cm1g_model *m) {
   for (int n = 0; n < m->c.npt; n++) {
     float *pta = f[CM1G_PTA0 + n];
     const float *pt3d = f[CM1G_PT3D0 + n];
-    for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+    for3(m, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
       pta[IXD(P, i, j, k)] = pt3d[IXD(P, i, j, k)];
     });
   }
   return check_launch();
 }
 
-extern "C" int cm1g_step(cm1g_model *m, float *cflmax) {
-  ARGCHK(m, "cm1g_step: null model");
+int step_body(cm1g_model *m) {
   int rc = cm1g_solve1(m);
   if (!rc) rc = cm1g_solve2(m);
   if (!rc) rc = cm1g_mp_driver(m);
   if (!rc) rc = cm1g_solve3(m);
+  return rc;
+}
+
+// the buffer arrangement a step starts from: every field pointer and the
+// scalar updates' partner
+std::vector<float *> field_map(const cm1g_model *m) {
+  std::vector<float *> v(m->f, m->f + CM1G_NFIELDS);
+  v.push_back(m->thb);
+  return v;
+}
+void set_field_map(cm1g_model *m, const std::vector<float *> &v) {
+  for (int i = 0; i < CM1G_NFIELDS; i++) m->f[i] = v[i];
+  m->thb = v[CM1G_NFIELDS];
+}
+
+void graphs_clear(cm1g_model *m) {
+  for (auto &g : m->graphs) hipGraphExecDestroy(g.exec);
+  m->graphs.clear();
+}
+
+// One step as a graph: replay the one captured from this arrangement, or
+// capture it now (capturing runs the host side of the step, which leaves
+// the arrangement after the step, but launches nothing) and replay it.
+static int step_graph(cm1g_model *m) {
+  const std::vector<float *> key = field_map(m);
+  for (auto &g : m->graphs)
+    if (g.before == key) {
+      HIPCHK(hipGraphLaunch(g.exec, m->stream));
+      set_field_map(m, g.after);
+      return 0;
+    }
+  if (m->graphs.size() >= 16) graphs_clear(m);
+  HIPCHK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = step_body(m);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(m->stream, &graph);
+  if (rc || e != hipSuccess) {
+    // nothing ran: the device still holds the arrangement before the step
+    if (graph) hipGraphDestroy(graph);
+    set_field_map(m, key);
+    if (rc) return rc;
+    cm1g_set_error(std::string("cm1g_step: graph capture: ") + hipGetErrorString(e));
+    return CM1G_E_HIP;
+  }
+  cm1g_model::StepGraph g;
+  g.before = key;
+  g.after = field_map(m);
+  const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    set_field_map(m, key);
+    cm1g_set_error(std::string("cm1g_step: graph instantiate: ") + hipGetErrorString(ei));
+    return CM1G_E_HIP;
+  }
+  m->graphs.push_back(g);
+  HIPCHK(hipGraphLaunch(g.exec, m->stream));
+  return 0;
+}
+
+extern "C" int cm1g_set_graphs(cm1g_model *m, int on) {
+  ARGCHK(m && (on == 0 || on == 1), "cm1g_set_graphs: bad argument");
+  m->no_graphs = !on;
+  if (!on) graphs_clear(m);
+  return 0;
+}
+
+extern "C" int cm1g_step(cm1g_model *m, float *cflmax) {
+  ARGCHK(m, "cm1g_step: null model");
+  // a kernel timer records events between launches: time kernel by kernel
+  const bool graph = !m->no_graphs && m->kt_which == CM1G_KT_NONE;
+  int rc = graph ? step_graph(m) : step_body(m);
   if (rc) return rc;
   if (cflmax) {
     HIPCHK(hipMemcpyAsync(&m->cflmax, m->fred, sizeof(float), hipMemcpyDeviceToHost, m->stream));

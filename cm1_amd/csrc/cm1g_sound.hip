@@ -6,10 +6,6 @@
 // (src/misclibs.F:2726-2814).
 #include "cm1g_internal.h"
 
-// rows of columns per acoustic-substep workgroup (64 columns each)
-#ifndef CM1G_SOUND_TY
-#define CM1G_SOUND_TY 4
-#endif
 
 static void sound_nloop(int nrk, int nsound, float dt, int *nloop_out, float *dts_out) {
   int nloop;
@@ -145,16 +141,16 @@ __device__ __forceinline__ void sound_ring_point(const KP &P, const SubArgs &a, 
 // The a.nring workgroups after the column tiles do the averaging ring (one
 // thread per ring point and level); they fill the CUs the last tiles leave
 // idle.
-__global__ void __launch_bounds__(64 * CM1G_SOUND_TY) k_sound_sub(KP P, SubArgs a, ColGrid cg) {
+__device__ __forceinline__ void sound_sub(const KP &P, const SubArgs &a, const ColGrid &cg, int bx) {
   const int nmain = cg.per * NXCD;
-  if ((int)blockIdx.x >= nmain) {
+  if (bx >= nmain) {
     const int npts = P.ni + P.nj + 1;
-    const int t = ((int)blockIdx.x - nmain) * (int)blockDim.x + (int)threadIdx.x;
+    const int t = (bx - nmain) * (int)blockDim.x + (int)threadIdx.x;
     if (t < npts * P.nk) sound_ring_point(P, a, t % npts, 1 + t / npts);
     return;
   }
   int i, j;
-  if (!col_tile(cg, i, j)) return;
+  if (!col_tile_b(cg, bx, i, j)) return;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   if (i > ni || j > nj) return;
   const float dts = a.dts, rdz = P.rdz, cp = P.K.cp, smeps = P.K.smeps;
@@ -378,7 +374,7 @@ void sound_run(cm1g_model *m, int nrk) {
   // last substep writes u3d/v3d.
   const bool fold = (nrk >= 2);
   a.wa = wa;
-  const ColGrid cg = col_grid(ni, nj, CM1G_SOUND_TY);
+  const ColGrid cg = col_grid(ni, nj, m->tile_rows);
   const dim3 blk = col_threads(cg), grd = col_blocks(cg);
   {
     const int nthr = (int)blk.x, npts = (ni + nj + 1) * nk;
@@ -409,7 +405,8 @@ void sound_run(cm1g_model *m, int nrk) {
     a.first = (n == 1);
     a.last = (n == nloop);
     ktimer_mark(m, CM1G_KT_SOUND_SUBSTEP);
-    hipLaunchKernelGGL(k_sound_sub, dim3(grd.x + a.nring), blk, 0, m->stream, P, a, cg);
+    launch<256>(m, dim3(grd.x + a.nring), blk,
+                [=] __device__(int bx, int, int) { sound_sub(P, a, cg, bx); });
     ktimer_mark(m, CM1G_KT_SOUND_SUBSTEP);
     if (n < nloop) bc_fill(m, BC_P, a.ppdY);
     uprev = udst;
@@ -418,14 +415,14 @@ void sound_run(cm1g_model *m, int nrk) {
   if (!fold && (nloop & 1)) {
     // the last substep wrote the scratch pair: copy the updated range back
     const float *us = m->scr1, *vs = m->scr2;
-    for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+    for3(m, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
       int c = IXD(P, i, j, k);
       u3d[c] = us[c];
       v3d[c] = vs[c];
     });
   }
   if (nrk == 3)
-    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
       int c = IXD(P, i, j, k);
       ppx[c] = dum1[c];
     });
@@ -460,7 +457,7 @@ void sounde_run(cm1g_model *m, int nrk) {
 
   // prep copies (src/sounde.F:181-233)
   if (nrk == 1) {
-    for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+    for3(m, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
       int c = IXD(P, i, j, k);
       ppd[c] = ppx[c];
       rru[c] = 0.0f;
@@ -468,7 +465,7 @@ void sounde_run(cm1g_model *m, int nrk) {
       rrw[c] = 0.0f;
     });
   } else {
-    for3(m->stream, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+    for3(m, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
       int c = IXD(P, i, j, k);
       u3d[c] = ua[c];
       rru[c] = 0.0f;
@@ -487,7 +484,7 @@ void sounde_run(cm1g_model *m, int nrk) {
   for (int n = 1; n <= nloop; n++) {
     const bool last = (n == nloop);
     if (opw || ope)
-      for2(m->stream, 1, nj, 1, nk, [=] __device__(int j, int k) {
+      for2(m, 1, nj, 1, nk, [=] __device__(int j, int k) {
         if (opw)
           u3d[IXD(P, 1, j, k)] = u3d[IXD(P, 1, j, k)] + dts * (-radbcw[(j + 2) + k * P.py] *
               (u3d[IXD(P, 2, j, k)] - u3d[IXD(P, 1, j, k)]) * rdx * P.uh[1 + 2] + uten[IXD(P, 1, j, k)]);
@@ -496,7 +493,7 @@ void sounde_run(cm1g_model *m, int nrk) {
               (u3d[IXD(P, ni + 1, j, k)] - u3d[IXD(P, ni, j, k)]) * rdx * P.uh[ni + 2] + uten[IXD(P, ni + 1, j, k)]);
       });
     if (ops || opn)
-      for2(m->stream, 1, ni, 1, nk, [=] __device__(int i, int k) {
+      for2(m, 1, ni, 1, nk, [=] __device__(int i, int k) {
         if (ops)
           v3d[IXD(P, i, 1, k)] = v3d[IXD(P, i, 1, k)] + dts * (-radbcs[(i + 2) + k * P.px] *
               (v3d[IXD(P, i, 2, k)] - v3d[IXD(P, i, 1, k)]) * rdy * P.vh[1 + 2] + vten[IXD(P, i, 1, k)]);
@@ -507,7 +504,7 @@ void sounde_run(cm1g_model *m, int nrk) {
     // u, v, w forward and the velocity averages
     {
       const float tem1 = rdx * cp * 0.5f, tem2 = rdy * cp * 0.5f;
-      for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+      for3(m, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
         const int c = IXD(P, i, j, k);
         const float dtu = dts * dtu0[IX2D(P, i, j)], dtv = dts * dtv0[IX2D(P, i, j)];
         const float u = u3d[c] + dtu * (uten[c] - tem1 * P.uf[i + 2] * (ppd[c] - ppd[c - 1]) *
@@ -537,7 +534,7 @@ void sounde_run(cm1g_model *m, int nrk) {
     // pressure
     {
       const float temx = dts * 0.5f * rdx, temy = dts * 0.5f * rdy;
-      for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
         const int c = IXD(P, i, j, k);
         const float mh = P.mh[k], rr0 = P.rr0[k];
         float pk1 = 0.0f, pk2 = 0.0f;
@@ -571,7 +568,7 @@ void sounde_run(cm1g_model *m, int nrk) {
     if (n < nloop) bc_fill(m, BC_P, ppd);
   }
   if (nrk == 3)
-    for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
       int c = IXD(P, i, j, k);
       ppx[c] = dum1[c];
     });
@@ -587,7 +584,7 @@ void getdiv_both(cm1g_model *m, int want_t11, int want_divx, float *div) {
   const float *rru = m->f[CM1G_RRU], *rrv = m->f[CM1G_RRV], *rrw = m->f[CM1G_RRW];
   float *t11 = div ? div : m->f[CM1G_T11], *divx = m->f[CM1G_DIVX];
   const float smeps = P.K.smeps;
-  for3(m->stream, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+  for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
     int c = IXD(P, i, j, k);
     if (want_t11) {
       float d = ((rru[IXD(P, i + 1, j, k)] - rru[c]) * P.rdx * P.uh[i + 2] +
@@ -615,7 +612,7 @@ void scale_mass_fluxes(cm1g_model *m) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   float *rru = m->f[CM1G_RRU], *rrv = m->f[CM1G_RRV], *rrw = m->f[CM1G_RRW];
-  for3(m->stream, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+  for3(m, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
     const int c = IXD(P, i, j, k);
     if (j <= nj) rru[c] = rru[c] * P.rho0[k];
     if (i <= ni) rrv[c] = rrv[c] * P.rho0[k];
@@ -632,10 +629,10 @@ void scale_mass_fluxes(cm1g_model *m) {
 
 // calccflquick: per-level maximum of the Courant magnitude, then the max.
 #define CFL_NB 32
-__global__ void __launch_bounds__(256) k_cfl(KP P, float dt, const float *ua, const float *va,
-                                             const float *wa, float *out) {
+__device__ __forceinline__ void cfl_levels(const KP &P, float dt, const float *ua, const float *va,
+                                           const float *wa, float *out, int bx, int by) {
   __shared__ float red[256];
-  const int k = 1 + (int)blockIdx.x, b = (int)blockIdx.y;
+  const int k = 1 + bx, b = by;
   const float dtdx = 0.5f * dt * P.rdx, dtdy = 0.5f * dt * P.rdy, dtdz = 0.5f * dt * P.rdz;
   float cfl = 0.0f;
   for (int j = 1 + b; j <= P.nj; j += CFL_NB)
@@ -656,13 +653,17 @@ __global__ void __launch_bounds__(256) k_cfl(KP P, float dt, const float *ua, co
 }
 
 // maximum over the per-level partial maxima (exact in any order)
-__global__ void k_cfl_final(int n, float *out) {
+__device__ __forceinline__ void cfl_final(int n, float *out) {
   float fmax = -99999999.0f;
   for (int t = 1; t <= n; t++) fmax = fmaxf(fmax, out[t]);
   out[0] = fmax;
 }
 
 void cfl_run(cm1g_model *m, float dt, const float *u, const float *v, const float *w) {
-  hipLaunchKernelGGL(k_cfl, dim3(m->P.nk, CFL_NB), dim3(256), 0, m->stream, m->P, dt, u, v, w, m->fred);
-  hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(1), 0, m->stream, m->P.nk * CFL_NB, m->fred);
+  const KP P = m->P;
+  float *out = m->fred;
+  launch<256>(m, dim3(P.nk, CFL_NB), dim3(256),
+              [=] __device__(int bx, int by, int) { cfl_levels(P, dt, u, v, w, out, bx, by); });
+  const int n = P.nk * CFL_NB;
+  launch<64>(m, dim3(1), dim3(1), [=] __device__(int, int, int) { cfl_final(n, out); });
 }

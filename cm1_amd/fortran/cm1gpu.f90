@@ -213,6 +213,48 @@ module cm1gpu
       type(c_ptr), value :: m
       real(c_float), intent(out) :: cflmax
     end function
+    integer(c_int) function cm1g_set_graphs(m, on) bind(C, name="cm1g_set_graphs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function cm1g_set_tiles(m, ty) bind(C, name="cm1g_set_tiles")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: ty
+    end function
+    ! ensembles: members(n) are model handles
+    integer(c_int) function cm1g_ensemble_create(members, n, e) bind(C, name="cm1g_ensemble_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: members(*)
+      integer(c_int), value :: n
+      type(c_ptr), intent(out) :: e
+    end function
+    integer(c_int) function cm1g_ensemble_destroy(e) bind(C, name="cm1g_ensemble_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+    end function
+    integer(c_int) function cm1g_ensemble_step(e) bind(C, name="cm1g_ensemble_step")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+    end function
+    integer(c_int) function cm1g_ensemble_sync(e) bind(C, name="cm1g_ensemble_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+    end function
+    integer(c_int) function cm1g_ensemble_set_tiles(e, ty) bind(C, name="cm1g_ensemble_set_tiles")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: ty
+    end function
+    integer(c_int) function cm1g_ensemble_tune(e, steps, ty_best, ms_per_step) &
+        bind(C, name="cm1g_ensemble_tune")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: e
+      integer(c_int), value :: steps
+      integer(c_int), intent(out) :: ty_best
+      real(c_float), intent(out) :: ms_per_step(3)
+    end function
     integer(c_int) function cm1g_solve1(m) bind(C, name="cm1g_solve1")
       import :: c_int, c_ptr
       type(c_ptr), value :: m

@@ -105,6 +105,15 @@ class Model:
             N.check(self.L.cm1g_step(self.h, C.byref(cfl)), "cm1g_step")
         return cfl.value
 
+    def set_tiles(self, ty):
+        """Rows of 64 columns per workgroup of the column kernels (1, 2, 4)."""
+        N.check(self.L.cm1g_set_tiles(self.h, int(ty)), "cm1g_set_tiles")
+
+    def set_graphs(self, on):
+        """Replay each large step as a captured HIP graph (default) or launch
+        kernel by kernel (cm1g_set_graphs)."""
+        N.check(self.L.cm1g_set_graphs(self.h, 1 if on else 0), "cm1g_set_graphs")
+
     def sync(self):
         N.check(self.L.cm1g_sync(self.h), "cm1g_sync")
 
@@ -139,3 +148,46 @@ def setup(cfg, cols, fields, fresh=True):
         m.put(name, arr)
     m.getset(fresh)
     return m
+
+
+class Ensemble:
+    """Members (set-up :class:`Model` objects of any size) advanced together:
+    one launch per kernel of the large step for all members, the merged step
+    replayed as one HIP graph (cm1g_ensemble_*).  :meth:`sync` before using a
+    member directly."""
+
+    def __init__(self, members):
+        self.L = N.lib()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = C.c_void_p()
+        N.check(self.L.cm1g_ensemble_create(arr, len(self.members), C.byref(h)), "cm1g_ensemble_create")
+        self.h = h
+
+    def step(self, n=1):
+        for _ in range(n):
+            N.check(self.L.cm1g_ensemble_step(self.h), "cm1g_ensemble_step")
+
+    def sync(self):
+        N.check(self.L.cm1g_ensemble_sync(self.h), "cm1g_ensemble_sync")
+
+    def set_tiles(self, ty):
+        N.check(self.L.cm1g_ensemble_set_tiles(self.h, int(ty)), "cm1g_ensemble_set_tiles")
+
+    def tune(self, steps):
+        """Autotune the column-tile height; returns (best, {ty: ms/step})."""
+        best = C.c_int()
+        ms = (C.c_float * 3)()
+        N.check(self.L.cm1g_ensemble_tune(self.h, int(steps), C.byref(best), ms), "cm1g_ensemble_tune")
+        return best.value, {1: ms[0], 2: ms[1], 4: ms[2]}
+
+    def close(self):
+        if self.h:
+            self.L.cm1g_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

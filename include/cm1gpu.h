@@ -150,6 +150,41 @@ int cm1g_getset(cm1g_model *m, int fresh_start);
 /* One large step: solve1 + solve2 + mp_driver + solve3
  * (src/cm1.F:3313-3476).  Sets *cflmax (may be NULL). */
 int cm1g_step(cm1g_model *m, float *cflmax);
+/* No counterpart in the reference.  on=1 (the default): cm1g_step replays
+ * each large step as one captured HIP graph (captured on first use, one
+ * per arrangement of the exchanged scalar buffers), which removes the
+ * per-kernel launch cost that dominates small grids; on=0: kernel-by-kernel
+ * launches.  Results are identical either way. */
+int cm1g_set_graphs(cm1g_model *m, int on);
+/* No counterpart in the reference.  Rows of 64 columns per workgroup of the
+ * column kernels (advection, pdef, mass fluxes, acoustic substep): 1, 2 or
+ * 4 (the default).  Changes which workgroup computes which column, never a
+ * result. */
+int cm1g_set_tiles(cm1g_model *m, int ty);
+
+/* --- ensembles (no counterpart in the reference) ------------------------
+ * Members of any grid size and sounding, created and set up as single
+ * models, advanced together: each kernel of the large step runs once for
+ * all members (member-indexed workgroups), and the whole merged step is
+ * replayed as one captured HIP graph.  The members must share the switches
+ * that decide which kernels a step launches (psolver, imoist, ptype,
+ * advwenos/v, npt, apmasscon, open or periodic sides); sizes, soundings,
+ * dt, nsound-independent scalars may differ, nsound may not.  Results equal
+ * those of the members stepped one by one, bit for bit.  Work is queued on
+ * the ensemble's own stream: call cm1g_ensemble_sync before using a member
+ * through any other entry point, and do not step a member on its own while
+ * it belongs to a live ensemble. */
+typedef struct cm1g_ensemble cm1g_ensemble;
+int cm1g_ensemble_create(cm1g_model *const *members, int n, cm1g_ensemble **out);
+int cm1g_ensemble_destroy(cm1g_ensemble *e);   /* the members stay */
+int cm1g_ensemble_step(cm1g_ensemble *e);      /* one large step of every member */
+int cm1g_ensemble_sync(cm1g_ensemble *e);
+int cm1g_ensemble_set_tiles(cm1g_ensemble *e, int ty);
+/* Autotuner: times `steps` ensemble steps at each tile height (1, 2, 4),
+ * keeps the fastest in *ty_best and, if ms_per_step is non-NULL, stores the
+ * three timings.  The 4*(steps+1) steps it takes are ordinary steps of the
+ * run. */
+int cm1g_ensemble_tune(cm1g_ensemble *e, int steps, int *ty_best, float *ms_per_step);
 int cm1g_solve1(cm1g_model *m);   /* src/solve1.F:77-1791 (ten1, ppten) */
 int cm1g_solve2(cm1g_model *m);   /* src/solve2.F:77-2094 (RK3 loop) */
 int cm1g_mp_driver(cm1g_model *m);/* src/mp_driver.F:17-1706 (ptype 0/1) */
