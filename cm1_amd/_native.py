@@ -80,6 +80,20 @@ SIGNATURES = {
     "cm1g_ensemble_sync": (C.c_int, [C.c_void_p]),
     "cm1g_ensemble_set_tiles": (C.c_int, [C.c_void_p, C.c_int]),
     "cm1g_ensemble_tune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+    "cm1g_mlp_nparams": (C.c_int, [C.POINTER(C.c_int64)]),
+    "cm1g_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "cm1g_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "cm1g_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cm1g_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cm1g_mlp_set_norm": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cm1g_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "cm1g_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                     C.POINTER(C.c_float), C.c_void_p]),
+    "cm1g_mlp_adam": (C.c_int, [C.c_void_p, C.c_float]),
+    "cm1g_mlp_sync": (C.c_int, [C.c_void_p]),
+    "cm1g_set_surrogate": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cm1g_set_harvest": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]),
+    "cm1g_surrogate_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "cm1g_solve1": (C.c_int, [C.c_void_p]),
     "cm1g_solve2": (C.c_int, [C.c_void_p]),
     "cm1g_mp_driver": (C.c_int, [C.c_void_p]),

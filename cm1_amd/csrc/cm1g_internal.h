@@ -90,6 +90,15 @@ struct cm1g_model {
   // non-null while an ensemble records this member's step: launches are
   // appended here instead of run
   std::vector<struct LaunchRec> *rec;
+  // learned surrogate of the local Kessler step (cm1g_set_surrogate), and the
+  // harvest of (state, change) pairs from the plain one (cm1g_set_harvest)
+  struct cm1g_mlp *sur;
+  float *harv;
+  int64_t harv_cap;
+  int harv_keep;
+  unsigned long long *dcount;   // [0] pairs harvested [1] surrogate local steps [2] fallback cells
+                                // [3] harvest calls [4] cells out of the trained range [5] unusable proposals
+  int *colflag;                 // per column: the surrogate fell back to plain Kessler
 };
 
 // drop the captured steps (after anything that changes launch arguments)

@@ -5,6 +5,7 @@
 // with the reference's k order (so their double-precision column sums are
 // the reference's); pointwise routines run one cell per thread.
 #include "cm1g_internal.h"
+#include "cm1g_mlp.h"
 
 void pdefq_run(cm1g_model *m, float rmax, const float *rho, float *q) {
   const KP P = m->P;
@@ -31,6 +32,299 @@ void pdefq_run(cm1g_model *m, float rmax, const float *rho, float *q) {
 __device__ __forceinline__ float esl_of(float t, float prs) {
   float esl = 611.2f * exp_f(17.67f * (t - 273.15f) / (t - 29.65f));
   return fminf(esl, prs * 0.5f);
+}
+
+// kessler (src/kessler.F:8-138) at one cell
+__device__ __forceinline__ void kessler_cell(const KP &P, float dt, int c, int k, float *th3d, float *pp3d,
+                                             float *prs, float *qv, float *qc, float *qr, const float *rho) {
+  const Consts &K = P.K;
+  float th = th3d[c], pp = pp3d[c];
+  float t = (P.th0[k] + th) * (P.pi0[k] + pp);
+  float qvnew = qv[c], qcnew = qc[c], qrnew = qr[c];
+  float ar = 0.0f, cr = 0.0f, er = 0.0f;
+  if (qcnew > 0.0f) {
+    ar = fmaxf(0.001f * (qcnew - 0.001f), 0.0f);
+    ar = ar * dt;
+  }
+  if (qcnew > 0.0f && qrnew > 0.0f) {
+    cr = 2.2f * qcnew * pow_f(qrnew, 0.875f);
+    cr = cr * dt;
+  }
+  float p = prs[c];
+  if (qrnew > 0.0f) {
+    float esl = esl_of(t, p);
+    float qvs = K.eps * esl / (p - esl);
+    if (qvnew < qvs) {
+      float rq = rho[c] * qrnew;
+      er = (1.6f + 30.3922f * pow_f(rq, 0.2046f)) * (1.0f - (qvnew / qvs)) * pow_f(rq, 0.525f) /
+           ((2.03e4f + 9.584e6f / (qvs * p)) * rho[c]);
+      er = fminf(er * dt, qrnew);
+      if ((qvnew + er) > qvs) er = qvs - qvnew;
+    }
+  }
+  if ((ar + cr) > qcnew) {
+    float term1 = ar + cr;
+    ar = qcnew * ar / term1;
+    cr = qcnew * cr / term1;
+  }
+  qvnew = qvnew + er;
+  qcnew = qcnew - (ar + cr);
+  qrnew = qrnew + (ar + cr - er);
+  if (er > 1.0e-7f) {
+    float cpml = K.cp + K.cpv * qvnew + K.cpl * (qcnew + qrnew);
+    float cvml = K.cv + K.cvv * qvnew + K.cpl * (qcnew + qrnew);
+    float rm = K.rd + K.rv * qvnew;
+    th = th - er * ((K.lv1 - K.lv2 * t) / (K.cpdcv * cvml * (P.pi0[k] + pp)) -
+                    (P.th0[k] + th) * (K.rv / cvml) * (1.0f - K.rovcp * cpml / rm));
+    pp = pow_f(rho[c] * rm * (P.th0[k] + th) * K.rp00, K.rddcv) - P.pi0[k];
+    p = K.p00 * pow_f(P.pi0[k] + pp, K.cpdrd);
+    th3d[c] = th;
+    pp3d[c] = pp;
+    prs[c] = p;
+  }
+  qv[c] = qvnew;
+  qc[c] = qcnew;
+  qr[c] = qrnew;
+}
+
+// satadj (src/kessler.F:146-360, eqtset=2) at one cell
+__device__ __forceinline__ void satadj_cell(const KP &P, int c, int k, float *th3d, float *pp3d, float *prs,
+                                            float *qv, float *qc, float *qr, const float *rho, int *iflag) {
+  const Consts &K = P.K;
+  const float converge = 2.0f * K.tsmall;
+  float th = th3d[c], pp = pp3d[c], p = prs[c];
+  float tnew = (P.th0[k] + th) * (P.pi0[k] + pp);
+  float esl = esl_of(tnew, p);
+  float qvs = K.eps * esl / (p - esl);
+  float qv0 = qv[c], qc0 = qc[c];
+  if (qc0 > 1.0e-12f || qv0 > qvs) {
+    float qvnew = qv0, qcnew = qc0;
+    float ql = 0.0f;
+    ql = ql + qc0;
+    ql = ql + qr[c];
+    ql = fmaxf(0.0f, ql);
+    float qi = 0.0f;
+    float cvml = K.cv + K.cvv * qvnew + K.cpl * ql + K.cpi * qi;
+    float lhv = K.lv1 - K.lv2 * tnew;
+    float t1 = (lhv - K.rv * tnew) / cvml;
+    float d1 = t1 * 17.67f * 243.5f;
+    int n = 0;
+    float tlast = tnew, pnew = 0.0f;
+    bool doit = true;
+    while (doit) {
+      n = n + 1;
+      float tm = tnew - 29.65f;
+      float dqv = (qvs - qvnew) / (1.0f + d1 * qvs / (tm * tm));
+      dqv = fminf(dqv, qcnew);
+      if ((qvnew + dqv) < 1.0e-20f) dqv = 1.0e-20f - qvnew;
+      qvnew = qvnew + dqv;
+      qcnew = qcnew - dqv;
+      tnew = tnew - dqv * t1;
+      pnew = rho[c] * (K.rd + K.rv * qvnew) * tnew;
+      doit = false;
+      if (fabsf(tnew - tlast) > converge) {
+        tlast = tnew;
+        esl = esl_of(tnew, p);
+        qvs = K.eps * esl / (pnew - esl);
+        doit = true;
+      }
+      if (n == 100) {
+        atomicOr(iflag, 1);
+        doit = false;
+      }
+    }
+    prs[c] = pnew;
+    float ppn = pow_f(pnew * K.rp00, K.rovcp) - P.pi0[k];
+    pp3d[c] = ppn;
+    th3d[c] = tnew / (P.pi0[k] + ppn) - P.th0[k];
+    qc[c] = qcnew;
+    qv[c] = qvnew;
+  }
+}
+
+// ---------------------------------------------------------------- learned surrogate
+// The network replaces kessler (src/kessler.F:8-138: autoconversion,
+// accretion, rain evaporation); sedimentation before it and the saturation
+// adjustment after it stay exact.  Its 8 inputs at a cell, raw: T, p, rho,
+// qv, qc, qr, dt, z/ztop; its 4 outputs: the changes of theta, qv, qc, qr
+// over kessler.  (No counterpart in the reference.)
+struct KessAct {
+  bool ar, cr, er;  // autoconversion, accretion, evaporation possible
+  float t;          // temperature
+};
+// kessler's own conditions for each process (src/kessler.F:48-75)
+__device__ __forceinline__ KessAct kess_active(const KP &P, int c, int k, const float *th3d, const float *pp3d,
+                                               const float *prs, const float *qv, const float *qc,
+                                               const float *qr) {
+  KessAct a;
+  a.t = (P.th0[k] + th3d[c]) * (P.pi0[k] + pp3d[c]);
+  a.ar = qc[c] > 0.001f;
+  a.cr = qc[c] > 0.0f && qr[c] > 0.0f;
+  a.er = false;
+  if (qr[c] > 0.0f) {
+    const float p = prs[c], esl = esl_of(a.t, p);
+    a.er = qv[c] < P.K.eps * esl / (p - esl);
+  }
+  return a;
+}
+
+// Projection of the proposed changes onto what kessler can do at the cell:
+// cloud only loses (autoconversion, accretion) and at most all of it,
+// vapour only gains (evaporation) and only where rain evaporates, and rain
+// takes exactly the difference, dqr = -(dqv + dqc), so qv + qc + qr is kept
+// by construction; evaporation is limited so that no rain goes negative.
+// The network's own dqr (and dtheta) serve as diagnostics.  Returns false
+// (the column then falls back to plain kessler) for a non-finite proposal.
+__device__ __forceinline__ bool sur_project(const KessAct &a, float qv, float qc, float qr, const float *d,
+                                            float &nqv, float &nqc, float &nqr) {
+  for (int q = 1; q < 4; q++)
+    if (!isfinite(d[q])) return false;
+  float dc = (a.ar || a.cr) ? fminf(d[2], 0.0f) : 0.0f;
+  dc = fmaxf(dc, -qc);
+  float dv = a.er ? fmaxf(d[1], 0.0f) : 0.0f;
+  dv = fminf(dv, qr - dc);
+  nqv = qv + dv;
+  nqc = qc + dc;
+  nqr = qr - (dv + dc);
+  if (nqr < 0.0f) nqr = 0.0f;
+  return true;
+}
+
+static void surrogate_kessler(cm1g_model *m) {
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  const float dt = m->c.dt;
+  float *th3d = m->f[CM1G_TH3D], *pp3d = m->f[CM1G_PP3D], *rho = m->f[CM1G_RHO], *prs = m->f[CM1G_PRS];
+  float *qv = m->f[CM1G_Q3D0], *qc = m->f[CM1G_Q3D1], *qr = m->f[CM1G_Q3D2];
+  float *pqv = m->sy, *pqc = m->sz, *pqr = m->tx;  // the projected proposals
+  int *colflag = m->colflag;
+  unsigned long long *cnt = m->dcount;
+  const float *zh = m->kf[CM1G_K_ZH];
+  const float rztop = 1.0f / m->hk[CM1G_K_ZF][nk + 1];
+  const float *prm = mlp_params(m->sur), *nm = mlp_norm(m->sur);
+  for2(m, 1, ni, 1, nj, [=] __device__(int i, int j) {
+    colflag[(i - 1) + (j - 1) * ni] = 0;
+    if (i == 1 && j == 1) cnt[1] += 1ULL;
+  });
+  const int64_t n = (int64_t)ni * nj * nk;
+  auto cell = [=] __device__(int64_t s, int &i, int &j, int &k) {
+    i = 1 + (int)(s % ni);
+    j = 1 + (int)((s / ni) % nj);
+    k = 1 + (int)(s / ((int64_t)ni * nj));
+  };
+  auto inputs = [=] __device__(int c, int k, float t, float *x) {
+    const float raw[MLP_IN] = {t, prs[c], rho[c], qv[c], qc[c], qr[c], dt, zh[k] * rztop};
+    for (int q = 0; q < MLP_IN; q++) x[q] = (raw[q] - nm[q]) / nm[MLP_IN + q];
+  };
+  auto in = [=] __device__(int64_t s, float *x) {
+    int i, j, k;
+    cell(s, i, j, k);
+    const int c = IXD(P, i, j, k);
+    inputs(c, k, (P.th0[k] + th3d[c]) * (P.pi0[k] + pp3d[c]), x);
+  };
+  // cells kessler changes: outside the trusted input range, or with an
+  // unusable proposal, the column falls back
+  auto out = [=] __device__(int64_t s, const float *y) {
+    int i, j, k;
+    cell(s, i, j, k);
+    const int c = IXD(P, i, j, k);
+    const KessAct a = kess_active(P, c, k, th3d, pp3d, prs, qv, qc, qr);
+    if (!(a.ar || a.cr || a.er)) return;
+    float x[MLP_IN];
+    inputs(c, k, a.t, x);
+    bool inside = true;
+    for (int q = 0; q < MLP_IN; q++) inside = inside && x[q] >= nm[NM_XLO + q] && x[q] <= nm[NM_XHI + q];
+    float d[MLP_OUT];
+    for (int q = 0; q < MLP_OUT; q++) d[q] = y[q] * nm[2 * MLP_IN + MLP_OUT + q] + nm[2 * MLP_IN + q];
+    float v, w, r;
+    const bool ok = inside && sur_project(a, qv[c], qc[c], qr[c], d, v, w, r);
+    if (ok) {
+      pqv[c] = v;
+      pqc[c] = w;
+      pqr[c] = r;
+    } else {
+      atomicOr(&colflag[(i - 1) + (j - 1) * ni], 1);
+      atomicAdd(&cnt[inside ? 5 : 4], 1ULL);
+    }
+  };
+  const int64_t groups = (n + 16 * MLP_WAVES - 1) / (16 * MLP_WAVES);
+  const int nbx = (int)(groups < 1024 ? groups : 1024);
+  launch<512>(m, dim3(nbx), dim3(512), [=] __device__(int bx, int, int) { mlp_tiles(prm, n, in, out, bx, nbx); });
+  // apply the projected proposal as kessler applies its own result (the
+  // theta, pi', p update after evaporation above 1e-7), or plain kessler in
+  // flagged columns
+  for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    const Consts &K = P.K;
+    const int c = IXD(P, i, j, k);
+    if (colflag[(i - 1) + (j - 1) * ni]) {
+      kessler_cell(P, dt, c, k, th3d, pp3d, prs, qv, qc, qr, rho);
+      atomicAdd(&cnt[2], 1ULL);
+      return;
+    }
+    const KessAct a = kess_active(P, c, k, th3d, pp3d, prs, qv, qc, qr);
+    if (!(a.ar || a.cr || a.er)) return;
+    const float qvnew = pqv[c], qcnew = pqc[c], qrnew = pqr[c];
+    const float er = qvnew - qv[c];
+    if (er > 1.0e-7f) {
+      float th = th3d[c], pp = pp3d[c];
+      const float cpml = K.cp + K.cpv * qvnew + K.cpl * (qcnew + qrnew);
+      const float cvml = K.cv + K.cvv * qvnew + K.cpl * (qcnew + qrnew);
+      const float rm = K.rd + K.rv * qvnew;
+      th = th - er * ((K.lv1 - K.lv2 * a.t) / (K.cpdcv * cvml * (P.pi0[k] + pp)) -
+                      (P.th0[k] + th) * (K.rv / cvml) * (1.0f - K.rovcp * cpml / rm));
+      pp = pow_f(rho[c] * rm * (P.th0[k] + th) * K.rp00, K.rddcv) - P.pi0[k];
+      th3d[c] = th;
+      pp3d[c] = pp;
+      prs[c] = K.p00 * pow_f(P.pi0[k] + pp, K.cpdrd);
+    }
+    qv[c] = qvnew;
+    qc[c] = qcnew;
+    qr[c] = qrnew;
+  });
+}
+
+// (state entering kessler, its change over kessler) pairs of every cell
+// with cloud or rain or any change, and of one in `harv_keep` of the others,
+// appended to the harvest buffer while it has room
+static void harvest_pairs(cm1g_model *m) {
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  const float dt = m->c.dt;
+  const float *th3d = m->f[CM1G_TH3D], *rho = m->f[CM1G_RHO];
+  const float *qv = m->f[CM1G_Q3D0], *qc = m->f[CM1G_Q3D1], *qr = m->f[CM1G_Q3D2];
+  const float *b0 = m->sx, *b1 = m->sy, *b2 = m->sz, *b3 = m->tx, *b4 = m->ty, *b5 = m->tz;
+  float *buf = m->harv;
+  const int64_t cap = m->harv_cap;
+  const unsigned keep = (unsigned)m->harv_keep;
+  unsigned long long *cnt = m->dcount;
+  const float *zh = m->kf[CM1G_K_ZH];
+  const float rztop = 1.0f / m->hk[CM1G_K_ZF][nk + 1];
+  for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    const int c = IXD(P, i, j, k);
+    const float d0 = th3d[c] - b0[c], d1 = qv[c] - b1[c], d2 = qc[c] - b2[c], d3 = qr[c] - b3[c];
+    const bool active = b2[c] > 0.0f || b3[c] > 0.0f || d0 != 0.0f || d1 != 0.0f || d2 != 0.0f || d3 != 0.0f;
+    unsigned h = (unsigned)c * 2654435761u ^ (unsigned)cnt[3] * 2246822519u;
+    h ^= h >> 15;
+    h *= 2654435761u;
+    h ^= h >> 13;
+    if (!active && (h % keep) != 0u) return;
+    const unsigned long long idx = atomicAdd(&cnt[0], 1ULL);
+    if ((int64_t)idx >= cap) return;
+    float *o = buf + (int64_t)idx * 12;
+    o[0] = (P.th0[k] + b0[c]) * (P.pi0[k] + b4[c]);
+    o[1] = b5[c];
+    o[2] = rho[c];
+    o[3] = b1[c];
+    o[4] = b2[c];
+    o[5] = b3[c];
+    o[6] = dt;
+    o[7] = zh[k] * rztop;
+    o[8] = d0;
+    o[9] = d1;
+    o[10] = d2;
+    o[11] = d3;
+  });
+  launch<64>(m, dim3(1), dim3(1), [=] __device__(int, int, int) { cnt[3] += 1ULL; });
 }
 
 void kessler_branch_run(cm1g_model *m) {
@@ -96,110 +390,79 @@ void kessler_branch_run(cm1g_model *m) {
     prate[IX2D(P, i, j)] = pr;
     colbud[(i - 1) + (j - 1) * ni] = bud;
   });
-  // kessler (pointwise)
-  for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-    const Consts &K = P.K;
-    int c = IXD(P, i, j, k);
-    float th = th3d[c], pp = pp3d[c];
-    float t = (P.th0[k] + th) * (P.pi0[k] + pp);
-    float qvnew = qv[c], qcnew = qc[c], qrnew = qr[c];
-    float ar = 0.0f, cr = 0.0f, er = 0.0f;
-    if (qcnew > 0.0f) {
-      ar = fmaxf(0.001f * (qcnew - 0.001f), 0.0f);
-      ar = ar * dt;
-    }
-    if (qcnew > 0.0f && qrnew > 0.0f) {
-      cr = 2.2f * qcnew * pow_f(qrnew, 0.875f);
-      cr = cr * dt;
-    }
-    float p = prs[c];
-    if (qrnew > 0.0f) {
-      float esl = esl_of(t, p);
-      float qvs = K.eps * esl / (p - esl);
-      if (qvnew < qvs) {
-        float rq = rho[c] * qrnew;
-        er = (1.6f + 30.3922f * pow_f(rq, 0.2046f)) * (1.0f - (qvnew / qvs)) * pow_f(rq, 0.525f) /
-             ((2.03e4f + 9.584e6f / (qvs * p)) * rho[c]);
-        er = fminf(er * dt, qrnew);
-        if ((qvnew + er) > qvs) er = qvs - qvnew;
-      }
-    }
-    if ((ar + cr) > qcnew) {
-      float term1 = ar + cr;
-      ar = qcnew * ar / term1;
-      cr = qcnew * cr / term1;
-    }
-    qvnew = qvnew + er;
-    qcnew = qcnew - (ar + cr);
-    qrnew = qrnew + (ar + cr - er);
-    if (er > 1.0e-7f) {
-      float cpml = K.cp + K.cpv * qvnew + K.cpl * (qcnew + qrnew);
-      float cvml = K.cv + K.cvv * qvnew + K.cpl * (qcnew + qrnew);
-      float rm = K.rd + K.rv * qvnew;
-      th = th - er * ((K.lv1 - K.lv2 * t) / (K.cpdcv * cvml * (P.pi0[k] + pp)) -
-                      (P.th0[k] + th) * (K.rv / cvml) * (1.0f - K.rovcp * cpml / rm));
-      pp = pow_f(rho[c] * rm * (P.th0[k] + th) * K.rp00, K.rddcv) - P.pi0[k];
-      p = K.p00 * pow_f(P.pi0[k] + pp, K.cpdrd);
-      th3d[c] = th;
-      pp3d[c] = pp;
-      prs[c] = p;
-    }
-    qv[c] = qvnew;
-    qc[c] = qcnew;
-    qr[c] = qrnew;
-  });
+  if (m->harv) {
+    // the state entering kessler (after sedimentation)
+    float *b0 = m->sx, *b1 = m->sy, *b2 = m->sz, *b3 = m->tx, *b4 = m->ty, *b5 = m->tz;
+    for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      const int c = IXD(P, i, j, k);
+      b0[c] = th3d[c];
+      b1[c] = qv[c];
+      b2[c] = qc[c];
+      b3[c] = qr[c];
+      b4[c] = pp3d[c];
+      b5[c] = prs[c];
+    });
+  }
+  if (m->sur) {
+    surrogate_kessler(m);
+  } else {
+    // kessler (pointwise)
+    for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      kessler_cell(P, dt, IXD(P, i, j, k), k, th3d, pp3d, prs, qv, qc, qr, rho);
+    });
+  }
+  if (m->harv) harvest_pairs(m);
   // satadj (nrk=4), iterative saturation adjustment
   for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-    const Consts &K = P.K;
-    const float converge = 2.0f * K.tsmall;
-    int c = IXD(P, i, j, k);
-    float th = th3d[c], pp = pp3d[c], p = prs[c];
-    float tnew = (P.th0[k] + th) * (P.pi0[k] + pp);
-    float esl = esl_of(tnew, p);
-    float qvs = K.eps * esl / (p - esl);
-    float qv0 = qv[c], qc0 = qc[c];
-    if (qc0 > 1.0e-12f || qv0 > qvs) {
-      float qvnew = qv0, qcnew = qc0;
-      float ql = 0.0f;
-      ql = ql + qc0;
-      ql = ql + qr[c];
-      ql = fmaxf(0.0f, ql);
-      float qi = 0.0f;
-      float cvml = K.cv + K.cvv * qvnew + K.cpl * ql + K.cpi * qi;
-      float lhv = K.lv1 - K.lv2 * tnew;
-      float t1 = (lhv - K.rv * tnew) / cvml;
-      float d1 = t1 * 17.67f * 243.5f;
-      int n = 0;
-      float tlast = tnew, pnew = 0.0f;
-      bool doit = true;
-      while (doit) {
-        n = n + 1;
-        float tm = tnew - 29.65f;
-        float dqv = (qvs - qvnew) / (1.0f + d1 * qvs / (tm * tm));
-        dqv = fminf(dqv, qcnew);
-        if ((qvnew + dqv) < 1.0e-20f) dqv = 1.0e-20f - qvnew;
-        qvnew = qvnew + dqv;
-        qcnew = qcnew - dqv;
-        tnew = tnew - dqv * t1;
-        pnew = rho[c] * (K.rd + K.rv * qvnew) * tnew;
-        doit = false;
-        if (fabsf(tnew - tlast) > converge) {
-          tlast = tnew;
-          esl = esl_of(tnew, p);
-          qvs = K.eps * esl / (pnew - esl);
-          doit = true;
-        }
-        if (n == 100) {
-          atomicOr(iflag, 1);
-          doit = false;
-        }
-      }
-      prs[c] = pnew;
-      float ppn = pow_f(pnew * K.rp00, K.rovcp) - P.pi0[k];
-      pp3d[c] = ppn;
-      th3d[c] = tnew / (P.pi0[k] + ppn) - P.th0[k];
-      qc[c] = qcnew;
-      qv[c] = qvnew;
-    }
+    satadj_cell(P, IXD(P, i, j, k), k, th3d, pp3d, prs, qv, qc, qr, rho, iflag);
   });
+}
+
+// ---------------------------------------------------------------- C ABI
+extern "C" int cm1g_set_surrogate(cm1g_model *m, cm1g_mlp *net) {
+  if (!m) {
+    cm1g_set_error("cm1g_set_surrogate: null model");
+    return CM1G_E_ARG;
+  }
+  if (net && (m->c.imoist != 1 || m->c.ptype != 1)) {
+    cm1g_set_error("cm1g_set_surrogate: the surrogate replaces Kessler (imoist=1, ptype=1)");
+    return CM1G_E_ARG;
+  }
+  if (net && m->harv) {
+    cm1g_set_error("cm1g_set_surrogate: stop the harvest first");
+    return CM1G_E_ARG;
+  }
+  m->sur = net;
+  graphs_clear(m);
+  return 0;
+}
+
+extern "C" int cm1g_set_harvest(cm1g_model *m, float *buf, int64_t capacity, int keep_one_in) {
+  if (!m || (buf && (capacity < 1 || keep_one_in < 1))) {
+    cm1g_set_error("cm1g_set_harvest: bad argument");
+    return CM1G_E_ARG;
+  }
+  if (buf && m->sur) {
+    cm1g_set_error("cm1g_set_harvest: the harvest records plain Kessler; remove the surrogate first");
+    return CM1G_E_ARG;
+  }
+  HIPCHK(hipStreamSynchronize(m->stream));
+  m->harv = buf;
+  m->harv_cap = buf ? capacity : 0;
+  m->harv_keep = keep_one_in;
+  if (buf) HIPCHK(hipMemset(m->dcount, 0, sizeof(unsigned long long)));
+  graphs_clear(m);
+  return 0;
+}
+
+extern "C" int cm1g_surrogate_counts(cm1g_model *m, int64_t *out6) {
+  if (!m || !out6) {
+    cm1g_set_error("cm1g_surrogate_counts: null argument");
+    return CM1G_E_ARG;
+  }
+  unsigned long long h[6];
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(h, m->dcount, sizeof(h), hipMemcpyDeviceToHost));
+  for (int q = 0; q < 6; q++) out6[q] = (int64_t)h[q];
+  return 0;
 }

@@ -158,9 +158,14 @@ extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
     if (!rc && hipMalloc((void **)&m->dscal, 16 * sizeof(double)) != hipSuccess) rc = CM1G_E_NOMEM;
     if (!rc && hipMalloc((void **)&m->fred, 32 * (pz + 8) * sizeof(float)) != hipSuccess) rc = CM1G_E_NOMEM;
     if (!rc && hipMalloc((void **)&m->iflag, 16 * sizeof(int)) != hipSuccess) rc = CM1G_E_NOMEM;
+    if (!rc && hipMalloc((void **)&m->dcount, 8 * sizeof(unsigned long long)) != hipSuccess) rc = CM1G_E_NOMEM;
+    if (!rc && hipMalloc((void **)&m->colflag, (size_t)cfg->ni * cfg->nj * sizeof(int)) != hipSuccess)
+      rc = CM1G_E_NOMEM;
     if (!rc) {
       hipMemset(m->dscal, 0, 16 * sizeof(double));
       hipMemset(m->iflag, 0, 16 * sizeof(int));
+      hipMemset(m->dcount, 0, 8 * sizeof(unsigned long long));
+      hipMemset(m->colflag, 0, (size_t)cfg->ni * cfg->nj * sizeof(int));
     }
     if (rc) cm1g_set_error("device allocation failed");
   }
@@ -226,6 +231,8 @@ extern "C" int cm1g_destroy(cm1g_model *m) {
   if (m->dred) hipFree(m->dred);
   if (m->dscal) hipFree(m->dscal);
   if (m->iflag) hipFree(m->iflag);
+  if (m->dcount) hipFree(m->dcount);
+  if (m->colflag) hipFree(m->colflag);
   for (hipEvent_t e : m->kt_ev) hipEventDestroy(e);
   if (m->stream) hipStreamDestroy(m->stream);
   delete m;

@@ -255,6 +255,69 @@ module cm1gpu
       integer(c_int), intent(out) :: ty_best
       real(c_float), intent(out) :: ms_per_step(3)
     end function
+    ! learned Kessler surrogate; x, y, t, buf are device addresses
+    integer(c_int) function cm1g_mlp_nparams(n) bind(C, name="cm1g_mlp_nparams")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), intent(out) :: n
+    end function
+    integer(c_int) function cm1g_mlp_create(seed, net) bind(C, name="cm1g_mlp_create")
+      import :: c_int, c_ptr, c_int64_t
+      integer(c_int64_t), value :: seed
+      type(c_ptr), intent(out) :: net
+    end function
+    integer(c_int) function cm1g_mlp_destroy(net) bind(C, name="cm1g_mlp_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: net
+    end function
+    integer(c_int) function cm1g_mlp_get_params(net, host) bind(C, name="cm1g_mlp_get_params")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: net
+      real(c_float), intent(out) :: host(*)
+    end function
+    integer(c_int) function cm1g_mlp_set_params(net, host) bind(C, name="cm1g_mlp_set_params")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: net
+      real(c_float), intent(in) :: host(*)
+    end function
+    integer(c_int) function cm1g_mlp_set_norm(net, norm) bind(C, name="cm1g_mlp_set_norm")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: net
+      real(c_float), intent(in) :: norm(40)
+    end function
+    integer(c_int) function cm1g_mlp_forward(net, x, rows, y) bind(C, name="cm1g_mlp_forward")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: net, x, y
+      integer(c_int64_t), value :: rows
+    end function
+    integer(c_int) function cm1g_mlp_loss_grad(net, x, t, rows, loss, grad) bind(C, name="cm1g_mlp_loss_grad")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: net, x, t, loss, grad
+      integer(c_int64_t), value :: rows
+    end function
+    integer(c_int) function cm1g_mlp_adam(net, lr) bind(C, name="cm1g_mlp_adam")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: net
+      real(c_float), value :: lr
+    end function
+    integer(c_int) function cm1g_mlp_sync(net) bind(C, name="cm1g_mlp_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: net
+    end function
+    integer(c_int) function cm1g_set_surrogate(m, net) bind(C, name="cm1g_set_surrogate")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m, net
+    end function
+    integer(c_int) function cm1g_set_harvest(m, buf, capacity, keep_one_in) bind(C, name="cm1g_set_harvest")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: m, buf
+      integer(c_int64_t), value :: capacity
+      integer(c_int), value :: keep_one_in
+    end function
+    integer(c_int) function cm1g_surrogate_counts(m, out6) bind(C, name="cm1g_surrogate_counts")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: m
+      integer(c_int64_t), intent(out) :: out6(6)
+    end function
     integer(c_int) function cm1g_solve1(m) bind(C, name="cm1g_solve1")
       import :: c_int, c_ptr
       type(c_ptr), value :: m

@@ -185,6 +185,53 @@ int cm1g_ensemble_set_tiles(cm1g_ensemble *e, int ty);
  * three timings.  The 4*(steps+1) steps it takes are ordinary steps of the
  * run. */
 int cm1g_ensemble_tune(cm1g_ensemble *e, int steps, int *ty_best, float *ms_per_step);
+
+/* --- learned Kessler surrogate (no counterpart in the reference) --------
+ * A per-cell MLP, 8 -> 128 -> 128 -> 128 -> 4 with SiLU hidden layers, BF16
+ * operands and FP32 accumulation on the matrix cores, standing in for
+ * kessler (src/kessler.F:8-138: autoconversion, accretion, rain
+ * evaporation); sedimentation before it and satadj after it stay exact.
+ * Inputs (raw): T, p, rho, qv, qc, qr, dt, z/ztop of a cell entering
+ * kessler; outputs: the changes of theta, qv, qc, qr over kessler.
+ * Parameters are FP32, weights row-major [in][out] in the order
+ * W1 b1 W2 b2 W3 b3 W4 b4; normalisation norm[40] = xmean[8] xstd[8]
+ * ymean[4] ystd[4] xlo[8] xhi[8] (x = (raw - xmean)/xstd,
+ * change = y*ystd + ymean; [xlo, xhi] the range of normalised inputs
+ * trusted, -8..8 by default).  Device arrays below are device pointers. */
+typedef struct cm1g_mlp cm1g_mlp;
+int cm1g_mlp_nparams(int64_t *n);
+int cm1g_mlp_create(uint64_t seed, cm1g_mlp **out);   /* He-uniform weights, zero biases */
+int cm1g_mlp_destroy(cm1g_mlp *net);
+int cm1g_mlp_get_params(cm1g_mlp *net, float *host);
+int cm1g_mlp_set_params(cm1g_mlp *net, const float *host);
+int cm1g_mlp_set_norm(cm1g_mlp *net, const float *norm40);
+/* y[rows x 4] = net(x[rows x 8]) on normalised values (fused inference) */
+int cm1g_mlp_forward(cm1g_mlp *net, const float *x, int64_t rows, float *y);
+/* Mean squared error of net(x) against t[rows x 4] (normalised) and its
+ * gradient (kept for cm1g_mlp_adam); *loss and grad[nparams] (host) are
+ * filled when non-NULL. */
+int cm1g_mlp_loss_grad(cm1g_mlp *net, const float *x, const float *t, int64_t rows, float *loss, float *grad);
+int cm1g_mlp_adam(cm1g_mlp *net, float lr);          /* one Adam step (0.9, 0.999, 1e-8) */
+int cm1g_mlp_sync(cm1g_mlp *net);
+/* Replace kessler inside cm1g_step's Kessler branch with the network (NULL:
+ * plain kessler).  Cells where none of kessler's processes can act stay
+ * unchanged, as kessler leaves them.  Elsewhere the proposal is projected
+ * onto what kessler can do there: cloud only lost (at most all of it),
+ * vapour only gained where rain evaporates, rain taking the difference (so
+ * qv + qc + qr is kept by construction, no species negative); theta, pi'
+ * and p then follow from the evaporation as in kessler.  A column with a
+ * cell whose inputs leave [xlo, xhi] or whose proposal is not finite runs
+ * plain kessler instead. */
+int cm1g_set_surrogate(cm1g_model *m, cm1g_mlp *net);
+/* While buf is set, every Kessler branch of the plain model appends
+ * (inputs[8], changes over kessler[4]) float pairs to buf (device, capacity
+ * pairs): all cells with cloud, rain or any change, and one in keep_one_in
+ * of the others.  buf = NULL stops. */
+int cm1g_set_harvest(cm1g_model *m, float *buf, int64_t capacity, int keep_one_in);
+/* out6 = pairs harvested (may exceed the capacity), surrogate local steps,
+ * cells that fell back to plain Kessler, harvest calls, changing cells
+ * outside the trusted input range, unusable proposals */
+int cm1g_surrogate_counts(cm1g_model *m, int64_t *out6);
 int cm1g_solve1(cm1g_model *m);   /* src/solve1.F:77-1791 (ten1, ppten) */
 int cm1g_solve2(cm1g_model *m);   /* src/solve2.F:77-2094 (RK3 loop) */
 int cm1g_mp_driver(cm1g_model *m);/* src/mp_driver.F:17-1706 (ptype 0/1) */

@@ -1,0 +1,155 @@
+"""The learned Kessler surrogate on the GPU.
+
+- The MFMA training kernels (forward, backward) against the CPU oracle of
+  the same network (oracle/mlp_ref.py, float64 with the same BF16 operand
+  rounding), including the split-K weight-gradient path; the fused
+  inference kernel against the oracle's forward.
+- Training on pairs harvested from plain Kessler: the network explains most
+  of the variance of held-out local-step changes.
+- A surrogate whose inputs are all out of range falls back to plain Kessler
+  in every column the local step changes: the run equals the plain run bit
+  for bit.
+- A run with the trained surrogate in place of kessler keeps every species
+  non-negative and, against plain Kessler, accumulates rain much closer
+  than the same step without the network (a zero network: no
+  autoconversion, accretion or evaporation).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import mlp_ref as R
+from cm1_amd import surrogate as S
+from common import PROG, case, make_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / (np.linalg.norm(b) + 1e-300))
+
+
+@pytest.mark.parametrize("rows", [300, 2500], ids=["rows300", "rows2500-splitK"])
+def test_training_kernels_match_oracle(require_gpu, rows):
+    import torch
+    rng = np.random.default_rng(rows)
+    net = S.Net(seed=11)
+    p = net.params().astype(np.float64)
+    x = rng.standard_normal((rows, 8)).astype(np.float32)
+    t = rng.standard_normal((rows, 4)).astype(np.float32)
+    loss, g = net.loss_grad(torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda(), want_grad=True)
+    loss_r, g_r = R.loss_grad(p, x.astype(np.float64), t.astype(np.float64), rnd=True)
+    assert abs(loss - loss_r) <= 2.0e-4 * loss_r, (loss, loss_r)
+    offs = np.cumsum([0] + [int(np.prod(s)) for _, s in R.SIZES])
+    for (name, _), a, b in zip(R.SIZES, offs[:-1], offs[1:]):
+        e = rel(g[a:b], g_r[a:b])
+        print(f"{name}: rel L2 {e:.2e}")
+        assert e < 5.0e-3, (name, e)
+    net.close()
+
+
+def test_fused_inference_matches_oracle(require_gpu):
+    import torch
+    rng = np.random.default_rng(3)
+    net = S.Net(seed=5)
+    p = net.params().astype(np.float64)
+    x = rng.standard_normal((1000, 8)).astype(np.float32)
+    y = net.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    y_r = R.forward(p, x.astype(np.float64), rnd=True)[0]
+    e = rel(y, y_r)
+    print(f"fused forward vs oracle: rel L2 {e:.2e}")
+    assert e < 5.0e-3, e
+    net.close()
+
+
+@pytest.fixture(scope="module")
+def trained():
+    """Harvest from a plain moist run, train, and keep the run's state at the
+    end of the harvest for the comparisons below."""
+    import torch
+    torch.cuda.init()
+    cfgd, cols, fields = case(ni=48, nj=44, nk=30, amp=5.0, umax=20.0, seed=9)
+    m = make_gpu(cfgd, cols, fields)
+    pairs = S.harvest([m], steps=150, capacity=3_000_000, keep_one_in=10)
+    state = m.save_state()
+    n = pairs.shape[0]
+    norm = S.fit_norm(pairs)
+    x, t = S.normalise(pairs, norm)
+    perm = torch.randperm(n, device=x.device, generator=torch.Generator(device=x.device).manual_seed(0))
+    ntest = n // 10
+    te, tr = perm[:ntest], perm[ntest:]
+    net = S.Net(seed=1)
+    net.set_norm(norm)
+    losses = net.train(x[tr].contiguous(), t[tr].contiguous(), steps=6000, batch=16384, lr=1.0e-3,
+                       log_every=500)
+    yte = net.forward(x[te].contiguous())
+    mse = float(((yte - t[te]) ** 2).mean())
+    zero = float((t[te] ** 2).mean())
+    info = dict(pairs=n, losses=losses, mse=mse, zero=zero)
+    print(info)
+    return dict(net=net, norm=norm, case=(cfgd, cols, fields), state=state, info=info)
+
+
+def test_training_learns(require_gpu, trained):
+    info = trained["info"]
+    assert info["pairs"] > 10000, info
+    # normalised targets have unit variance: the zero predictor's error is ~1
+    assert info["mse"] < 0.2 * info["zero"], info
+
+
+def _fork(tr, net=None):
+    cfgd, cols, fields = tr["case"]
+    m = make_gpu(cfgd, cols, fields)
+    m.load_state(tr["state"])
+    if net is not None:
+        S.use(m, net)
+    return m
+
+
+def test_fallback_everywhere_equals_plain(require_gpu, trained):
+    net = S.Net(seed=2)
+    norm = trained["norm"].copy()
+    norm[8:16] = 1.0e-30  # every input lies far outside the training range
+    net.set_norm(norm)
+    a = _fork(trained, net)
+    b = _fork(trained)
+    a.step(3)
+    b.step(3)
+    for n, _ in PROG:
+        assert np.array_equal(a.get(n).view(np.uint32), b.get(n).view(np.uint32)), n
+    # every changing cell is out of range: each column with one falls back
+    # (cells the scheme leaves unchanged stay unchanged either way)
+    c = S.counts(a)
+    assert c["surrogate_steps"] == 3 and c["fallback_cells"] > 0 and c["out_of_range"] > 0, c
+    assert c["unusable"] == 0, c
+    net.close()
+
+
+def test_surrogate_run_against_plain(require_gpu, trained):
+    net = trained["net"]
+    zero = S.Net(seed=3)
+    zero.set_params(np.zeros(zero.nparams, np.float32))
+    zn = trained["norm"].copy()
+    zn[16:20] = 0.0
+    zero.set_norm(zn)
+    plain, sur, none = _fork(trained), _fork(trained, net), _fork(trained, zero)
+    r0 = plain.scalars()["train"]
+    nsteps = 60
+    for m in (plain, sur, none):
+        m.step(nsteps)
+    rain = {k: m.scalars()["train"] - r0 for k, m in (("plain", plain), ("sur", sur), ("none", none))}
+    for q in ("qa0", "qa1", "qa2"):
+        a = sur.interior(q)
+        assert np.all(np.isfinite(a)) and a.min() >= 0.0, q
+    c = S.counts(sur)
+    ncell = sur.ni * sur.nj * sur.nk
+    err_sur = abs(rain["sur"] - rain["plain"]) / rain["plain"]
+    err_none = abs(rain["none"] - rain["plain"]) / rain["plain"]
+    print(f"rain over {nsteps} steps: plain {rain['plain']:.4e} surrogate {rain['sur']:.4e} "
+          f"(err {err_sur:.3f}) without network {rain['none']:.4e} (err {err_none:.3f}); "
+          f"fallback cells {c['fallback_cells']} of {nsteps * ncell} ({c['out_of_range']} changing cells out "
+          f"of range, {c['unusable']} unusable proposals)")
+    assert rain["plain"] > 0.0
+    assert err_sur < 0.5 * err_none, (err_sur, err_none)
+    zero.close()
