@@ -154,6 +154,163 @@ def bench_ensemble(args):
     return 0
 
 
+def workload_spinup(ni=128, nj=128, nk=40, seed=7):
+    """Moist supercell case for the surrogate: 128x128x40, dx=dy=1 km,
+    dz=500 m, dt=6 s, nsound=6, Kessler, four seeded +3 K bubbles."""
+    from cm1_amd import init as hinit
+    dx = 1000.0
+    rng = np.random.default_rng(seed)
+    bubbles = [dict(x=float(rng.uniform(0.2, 0.8) * ni * dx), y=float(rng.uniform(0.2, 0.8) * nj * dx),
+                    z=1400.0, amp=3.0, rh=10000.0, rv=1400.0) for _ in range(4)]
+    cols, fields, dzn = hinit.initial_state(ni, nj, nk, dx, dx, 500.0, umax=25.0, bubbles=bubbles)
+    cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=2, ebc=2, sbc=2, nbc=2, imoist=1, ptype=1,
+                psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4, nsound=6, npt=0,
+                pdtra=1, dx=dx, dy=dx, dz=dzn, dt=6.0, alph=0.6, kdiv=0.1, csmax=350.0)
+    return cfgd, cols, fields
+
+
+def bench_surrogate(args):
+    """Learned Kessler surrogate: harvest (state, change) pairs from a plain
+    spin-up, train the MLP on the GPU, then (a) the rain accumulated over
+    60 further steps with the network in place of kessler against plain
+    Kessler (and against no network), (b) the step time of the 1024x1024x96
+    supercell with and without the surrogate, (c) a 48-member ensemble,
+    batched with the surrogate, against its members stepped one by one with
+    plain Kessler.  Prints one JSON line."""
+    import torch
+    from cm1_amd import model
+    from cm1_amd import surrogate as S
+    import cm1_amd._native as N
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    cfgd, cols, fields = workload_spinup()
+    m = model.setup(model.make_config(**cfgd), cols, fields)
+    t0 = time.perf_counter()
+    # the first half spins the storms up; the second half is harvested
+    m.step(args.spinup // 2)
+    pairs = S.harvest([m], steps=args.spinup - args.spinup // 2, capacity=12_000_000, keep_one_in=20)
+    state = m.save_state()
+    log(f"surrogate: {pairs.shape[0]} pairs from the last {args.spinup - args.spinup // 2} of {args.spinup} "
+        f"plain steps ({time.perf_counter() - t0:.1f} s)")
+    norm = S.fit_norm(pairs)
+    x, t = S.normalise(pairs, norm)
+    net = S.Net(seed=1)
+    net.set_norm(norm)
+    t0 = time.perf_counter()
+    losses = net.train(x, t, steps=args.train_steps, batch=65536, lr=1.0e-3, log_every=2000)
+    t_train = time.perf_counter() - t0
+    log(f"surrogate: trained {args.train_steps} Adam steps of 65536 in {t_train:.1f} s, losses {losses}")
+    del x, t, pairs
+
+    def fork(n=None):
+        f = model.setup(model.make_config(**cfgd), cols, fields)
+        f.load_state(state)
+        if n is not None:
+            S.use(f, n)
+        return f
+
+    zero = S.Net(seed=2)
+    zero.set_params(np.zeros(zero.nparams, np.float32))
+    zn = norm.copy()
+    zn[16:20] = 0.0
+    zero.set_norm(zn)
+    runs = {"plain": fork(), "surrogate": fork(net), "no_network": fork(zero)}
+    r0 = runs["plain"].scalars()["train"]
+    rain = {}
+    for k, f in runs.items():
+        f.step(60)
+        rain[k] = f.scalars()["train"] - r0
+    c = S.counts(runs["surrogate"])
+    ncell = cfgd["ni"] * cfgd["nj"] * cfgd["nk"]
+    err = abs(rain["surrogate"] - rain["plain"]) / rain["plain"]
+    err0 = abs(rain["no_network"] - rain["plain"]) / rain["plain"]
+    log(f"surrogate: rain over 60 steps plain {rain['plain']:.4e} surrogate {rain['surrogate']:.4e} "
+        f"(err {err:.4f}) no network (err {err0:.4f}); fallback cells {c['fallback_cells']} of {60 * ncell}")
+    for f in runs.values():
+        f.close()
+
+    # (b) the 1024x1024x96 supercell step with and without the surrogate
+    cfg1, cols1, fields1 = workload_config1()
+    m1 = model.setup(model.make_config(**cfg1), cols1, fields1)
+    del fields1
+
+    def timed(mm, nsteps):
+        mm.step(1)
+        mm.sync()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        mm.step(nsteps)
+        mm.sync()
+        return (time.perf_counter() - t1) * 1000.0 / nsteps
+
+    ms_plain = timed(m1, args.steps)
+    S.use(m1, net)
+    ms_sur = timed(m1, args.steps)
+    cells1 = cfg1["ni"] * cfg1["nj"] * cfg1["nk"]
+    log(f"surrogate: 1024x1024x96 step {ms_plain:.2f} ms plain, {ms_sur:.2f} ms with the surrogate")
+    m1.close()
+
+    # (c) ensemble: batched with the surrogate vs one by one with plain Kessler
+    work = workload_ensemble(48)
+    members = [model.setup(model.make_config(**c_), co, fi) for c_, co, fi in work]
+    del work
+    cells_e = sum(mm.ni * mm.nj * mm.nk for mm in members)
+    L = N.lib()
+    for mm in members:
+        N.check(L.cm1g_step(mm.h, None), "cm1g_step")
+    for mm in members:
+        mm.sync()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        for mm in members:
+            N.check(L.cm1g_step(mm.h, None), "cm1g_step")
+    for mm in members:
+        mm.sync()
+    ms_seq = (time.perf_counter() - t1) * 1000.0 / args.steps
+    for mm in members:
+        S.use(mm, net)
+    e = model.Ensemble(members)
+    e.step(1)
+    e.sync()
+    t1 = time.perf_counter()
+    e.step(args.steps)
+    e.sync()
+    ms_ens = (time.perf_counter() - t1) * 1000.0 / args.steps
+    e.close()
+    log(f"surrogate: ensemble {ms_seq:.2f} ms one by one (plain), {ms_ens:.2f} ms batched with the surrogate")
+    line = {
+        "metric": "cell-updates/s per RK3 step (FP32), Kessler surrogate in place of kessler",
+        "value": cells1 / (ms_sur / 1000.0),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": ms_sur,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (network: bf16 operands, f32 accumulation)",
+        "data": "synthetic",
+        "config": {"workload": "supercell 1024x1024x96 (as the main bench) with the surrogate; rain test "
+                               "on a spun-up 128x128x40 case; 48-member ensemble",
+                   "ms_per_step_plain_kessler": round(ms_plain, 3),
+                   "harvest_steps": args.spinup, "train_steps": args.train_steps, "batch": 65536,
+                   "train_seconds": round(t_train, 2), "final_loss": losses[-1][1] if losses else None,
+                   "rain_60_steps": {k: v for k, v in rain.items()},
+                   "rain_rel_error_surrogate": round(err, 5),
+                   "rain_rel_error_no_network": round(err0, 5),
+                   "fallback_cell_fraction": round(c["fallback_cells"] / (60.0 * ncell), 5),
+                   "ensemble_cells": cells_e,
+                   "ensemble_ms_per_step_one_by_one_plain": round(ms_seq, 3),
+                   "ensemble_ms_per_step_batched_surrogate": round(ms_ens, 3),
+                   "ensemble_gain": round(ms_seq / ms_ens, 3)},
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def cpu_baseline(threads):
     """Reference CM1 (FP32 OpenMP build) on a 256x256x96 sample of the case."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -222,6 +379,10 @@ def main():
                     help="first run a plain 1 GiB device copy (reference kernel for HBM counters)")
     ap.add_argument("--ensemble", type=int, default=0,
                     help="benchmark an ensemble of this many members instead (batched steps)")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="train and benchmark the learned Kessler surrogate instead")
+    ap.add_argument("--spinup", type=int, default=300, help="plain steps harvested for the surrogate")
+    ap.add_argument("--train-steps", type=int, default=10000, help="Adam steps for the surrogate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -230,6 +391,8 @@ def main():
 
     if args.ensemble > 0:
         return bench_ensemble(args)
+    if args.surrogate:
+        return bench_surrogate(args)
 
     import torch
     from cm1_amd import model

@@ -152,18 +152,18 @@ struct KessAct {
   bool ar, cr, er;  // autoconversion, accretion, evaporation possible
   float t;          // temperature
 };
-// kessler's own conditions for each process (src/kessler.F:48-75)
-__device__ __forceinline__ KessAct kess_active(const KP &P, int c, int k, const float *th3d, const float *pp3d,
-                                               const float *prs, const float *qv, const float *qc,
-                                               const float *qr) {
+// kessler's own conditions for each process (src/kessler.F:48-75), from a
+// cell's theta', pi', p, qv, qc, qr at level k
+__device__ __forceinline__ KessAct kess_active(const KP &P, int k, float th, float pp, float p, float qv,
+                                               float qc, float qr) {
   KessAct a;
-  a.t = (P.th0[k] + th3d[c]) * (P.pi0[k] + pp3d[c]);
-  a.ar = qc[c] > 0.001f;
-  a.cr = qc[c] > 0.0f && qr[c] > 0.0f;
+  a.t = (P.th0[k] + th) * (P.pi0[k] + pp);
+  a.ar = qc > 0.001f;
+  a.cr = qc > 0.0f && qr > 0.0f;
   a.er = false;
-  if (qr[c] > 0.0f) {
-    const float p = prs[c], esl = esl_of(a.t, p);
-    a.er = qv[c] < P.K.eps * esl / (p - esl);
+  if (qr > 0.0f) {
+    const float esl = esl_of(a.t, p);
+    a.er = qv < P.K.eps * esl / (p - esl);
   }
   return a;
 }
@@ -207,49 +207,64 @@ static void surrogate_kessler(cm1g_model *m) {
     if (i == 1 && j == 1) cnt[1] += 1ULL;
   });
   const int64_t n = (int64_t)ni * nj * nk;
-  auto cell = [=] __device__(int64_t s, int &i, int &j, int &k) {
-    i = 1 + (int)(s % ni);
-    j = 1 + (int)((s / ni) % nj);
-    k = 1 + (int)(s / ((int64_t)ni * nj));
+  // (n < 2^31: cm1g_set_surrogate checks it)
+  auto cell = [=] __device__(int64_t s64, int &i, int &j, int &k) {
+    const int s = (int)s64, r = s / ni;
+    i = 1 + (s - r * ni);
+    j = 1 + (r % nj);
+    k = 1 + r / nj;
   };
-  auto inputs = [=] __device__(int c, int k, float t, float *x) {
-    const float raw[MLP_IN] = {t, prs[c], rho[c], qv[c], qc[c], qr[c], dt, zh[k] * rztop};
-    for (int q = 0; q < MLP_IN; q++) x[q] = (raw[q] - nm[q]) / nm[MLP_IN + q];
-  };
-  auto in = [=] __device__(int64_t s, float *x) {
+  // raw values of a cell: theta, pi', p, rho, qv, qc, qr
+  auto load = [=] __device__(int64_t s, float *r) {
     int i, j, k;
     cell(s, i, j, k);
     const int c = IXD(P, i, j, k);
-    inputs(c, k, (P.th0[k] + th3d[c]) * (P.pi0[k] + pp3d[c]), x);
+    r[0] = th3d[c];
+    r[1] = pp3d[c];
+    r[2] = prs[c];
+    r[3] = rho[c];
+    r[4] = qv[c];
+    r[5] = qc[c];
+    r[6] = qr[c];
+  };
+  auto prep = [=] __device__(int64_t s, const float *r, float *x) {
+    int i, j, k;
+    cell(s, i, j, k);
+    const float raw[MLP_IN] = {(P.th0[k] + r[0]) * (P.pi0[k] + r[1]), r[2], r[3], r[4], r[5], r[6], dt,
+                               zh[k] * rztop};
+    for (int q = 0; q < MLP_IN; q++) x[q] = (raw[q] - nm[q]) / nm[MLP_IN + q];
   };
   // cells kessler changes: outside the trusted input range, or with an
   // unusable proposal, the column falls back
-  auto out = [=] __device__(int64_t s, const float *y) {
+  auto out = [=] __device__(int64_t s, const float *y, const float *r) {
     int i, j, k;
     cell(s, i, j, k);
     const int c = IXD(P, i, j, k);
-    const KessAct a = kess_active(P, c, k, th3d, pp3d, prs, qv, qc, qr);
+    const KessAct a = kess_active(P, k, r[0], r[1], r[2], r[4], r[5], r[6]);
     if (!(a.ar || a.cr || a.er)) return;
-    float x[MLP_IN];
-    inputs(c, k, a.t, x);
+    const float raw[MLP_IN] = {a.t, r[2], r[3], r[4], r[5], r[6], dt, zh[k] * rztop};
     bool inside = true;
-    for (int q = 0; q < MLP_IN; q++) inside = inside && x[q] >= nm[NM_XLO + q] && x[q] <= nm[NM_XHI + q];
+    for (int q = 0; q < MLP_IN; q++) {
+      const float x = (raw[q] - nm[q]) / nm[MLP_IN + q];
+      inside = inside && x >= nm[NM_XLO + q] && x <= nm[NM_XHI + q];
+    }
     float d[MLP_OUT];
     for (int q = 0; q < MLP_OUT; q++) d[q] = y[q] * nm[2 * MLP_IN + MLP_OUT + q] + nm[2 * MLP_IN + q];
-    float v, w, r;
-    const bool ok = inside && sur_project(a, qv[c], qc[c], qr[c], d, v, w, r);
+    float v, w, rr;
+    const bool ok = inside && sur_project(a, r[4], r[5], r[6], d, v, w, rr);
     if (ok) {
       pqv[c] = v;
       pqc[c] = w;
-      pqr[c] = r;
+      pqr[c] = rr;
     } else {
       atomicOr(&colflag[(i - 1) + (j - 1) * ni], 1);
       atomicAdd(&cnt[inside ? 5 : 4], 1ULL);
     }
   };
-  const int64_t groups = (n + 16 * MLP_WAVES - 1) / (16 * MLP_WAVES);
+  const int64_t groups = (n + 16 * MLP_G * MLP_WAVES - 1) / (16 * MLP_G * MLP_WAVES);
   const int nbx = (int)(groups < 1024 ? groups : 1024);
-  launch<512>(m, dim3(nbx), dim3(512), [=] __device__(int bx, int, int) { mlp_tiles(prm, n, in, out, bx, nbx); });
+  launch<512>(m, dim3(nbx), dim3(512),
+              [=] __device__(int bx, int, int) { mlp_tiles(prm, n, load, prep, out, bx, nbx); });
   // apply the projected proposal as kessler applies its own result (the
   // theta, pi', p update after evaporation above 1e-7), or plain kessler in
   // flagged columns
@@ -261,7 +276,7 @@ static void surrogate_kessler(cm1g_model *m) {
       atomicAdd(&cnt[2], 1ULL);
       return;
     }
-    const KessAct a = kess_active(P, c, k, th3d, pp3d, prs, qv, qc, qr);
+    const KessAct a = kess_active(P, k, th3d[c], pp3d[c], prs[c], qv[c], qc[c], qr[c]);
     if (!(a.ar || a.cr || a.er)) return;
     const float qvnew = pqv[c], qcnew = pqc[c], qrnew = pqr[c];
     const float er = qvnew - qv[c];
@@ -430,6 +445,10 @@ extern "C" int cm1g_set_surrogate(cm1g_model *m, cm1g_mlp *net) {
   }
   if (net && m->harv) {
     cm1g_set_error("cm1g_set_surrogate: stop the harvest first");
+    return CM1G_E_ARG;
+  }
+  if (net && (int64_t)m->c.ni * m->c.nj * m->c.nk >= (1LL << 31)) {
+    cm1g_set_error("cm1g_set_surrogate: more than 2^31 cells");
     return CM1G_E_ARG;
   }
   m->sur = net;

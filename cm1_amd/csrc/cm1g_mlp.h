@@ -26,76 +26,114 @@ __device__ __forceinline__ unsigned short bf16_rne(float f) {
   u += 0x7fffu + ((u >> 16) & 1u);
   return (unsigned short)(u >> 16);
 }
-__device__ __forceinline__ float silu_f(float z) { return z / (1.0f + expf(-z)); }
+// SiLU with the hardware exponential and reciprocal (training and inference
+// use the same form)
+__device__ __forceinline__ float silu_f(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
-// LDS of the fused inference: BF16 weights transposed to [out][in] (the four
-// inputs a lane feeds one matrix instruction are then contiguous), rows
-// padded by 4 against bank conflicts; per wavefront a 16-sample activation
-// block and the 16 x 4 outputs.
-constexpr int MLP_P1 = 20, MLP_PH = 132;
-constexpr int MLP_WAVES = 8;  // wavefronts per workgroup (512 threads)
+// Fused inference.  Each layer is computed transposed, Y^T = W^T X^T: the
+// weight block is the first matrix operand and the 16-sample activation
+// block the second, so a lane's result, Y(sample l%16, outputs 16nt +
+// 4(l/16) + t), is exactly the second operand the next layer needs for
+// k-step nt.  Activations therefore stay in registers from layer to layer;
+// only the weights sit in LDS, as BF16 in fragment order (for output tile
+// nt and k-step ks the 64 lanes' 4-element fragments lie consecutively,
+// lane l at ((nt*KS + ks)*64 + l)*4, one 8-byte load per lane).
+constexpr int MLP_WAVES = 8;          // wavefronts per workgroup (512 threads)
+constexpr int MLP_G = 1;              // 16-sample groups per wavefront and round
+constexpr int MLP_F1 = 8 * 1 * 256;   // layer 1: 8 output tiles x 1 k-step x 64 lanes x 4
+constexpr int MLP_FH = 8 * 8 * 256;   // hidden layers: 8 x 8
+constexpr int MLP_F4 = 1 * 8 * 256;   // output layer: 1 (4 of 16 outputs used) x 8
 struct MlpLds {
-  unsigned short w1[MLP_H * MLP_P1];
-  unsigned short w2[MLP_H * MLP_PH];
-  unsigned short w3[MLP_H * MLP_PH];
-  unsigned short w4[16 * MLP_PH];
+  bf16x4 w1[MLP_F1 / 4];
+  bf16x4 w2[MLP_FH / 4];
+  bf16x4 w3[MLP_FH / 4];
+  bf16x4 w4[MLP_F4 / 4];
   float b[3 * MLP_H + 16];
-  unsigned short act[MLP_WAVES][16 * MLP_PH];
-  float out[MLP_WAVES][16 * MLP_OUT];
 };
 
-// one hidden or output layer of a 16-sample block: act (16 x K, BF16) times
-// W^T rows of `w` (pitch pitch), nt output tiles of 16; SiLU on hidden layers
-// writes back into act, the last layer leaves its 16 x 4 block in out
-template <int KSTEPS, int NT, bool LAST>
-__device__ __forceinline__ void mlp_layer(unsigned short *act, const unsigned short *w, int pitch, const float *bias,
-                                          float *out, int l) {
-  bf16x4 a[KSTEPS];
-  for (int ks = 0; ks < KSTEPS; ks++)
-    for (int t = 0; t < 4; t++) a[ks][t] = (short)act[(l & 15) * MLP_PH + 16 * ks + 4 * (l >> 4) + t];
-  __syncthreads();
-  for (int nt = 0; nt < NT; nt++) {
-    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int ks = 0; ks < KSTEPS; ks++) {
-      bf16x4 b;
-      const unsigned short *pb = &w[(16 * nt + (l & 15)) * pitch + 16 * ks + 4 * (l >> 4)];
-      for (int t = 0; t < 4; t++) b[t] = (short)pb[t];
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ks], b, acc, 0, 0, 0);
-    }
-    const int col = 16 * nt + (l & 15);
-    for (int t = 0; t < 4; t++) {
-      const int row = 4 * (l >> 4) + t;
-      const float z = acc[t] + bias[col];
-      if (LAST) {
-        if (col < MLP_OUT) out[row * MLP_OUT + col] = z;
-      } else {
-        act[row * MLP_PH + col] = bf16_rne(silu_f(z));
-      }
-    }
-  }
-  __syncthreads();
+// fragment-order index of element (k, n) of a weight with KS k-steps
+__device__ __forceinline__ int mlp_wfrag(int k, int n, int ks_count) {
+  return (((n >> 4) * ks_count + (k >> 4)) * 64 + (n & 15) + 16 * ((k & 15) >> 2)) * 4 + (k & 3);
 }
 
-// Run the network over samples 0..n-1: in(s, x) gives the 8 normalised
-// inputs of sample s, out(s, y) takes its 4 normalised outputs.  Workgroup
-// bx of nbx (512 threads) stages the weights once, then its wavefronts take
-// 16-sample blocks bx*8 + w, bx*8 + w + nbx*8, ...
-template <class IN, class OUT>
-__device__ void mlp_tiles(const float *__restrict__ prm, int64_t n, const IN &in, const OUT &out, int bx, int nbx) {
+__device__ __forceinline__ bf16x4 mlp_pack(float a, float b, float c, float d) {
+  bf16x4 v;
+  v[0] = (short)bf16_rne(a);
+  v[1] = (short)bf16_rne(b);
+  v[2] = (short)bf16_rne(c);
+  v[3] = (short)bf16_rne(d);
+  return v;
+}
+
+// one hidden layer for MLP_G groups: x[g][ks] -> x[g][nt] = SiLU(W^T x + b)
+template <int KS>
+__device__ __forceinline__ void mlp_hidden(bf16x4 (&x)[MLP_G][8], const bf16x4 *wf, const float *bias, int l) {
+  floatx4 acc[MLP_G][8];
+#pragma unroll
+  for (int g = 0; g < MLP_G; g++)
+#pragma unroll
+    for (int nt = 0; nt < 8; nt++) acc[g][nt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  // the next k-step's weight fragments load while this one's matrix
+  // instructions run; the scheduling barrier keeps the compiler from
+  // hoisting every k-step's loads at once (which would exceed the registers)
+  bf16x4 wc[8], wn[8];
+#pragma unroll
+  for (int nt = 0; nt < 8; nt++) wc[nt] = wf[(nt * KS) * 64 + l];
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) {
+    if (ks + 1 < KS) {
+#pragma unroll
+      for (int nt = 0; nt < 8; nt++) wn[nt] = wf[(nt * KS + ks + 1) * 64 + l];
+    }
+#pragma unroll
+    for (int nt = 0; nt < 8; nt++)
+#pragma unroll
+      for (int g = 0; g < MLP_G; g++)
+        acc[g][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wc[nt], x[g][ks], acc[g][nt], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int nt = 0; nt < 8; nt++) wc[nt] = wn[nt];
+  }
+#pragma unroll
+  for (int nt = 0; nt < 8; nt++) {
+    const int c0 = 16 * nt + 4 * (l >> 4);
+    const float b0 = bias[c0], b1 = bias[c0 + 1], b2 = bias[c0 + 2], b3 = bias[c0 + 3];
+#pragma unroll
+    for (int g = 0; g < MLP_G; g++)
+      x[g][nt] = mlp_pack(silu_f(acc[g][nt][0] + b0), silu_f(acc[g][nt][1] + b1), silu_f(acc[g][nt][2] + b2),
+                          silu_f(acc[g][nt][3] + b3));
+  }
+}
+
+// Run the network over samples 0..n-1.  load(s, r) fills r[MLP_KEEP] with
+// the raw values sample s needs (loads only), prep(s, r, x) turns them into
+// the 8 normalised inputs, out(s, y, r) takes the 4 normalised outputs.
+// In a wavefront, lanes l and l+16 (l < 16) of group g work on sample
+// base + 16g + l: both load and prepare it (each keeps its half of the
+// inputs), lane l takes the outputs.  The loads of the next round are
+// issued before the current round's layers, so their latency overlaps the
+// matrix work.  Workgroup bx of nbx stages the weights once; its
+// wavefronts then run independently.
+constexpr int MLP_KEEP = 8;
+template <class LOAD, class PREP, class OUT>
+__device__ void mlp_tiles(const float *__restrict__ prm, int64_t n, const LOAD &load, const PREP &prep,
+                          const OUT &out, int bx, int nbx) {
   __shared__ MlpLds S;
   const int tid = (int)threadIdx.x, l = tid & 63, w = tid >> 6;
-  for (int e = tid; e < MLP_H * 16; e += 512) {  // e = k*128 + n over the padded 16 inputs
+  unsigned short *w1 = (unsigned short *)S.w1, *w2 = (unsigned short *)S.w2, *w3 = (unsigned short *)S.w3,
+                 *w4 = (unsigned short *)S.w4;
+  for (int e = tid; e < 16 * MLP_H; e += 512) {  // layer 1, inputs padded to 16
     const int k = e / MLP_H, nn = e % MLP_H;
-    S.w1[nn * MLP_P1 + k] = k < MLP_IN ? bf16_rne(prm[OW1 + k * MLP_H + nn]) : (unsigned short)0;
+    w1[mlp_wfrag(k, nn, 1)] = k < MLP_IN ? bf16_rne(prm[OW1 + k * MLP_H + nn]) : (unsigned short)0;
   }
   for (int e = tid; e < MLP_H * MLP_H; e += 512) {
     const int k = e / MLP_H, nn = e % MLP_H;
-    S.w2[nn * MLP_PH + k] = bf16_rne(prm[OW2 + e]);
-    S.w3[nn * MLP_PH + k] = bf16_rne(prm[OW3 + e]);
+    w2[mlp_wfrag(k, nn, 8)] = bf16_rne(prm[OW2 + e]);
+    w3[mlp_wfrag(k, nn, 8)] = bf16_rne(prm[OW3 + e]);
   }
-  for (int e = tid; e < MLP_H * 16; e += 512) {  // e = k*16 + n over 16 padded outputs
+  for (int e = tid; e < MLP_H * 16; e += 512) {  // output layer, outputs padded to 16
     const int k = e / 16, nn = e % 16;
-    S.w4[nn * MLP_PH + k] = nn < MLP_OUT ? bf16_rne(prm[OW4 + k * MLP_OUT + nn]) : (unsigned short)0;
+    w4[mlp_wfrag(k, nn, 8)] = nn < MLP_OUT ? bf16_rne(prm[OW4 + k * MLP_OUT + nn]) : (unsigned short)0;
   }
   for (int e = tid; e < 3 * MLP_H + 16; e += 512) {
     float v = 0.0f;
@@ -106,27 +144,60 @@ __device__ void mlp_tiles(const float *__restrict__ prm, int64_t n, const IN &in
     S.b[e] = v;
   }
   __syncthreads();
-  unsigned short *act = S.act[w];
-  float *o = S.out[w];
-  const int64_t nblk = (n + 15) / 16;
-  // every wavefront runs the same number of rounds (the barriers inside)
-  for (int64_t base = (int64_t)bx * MLP_WAVES; base < nblk; base += (int64_t)nbx * MLP_WAVES) {
-    const int64_t blk = base + w;
-    const int64_t s = blk * 16 + l;
-    const bool live = (blk < nblk) && (l < 16) && (s < n);
-    if (l < 16) {
-      float x[MLP_IN];
-      for (int k = 0; k < MLP_IN; k++) x[k] = 0.0f;
-      if (live) in(s, x);
-      for (int k = 0; k < MLP_IN; k++) act[l * MLP_PH + k] = bf16_rne(x[k]);
-      for (int k = MLP_IN; k < 16; k++) act[l * MLP_PH + k] = 0;
+  constexpr int PER = 16 * MLP_G;  // samples per wavefront and round
+  const int64_t stride = (int64_t)nbx * MLP_WAVES * PER;
+  const bool loader = l < 32;
+  int64_t base = ((int64_t)bx * MLP_WAVES + w) * PER;
+  float r[MLP_G][MLP_KEEP];
+  auto fetch = [&](int64_t b0, float (&rr)[MLP_G][MLP_KEEP]) {
+#pragma unroll
+    for (int g = 0; g < MLP_G; g++) {
+      for (int q = 0; q < MLP_KEEP; q++) rr[g][q] = 0.0f;
+      const int64_t s = b0 + 16 * g + (l & 15);
+      if (loader && s < n) load(s, rr[g]);
     }
-    __syncthreads();
-    mlp_layer<1, 8, false>(act, S.w1, MLP_P1, S.b, o, l);
-    mlp_layer<8, 8, false>(act, S.w2, MLP_PH, S.b + MLP_H, o, l);
-    mlp_layer<8, 8, false>(act, S.w3, MLP_PH, S.b + 2 * MLP_H, o, l);
-    mlp_layer<8, 1, true>(act, S.w4, MLP_PH, S.b + 3 * MLP_H, o, l);
-    if (live) out(s, &o[l * MLP_OUT]);
-    __syncthreads();
+  };
+  fetch(base, r);
+  for (; base < n; base += stride) {
+    bf16x4 x[MLP_G][8];
+#pragma unroll
+    for (int g = 0; g < MLP_G; g++) {
+      const int64_t s = base + 16 * g + (l & 15);
+      float xin[MLP_IN];
+      for (int q = 0; q < MLP_IN; q++) xin[q] = 0.0f;
+      if (loader && s < n) prep(s, r[g], xin);
+      const int h = (l >> 4) & 1;  // lanes 0-15 take inputs 0-3, lanes 16-31 inputs 4-7
+      x[g][0] = loader ? mlp_pack(xin[4 * h], xin[4 * h + 1], xin[4 * h + 2], xin[4 * h + 3])
+                       : mlp_pack(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    float rn[MLP_G][MLP_KEEP];
+    fetch(base + stride, rn);
+    mlp_hidden<1>(x, S.w1, S.b, l);
+    mlp_hidden<8>(x, S.w2, S.b + MLP_H, l);
+    mlp_hidden<8>(x, S.w3, S.b + 2 * MLP_H, l);
+    // output layer: one tile of 16 outputs (4 used); lane l < 16 ends up
+    // with outputs 0..3 of its sample
+    floatx4 acc[MLP_G];
+#pragma unroll
+    for (int g = 0; g < MLP_G; g++) acc[g] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++) {
+      const bf16x4 wv = S.w4[ks * 64 + l];
+#pragma unroll
+      for (int g = 0; g < MLP_G; g++)
+        acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wv, x[g][ks], acc[g], 0, 0, 0);
+    }
+#pragma unroll
+    for (int g = 0; g < MLP_G; g++) {
+      const int64_t s = base + 16 * g + l;
+      if (l < 16 && s < n) {
+        const float y[MLP_OUT] = {acc[g][0] + S.b[3 * MLP_H], acc[g][1] + S.b[3 * MLP_H + 1],
+                                  acc[g][2] + S.b[3 * MLP_H + 2], acc[g][3] + S.b[3 * MLP_H + 3]};
+        out(s, y, r[g]);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < MLP_G; g++)
+      for (int q = 0; q < MLP_KEEP; q++) r[g][q] = rn[g][q];
   }
 }

@@ -204,9 +204,9 @@ __global__ void __launch_bounds__(256) k_adam(float *__restrict__ p, const float
 }
 
 // fused inference over a row-major [n x 8] input, [n x 4] output
-template <class IN, class OUT>
-__global__ void __launch_bounds__(512) k_mlp(const float *prm, int64_t n, IN in, OUT out) {
-  mlp_tiles(prm, n, in, out, (int)blockIdx.x, (int)gridDim.x);
+template <class LOAD, class PREP, class OUT>
+__global__ void __launch_bounds__(512) k_mlp(const float *prm, int64_t n, LOAD load, PREP prep, OUT out) {
+  mlp_tiles(prm, n, load, prep, out, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ------------------------------------------------------------ C ABI
@@ -307,16 +307,19 @@ extern "C" int cm1g_mlp_set_norm(cm1g_mlp *n, const float *norm) {
 extern "C" int cm1g_mlp_forward(cm1g_mlp *n, const float *x, int64_t rows, float *y) {
   ARGCHK(n && x && y && rows >= 0, "cm1g_mlp_forward: bad argument");
   if (rows == 0) return 0;
-  auto in = [=] __device__(int64_t s, float *v) {
-    for (int k = 0; k < MLP_IN; k++) v[k] = x[s * MLP_IN + k];
+  auto load = [=] __device__(int64_t s, float *r) {
+    for (int k = 0; k < MLP_IN; k++) r[k] = x[s * MLP_IN + k];
   };
-  auto out = [=] __device__(int64_t s, const float *v) {
+  auto prep = [=] __device__(int64_t, const float *r, float *v) {
+    for (int k = 0; k < MLP_IN; k++) v[k] = r[k];
+  };
+  auto out = [=] __device__(int64_t s, const float *v, const float *) {
     for (int k = 0; k < MLP_OUT; k++) y[s * MLP_OUT + k] = v[k];
   };
-  const int64_t groups = (rows + 16 * MLP_WAVES - 1) / (16 * MLP_WAVES);
+  const int64_t groups = (rows + 16 * MLP_G * MLP_WAVES - 1) / (16 * MLP_G * MLP_WAVES);
   const int nbx = (int)(groups < 1024 ? groups : 1024);
-  hipLaunchKernelGGL((k_mlp<decltype(in), decltype(out)>), dim3(nbx), dim3(512), 0, n->stream, n->p, rows, in,
-                     out);
+  hipLaunchKernelGGL((k_mlp<decltype(load), decltype(prep), decltype(out)>), dim3(nbx), dim3(512), 0, n->stream,
+                     n->p, rows, load, prep, out);
   HIPCHK(hipGetLastError());
   return 0;
 }

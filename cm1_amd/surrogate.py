@@ -1,12 +1,13 @@
-"""Learned surrogate for the local Kessler step (kessler + satadj), trained on
-the GPU by the library's own MLP kernels (cm1g_mlp_*).
+"""Learned surrogate for kessler (autoconversion, accretion, rain
+evaporation), trained on the GPU by the library's own MLP kernels
+(cm1g_mlp_*); sedimentation and the saturation adjustment stay exact.
 
 Workflow: harvest (state, change) pairs from plain-Kessler runs
 (:func:`harvest`), fit the normalisation (:func:`fit_norm`), train
 (:meth:`Net.train`), then hand the network to a model
-(``Model`` + :func:`use`): its local Kessler step becomes the network's, with
-the positivity / total-water projection and the per-column fallback to
-plain Kessler done in the kernels.  Device buffers are torch tensors (torch
+(``Model`` + :func:`use`): its kessler becomes the network's, with the
+projection (water conserved by construction, no negative species) and the
+per-column fallback to plain kessler done in the kernels.  Device buffers are torch tensors (torch
 provides memory and nothing else here).
 """
 import ctypes as C
@@ -80,9 +81,10 @@ class Net:
                                           g.ctypes.data if want_grad else None), "cm1g_mlp_loss_grad")
         return (loss.value, g) if want_grad else loss.value
 
-    def train(self, x, t, steps, batch=65536, lr=1.0e-3, seed=0, log_every=0):
+    def train(self, x, t, steps, batch=65536, lr=1.0e-3, seed=0, log_every=0, lr_end=0.05):
         """Adam on minibatches drawn (with replacement) from normalised x, t
-        (torch cuda tensors); returns the loss every `log_every` steps."""
+        (torch cuda tensors), the rate decaying from lr to lr*lr_end along a
+        cosine; returns the loss every `log_every` steps."""
         import torch
         gen = torch.Generator(device=x.device)
         gen.manual_seed(seed)
@@ -96,7 +98,8 @@ class Net:
             loss = C.c_float()
             N.check(self.L.cm1g_mlp_loss_grad(self.h, _ptr(xb), _ptr(tb), batch,
                                               C.byref(loss) if want else None, None), "cm1g_mlp_loss_grad")
-            N.check(self.L.cm1g_mlp_adam(self.h, C.c_float(lr)), "cm1g_mlp_adam")
+            rate = lr * (lr_end + (1.0 - lr_end) * 0.5 * (1.0 + np.cos(np.pi * s / max(1, steps - 1))))
+            N.check(self.L.cm1g_mlp_adam(self.h, C.c_float(rate)), "cm1g_mlp_adam")
             if want:
                 losses.append((s, loss.value))
         N.check(self.L.cm1g_mlp_sync(self.h), "cm1g_mlp_sync")
@@ -105,8 +108,9 @@ class Net:
 
 def harvest(models, steps, capacity, keep_one_in=20):
     """Run plain-Kessler models `steps` large steps each, collecting the
-    (inputs, changes) pairs of their local Kessler steps; returns a torch
-    cuda tensor [n, 12]."""
+    (inputs, changes over kessler) pairs of their Kessler branches; returns
+    a torch cuda tensor [n, 12] (at most `capacity` rows: later steps are
+    dropped once it is full)."""
     import torch
     L = N.lib()
     buf = torch.empty((capacity, 12), dtype=torch.float32, device="cuda")
@@ -154,7 +158,7 @@ def normalise(pairs, norm):
 
 
 def use(model, net):
-    """Make `net` the local Kessler step of `model` (None: plain Kessler)."""
+    """Make `net` the kessler of `model` (None: plain kessler)."""
     N.check(N.lib().cm1g_set_surrogate(model.h, net.h if net is not None else None), "cm1g_set_surrogate")
 
 
