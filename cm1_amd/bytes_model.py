@@ -39,7 +39,7 @@ SOUND_SETUP = 7
 SOUND_PREP = {1: 6, 2: 13, 3: 13}
 STEP_BASE = 534
 
-# the fused acoustic substep kernel (k_sound_sub) evaluates pk1, pk2, qk and
+# the fused acoustic substep kernel (sound_sub) evaluates pk1, pk2, qk and
 # mm on the fly, so it touches 23 distinct fields: u, v, w, ppd, pp3d, rru,
 # rrv, rrw read and written (16) and uten, vten, wten, thv, ppten, ppterm,
 # piadv read (7)

@@ -1348,7 +1348,7 @@ extern "C" int cm1g_ktimer_stop(cm1g_model *m, double *total_ms, int64_t *launch
     // u, v, w, ppd, pp3d, rru, rrv, rrw read and written (16); uten, vten,
     // wten, thv, ppten, ppterm, piadv read (7)
     fields = 23.0;
-    nm = "k_sound_sub";
+    nm = "sound_sub";
   }
   *total_ms = tot;
   *launches = (int64_t)npair;
