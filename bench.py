@@ -311,6 +311,63 @@ def bench_surrogate(args):
     return 0
 
 
+def workload_cbl(ni=1536, nj=1536, nk=128, seed=2026):
+    """Dry convective boundary layer: 1536x1536x128, dx=dy=dz=20 m,
+    doubly periodic, dt=0.25 s, explicit acoustic solver (psolver=2), WENO5
+    scalar advection on every stage, 300 K mixed layer to 1 km then 8 K/km,
+    u=10 m/s, +-0.1 K noise below 500 m, six Gaussian tracer blobs
+    (sigma 200 m).  (The surface heat flux needs the SGS turbulence scheme,
+    which is not on this path.)"""
+    from cm1_amd import init as hinit
+    cols, fields, dzn = hinit.cbl_state(ni, nj, nk, 20.0, 20.0, 20.0, seed=seed)
+    cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=1, ebc=1, sbc=1, nbc=1, imoist=0, ptype=0,
+                psolver=2, advwenos=1, advwenov=0, apmasscon=1, irbc=4, nsound=0, npt=6,
+                pdtra=1, dx=20.0, dy=20.0, dz=dzn, dt=0.25, alph=0.6, kdiv=0.1, csmax=350.0)
+    return cfgd, cols, fields
+
+
+def bench_cbl(args):
+    """Throughput of the dry CBL workload (one JSON line)."""
+    import torch
+    from cm1_amd import model
+    t0 = time.perf_counter()
+    cfgd, cols, fields = workload_cbl()
+    cfg = model.make_config(**cfgd)
+    m = model.setup(cfg, cols, fields)
+    del fields
+    print(f"cbl: set-up {time.perf_counter() - t0:.1f} s, nsound {cfg.nsound}", file=sys.stderr, flush=True)
+    tm0, _ = m.budgets()
+    m.step(max(1, args.warmup))
+    m.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    cfl = m.step(args.steps)
+    m.sync()
+    ms = (time.perf_counter() - t1) * 1000.0 / args.steps
+    tm1, _ = m.budgets()
+    cells = cfgd["ni"] * cfgd["nj"] * cfgd["nk"]
+    line = {
+        "metric": "cell-updates/s per RK3 step (FP32)",
+        "value": cells / (ms / 1000.0),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": max(1, args.warmup),
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "dry CBL 1536x1536x128 dx=dz=20m dt=0.25s periodic sounde WENO5 6 tracers "
+                               "(no SGS turbulence / surface flux)",
+                   "nsound": cfg.nsound, "cflmax": cfl,
+                   "dry_mass_rel_change": (tm1 - tm0) / tm0},
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def cpu_baseline(threads):
     """Reference CM1 (FP32 OpenMP build) on a 256x256x96 sample of the case."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -379,6 +436,8 @@ def main():
                     help="first run a plain 1 GiB device copy (reference kernel for HBM counters)")
     ap.add_argument("--ensemble", type=int, default=0,
                     help="benchmark an ensemble of this many members instead (batched steps)")
+    ap.add_argument("--cbl", action="store_true",
+                    help="benchmark the dry convective-boundary-layer workload (1536x1536x128) instead")
     ap.add_argument("--surrogate", action="store_true",
                     help="train and benchmark the learned Kessler surrogate instead")
     ap.add_argument("--spinup", type=int, default=300, help="plain steps harvested for the surrogate")
@@ -393,6 +452,8 @@ def main():
         return bench_ensemble(args)
     if args.surrogate:
         return bench_surrogate(args)
+    if args.cbl:
+        return bench_cbl(args)
 
     import torch
     from cm1_amd import model

@@ -170,3 +170,50 @@ def initial_state(ni, nj, nk, dx, dy, dz, *, stretch=None, imoist=1, umax=35.0,
             a[1:nk + 1, 3:nj + 3, 3:ni + 3] = tracers[n]
         f[f"pta{n}"] = a
     return cols, f, dzn
+
+
+def cbl_sounding(zh, nk, th_ml=300.0, z_inv=1000.0, gamma=0.008, prs_sfc=100000.0):
+    """Dry convective-boundary-layer base state: theta = th_ml up to z_inv,
+    then th_ml + gamma (z - z_inv); pi0 hydrostatic as in wk_sounding."""
+    g, cp, rd = float(G), float(CP), float(RD)
+    pi_sfc = (prs_sfc / 1.0e5) ** (rd / cp)
+    th0 = np.zeros(nk + 3)
+    for k in range(1, nk + 1):
+        z = float(zh[k])
+        th0[k] = th_ml + (gamma * (z - z_inv) if z > z_inv else 0.0)
+    th0[0], th0[nk + 1] = th0[1], th0[nk]
+    pi0 = np.zeros(nk + 3)
+    pi0[1] = pi_sfc - g * float(zh[1]) / (cp * 0.5 * (th_ml + th0[1]))
+    for k in range(2, nk + 1):
+        pi0[k] = pi0[k - 1] - g * float(zh[k] - zh[k - 1]) / (cp * 0.5 * (th0[k] + th0[k - 1]))
+    return dict(th0=th0.astype(F32), pi0=pi0.astype(F32), qv0=np.zeros(nk + 3, F32))
+
+
+def cbl_state(ni, nj, nk, dx, dy, dz, *, u=10.0, npt=6, blob_sigma=200.0, noise_amp=0.1,
+              noise_zmax=500.0, seed=0):
+    """Columns and t=0 fields of the dry CBL case: the cbl_sounding base state,
+    uniform u, +-noise_amp K theta noise below noise_zmax, and npt passive
+    tracers initialised as seeded Gaussian blobs (amplitude 1, blob_sigma)."""
+    cols, dzn = vertical_grid(nk, dz, None)
+    cols.update(cbl_sounding(cols["zh"], nk))
+    cols["u0"] = np.zeros(nk + 3, F32)
+    cols["u0"][1:nk + 1] = u
+    f = {n: blank(ni, nj, nk) for n in ("ua", "va", "wa", "tha", "ppi")}
+    f["ua"][1:nk + 1] = u
+    rng = np.random.default_rng(seed)
+    zh = cols["zh"][1:nk + 1].astype(np.float64)
+    nzk = int(np.count_nonzero(zh < noise_zmax))
+    f["tha"][1:nzk + 1, 3:nj + 3, 3:ni + 3] = rng.uniform(-noise_amp, noise_amp,
+                                                          size=(nzk, nj, ni)).astype(F32)
+    xh = ((np.arange(1, ni + 1) - 0.5) * dx).astype(np.float32)
+    yh = ((np.arange(1, nj + 1) - 0.5) * dy).astype(np.float32)
+    for n in range(npt):
+        cx, cy = rng.uniform(0.2, 0.8) * ni * dx, rng.uniform(0.2, 0.8) * nj * dy
+        cz = rng.uniform(0.2, 0.6) * float(zh[-1])
+        gx = np.exp(-0.5 * ((xh - cx) / blob_sigma) ** 2).astype(F32)
+        gy = np.exp(-0.5 * ((yh - cy) / blob_sigma) ** 2).astype(F32)
+        gz = np.exp(-0.5 * ((zh - cz) / blob_sigma) ** 2).astype(F32)
+        a = blank(ni, nj, nk)
+        a[1:nk + 1, 3:nj + 3, 3:ni + 3] = gz[:, None, None] * (gy[:, None] * gx[None, :])[None, :, :]
+        f[f"pta{n}"] = a
+    return cols, f, dzn
