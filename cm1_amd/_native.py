@@ -114,6 +114,8 @@ SIGNATURES = {
                                     C.POINTER(C.c_float)]),
     "cm1g_getdiv": (C.c_int, [C.c_void_p, C.c_int]),
     "cm1g_getdivx": (C.c_int, [C.c_void_p]),
+    "cm1g_maxmin": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int32),
+                              C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
     "cm1g_state_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "cm1g_state_save": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "cm1g_state_load": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),

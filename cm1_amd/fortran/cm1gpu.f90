@@ -437,6 +437,13 @@ module cm1gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: m
     end function
+    integer(c_int) function cm1g_maxmin(m, id, stag, vmax, locmax, vmin, locmin) bind(C, name="cm1g_maxmin")
+      import :: c_int, c_ptr, c_float, c_int32_t
+      type(c_ptr), value :: m
+      integer(c_int), value :: id, stag
+      real(c_float), intent(out) :: vmax, vmin
+      integer(c_int32_t), intent(out) :: locmax(3), locmin(3)
+    end function
     integer(c_int) function cm1g_scalars(m, out4) bind(C, name="cm1g_scalars")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: m

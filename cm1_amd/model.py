@@ -129,6 +129,15 @@ class Model:
         buf = np.ascontiguousarray(buf, np.uint8)
         N.check(self.L.cm1g_state_load(self.h, buf.ctypes.data, buf.size), "cm1g_state_load")
 
+    def maxmin(self, name, stag=None):
+        """Device maxmin of a field's interior: (vmax, (i,j,k)), (vmin, (i,j,k))."""
+        st = {None: 0, "u": 1, "v": 2, "w": 3}[stag]
+        vmax, vmin = C.c_float(), C.c_float()
+        lmax, lmin = (C.c_int32 * 3)(), (C.c_int32 * 3)()
+        N.check(self.L.cm1g_maxmin(self.h, N.FID[name], st, C.byref(vmax), lmax, C.byref(vmin), lmin),
+                "cm1g_maxmin")
+        return (vmax.value, tuple(lmax)), (vmin.value, tuple(lmin))
+
     def budgets(self):
         a, b = C.c_double(), C.c_double()
         N.check(self.L.cm1g_budgets(self.h, C.byref(a), C.byref(b)), "cm1g_budgets")

@@ -267,6 +267,12 @@ int cm1g_calccflquick(cm1g_model *m, float dt, int u_id, int v_id, int w_id, flo
  * (rrw = 0 at k = 1 and nk+1) and computes divx from them. */
 int cm1g_getdiv(cm1g_model *m, int div_id);
 int cm1g_getdivx(cm1g_model *m);
+/* maxmin(izz,jzz,kzz,f,...), src/maxmin.F:5-119: the largest and smallest
+ * interior value of field id (izz, jzz, kzz = ni, nj, nk plus one along the
+ * staggered direction of stag, as statpack passes them, src/statpack.F:138,
+ * 214) and their (i,j,k): among equal extremes the smallest k, then j, then
+ * i, as maxmin's strict comparisons pick them.  Reduced on the device. */
+int cm1g_maxmin(cm1g_model *m, int id, int stag, float *vmax, int32_t *locmax, float *vmin, int32_t *locmin);
 /* dtsmall: nsound for dt (src/misclibs.F:818-915) */
 int cm1g_dtsmall(const cm1g_config *cfg, int32_t *nsound);
 /* Domain budgets in double precision: dry-air mass (calcmass,
