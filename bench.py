@@ -170,94 +170,112 @@ def workload_spinup(ni=128, nj=128, nk=40, seed=7):
 
 
 def bench_surrogate(args):
-    """Learned Kessler surrogate: harvest (state, change) pairs from a plain
-    spin-up, train the MLP on the GPU, then (a) the rain accumulated over
-    60 further steps with the network in place of kessler against plain
-    Kessler (and against no network), (b) the step time of the 1024x1024x96
-    supercell with and without the surrogate, (c) a 48-member ensemble,
-    batched with the surrogate, against its members stepped one by one with
-    plain Kessler.  Prints one JSON line."""
+    """Learned Kessler surrogate (one JSON line).
+
+    (a) The 1024x1024x96 supercell itself: a plain spin-up of --spinup steps,
+    (state, change-over-kessler) pairs harvested on 6 steps spread over its
+    second half, the MLP trained on the GPU, then 60 further steps from the
+    end of the spin-up with plain kessler, with the network in its place
+    and with a zero network (no autoconversion, accretion or evaporation):
+    the domain rain accumulation and step time of each.
+    (b) A 128x128x40 dx=1 km case (dt=6 s, as the ensemble's members): the
+    same harvest, training and rain comparison, then a 48-member ensemble
+    batched with that network against its members stepped one by one with
+    plain Kessler."""
     import torch
     from cm1_amd import model
     from cm1_amd import surrogate as S
     import cm1_amd._native as N
+    L = N.lib()
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    cfgd, cols, fields = workload_spinup()
-    m = model.setup(model.make_config(**cfgd), cols, fields)
-    t0 = time.perf_counter()
-    # the first half spins the storms up; the second half is harvested
-    m.step(args.spinup // 2)
-    pairs = S.harvest([m], steps=args.spinup - args.spinup // 2, capacity=12_000_000, keep_one_in=20)
-    state = m.save_state()
-    log(f"surrogate: {pairs.shape[0]} pairs from the last {args.spinup - args.spinup // 2} of {args.spinup} "
-        f"plain steps ({time.perf_counter() - t0:.1f} s)")
-    norm = S.fit_norm(pairs)
-    x, t = S.normalise(pairs, norm)
-    net = S.Net(seed=1)
-    net.set_norm(norm)
-    t0 = time.perf_counter()
-    losses = net.train(x, t, steps=args.train_steps, batch=65536, lr=1.0e-3, log_every=2000)
-    t_train = time.perf_counter() - t0
-    log(f"surrogate: trained {args.train_steps} Adam steps of 65536 in {t_train:.1f} s, losses {losses}")
-    del x, t, pairs
+    def harvest_during(m, nsteps, at, capacity, keep):
+        buf = torch.empty((capacity, 12), dtype=torch.float32, device="cuda")
+        used = 0
+        for s_ in range(nsteps):
+            if s_ in at and used < capacity:
+                N.check(L.cm1g_set_harvest(m.h, S._ptr(buf[used:]), capacity - used, keep), "cm1g_set_harvest")
+                m.step(1)
+                c = S.counts(m)
+                N.check(L.cm1g_set_harvest(m.h, None, 0, 1), "cm1g_set_harvest")
+                used += min(c["harvested"], capacity - used)
+            else:
+                m.step(1)
+        return buf[:used]
 
-    def fork(n=None):
-        f = model.setup(model.make_config(**cfgd), cols, fields)
-        f.load_state(state)
-        if n is not None:
-            S.use(f, n)
-        return f
+    def train(pairs):
+        norm = S.fit_norm(pairs)
+        x, t = S.normalise(pairs, norm)
+        net = S.Net(seed=1)
+        net.set_norm(norm)
+        t0 = time.perf_counter()
+        losses = net.train(x, t, steps=args.train_steps, batch=65536, lr=1.0e-3, log_every=2000)
+        return net, norm, losses, time.perf_counter() - t0
 
-    zero = S.Net(seed=2)
-    zero.set_params(np.zeros(zero.nparams, np.float32))
-    zn = norm.copy()
-    zn[16:20] = 0.0
-    zero.set_norm(zn)
-    runs = {"plain": fork(), "surrogate": fork(net), "no_network": fork(zero)}
-    r0 = runs["plain"].scalars()["train"]
-    rain = {}
-    for k, f in runs.items():
-        f.step(60)
-        rain[k] = f.scalars()["train"] - r0
-    c = S.counts(runs["surrogate"])
-    ncell = cfgd["ni"] * cfgd["nj"] * cfgd["nk"]
-    err = abs(rain["surrogate"] - rain["plain"]) / rain["plain"]
-    err0 = abs(rain["no_network"] - rain["plain"]) / rain["plain"]
-    log(f"surrogate: rain over 60 steps plain {rain['plain']:.4e} surrogate {rain['surrogate']:.4e} "
-        f"(err {err:.4f}) no network (err {err0:.4f}); fallback cells {c['fallback_cells']} of {60 * ncell}")
-    for f in runs.values():
-        f.close()
+    def compare(m, state, net, norm, nsteps=60):
+        zero = S.Net(seed=2)
+        zero.set_params(np.zeros(zero.nparams, np.float32))
+        zn = norm.copy()
+        zn[16:20] = 0.0
+        zero.set_norm(zn)
+        out = {}
+        for name, n in (("plain", None), ("surrogate", net), ("no_network", zero)):
+            m.load_state(state)
+            S.use(m, n)
+            r0 = m.scalars()["train"]
+            c0 = S.counts(m)["fallback_cells"]
+            m.step(1)
+            m.sync()
+            t1 = time.perf_counter()
+            m.step(nsteps - 1)
+            m.sync()
+            ms = (time.perf_counter() - t1) * 1000.0 / (nsteps - 1)
+            out[name] = dict(rain=m.scalars()["train"] - r0, ms=ms,
+                             fallback=S.counts(m)["fallback_cells"] - c0)
+        S.use(m, None)
+        zero.close()
+        p = out["plain"]["rain"]
+        for k in out:
+            out[k]["rel_error"] = abs(out[k]["rain"] - p) / p if p > 0 else None
+        return out
 
-    # (b) the 1024x1024x96 supercell step with and without the surrogate
+    # (a) the 1024x1024x96 supercell
     cfg1, cols1, fields1 = workload_config1()
-    m1 = model.setup(model.make_config(**cfg1), cols1, fields1)
+    m = model.setup(model.make_config(**cfg1), cols1, fields1)
     del fields1
-
-    def timed(mm, nsteps):
-        mm.step(1)
-        mm.sync()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        mm.step(nsteps)
-        mm.sync()
-        return (time.perf_counter() - t1) * 1000.0 / nsteps
-
-    ms_plain = timed(m1, args.steps)
-    S.use(m1, net)
-    ms_sur = timed(m1, args.steps)
+    n1 = args.spinup
+    at = {n1 // 2 + q * (n1 - n1 // 2) // 6 for q in range(6)}
+    t0 = time.perf_counter()
+    pairs = harvest_during(m, n1, at, 16_000_000, 50)
+    state = m.save_state()
+    log(f"surrogate (a): {n1} plain steps in {time.perf_counter() - t0:.1f} s, {pairs.shape[0]} pairs")
+    net_a, norm_a, losses_a, tt_a = train(pairs)
+    del pairs
+    log(f"surrogate (a): trained in {tt_a:.1f} s, losses {losses_a}")
+    res_a = compare(m, state, net_a, norm_a)
+    log(f"surrogate (a): {res_a}")
     cells1 = cfg1["ni"] * cfg1["nj"] * cfg1["nk"]
-    log(f"surrogate: 1024x1024x96 step {ms_plain:.2f} ms plain, {ms_sur:.2f} ms with the surrogate")
-    m1.close()
+    ncell1 = cells1 * 60
+    m.close()
+    del state
 
-    # (c) ensemble: batched with the surrogate vs one by one with plain Kessler
+    # (b) the 128x128x40 case and the ensemble
+    cfgd, cols, fields = workload_spinup()
+    mb = model.setup(model.make_config(**cfgd), cols, fields)
+    nb = 300
+    pairs = harvest_during(mb, nb, set(range(nb // 2, nb)), 12_000_000, 20)
+    state = mb.save_state()
+    net_b, norm_b, losses_b, tt_b = train(pairs)
+    del pairs
+    res_b = compare(mb, state, net_b, norm_b)
+    log(f"surrogate (b): {res_b}")
+    mb.close()
     work = workload_ensemble(48)
     members = [model.setup(model.make_config(**c_), co, fi) for c_, co, fi in work]
     del work
     cells_e = sum(mm.ni * mm.nj * mm.nk for mm in members)
-    L = N.lib()
     for mm in members:
         N.check(L.cm1g_step(mm.h, None), "cm1g_step")
     for mm in members:
@@ -270,7 +288,7 @@ def bench_surrogate(args):
         mm.sync()
     ms_seq = (time.perf_counter() - t1) * 1000.0 / args.steps
     for mm in members:
-        S.use(mm, net)
+        S.use(mm, net_b)
     e = model.Ensemble(members)
     e.step(1)
     e.sync()
@@ -279,13 +297,14 @@ def bench_surrogate(args):
     e.sync()
     ms_ens = (time.perf_counter() - t1) * 1000.0 / args.steps
     e.close()
-    log(f"surrogate: ensemble {ms_seq:.2f} ms one by one (plain), {ms_ens:.2f} ms batched with the surrogate")
+    log(f"surrogate (b): ensemble {ms_seq:.2f} ms one by one (plain), {ms_ens:.2f} ms batched with the surrogate")
+    ms_sur = res_a["surrogate"]["ms"]
     line = {
         "metric": "cell-updates/s per RK3 step (FP32), Kessler surrogate in place of kessler",
         "value": cells1 / (ms_sur / 1000.0),
         "unit": "cell-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": 59,
         "warmup": 1,
         "ms_per_step": ms_sur,
         "higher_is_better": True,
@@ -293,15 +312,12 @@ def bench_surrogate(args):
         "vs_baseline": None,
         "dtype": "f32 (network: bf16 operands, f32 accumulation)",
         "data": "synthetic",
-        "config": {"workload": "supercell 1024x1024x96 (as the main bench) with the surrogate; rain test "
-                               "on a spun-up 128x128x40 case; 48-member ensemble",
-                   "ms_per_step_plain_kessler": round(ms_plain, 3),
-                   "harvest_steps": args.spinup, "train_steps": args.train_steps, "batch": 65536,
-                   "train_seconds": round(t_train, 2), "final_loss": losses[-1][1] if losses else None,
-                   "rain_60_steps": {k: v for k, v in rain.items()},
-                   "rain_rel_error_surrogate": round(err, 5),
-                   "rain_rel_error_no_network": round(err0, 5),
-                   "fallback_cell_fraction": round(c["fallback_cells"] / (60.0 * ncell), 5),
+        "config": {"workload": f"supercell 1024x1024x96 (as the main bench) after {n1} plain steps, 60 steps "
+                               "with the surrogate; plus a 128x128x40 case and a 48-member ensemble",
+                   "train_steps": args.train_steps, "batch": 65536,
+                   "a_1024": {"train_seconds": round(tt_a, 2), "final_loss": losses_a[-1][1],
+                              "runs": res_a, "fallback_fraction": res_a["surrogate"]["fallback"] / ncell1},
+                   "b_128": {"train_seconds": round(tt_b, 2), "final_loss": losses_b[-1][1], "runs": res_b},
                    "ensemble_cells": cells_e,
                    "ensemble_ms_per_step_one_by_one_plain": round(ms_seq, 3),
                    "ensemble_ms_per_step_batched_surrogate": round(ms_ens, 3),
@@ -440,7 +456,8 @@ def main():
                     help="benchmark the dry convective-boundary-layer workload (1536x1536x128) instead")
     ap.add_argument("--surrogate", action="store_true",
                     help="train and benchmark the learned Kessler surrogate instead")
-    ap.add_argument("--spinup", type=int, default=300, help="plain steps harvested for the surrogate")
+    ap.add_argument("--spinup", type=int, default=1000,
+                    help="plain 1024x1024x96 steps before the surrogate comparison")
     ap.add_argument("--train-steps", type=int, default=10000, help="Adam steps for the surrogate")
     args = ap.parse_args()
     if args.gpus != 1:
