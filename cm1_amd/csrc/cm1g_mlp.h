@@ -4,6 +4,8 @@
 #include "cm1g_internal.h"
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8s __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int MLP_IN = 8, MLP_H = 128, MLP_OUT = 4;
@@ -30,7 +32,9 @@ __device__ __forceinline__ unsigned short bf16_rne(float f) {
 // use the same form)
 __device__ __forceinline__ float silu_f(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
-// Fused inference.  Each layer is computed transposed, Y^T = W^T X^T: the
+// Fused inference (the first layer with v_mfma_f32_16x16x16_bf16, the
+// others with v_mfma_f32_16x16x32_bf16).  Each layer is computed
+// transposed, Y^T = W^T X^T: the
 // weight block is the first matrix operand and the 16-sample activation
 // block the second, so a lane's result, Y(sample l%16, outputs 16nt +
 // 4(l/16) + t), is exactly the second operand the next layer needs for
@@ -40,14 +44,14 @@ __device__ __forceinline__ float silu_f(float z) { return z * __builtin_amdgcn_r
 // lane l at ((nt*KS + ks)*64 + l)*4, one 8-byte load per lane).
 constexpr int MLP_WAVES = 8;          // wavefronts per workgroup (512 threads)
 constexpr int MLP_G = 1;              // 16-sample groups per wavefront and round
-constexpr int MLP_F1 = 8 * 1 * 256;   // layer 1: 8 output tiles x 1 k-step x 64 lanes x 4
-constexpr int MLP_FH = 8 * 8 * 256;   // hidden layers: 8 x 8
-constexpr int MLP_F4 = 1 * 8 * 256;   // output layer: 1 (4 of 16 outputs used) x 8
+constexpr int MLP_F1 = 8 * 1 * 256;   // layer 1: 8 output tiles x 1 k-step of 16 x 64 lanes x 4
+constexpr int MLP_FH = 8 * 4 * 512;   // hidden layers: 8 x 4 k-steps of 32 x 64 lanes x 8
+constexpr int MLP_F4 = 1 * 4 * 512;   // output layer: 1 (4 of 16 outputs used) x 4
 struct MlpLds {
   bf16x4 w1[MLP_F1 / 4];
-  bf16x4 w2[MLP_FH / 4];
-  bf16x4 w3[MLP_FH / 4];
-  bf16x4 w4[MLP_F4 / 4];
+  bf16x8s w2[MLP_FH / 8];
+  bf16x8s w3[MLP_FH / 8];
+  bf16x8s w4[MLP_F4 / 8];
   float b[3 * MLP_H + 16];
 };
 
@@ -56,53 +60,97 @@ __device__ __forceinline__ int mlp_wfrag(int k, int n, int ks_count) {
   return (((n >> 4) * ks_count + (k >> 4)) * 64 + (n & 15) + 16 * ((k & 15) >> 2)) * 4 + (k & 3);
 }
 
+// The 32-deep instruction (v_mfma_f32_16x16x32_bf16) reads lane l's 8
+// second-operand values as k = 8(l/16) + p, p = 0..7.  A lane holds the
+// previous layer's outputs 16nt + 4(l/16) + t for two consecutive tiles,
+// i.e. inputs 4(l/16) + t and 16 + 4(l/16) + t of a 32-input block: so the
+// weights are staged with the same permutation of k, and position p of
+// lane l's fragment is input 4(l/16) + p (p < 4) or 16 + 4(l/16) + p - 4.
+__device__ __forceinline__ int mlp_wfrag32(int k, int n, int kb_count) {
+  const int kl = k & 31;
+  const int g = (kl & 15) >> 2, p = (kl & 3) + (kl >= 16 ? 4 : 0);
+  return (((n >> 4) * kb_count + (k >> 5)) * 64 + (n & 15) + 16 * g) * 8 + p;
+}
+
+// four floats to BF16 (round to nearest, ties to even: the hardware's
+// packed conversion, the same rounding as bf16_rne for finite values)
 __device__ __forceinline__ bf16x4 mlp_pack(float a, float b, float c, float d) {
   bf16x4 v;
-  v[0] = (short)bf16_rne(a);
-  v[1] = (short)bf16_rne(b);
-  v[2] = (short)bf16_rne(c);
-  v[3] = (short)bf16_rne(d);
+  v[0] = __builtin_bit_cast(short, (__bf16)a);
+  v[1] = __builtin_bit_cast(short, (__bf16)b);
+  v[2] = __builtin_bit_cast(short, (__bf16)c);
+  v[3] = __builtin_bit_cast(short, (__bf16)d);
   return v;
 }
 
-// one hidden layer for MLP_G groups: x[g][ks] -> x[g][nt] = SiLU(W^T x + b)
-template <int KS>
-__device__ __forceinline__ void mlp_hidden(bf16x4 (&x)[MLP_G][8], const bf16x4 *wf, const float *bias, int l) {
+// accumulators start from the bias of the outputs each lane receives
+__device__ __forceinline__ void mlp_bias_init(floatx4 (&acc)[MLP_G][8], const float *bias, int l) {
+#pragma unroll
+  for (int nt = 0; nt < 8; nt++) {
+    const int c0 = 16 * nt + 4 * (l >> 4);
+    const floatx4 b = {bias[c0], bias[c0 + 1], bias[c0 + 2], bias[c0 + 3]};
+#pragma unroll
+    for (int g = 0; g < MLP_G; g++) acc[g][nt] = b;
+  }
+}
+__device__ __forceinline__ void mlp_silu_out(const floatx4 (&acc)[MLP_G][8], bf16x4 (&x)[MLP_G][8]) {
+#pragma unroll
+  for (int nt = 0; nt < 8; nt++)
+#pragma unroll
+    for (int g = 0; g < MLP_G; g++)
+      x[g][nt] = mlp_pack(silu_f(acc[g][nt][0]), silu_f(acc[g][nt][1]), silu_f(acc[g][nt][2]),
+                          silu_f(acc[g][nt][3]));
+}
+// lane l's 8 inputs of 32-input block kb: outputs of tiles 2kb and 2kb+1
+__device__ __forceinline__ bf16x8 mlp_join(const bf16x4 &lo, const bf16x4 &hi) {
+  const bf16x8s v = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+// first layer (8 inputs padded to one 16-deep step): x[g][0] -> x[g][nt]
+__device__ __forceinline__ void mlp_first(bf16x4 (&x)[MLP_G][8], const bf16x4 *wf, const float *bias, int l) {
   floatx4 acc[MLP_G][8];
+  mlp_bias_init(acc, bias, l);
 #pragma unroll
-  for (int g = 0; g < MLP_G; g++)
+  for (int nt = 0; nt < 8; nt++) {
+    const bf16x4 w = wf[nt * 64 + l];
 #pragma unroll
-    for (int nt = 0; nt < 8; nt++) acc[g][nt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-  // the next k-step's weight fragments load while this one's matrix
-  // instructions run; the scheduling barrier keeps the compiler from
-  // hoisting every k-step's loads at once (which would exceed the registers)
-  bf16x4 wc[8], wn[8];
+    for (int g = 0; g < MLP_G; g++)
+      acc[g][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w, x[g][0], acc[g][nt], 0, 0, 0);
+  }
+  mlp_silu_out(acc, x);
+}
+
+// a hidden layer (128 -> 128) in four 32-deep steps.  The next step's
+// weight fragments load while this one's matrix instructions run; the
+// scheduling barrier keeps the compiler from hoisting every step's loads at
+// once (which would exceed the registers).
+__device__ __forceinline__ void mlp_hidden(bf16x4 (&x)[MLP_G][8], const bf16x8s *wf, const float *bias, int l) {
+  floatx4 acc[MLP_G][8];
+  mlp_bias_init(acc, bias, l);
+  bf16x8s wc[8], wn[8];
 #pragma unroll
-  for (int nt = 0; nt < 8; nt++) wc[nt] = wf[(nt * KS) * 64 + l];
+  for (int nt = 0; nt < 8; nt++) wc[nt] = wf[(nt * 4) * 64 + l];
 #pragma unroll
-  for (int ks = 0; ks < KS; ks++) {
-    if (ks + 1 < KS) {
+  for (int kb = 0; kb < 4; kb++) {
+    if (kb + 1 < 4) {
 #pragma unroll
-      for (int nt = 0; nt < 8; nt++) wn[nt] = wf[(nt * KS + ks + 1) * 64 + l];
+      for (int nt = 0; nt < 8; nt++) wn[nt] = wf[(nt * 4 + kb + 1) * 64 + l];
     }
+    bf16x8 xb[MLP_G];
+#pragma unroll
+    for (int g = 0; g < MLP_G; g++) xb[g] = mlp_join(x[g][2 * kb], x[g][2 * kb + 1]);
 #pragma unroll
     for (int nt = 0; nt < 8; nt++)
 #pragma unroll
       for (int g = 0; g < MLP_G; g++)
-        acc[g][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wc[nt], x[g][ks], acc[g][nt], 0, 0, 0);
+        acc[g][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, wc[nt]), xb[g],
+                                                              acc[g][nt], 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int nt = 0; nt < 8; nt++) wc[nt] = wn[nt];
   }
-#pragma unroll
-  for (int nt = 0; nt < 8; nt++) {
-    const int c0 = 16 * nt + 4 * (l >> 4);
-    const float b0 = bias[c0], b1 = bias[c0 + 1], b2 = bias[c0 + 2], b3 = bias[c0 + 3];
-#pragma unroll
-    for (int g = 0; g < MLP_G; g++)
-      x[g][nt] = mlp_pack(silu_f(acc[g][nt][0] + b0), silu_f(acc[g][nt][1] + b1), silu_f(acc[g][nt][2] + b2),
-                          silu_f(acc[g][nt][3] + b3));
-  }
+  mlp_silu_out(acc, x);
 }
 
 // Run the network over samples 0..n-1.  load(s, r) fills r[MLP_KEEP] with
@@ -128,12 +176,12 @@ __device__ void mlp_tiles(const float *__restrict__ prm, int64_t n, const LOAD &
   }
   for (int e = tid; e < MLP_H * MLP_H; e += 512) {
     const int k = e / MLP_H, nn = e % MLP_H;
-    w2[mlp_wfrag(k, nn, 8)] = bf16_rne(prm[OW2 + e]);
-    w3[mlp_wfrag(k, nn, 8)] = bf16_rne(prm[OW3 + e]);
+    w2[mlp_wfrag32(k, nn, 4)] = bf16_rne(prm[OW2 + e]);
+    w3[mlp_wfrag32(k, nn, 4)] = bf16_rne(prm[OW3 + e]);
   }
   for (int e = tid; e < MLP_H * 16; e += 512) {  // output layer, outputs padded to 16
     const int k = e / 16, nn = e % 16;
-    w4[mlp_wfrag(k, nn, 8)] = nn < MLP_OUT ? bf16_rne(prm[OW4 + k * MLP_OUT + nn]) : (unsigned short)0;
+    w4[mlp_wfrag32(k, nn, 4)] = nn < MLP_OUT ? bf16_rne(prm[OW4 + k * MLP_OUT + nn]) : (unsigned short)0;
   }
   for (int e = tid; e < 3 * MLP_H + 16; e += 512) {
     float v = 0.0f;
@@ -172,20 +220,21 @@ __device__ void mlp_tiles(const float *__restrict__ prm, int64_t n, const LOAD &
     }
     float rn[MLP_G][MLP_KEEP];
     fetch(base + stride, rn);
-    mlp_hidden<1>(x, S.w1, S.b, l);
-    mlp_hidden<8>(x, S.w2, S.b + MLP_H, l);
-    mlp_hidden<8>(x, S.w3, S.b + 2 * MLP_H, l);
+    mlp_first(x, S.w1, S.b, l);
+    mlp_hidden(x, S.w2, S.b + MLP_H, l);
+    mlp_hidden(x, S.w3, S.b + 2 * MLP_H, l);
     // output layer: one tile of 16 outputs (4 used); lane l < 16 ends up
     // with outputs 0..3 of its sample
     floatx4 acc[MLP_G];
 #pragma unroll
     for (int g = 0; g < MLP_G; g++) acc[g] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int ks = 0; ks < 8; ks++) {
-      const bf16x4 wv = S.w4[ks * 64 + l];
+    for (int kb = 0; kb < 4; kb++) {
+      const bf16x8 wv = __builtin_bit_cast(bf16x8, S.w4[kb * 64 + l]);
 #pragma unroll
       for (int g = 0; g < MLP_G; g++)
-        acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wv, x[g][ks], acc[g], 0, 0, 0);
+        acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv, mlp_join(x[g][2 * kb], x[g][2 * kb + 1]), acc[g],
+                                                         0, 0, 0);
     }
 #pragma unroll
     for (int g = 0; g < MLP_G; g++) {
