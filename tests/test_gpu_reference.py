@@ -7,7 +7,7 @@ contraction and evaluate exp/pow with the host C library's method
 (cm1g_libm.h), so every prognostic field (u, v, w, theta', pi', qv, qc, qr,
 rho, prs, and the passive tracers where present) must equal the reference
 build's output bit for bit; the relative L2 distance (0) is printed beside
-the north-star bound of 1e-5.
+the 1e-5 tolerance a single-precision port is held to.
 """
 import glob
 import os
