@@ -87,6 +87,11 @@ struct cm1g_model {
   std::vector<StepGraph> graphs;
   int no_graphs;                // 1: cm1g_step launches kernel by kernel
   int tile_rows;                // rows of 64 columns per workgroup of the column kernels (1, 2 or 4)
+  // 1 while ua, va, wa, ppi equal u3d, v3d, w3d, pp3d wherever the first
+  // acoustic substep reads them (set by solve3 and getset, cleared by
+  // anything that may write either): the first RK stage's acoustic
+  // substeps may then read the a-arrays directly
+  int synced;
   // non-null while an ensemble records this member's step: launches are
   // appended here instead of run
   std::vector<struct LaunchRec> *rec;
@@ -288,7 +293,7 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
               const float *sten0, const float *thterm_div, float *out_sten, float *out_new,
               int pdef, int pdefweno, float dt, double weps, int smeps_flush);
 // acoustic (cm1g_sound.hip)
-void sound_run(cm1g_model *m, int nrk);
+void sound_run(cm1g_model *m, int nrk, bool fold1 = false);
 void sounde_run(cm1g_model *m, int nrk);
 void getdiv_both(cm1g_model *m, int want_t11, int want_divx = 1, float *div = nullptr);
 void scale_mass_fluxes(cm1g_model *m);

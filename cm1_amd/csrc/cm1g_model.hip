@@ -361,6 +361,7 @@ static float *box_origin(const cm1g_model *m, int id) { return m->f[id] + (CM1G_
 
 extern "C" int cm1g_put_field(cm1g_model *m, int id, const float *host) {
   ARGCHK(field_ok(m, id) && host, "cm1g_put_field: bad field id or null pointer");
+  m->synced = 0;
   const size_t w = (size_t)m->P.ni + 7, rows = (size_t)m->P.py * (size_t)(m->P.nk + 3);
   HIPCHK(hipStreamSynchronize(m->stream));
   HIPCHK(hipMemcpy2D(box_origin(m, id), (size_t)m->P.px * sizeof(float), host, w * sizeof(float),
@@ -391,6 +392,7 @@ static int array_shape(const cm1g_model *m, int stag, size_t *nx, size_t *ny, si
 
 extern "C" int cm1g_put_array(cm1g_model *m, int id, int stag, const float *host) {
   ARGCHK(field_ok(m, id) && host, "cm1g_put_array: bad field id or null pointer");
+  m->synced = 0;
   size_t nx, ny, nz;
   ARGCHK(array_shape(m, stag, &nx, &ny, &nz) == 0, "cm1g_put_array: bad staggering");
   HIPCHK(hipStreamSynchronize(m->stream));
@@ -504,6 +506,7 @@ extern "C" int cm1g_state_save(cm1g_model *m, void *host, int64_t nbytes) {
 
 extern "C" int cm1g_state_load(cm1g_model *m, const void *host, int64_t nbytes) {
   ARGCHK(m && host, "cm1g_state_load: null argument");
+  m->synced = 0;
   int64_t need = 0;
   cm1g_state_bytes(m, &need);
   ARGCHK(nbytes >= need, "cm1g_state_load: buffer too small");
@@ -531,6 +534,7 @@ extern "C" int cm1g_state_load(cm1g_model *m, const void *host, int64_t nbytes) 
 
 extern "C" int cm1g_field_ptr(cm1g_model *m, int id, float **dptr) {
   ARGCHK(field_ok(m, id) && dptr, "cm1g_field_ptr: bad field id");
+  m->synced = 0;
   *dptr = m->f[id];
   return 0;
 }
@@ -718,6 +722,7 @@ extern "C" int cm1g_getset(cm1g_model *m, int fresh_start) {
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(m->stream));
   HIPCHK(hipMemcpy(&m->mass1, m->dscal, sizeof(double), hipMemcpyDeviceToHost));
+  m->synced = 1;
   return 0;
 }
 
@@ -869,10 +874,13 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
     float *th3d = f[CM1G_TH3D];
     const float *qv = f[CM1G_Q3D0], *qc = f[CM1G_Q3D1], *qr = f[CM1G_Q3D2];
     const float dttmp = dt / (float)(3 + 1 - nrk);
-    // thermodynamic prep (src/solve2.F:302-450)
-    for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+    // thermodynamic prep (src/solve2.F:302-450), with sadv = pi0 + pi'
+    // (src/solve2.F:1027-1050) over the interior and its 1-wide ring
+    for3(m, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
       const Consts &K = P.K;
       int c = IXD(P, i, j, k);
+      sadv[c] = P.pi0[k] + pp3d[c];
+      if (i < 1 || i > ni || j < 1 || j > nj) return;
       float th = th3d[c];
       if (moist) {
         float ql = qc[c];
@@ -903,13 +911,8 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
     // tendencies: the pre-RK values (src/solve2.F:459-471) plus advection,
     // each component's own mass flux formed from its velocity
     adv_uvw(m, nrk, 7, 1, 1, 1);
-    // sadv = pi0 + pi' (src/solve2.F:1027-1050)
-    for3(m, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
-      int c = IXD(P, i, j, k);
-      sadv[c] = P.pi0[k] + pp3d[c];
-    });
     if (m->c.psolver == 2) sounde_run(m, nrk);
-    else sound_run(m, nrk);
+    else sound_run(m, nrk, nrk == 1 && m->synced);
     if (nrk == 3) cfl_run(m, dt, u3d, v3d, w3d);
     if (m->c.irbc == 4 && (P.ibw || P.ibe || P.ibs || P.ibn))
       radbc4(m, f[CM1G_UA], u3d, f[CM1G_VA], v3d, dttmp);
@@ -996,12 +999,14 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
     if (m->c.pdtra == 1) pdefq_run(m, 0.0f, rho, f[CM1G_PT3D0 + n]);
     bc_fill(m, BC_S, f[CM1G_PT3D0 + n]);
   }
+  m->synced = 0;
   return check_launch();
 }
 
 // --------------------------------------------------------------- mp_driver
 extern "C" int cm1g_mp_driver(cm1g_model *m) {
   ARGCHK(m, "cm1g_mp_driver: null model");
+  m->synced = 0;
   if (m->c.imoist != 1) return 0;
   if (m->c.ptype == 0) {
     pdefq_run(m, 0.0f, m->f[CM1G_RHO], m->f[CM1G_Q3D0]);
@@ -1085,6 +1090,8 @@ extern "C" int cm1g_solve3(cm1g_model *m) {
       pta[IXD(P, i, j, k)] = pt3d[IXD(P, i, j, k)];
     });
   }
+  // ua..qa, pta now equal u3d..q3d, pt3d where the next stage reads them
+  m->synced = 1;
   return check_launch();
 }
 
@@ -1096,16 +1103,18 @@ int step_body(cm1g_model *m) {
   return rc;
 }
 
-// the buffer arrangement a step starts from: every field pointer and the
-// scalar updates' partner
+// the buffer arrangement a step starts from: every field pointer, the
+// scalar updates' partner, and whether the a-arrays equal the 3d-arrays
 std::vector<float *> field_map(const cm1g_model *m) {
   std::vector<float *> v(m->f, m->f + CM1G_NFIELDS);
   v.push_back(m->thb);
+  v.push_back((float *)(uintptr_t)m->synced);  // decides the first acoustic stage's launches
   return v;
 }
 void set_field_map(cm1g_model *m, const std::vector<float *> &v) {
   for (int i = 0; i < CM1G_NFIELDS; i++) m->f[i] = v[i];
   m->thb = v[CM1G_NFIELDS];
+  m->synced = (int)(uintptr_t)v[CM1G_NFIELDS + 1];
 }
 
 void graphs_clear(cm1g_model *m) {
@@ -1183,6 +1192,7 @@ extern "C" int cm1g_step(cm1g_model *m, float *cflmax) {
 extern "C" int cm1g_advs(cm1g_model *m, int nrk, int s_id, int s0_id, int sten_id, int pdef,
                          int pdefweno, float dt, double weps) {
   ARGCHK(field_ok(m, s_id) && field_ok(m, s0_id) && field_ok(m, sten_id), "cm1g_advs: bad field id");
+  m->synced = 0;
   ARGCHK(nrk >= 1 && nrk <= 3, "cm1g_advs: nrk must be 1..3");
   // the model's rru/rrv/rrw are used as given (no rho0 scaling here)
   float *sten = m->f[sten_id];
@@ -1209,6 +1219,7 @@ extern "C" int cm1g_advw(cm1g_model *m, int nrk) {
 }
 extern "C" int cm1g_sound(cm1g_model *m, int nrk, float dttmp) {
   ARGCHK(m && nrk >= 1 && nrk <= 3, "cm1g_sound: bad argument");
+  m->synced = 0;
   (void)dttmp;
   if (m->c.psolver == 2) sounde_run(m, nrk);
   else sound_run(m, nrk);
@@ -1216,6 +1227,7 @@ extern "C" int cm1g_sound(cm1g_model *m, int nrk, float dttmp) {
 }
 extern "C" int cm1g_kessler_branch(cm1g_model *m) {
   ARGCHK(m, "null model");
+  m->synced = 0;
   kessler_branch_run(m);
   train_accumulate(m);
   return check_launch();
@@ -1227,6 +1239,7 @@ extern "C" int cm1g_pdefq(cm1g_model *m, float rmax, int q_id) {
 }
 extern "C" int cm1g_bc(cm1g_model *m, int id, int kind) {
   ARGCHK(field_ok(m, id) && kind >= 0 && kind <= 5, "cm1g_bc: bad argument");
+  m->synced = 0;
   bc_fill(m, kind, m->f[id]);
   return check_launch();
 }

@@ -314,7 +314,7 @@ __device__ __forceinline__ void sound_sub(const KP &P, const SubArgs &a, const C
   }
 }
 
-void sound_run(cm1g_model *m, int nrk) {
+void sound_run(cm1g_model *m, int nrk, bool fold1) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   const float dt = m->c.dt, alph = m->c.alph, kdiv = m->c.kdiv;
@@ -372,7 +372,10 @@ void sound_run(cm1g_model *m, int nrk) {
   // averages itself.  u, v and ppd alternate between the state arrays and
   // scratch arrays so that neighbouring columns read the old values; the
   // last substep writes u3d/v3d.
-  const bool fold = (nrk >= 2);
+  // (at the first stage u3d..pp3d equal ua..ppi where the substep reads
+  // them when the caller says so, fold1: the folded form then needs no
+  // copy of the scratch pair back into u3d/v3d)
+  const bool fold = (nrk >= 2) || fold1;
   a.wa = wa;
   const ColGrid cg = col_grid(ni, nj, m->tile_rows);
   const dim3 blk = col_threads(cg), grd = col_blocks(cg);
