@@ -28,6 +28,44 @@ void pdefq_run(cm1g_model *m, float rmax, const float *rho, float *q) {
   });
 }
 
+// pdefq of qv (rmax 0), qc and qr (rmax qsmall) in one column pass: the
+// three species' double-precision sums run side by side, each in its own k
+// order, so every species gets exactly its own pdefq's result while rho is
+// read once
+static void pdefq3_run(cm1g_model *m, const float *rho, float *qv, float *qc, float *qr) {
+  const KP P = m->P;
+  const int nk = P.nk;
+  const float rc = P.K.qsmall;
+  for2(m, 1, P.ni, 1, P.nj, [=] __device__(int i, int j) {
+    double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0, c1 = 0.0, c2 = 0.0;
+    for (int k = 1; k <= nk; k++) {
+      const int c = IXD(P, i, j, k);
+      const float r = rho[c];
+      float v = qv[c], w = qc[c], x = qr[c];
+      a1 = a1 + (double)(r * v);
+      b1 = b1 + (double)(r * w);
+      c1 = c1 + (double)(r * x);
+      if (v < 0.0f) { v = 0.0f; qv[c] = 0.0f; }
+      if (w < rc) { w = 0.0f; qc[c] = 0.0f; }
+      if (x < rc) { x = 0.0f; qr[c] = 0.0f; }
+      a2 = a2 + (double)(r * v);
+      b2 = b2 + (double)(r * w);
+      c2 = c2 + (double)(r * x);
+    }
+    double ta = (a1 + 1.0e-20) / (a2 + 1.0e-20), tb = (b1 + 1.0e-20) / (b2 + 1.0e-20),
+           tc = (c1 + 1.0e-20) / (c2 + 1.0e-20);
+    if (ta < 0.0) ta = 1.0;
+    if (tb < 0.0) tb = 1.0;
+    if (tc < 0.0) tc = 1.0;
+    for (int k = 1; k <= nk; k++) {
+      const int c = IXD(P, i, j, k);
+      qv[c] = (float)(ta * (double)qv[c]);
+      qc[c] = (float)(tb * (double)qc[c]);
+      qr[c] = (float)(tc * (double)qr[c]);
+    }
+  });
+}
+
 // esl with the 171023 cap, and the saturation mixing ratio
 __device__ __forceinline__ float esl_of(float t, float prs) {
   float esl = 611.2f * exp_f(17.67f * (t - 273.15f) / (t - 29.65f));
@@ -359,9 +397,7 @@ void kessler_branch_run(cm1g_model *m) {
   // Kessler branch these feed only budget diagnostics (geterain, the rr
   // argument of kessler/satadj) and the other schemes' tendency checks, and
   // kessler recomputes T itself, so nothing on this path reads them.
-  pdefq_run(m, 0.0f, rho, qv);
-  pdefq_run(m, P.K.qsmall, rho, qc);
-  pdefq_run(m, P.K.qsmall, rho, qr);
+  pdefq3_run(m, rho, qv, qc, qr);
   // k_fallout
   for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
     int c = IXD(P, i, j, k);
@@ -417,6 +453,16 @@ void kessler_branch_run(cm1g_model *m) {
       b4[c] = pp3d[c];
       b5[c] = prs[c];
     });
+  }
+  if (!m->sur && !m->harv) {
+    // kessler then satadj (nrk=4) at each cell: both are pointwise, so one
+    // pass reads the cell's state once
+    for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      const int c = IXD(P, i, j, k);
+      kessler_cell(P, dt, c, k, th3d, pp3d, prs, qv, qc, qr, rho);
+      satadj_cell(P, c, k, th3d, pp3d, prs, qv, qc, qr, rho, iflag);
+    });
+    return;
   }
   if (m->sur) {
     surrogate_kessler(m);
