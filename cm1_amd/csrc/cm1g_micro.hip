@@ -16,11 +16,17 @@ void pdefq_run(cm1g_model *m, float rmax, const float *rho, float *q) {
       int c = IXD(P, i, j, k);
       float qq = q[c], r = rho[c];
       t1 = t1 + (double)(r * qq);
-      if (qq < rmax) { qq = 0.0f; q[c] = 0.0f; }
+      // a value that is +0 already is not stored again
+      if (qq < rmax) {
+        if (__float_as_uint(qq) != 0u) q[c] = 0.0f;
+        qq = 0.0f;
+      }
       t2 = t2 + (double)(r * qq);
     }
     double t3 = (t1 + 1.0e-20) / (t2 + 1.0e-20);
     if (t3 < 0.0) t3 = 1.0;
+    // a factor of exactly 1 leaves every value unchanged bit for bit
+    if (t3 == 1.0) return;
     for (int k = 1; k <= nk; k++) {
       int c = IXD(P, i, j, k);
       q[c] = (float)(t3 * (double)q[c]);
@@ -45,9 +51,16 @@ static void pdefq3_run(cm1g_model *m, const float *rho, float *qv, float *qc, fl
       a1 = a1 + (double)(r * v);
       b1 = b1 + (double)(r * w);
       c1 = c1 + (double)(r * x);
+      // a value that is +0 already is not stored again
       if (v < 0.0f) { v = 0.0f; qv[c] = 0.0f; }
-      if (w < rc) { w = 0.0f; qc[c] = 0.0f; }
-      if (x < rc) { x = 0.0f; qr[c] = 0.0f; }
+      if (w < rc) {
+        if (__float_as_uint(w) != 0u) qc[c] = 0.0f;
+        w = 0.0f;
+      }
+      if (x < rc) {
+        if (__float_as_uint(x) != 0u) qr[c] = 0.0f;
+        x = 0.0f;
+      }
       a2 = a2 + (double)(r * v);
       b2 = b2 + (double)(r * w);
       c2 = c2 + (double)(r * x);
@@ -57,11 +70,15 @@ static void pdefq3_run(cm1g_model *m, const float *rho, float *qv, float *qc, fl
     if (ta < 0.0) ta = 1.0;
     if (tb < 0.0) tb = 1.0;
     if (tc < 0.0) tc = 1.0;
+    // a factor of exactly 1 (nothing clipped in the column, the usual case)
+    // leaves every value unchanged bit for bit: that species is skipped
+    const bool sa = ta != 1.0, sb = tb != 1.0, sc = tc != 1.0;
+    if (!(sa || sb || sc)) return;
     for (int k = 1; k <= nk; k++) {
       const int c = IXD(P, i, j, k);
-      qv[c] = (float)(ta * (double)qv[c]);
-      qc[c] = (float)(tb * (double)qc[c]);
-      qr[c] = (float)(tc * (double)qr[c]);
+      if (sa) qv[c] = (float)(ta * (double)qv[c]);
+      if (sb) qc[c] = (float)(tb * (double)qc[c]);
+      if (sc) qr[c] = (float)(tc * (double)qr[c]);
     }
   });
 }
@@ -398,18 +415,20 @@ void kessler_branch_run(cm1g_model *m) {
   // argument of kessler/satadj) and the other schemes' tendency checks, and
   // kessler recomputes T itself, so nothing on this path reads them.
   pdefq3_run(m, rho, qv, qc, qr);
-  // k_fallout
-  for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
-    int c = IXD(P, i, j, k);
-    float v = 0.0f;
-    if (qr[c] > 1.0e-12f) v = 14.34f * pow_f(rho[c] * qr[c], 0.1346f) * sqrtf(1.15f / rho[c]);
-    vr[c] = v;
-  });
-  // fallout: per column, variable number of sedimentation substeps
+  // fallout: per column, the fall speed (k_fallout, src/kessler.F:368-396)
+  // and its maximum Courant number in one upward pass, then a variable
+  // number of sedimentation substeps
   for2(m, 1, ni, 1, nj, [=] __device__(int i, int j) {
     const float rdz = P.rdz;
     float crmax = 0.0f;
-    for (int k = 1; k <= nk; k++) crmax = fmaxf(crmax, vr[IXD(P, i, j, k)] * dt * rdz * P.mh[k]);
+    for (int k = 1; k <= nk; k++) {
+      const int c = IXD(P, i, j, k);
+      const float q = qr[c], r = rho[c];
+      float v = 0.0f;
+      if (q > 1.0e-12f) v = 14.34f * pow_f(r * q, 0.1346f) * sqrtf(1.15f / r);
+      vr[c] = v;
+      crmax = fmaxf(crmax, v * dt * rdz * P.mh[k]);
+    }
     int nfall = (int)(crmax + 1.0f);
     if (nfall < 1) nfall = 1;
     for (int k = nk - 1; k >= 1; k--)
