@@ -162,6 +162,53 @@ def test_pdefq(require_gpu):
         check(g, o, ["q3d1"])
 
 
+def _signed_zero_columns(shape, lo, hi, seed):
+    """Columns of four kinds, side by side in i: all values well inside
+    [lo, hi] (nothing clipped, the rescale factor is exactly 1), +0 and -0
+    mixed with positive values, a few small negatives among positive values,
+    and values straddling the clipping threshold."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(lo, hi, size=shape).astype(np.float32)
+    kind = np.arange(shape[2]) % 4
+    z = rng.random(shape) < 0.5
+    a[:, :, kind == 1] = np.where(z[:, :, kind == 1], np.float32(0.0), a[:, :, kind == 1])
+    sgn = rng.random(shape) < 0.5
+    a[:, :, kind == 1] = np.where(sgn[:, :, kind == 1] & z[:, :, kind == 1], np.float32(-0.0), a[:, :, kind == 1])
+    neg = rng.random(shape) < 0.05
+    a[:, :, kind == 2] = np.where(neg[:, :, kind == 2], -a[:, :, kind == 2] * np.float32(1.0e-3), a[:, :, kind == 2])
+    a[:, :, kind == 3] = rng.uniform(-1.0e-14, 3.0e-14, size=a[:, :, kind == 3].shape).astype(np.float32)
+    return a
+
+
+def test_pdefq_signed_zeros_and_unclipped_columns(require_gpu):
+    """pdefq stores only what changes (a +0 is not stored again, a column
+    whose rescale factor is exactly 1 is not rewritten): the result still
+    equals the oracle's bit for bit, -0 included."""
+    cfgd, g, o = pair()
+    a = _signed_zero_columns((o.pz, o.py, o.px), 1.0e-5, 1.0e-3, 21)
+    for rmax in (0.0, 1.0e-14, 2.0e-5):
+        o.f("q3d1")[:] = a
+        g.put("q3d1", a)
+        gcall(g, "cm1g_pdefq", C.c_float(rmax), N.FID["q3d1"])
+        o.L.cm1o_pdefq(o.h, C.c_float(rmax), pyoracle.FID["q3d1"])
+        check(g, o, ["q3d1"])
+
+
+def test_kessler_branch_signed_zeros(require_gpu):
+    """The Kessler branch's three-species pdefq pass on columns with -0,
+    +0, tiny negatives and unclipped values, bit for bit."""
+    cfgd, g, o = pair(ni=24, nj=20, nk=12)
+    for n, seed in (("q3d0", 1), ("q3d1", 2), ("q3d2", 3)):
+        lo, hi = (1.0e-4, 1.0e-2) if n == "q3d0" else (1.0e-6, 1.0e-3)
+        a = _signed_zero_columns((o.pz, o.py, o.px), lo, hi, seed)
+        o.f(n)[:] = a
+        g.put(n, a)
+    gcall(g, "cm1g_kessler_branch")
+    o.L.cm1o_kessler_branch(o.h)
+    check(g, o, ["q3d0", "q3d1", "q3d2", "th3d", "pp3d", "prs"])
+    assert g.scalars()["train"] == o.scalars()["train"]
+
+
 def test_getdiv_getdivx_cfl(require_gpu):
     cfgd, g, o = pair()
     gcall(g, "cm1g_getdiv", N.FID["dum4"])
