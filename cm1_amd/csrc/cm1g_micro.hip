@@ -415,24 +415,27 @@ void kessler_branch_run(cm1g_model *m) {
   // argument of kessler/satadj) and the other schemes' tendency checks, and
   // kessler recomputes T itself, so nothing on this path reads them.
   pdefq3_run(m, rho, qv, qc, qr);
-  // fallout: per column, the fall speed (k_fallout, src/kessler.F:368-396)
-  // and its maximum Courant number in one upward pass, then a variable
-  // number of sedimentation substeps
+  // fallout: per column, one downward pass for the fall speed (k_fallout,
+  // src/kessler.F:368-396), its maximum Courant number (taken over the
+  // speeds before the fill, as the reference does; the maximum does not
+  // depend on the order) and the fill of near-empty levels from the level
+  // above (src/kessler.F:446-448); then a variable number of sedimentation
+  // substeps
   for2(m, 1, ni, 1, nj, [=] __device__(int i, int j) {
     const float rdz = P.rdz;
-    float crmax = 0.0f;
-    for (int k = 1; k <= nk; k++) {
+    float crmax = 0.0f, vabove = 0.0f;
+    for (int k = nk; k >= 1; k--) {
       const int c = IXD(P, i, j, k);
       const float q = qr[c], r = rho[c];
       float v = 0.0f;
       if (q > 1.0e-12f) v = 14.34f * pow_f(r * q, 0.1346f) * sqrtf(1.15f / r);
-      vr[c] = v;
       crmax = fmaxf(crmax, v * dt * rdz * P.mh[k]);
+      if (k < nk && q < 1.0e-8f) v = vabove;
+      vr[c] = v;
+      vabove = v;
     }
     int nfall = (int)(crmax + 1.0f);
     if (nfall < 1) nfall = 1;
-    for (int k = nk - 1; k >= 1; k--)
-      if (qr[IXD(P, i, j, k)] < 1.0e-8f) vr[IXD(P, i, j, k)] = vr[IXD(P, i, j, k + 1)];
     float dtfall = dt / (float)nfall;
     double bud = 0.0;
     float rn = rain[IX2D(P, i, j)], pr = prate[IX2D(P, i, j)];
