@@ -30,48 +30,13 @@ HBM_PEAK_GBS = 8000.0
 # in this summary
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_supercell_1024x1024x96.md")
 
-STRETCH = dict(dz_bot=50.0, dz_top=500.0, str_bot=500.0, str_top=17000.0, ztop=30000.0)
+from cm1_amd import workloads  # noqa: E402
+from cm1_amd.workloads import CONFIG1_STRETCH as STRETCH  # noqa: E402
 
-
-def workload_config1(ni=1024, nj=1024, nk=96, seed=2024):
-    from cm1_amd import init as hinit
-    dx = dy = 250.0
-    rng = np.random.default_rng(seed)
-    bubbles = []
-    for _ in range(8):
-        bubbles.append(dict(x=float(rng.uniform(0.1, 0.9) * ni * dx),
-                            y=float(rng.uniform(0.1, 0.9) * nj * dy), z=1400.0,
-                            amp=float(rng.choice([-2.0, 2.0])), rh=10000.0, rv=1400.0))
-    cols, fields, dzn = hinit.initial_state(ni, nj, nk, dx, dy, 500.0, stretch=STRETCH,
-                                            umax=30.0, bubbles=bubbles)
-    cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=2, ebc=2, sbc=2, nbc=2, imoist=1, ptype=1,
-                psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4, nsound=8, npt=0,
-                pdtra=1, dx=dx, dy=dy, dz=dzn, dt=1.5, alph=0.6, kdiv=0.1, csmax=350.0)
-    return cfgd, cols, fields
-
-
-def workload_ensemble(n=48, seed=2025):
-    """Batched supercell ensemble: n members with (nx, ny) drawn from
-    {96,128,192,256}, nz from {40,56,64}, dx=dy=1 km, a 20 km model top, dt=6 s,
-    nsound=6; per-member Weisman-Klemp sounding with qv scaled by U(0.9,1.1),
-    shear u=U(10,30) tanh(z/3 km), and a centred U(1,3) K bubble."""
-    from cm1_amd import init as hinit
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(n):
-        ni, nj = (int(rng.choice([96, 128, 192, 256])) for _ in range(2))
-        nk = int(rng.choice([40, 56, 64]))
-        qvs, umax, amp = float(rng.uniform(0.9, 1.1)), float(rng.uniform(10, 30)), float(rng.uniform(1, 3))
-        dx = 1000.0
-        dz = 20000.0 / nk
-        bub = [dict(x=0.5 * ni * dx, y=0.5 * nj * dx, z=1400.0, amp=amp, rh=10000.0, rv=1400.0)]
-        cols, fields, dzn = hinit.initial_state(ni, nj, nk, dx, dx, dz, umax=umax, bubbles=bub,
-                                                qv_scale=qvs)
-        cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=2, ebc=2, sbc=2, nbc=2, imoist=1, ptype=1,
-                    psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4, nsound=6, npt=0,
-                    pdtra=1, dx=dx, dy=dx, dz=dzn, dt=6.0, alph=0.6, kdiv=0.1, csmax=350.0)
-        out.append((cfgd, cols, fields))
-    return out
+workload_config1 = workloads.config1
+workload_ensemble = workloads.ensemble
+workload_spinup = workloads.spinup
+workload_cbl = workloads.cbl
 
 
 def bench_ensemble(args):
@@ -152,21 +117,6 @@ def bench_ensemble(args):
     }
     print(json.dumps(line))
     return 0
-
-
-def workload_spinup(ni=128, nj=128, nk=40, seed=7):
-    """Moist supercell case for the surrogate: 128x128x40, dx=dy=1 km,
-    dz=500 m, dt=6 s, nsound=6, Kessler, four seeded +3 K bubbles."""
-    from cm1_amd import init as hinit
-    dx = 1000.0
-    rng = np.random.default_rng(seed)
-    bubbles = [dict(x=float(rng.uniform(0.2, 0.8) * ni * dx), y=float(rng.uniform(0.2, 0.8) * nj * dx),
-                    z=1400.0, amp=3.0, rh=10000.0, rv=1400.0) for _ in range(4)]
-    cols, fields, dzn = hinit.initial_state(ni, nj, nk, dx, dx, 500.0, umax=25.0, bubbles=bubbles)
-    cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=2, ebc=2, sbc=2, nbc=2, imoist=1, ptype=1,
-                psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4, nsound=6, npt=0,
-                pdtra=1, dx=dx, dy=dx, dz=dzn, dt=6.0, alph=0.6, kdiv=0.1, csmax=350.0)
-    return cfgd, cols, fields
 
 
 def bench_surrogate(args):
@@ -325,21 +275,6 @@ def bench_surrogate(args):
     }
     print(json.dumps(line))
     return 0
-
-
-def workload_cbl(ni=1536, nj=1536, nk=128, seed=2026):
-    """Dry convective boundary layer: 1536x1536x128, dx=dy=dz=20 m,
-    doubly periodic, dt=0.25 s, explicit acoustic solver (psolver=2), WENO5
-    scalar advection on every stage, 300 K mixed layer to 1 km then 8 K/km,
-    u=10 m/s, +-0.1 K noise below 500 m, six Gaussian tracer blobs
-    (sigma 200 m).  (The surface heat flux needs the SGS turbulence scheme,
-    which is not on this path.)"""
-    from cm1_amd import init as hinit
-    cols, fields, dzn = hinit.cbl_state(ni, nj, nk, 20.0, 20.0, 20.0, seed=seed)
-    cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=1, ebc=1, sbc=1, nbc=1, imoist=0, ptype=0,
-                psolver=2, advwenos=1, advwenov=0, apmasscon=1, irbc=4, nsound=0, npt=6,
-                pdtra=1, dx=20.0, dy=20.0, dz=dzn, dt=0.25, alph=0.6, kdiv=0.1, csmax=350.0)
-    return cfgd, cols, fields
 
 
 def bench_cbl(args):

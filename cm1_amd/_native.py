@@ -119,7 +119,7 @@ SIGNATURES = {
     "cm1g_state_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "cm1g_state_save": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "cm1g_state_load": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
-    "cm1g_dtsmall": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_int32)]),
+    "cm1g_dtsmall": (C.c_int, [C.POINTER(Config), C.c_float, C.POINTER(C.c_int32)]),
     "cm1g_budgets": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "cm1g_scalars": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
 }

@@ -19,16 +19,26 @@ __device__ __forceinline__ uint32_t ordered(float v) {
 // maxmin takes, level by level, the first point (in j, then i) that beats
 // the running extreme, then the first level (in k) that beats the others:
 // among equal extremes the one with the smallest k, then j, then i.  Keys
-// (ordered value, position) with the position inverted for the maximum
-// make that one atomic maximum each; NaNs never win a comparison there and
-// are skipped here.
-__device__ __forceinline__ void mm_cell(float v, uint32_t pos, unsigned long long *key) {
+// (ordered value, position) with the position inverted make that a plain
+// maximum of 64-bit keys for the maximum and for the minimum; NaNs never win
+// a comparison there and are skipped here.
+__device__ __forceinline__ void mm_cell(float v, uint32_t pos, unsigned long long &kmax,
+                                        unsigned long long &kmin) {
   if (v != v) return;
   v = v + 0.0f;  // -0 and +0 compare equal: one key for both
-  const unsigned long long kmax = ((unsigned long long)ordered(v) << 32) | (0xffffffffu - pos);
-  const unsigned long long kmin = ((unsigned long long)(0xffffffffu - ordered(v)) << 32) | (0xffffffffu - pos);
-  atomicMax(&key[0], kmax);
-  atomicMax(&key[1], kmin);
+  const unsigned long long a = ((unsigned long long)ordered(v) << 32) | (0xffffffffu - pos);
+  const unsigned long long b = ((unsigned long long)(0xffffffffu - ordered(v)) << 32) | (0xffffffffu - pos);
+  kmax = a > kmax ? a : kmax;
+  kmin = b > kmin ? b : kmin;
+}
+
+// maximum of a 64-bit key over the 64 lanes of a wavefront (butterfly)
+__device__ __forceinline__ unsigned long long wave_max(unsigned long long x) {
+  for (int d = 32; d >= 1; d >>= 1) {
+    const unsigned long long y = __shfl_xor(x, d, 64);
+    x = y > x ? y : x;
+  }
+  return x;
 }
 
 extern "C" int cm1g_maxmin(cm1g_model *m, int id, int stag, float *vmax, int32_t *locmax, float *vmin,
@@ -46,9 +56,25 @@ extern "C" int cm1g_maxmin(cm1g_model *m, int id, int stag, float *vmax, int32_t
     key[0] = 0ULL;
     key[1] = 0ULL;
   });
-  for3(m, 1, izz, 1, jzz, 1, kzz, [=] __device__(int i, int j, int k) {
-    const uint32_t pos = (uint32_t)(((int64_t)(k - 1) * jzz + (j - 1)) * izz + (i - 1));
-    mm_cell(f[IXD(P, i, j, k)], pos, key);
+  // each workgroup walks rows (j, k) of the interior, lanes along i; every
+  // lane keeps its own two keys, a wavefront combines its lanes' keys with
+  // shuffles and issues one atomic maximum per key
+  const int rows = jzz * kzz;
+  const int nblk = rows < 4096 ? rows : 4096;
+  launch<256>(m, dim3(nblk), dim3(256), [=] __device__(int bx, int, int) {
+    unsigned long long kmax = 0ULL, kmin = 0ULL;
+    for (int r = bx; r < rows; r += nblk) {
+      const int j = 1 + r % jzz, k = 1 + r / jzz;
+      const uint32_t row0 = (uint32_t)r * (uint32_t)izz;
+      for (int i = 1 + (int)threadIdx.x; i <= izz; i += 256)
+        mm_cell(f[IXD(P, i, j, k)], row0 + (uint32_t)(i - 1), kmax, kmin);
+    }
+    kmax = wave_max(kmax);
+    kmin = wave_max(kmin);
+    if ((threadIdx.x & 63) == 0) {
+      if (kmax) atomicMax(&key[0], kmax);
+      if (kmin) atomicMax(&key[1], kmin);
+    }
   });
   unsigned long long h[2];
   HIPCHK(hipMemcpyAsync(h, key, sizeof(h), hipMemcpyDeviceToHost, m->stream));

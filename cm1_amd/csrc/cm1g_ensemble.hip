@@ -86,6 +86,10 @@ extern "C" int cm1g_ensemble_destroy(cm1g_ensemble *e) {
 extern "C" int cm1g_ensemble_sync(cm1g_ensemble *e) {
   ARGCHK(e, "cm1g_ensemble_sync: null ensemble");
   HIPCHK(hipStreamSynchronize(e->stream));
+  for (auto *m : e->mem) {
+    int rc = satadj_check(m);
+    if (rc) return rc;
+  }
   return 0;
 }
 

@@ -92,6 +92,10 @@ struct cm1g_model {
   // anything that may write either): the first RK stage's acoustic
   // substeps may then read the a-arrays directly
   int synced;
+  // 1 once cm1g_field_ptr has handed out the address of one of those eight
+  // arrays: the caller may then write them where the library cannot see
+  // it, so synced is never set again on this handle
+  int ext_writers;
   // non-null while an ensemble records this member's step: launches are
   // appended here instead of run
   std::vector<struct LaunchRec> *rec;
@@ -110,6 +114,7 @@ struct cm1g_model {
 void graphs_clear(cm1g_model *m);
 // the host side of one large step (solve1, solve2, mp_driver, solve3)
 int step_body(cm1g_model *m);
+int satadj_check(cm1g_model *m);
 // every field pointer and the scalar updates' partner, and back
 std::vector<float *> field_map(const cm1g_model *m);
 void set_field_map(cm1g_model *m, const std::vector<float *> &v);

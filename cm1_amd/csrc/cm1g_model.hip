@@ -65,11 +65,14 @@ static int upload_column(cm1g_model *m, int id) {
   return 0;
 }
 
-extern "C" int cm1g_dtsmall(const cm1g_config *c, int32_t *nsound) {
+extern "C" int cm1g_dtsmall(const cm1g_config *c, float min_dz, int32_t *nsound) {
   ARGCHK(c && nsound, "cm1g_dtsmall: null argument");
   float dtsm;
   if (c->psolver == 2) {
-    float mn = fminf(fminf(c->dx, c->dy), c->dz);
+    // min( min_dx , min_dy , min_dz ): the smallest level thickness on a
+    // stretched grid (src/param.F:8522-8541); the namelist dz on a uniform one
+    const float mdz = (min_dz > 0.0f) ? min_dz : c->dz;
+    float mn = fminf(fminf(c->dx, c->dy), mdz);
     dtsm = 0.50f * mn / 360.0f;
   } else {
     dtsm = 0.60f * fminf(c->dx, c->dy) / 360.0f;
@@ -271,19 +274,35 @@ extern "C" int cm1g_metrics(cm1g_model *m) {
     c1[k] = 1.0f - c2[k];
   }
   // three-point extrapolation weights to zf(1) from zh(1..3) and to
-  // zf(nk+1) from zh(nk..nk-2): the m=0 Fornberg weights of
-  // src/param.F:8980-9060, in double precision, stored as REAL.
+  // zf(nk+1) from zh(nk..nk-2): Fornberg's recurrence as
+  // src/param.F:8988-9050 runs it (nn = 1, 2): x0 and alpha are REAL
+  // quotients of the heights by the end-level thickness, the recurrence
+  // is double precision, the m = 0 weights are stored as REAL.
   float w6[6];
   for (int nn = 0; nn < 2; nn++) {
+    float dsf;
     double x0, al[3];
-    if (nn == 0) { x0 = zf[1]; al[0] = zh[1]; al[1] = zh[2]; al[2] = zh[3]; }
-    else { x0 = zf[nk + 1]; al[0] = zh[nk]; al[1] = zh[nk - 1]; al[2] = zh[nk - 2]; }
-    for (int a = 0; a < 3; a++) {
-      double p = 1.0;
-      for (int b = 0; b < 3; b++)
-        if (b != a) p *= (x0 - al[b]) / (al[a] - al[b]);
-      w6[nn * 3 + a] = (float)p;
+    if (nn == 0) {
+      dsf = zf[2] - zf[1];
+      x0 = (double)(zf[1] / dsf);
+      al[0] = (double)(zh[1] / dsf); al[1] = (double)(zh[2] / dsf); al[2] = (double)(zh[3] / dsf);
+    } else {
+      dsf = zf[nk + 1] - zf[nk];
+      x0 = (double)(zf[nk + 1] / dsf);
+      al[0] = (double)(zh[nk] / dsf); al[1] = (double)(zh[nk - 1] / dsf); al[2] = (double)(zh[nk - 2] / dsf);
     }
+    // delta(n,0,nu) for n = 1, 2 (src/param.F:9013-9034)
+    double b1 = 1.0, b2 = 1.0, b3;
+    b3 = al[1] - al[0]; b2 = b2 * b3;
+    const double d10 = ((al[1] - x0) * 1.0 - 0.0) / b3;
+    const double d11 = (b1 / b2) * (0.0 - (al[0] - x0) * 1.0);
+    b1 = b2;
+    b2 = 1.0;
+    b3 = al[2] - al[0]; b2 = b2 * b3;
+    w6[nn * 3 + 0] = (float)(((al[2] - x0) * d10 - 0.0) / b3);
+    b3 = al[2] - al[1]; b2 = b2 * b3;
+    w6[nn * 3 + 1] = (float)(((al[2] - x0) * d11 - 0.0) / b3);
+    w6[nn * 3 + 2] = (float)((b1 / b2) * (0.0 - (al[1] - x0) * d11));
   }
   m->P.cgs1 = w6[0]; m->P.cgs2 = w6[1]; m->P.cgs3 = w6[2];
   m->P.cgt1 = w6[3]; m->P.cgt2 = w6[4]; m->P.cgt3 = w6[5];
@@ -535,14 +554,35 @@ extern "C" int cm1g_state_load(cm1g_model *m, const void *host, int64_t nbytes) 
 extern "C" int cm1g_field_ptr(cm1g_model *m, int id, float **dptr) {
   ARGCHK(field_ok(m, id) && dptr, "cm1g_field_ptr: bad field id");
   m->synced = 0;
+  // writes through the address are invisible to the library: the first
+  // acoustic stage stops relying on the a-arrays equalling the 3d-arrays
+  if (id == CM1G_UA || id == CM1G_VA || id == CM1G_WA || id == CM1G_PPI || id == CM1G_U3D ||
+      id == CM1G_V3D || id == CM1G_W3D || id == CM1G_PP3D) {
+    if (!m->ext_writers) graphs_clear(m);
+    m->ext_writers = 1;
+  }
   *dptr = m->f[id];
+  return 0;
+}
+
+// satadj's non-convergence flag (set by the kernels, never cleared: the
+// reference stops the run there, src/kessler.F:343-356, so every later
+// check of a model that hit it reports it again)
+int satadj_check(cm1g_model *m) {
+  int flag = 0;
+  HIPCHK(hipMemcpyAsync(&flag, m->iflag, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  if (flag) {
+    cm1g_set_error("satadj: convergence cannot be reached (src/kessler.F:343-356)");
+    return CM1G_E_SATADJ;
+  }
   return 0;
 }
 
 extern "C" int cm1g_sync(cm1g_model *m) {
   ARGCHK(m, "cm1g_sync: null model");
   HIPCHK(hipStreamSynchronize(m->stream));
-  return 0;
+  return satadj_check(m);
 }
 
 static int check_launch() {
@@ -722,7 +762,7 @@ extern "C" int cm1g_getset(cm1g_model *m, int fresh_start) {
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(m->stream));
   HIPCHK(hipMemcpy(&m->mass1, m->dscal, sizeof(double), hipMemcpyDeviceToHost));
-  m->synced = 1;
+  m->synced = !m->ext_writers;
   return 0;
 }
 
@@ -1091,7 +1131,7 @@ extern "C" int cm1g_solve3(cm1g_model *m) {
     });
   }
   // ua..qa, pta now equal u3d..q3d, pt3d where the next stage reads them
-  m->synced = 1;
+  m->synced = !m->ext_writers;
   return check_launch();
 }
 
@@ -1176,13 +1216,15 @@ extern "C" int cm1g_step(cm1g_model *m, float *cflmax) {
   if (rc) return rc;
   if (cflmax) {
     HIPCHK(hipMemcpyAsync(&m->cflmax, m->fred, sizeof(float), hipMemcpyDeviceToHost, m->stream));
-    int flag = 0;
-    HIPCHK(hipMemcpyAsync(&flag, m->iflag, sizeof(int), hipMemcpyDeviceToHost, m->stream));
-    HIPCHK(hipStreamSynchronize(m->stream));
+    rc = satadj_check(m);
     *cflmax = m->cflmax;
-    if (flag) {
-      cm1g_set_error("satadj: convergence cannot be reached (src/kessler.F:343-356)");
-      return CM1G_E_SATADJ;
+    if (rc) return rc;
+    // the reference's MPI build stops once the Courant number reaches 1.5
+    // (src/solve2.F:2082, src/cm1.F:4385-4397); the step's results stay in
+    // place, as the reference writes its output before stopping
+    if (m->cflmax >= 1.50f) {
+      cm1g_set_error("cm1g_step: Courant number has exceeded 1.5 (src/cm1.F:4385-4397)");
+      return CM1G_E_CFL;
     }
   }
   return 0;

@@ -162,9 +162,10 @@ module cm1gpu
     type(c_ptr) function cm1g_last_error() bind(C, name="cm1g_last_error")
       import :: c_ptr
     end function
-    integer(c_int) function cm1g_dtsmall(cfg, nsound) bind(C, name="cm1g_dtsmall")
-      import :: c_int, c_int32_t, cm1g_config
+    integer(c_int) function cm1g_dtsmall(cfg, min_dz, nsound) bind(C, name="cm1g_dtsmall")
+      import :: c_int, c_int32_t, c_float, cm1g_config
       type(cm1g_config), intent(in) :: cfg
+      real(c_float), value :: min_dz
       integer(c_int32_t), intent(out) :: nsound
     end function
     integer(c_int) function cm1g_put_column(m, id, col) bind(C, name="cm1g_put_column")

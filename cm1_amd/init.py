@@ -30,52 +30,80 @@ EPS = RD / RV
 REPS = RV / RD
 
 
+def _nint(x):
+    """Fortran NINT of a REAL: nearest integer, halves away from zero."""
+    x = float(x)
+    return int(math.floor(x + 0.5)) if x >= 0.0 else -int(math.floor(-x + 0.5))
+
+
 def vertical_grid(nk, dz, stretch=None):
-    """zf, zh, mh, mf columns (k=0..nk+2) and the nominal dz.
+    """zf, zh, mh, mf columns (k=0..nk+2) and the namelist dz.
+
+    Every operation is single precision, in the reference's order
+    (``src/param.F:7790-7810`` for the unstretched levels,
+    ``src/param.F:7819-7950`` for ``stretch_z=1``,
+    ``src/param.F:8140-8170`` for zh, mh and mf).  ``dz`` is the namelist
+    value: the reference keeps it as the metric scale (``rdz = 1/dz``,
+    ``mh = dz/(zf(k+1)-zf(k))``) also on a stretched grid.
 
     ``stretch`` = dict(dz_bot, dz_top, str_bot, str_top, ztop) selects
-    stretch_z=1."""
-    zf = np.zeros(nk + 3, np.float64)
-    if stretch is None:
-        for k in range(0, nk + 3):
-            zf[k] = float(F32(dz) * F32(k - 1))
-        dzn = dz
-    else:
-        dzb, dzt = stretch["dz_bot"], stretch["dz_top"]
-        sb, st, zt = stretch["str_bot"], stretch["str_top"], stretch["ztop"]
-        nk1 = int(round(sb / dzb))
-        nk3 = int(round((zt - st) / dzt))
-        nk2 = nk - (nk1 + nk3)
-        if nk2 <= 1 or nk1 < 0 or nk3 < 0:
-            raise ValueError("inconsistent stretched grid")
-        nominal = (st - sb) / nk2
-        c2 = (nominal - dzb) / (nominal * nominal * float(nk2 - 1))
-        c1 = (dzb / nominal) - c2 * nominal
-        for k in range(1, nk1 + 2):
-            zf[k] = (k - 1) * dzb
-        for k in range(nk1 + 1, nk1 + nk2 + 2):
-            zf[k] = zf[nk1 + 1] + (c1 + c2 * float(k - 1 - nk1) * nominal) * float(k - 1 - nk1) * nominal
-        for k in range(nk1 + nk2 + 2, nk1 + nk2 + nk3 + 2):
-            zf[k] = zf[k - 1] + dzt
-        dzn = zt / nk
-        zf[0] = -zf[2]
-        zf[nk + 2] = zf[nk + 1] + (zf[nk + 1] - zf[nk])
-    zf = zf.astype(F32)
+    stretch_z=1; settings the reference stops on raise ValueError."""
+    dz = F32(dz)
+    zf = np.zeros(nk + 3, F32)
+    for k in range(0, nk + 3):
+        zf[k] = dz * F32(k - 1)
     zh = np.zeros(nk + 3, F32)
     mh = np.ones(nk + 3, F32)
     mf = np.ones(nk + 3, F32)
-    for k in range(0, nk + 2):
-        zh[k] = F32(0.5) * (zf[k] + zf[k + 1])
-    if stretch is not None:
+    if stretch is None:
         for k in range(0, nk + 2):
-            mh[k] = F32(dzn) / (zf[k + 1] - zf[k])
-        zh[0] = -zh[1]
-        zh[nk + 1] = zh[nk] + F32(2.0) * (zf[nk + 1] - zh[nk])
-        for k in range(1, nk + 2):
-            mf[k] = F32(dzn) / (zh[k] - zh[k - 1])
-        mf[0] = mf[1]
-        mf[nk + 2] = mf[nk + 1]
-    return dict(zf=zf, zh=zh, mh=mh, mf=mf), float(dzn)
+            zh[k] = F32(0.5) * (zf[k] + zf[k + 1])
+        return dict(zf=zf, zh=zh, mh=mh, mf=mf), float(dz)
+    dzb, dzt = F32(stretch["dz_bot"]), F32(stretch["dz_top"])
+    sb, st, zt = F32(stretch["str_bot"]), F32(stretch["str_top"]), F32(stretch["ztop"])
+    nominal = F32(0.5) * (dzb + dzt)
+    nk1 = _nint(sb / dzb)
+    nk3 = _nint((zt - st) / dzt)
+    nk2 = nk - (nk1 + nk3)
+    # the reference's consistency checks (src/param.F:7827-7911)
+    if abs(F32(nk1) * dzb - sb) > F32(0.01):
+        raise ValueError("stretched grid: depth of bottom layer does not divide by dz_bot")
+    if abs(F32(nk2) * nominal - (st - sb)) >= F32(0.01):
+        raise ValueError("stretched grid: depth of middle layer does not divide by nominal_dz")
+    if abs(F32(nk3) * dzt - (zt - st)) > F32(0.01):
+        raise ValueError("stretched grid: depth of top layer does not divide by dz_top")
+    test_len = dzb * F32(nk1) + F32(0.5) * (dzb + dzt) * F32(nk2) + dzt * F32(nk3)
+    if nk1 < 0 or nk2 < 0 or nk3 < 0 or abs(zt - test_len) > F32(1.0e-6) * zt:
+        raise ValueError(f"stretched grid: nk1,nk2,nk3 = {nk1},{nk2},{nk3} do not give ztop")
+    nominal = (st - sb) / F32(nk2)
+    c2 = (nominal - dzb) / (nominal * nominal * F32(nk2 - 1))
+    c1 = (dzb / nominal) - c2 * nominal
+    for k in range(1, nk1 + 2):
+        zf[k] = F32(k - 1) * dzb
+    for k in range(nk1 + 1, nk1 + nk2 + 2):
+        m = F32(k - 1 - nk1)
+        zf[k] = zf[nk1 + 1] + (c1 + c2 * m * nominal) * m * nominal
+    for k in range(nk1 + nk2 + 2, nk1 + nk2 + nk3 + 2):
+        zf[k] = zf[k - 1] + dzt
+    zf[0] = -zf[2]
+    zf[nk + 2] = zf[nk + 1] + (zf[nk + 1] - zf[nk])
+    for k in range(0, nk + 2):
+        zh[k] = F32(0.5) * (zf[k + 1] + zf[k])
+        mh[k] = dz / (zf[k + 1] - zf[k])
+    zh[0] = -zh[1]
+    zh[nk + 1] = zh[nk] + F32(2.0) * (zf[nk + 1] - zh[nk])
+    for k in range(1, nk + 2):
+        mf[k] = dz / (zh[k] - zh[k - 1])
+    mf[0] = mf[1]
+    mf[nk + 2] = mf[nk + 1]
+    return dict(zf=zf, zh=zh, mh=mh, mf=mf), float(dz)
+
+
+def min_dz(zf, nk):
+    """Smallest level thickness, min over k=1..nk of zf(k+1)-zf(k) in single
+    precision (``src/param.F:8522-8541``); dtsmall uses it for psolver 2."""
+    zf = np.asarray(zf, F32)
+    return float(np.min(zf[2:nk + 2] - zf[1:nk + 1]))
 
 
 def rslf(p, t):

@@ -11,7 +11,10 @@ import numpy as np
 from . import _native as N
 
 
-def make_config(**kw):
+def make_config(min_dz=0.0, **kw):
+    """Config from namelist-style keywords; nsound <= 0 asks cm1g_dtsmall,
+    which for psolver 2 needs the grid's smallest level thickness min_dz
+    (0: an unstretched grid of spacing dz)."""
     d = dict(ni=64, nj=64, nk=40, wbc=2, ebc=2, sbc=2, nbc=2, imoist=1, ptype=1,
              psolver=3, advwenos=0, advwenov=0, apmasscon=1, irbc=4, nsound=0, npt=0,
              pdtra=1, dx=1000.0, dy=1000.0, dz=500.0, dt=6.0, alph=0.6, kdiv=0.1,
@@ -20,7 +23,7 @@ def make_config(**kw):
     cfg = N.Config(**d)
     if cfg.nsound <= 0:
         ns = C.c_int32()
-        N.check(N.lib().cm1g_dtsmall(C.byref(cfg), C.byref(ns)), "cm1g_dtsmall")
+        N.check(N.lib().cm1g_dtsmall(C.byref(cfg), float(min_dz), C.byref(ns)), "cm1g_dtsmall")
         cfg.nsound = ns.value
     return cfg
 

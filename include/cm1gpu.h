@@ -137,9 +137,17 @@ int cm1g_get_array(cm1g_model *m, int id, int stag, float *host);
 int cm1g_put_2d(cm1g_model *m, int id, const float *host);
 int cm1g_get_2d(cm1g_model *m, int id, float *host);
 /* Device address of a field (for zero-copy use by a caller that already
- * holds device memory, e.g. a Fortran host with its own device mirrors). */
+ * holds device memory, e.g. a Fortran host with its own device mirrors).
+ * The address is valid until the next call that runs kernels on the handle:
+ * the scalar updates exchange th3d, q3d and pt3d with partner buffers, so
+ * query it again after each step or routine call.  Once the address of
+ * ua, va, wa, ppi, u3d, v3d, w3d or pp3d has been handed out, the caller may
+ * write those arrays, and every later step reads u3d..pp3d as the reference
+ * does instead of assuming they equal ua..ppi after solve3 (correct for any
+ * writes, at the cost of the first stage's copies). */
 int cm1g_field_ptr(cm1g_model *m, int id, float **dptr);
-/* Wait for all queued work of this handle. */
+/* Wait for all queued work of this handle; CM1G_E_SATADJ if a saturation
+ * adjustment of any step so far failed to converge (src/kessler.F:343-356). */
 int cm1g_sync(cm1g_model *m);
 
 /* --- the time step ---------------------------------------------------- */
@@ -148,7 +156,16 @@ int cm1g_sync(cm1g_model *m);
  * (src/init3d.F:2032-2872).  fresh_start=0 is the restart path. */
 int cm1g_getset(cm1g_model *m, int fresh_start);
 /* One large step: solve1 + solve2 + mp_driver + solve3
- * (src/cm1.F:3313-3476).  Sets *cflmax (may be NULL). */
+ * (src/cm1.F:3313-3476).  With cflmax non-NULL the step is waited for, the
+ * Courant maximum is returned in *cflmax, and the step returns
+ * CM1G_E_SATADJ if the saturation adjustment has failed to converge
+ * (src/kessler.F:343-356) or CM1G_E_CFL if cflmax >= 1.5, where the
+ * reference's MPI build stops (src/solve2.F:2082, src/cm1.F:4385-4397; its
+ * serial build does not check, src/misclibs.F:2810); the step's results are
+ * in place either way.  With cflmax NULL nothing is read back and the step
+ * is only queued; cm1g_sync then reports a failed saturation adjustment.
+ * A model whose adjustment failed keeps reporting it (the reference stops
+ * the run there). */
 int cm1g_step(cm1g_model *m, float *cflmax);
 /* No counterpart in the reference.  on=1 (the default): cm1g_step replays
  * each large step as one captured HIP graph (captured on first use, one
@@ -178,7 +195,7 @@ typedef struct cm1g_ensemble cm1g_ensemble;
 int cm1g_ensemble_create(cm1g_model *const *members, int n, cm1g_ensemble **out);
 int cm1g_ensemble_destroy(cm1g_ensemble *e);   /* the members stay */
 int cm1g_ensemble_step(cm1g_ensemble *e);      /* one large step of every member */
-int cm1g_ensemble_sync(cm1g_ensemble *e);
+int cm1g_ensemble_sync(cm1g_ensemble *e);     /* wait; CM1G_E_SATADJ if a member's adjustment failed */
 int cm1g_ensemble_set_tiles(cm1g_ensemble *e, int ty);
 /* Autotuner: times `steps` ensemble steps at each tile height (1, 2, 4),
  * keeps the fastest in *ty_best and, if ms_per_step is non-NULL, stores the
@@ -273,8 +290,10 @@ int cm1g_getdivx(cm1g_model *m);
  * 214) and their (i,j,k): among equal extremes the smallest k, then j, then
  * i, as maxmin's strict comparisons pick them.  Reduced on the device. */
 int cm1g_maxmin(cm1g_model *m, int id, int stag, float *vmax, int32_t *locmax, float *vmin, int32_t *locmin);
-/* dtsmall: nsound for dt (src/misclibs.F:818-915) */
-int cm1g_dtsmall(const cm1g_config *cfg, int32_t *nsound);
+/* dtsmall: nsound for dt (src/misclibs.F:818-915).  min_dz is the smallest
+ * level thickness, min over k of zf(k+1)-zf(k) (src/param.F:8522-8541),
+ * which psolver 2 uses; min_dz <= 0 takes cfg->dz (an unstretched grid). */
+int cm1g_dtsmall(const cm1g_config *cfg, float min_dz, int32_t *nsound);
 /* Domain budgets in double precision: dry-air mass (calcmass,
  * src/misclibs.F:3134-3191) and total water qv+qc+qr (totmois,
  * src/misclibs.F:3199-3264). */

@@ -106,17 +106,23 @@ def main():
     rebuilt = set()
 
     def compile_one(f):
+        # the object is written under a temporary name and renamed, so a
+        # compile that is interrupted never leaves a stub that a later
+        # incremental build would take as up to date
         obj = os.path.join(bdir, f[:-2] + ".o")
-        cmd = ["flang"] + fflags + ["-c", f[:-2] + ".f90", "-o", obj]
+        tmp = obj + ".part"
+        cmd = ["flang"] + fflags + ["-c", f[:-2] + ".f90", "-o", tmp]
         r = subprocess.run(cmd, cwd=bdir, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"{f}: {r.stderr[-3000:]}")
+        os.replace(tmp, obj)
         return f
 
     for lv in level:
         todo = [f for f in lv
                 if changed[f] or (deps[f] & rebuilt)
-                or not os.path.exists(os.path.join(bdir, f[:-2] + ".o"))]
+                or not os.path.exists(os.path.join(bdir, f[:-2] + ".o"))
+                or os.path.getsize(os.path.join(bdir, f[:-2] + ".o")) == 0]
         with cf.ThreadPoolExecutor(args.j) as ex:
             for f in ex.map(compile_one, todo):
                 rebuilt.add(f)

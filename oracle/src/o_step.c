@@ -116,19 +116,38 @@ void cm1o_set_extrap(cm1o_model *M, const float *cgs6) {
     M->cgs1 = cgs6[0]; M->cgs2 = cgs6[1]; M->cgs3 = cgs6[2];
     M->cgt1 = cgs6[3]; M->cgt2 = cgs6[4]; M->cgt3 = cgs6[5];
   } else {
-    /* 2nd-order (three-point Lagrange) extrapolation weights to the
-     * surface zf(1) from zh(1..3) and to the top zf(nk+1) from
-     * zh(nk..nk-2): the m=0 Fornberg weights of src/param.F:8980-9060,
-     * evaluated in double precision and stored in REAL. */
+    /* 2nd-order extrapolation weights to the surface zf(1) from zh(1..3)
+     * and to the top zf(nk+1) from zh(nk..nk-2): Fornberg's recurrence as
+     * src/param.F:8988-9050 runs it (nn = 1, 2), with x0 and alpha the
+     * REAL quotients of the heights by the end-level thickness, the
+     * recurrence in double precision, the m = 0 weights stored as REAL.
+     * sigmaf(k) = zf(k) and sigma(k) = zh(k) for k = 1..nk+1 (no terrain). */
     for (int nn = 1; nn <= 2; nn++) {
+      float dsf;
       double x0, al[3], w[3];
-      if (nn == 1) { x0 = zf[1]; al[0] = zh[1]; al[1] = zh[2]; al[2] = zh[3]; }
-      else { x0 = zf[nk + 1]; al[0] = zh[nk]; al[1] = zh[nk - 1]; al[2] = zh[nk - 2]; }
-      for (int a = 0; a < 3; a++) {
-        double p = 1.0;
-        for (int b = 0; b < 3; b++)
-          if (b != a) p *= (x0 - al[b]) / (al[a] - al[b]);
-        w[a] = p;
+      if (nn == 1) {
+        dsf = zf[2] - zf[1];
+        x0 = (double)(zf[1] / dsf);
+        al[0] = (double)(zh[1] / dsf); al[1] = (double)(zh[2] / dsf); al[2] = (double)(zh[3] / dsf);
+      } else {
+        dsf = zf[nk + 1] - zf[nk];
+        x0 = (double)(zf[nk + 1] / dsf);
+        al[0] = (double)(zh[nk] / dsf); al[1] = (double)(zh[nk - 1] / dsf); al[2] = (double)(zh[nk - 2] / dsf);
+      }
+      {
+        /* delta(n,0,nu) for n = 1, 2 (src/param.F:9013-9034) */
+        double b1 = 1.0, b2, b3, d10, d11;
+        b2 = 1.0;
+        b3 = al[1] - al[0]; b2 = b2 * b3;
+        d10 = ((al[1] - x0) * 1.0 - 0.0) / b3;
+        d11 = (b1 / b2) * (0.0 - (al[0] - x0) * 1.0);
+        b1 = b2;
+        b2 = 1.0;
+        b3 = al[2] - al[0]; b2 = b2 * b3;
+        w[0] = ((al[2] - x0) * d10 - 0.0) / b3;
+        b3 = al[2] - al[1]; b2 = b2 * b3;
+        w[1] = ((al[2] - x0) * d11 - 0.0) / b3;
+        w[2] = (b1 / b2) * (0.0 - (al[1] - x0) * d11);
       }
       if (nn == 1) { M->cgs1 = (float)w[0]; M->cgs2 = (float)w[1]; M->cgs3 = (float)w[2]; }
       else { M->cgt1 = (float)w[0]; M->cgt2 = (float)w[1]; M->cgt3 = (float)w[2]; }

@@ -7,6 +7,11 @@ import numpy as np
 import pyoracle
 from cm1_amd import init as hinit
 
+# a small stretch_z = 1 grid for nk = 30: 100 m levels to 500 m, analytic
+# stretching to 500 m levels at 5 km, 10 km top (nk1, nk2, nk3 = 5, 15, 10);
+# the namelist dz stays 500 m
+SMALL_STRETCH = dict(dz_bot=100.0, dz_top=500.0, str_bot=500.0, str_top=5000.0, ztop=10000.0)
+
 # prognostic fields compared after stepping: (gpu/oracle name, staggering)
 PROG = [("ua", "u"), ("va", "v"), ("wa", "w"), ("tha", None), ("ppi", None),
         ("qa0", None), ("qa1", None), ("qa2", None), ("rho", None), ("prs", None)]
@@ -82,11 +87,17 @@ def load_fixture(path):
     irdamp = opts[3] if len(opts) > 3 else 0
     npt = opts[4] if len(opts) > 4 else 0
     rdalpha, zd = [float(v) for v in z["rdamp"]] if "rdamp" in z else (3.3333333333e-3, 15000.0)
+    dx, dy, dz, stretch = 1000.0, 1000.0, 500.0, None
+    if "grid" in z:
+        g = [float(v) for v in z["grid"]]
+        dx, dy, dz = g[:3]
+        if int(g[3]) == 1:
+            stretch = dict(zip(("dz_bot", "dz_top", "str_bot", "str_top", "ztop"), g[4:9]))
     cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=bc, ebc=bc, sbc=bc, nbc=bc, imoist=1, ptype=1,
                 psolver=psolver, advwenos=advwenos, advwenov=advwenov, apmasscon=1, irbc=4,
-                nsound=0, npt=npt, pdtra=1, dx=1000.0, dy=1000.0, dz=500.0, dt=dt, alph=0.6,
+                nsound=0, npt=npt, pdtra=1, dx=dx, dy=dy, dz=dz, dt=dt, alph=0.6,
                 kdiv=0.1, csmax=350.0, irdamp=irdamp, rdalpha=rdalpha, zd=zd)
-    cols, _ = hinit.vertical_grid(nk, 500.0)
+    cols, _ = hinit.vertical_grid(nk, dz, stretch)
     for n in ("pi0", "th0", "qv0"):
         c = np.zeros(nk + 3, np.float32)
         c[1:nk + 1] = z[n]
@@ -117,10 +128,13 @@ def load_fixture(path):
     return cfgd, cols, f, nsteps, target, z
 
 
-def nsound_for(cfgd):
-    """dtsmall (src/misclibs.F:818-915) for psolver=2 or 3 (3-D), in Python."""
+def nsound_for(cfgd, zf=None):
+    """dtsmall (src/misclibs.F:818-915) for psolver=2 or 3 (3-D), in Python;
+    zf: the grid's zf column, whose smallest level thickness psolver 2 uses
+    (src/param.F:8522-8541); None for an unstretched grid of spacing dz."""
     if cfgd.get("psolver", 3) == 2:
-        mn = min(cfgd["dx"], cfgd["dy"], float(np.min(cfgd["dz"])))
+        mdz = cfgd["dz"] if zf is None else hinit.min_dz(zf, cfgd["nk"])
+        mn = min(cfgd["dx"], cfgd["dy"], mdz)
         dtsm = np.float32(0.50) * np.float32(mn) / np.float32(360.0)
     else:
         dtsm = np.float32(0.60) * np.float32(min(cfgd["dx"], cfgd["dy"])) / np.float32(360.0)

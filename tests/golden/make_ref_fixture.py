@@ -12,7 +12,14 @@ stores its GrADS output at t=0 and after N large steps as float32 arrays:
   advwenov, irdamp, npt), and ``rdamp``: (rdalpha, zd) of the Rayleigh
   damper;
 * with npt > 0 (iptra = 1): ``t0_ptN``/``tN_ptN``, the passive tracers the
-  reference initialises (src/init3d.F:328-349) and advects.
+  reference initialises (src/init3d.F:328-349) and advects;
+* ``grid``: dx, dy, the namelist dz, stretch_z and (dz_bot, dz_top, str_bot,
+  str_top, ztop), and ``zh``: the scalar-level heights the reference wrote
+  (m), which pin the vertical-grid set-up.
+
+``python tests/golden/make_ref_fixture.py stretched`` writes the fixtures of
+the stretched 96-level grid of the 1024x1024x96 supercell (dx = 250 m) on a
+small horizontal domain.
 
 Usage: python tests/golden/make_ref_fixture.py [name ni nj nk nsteps dt wbc
 [psolver advwenos advwenov [irdamp zd [npt]]]]
@@ -31,18 +38,25 @@ import refrun  # noqa: E402
 OPT_KEYS = ("psolver", "advwenos", "advwenov", "irdamp", "npt")
 OPT_DEFAULTS = dict(psolver=3, advwenos=0, advwenov=0, irdamp=0, npt=0)
 RDALPHA = 3.3333333333e-3
+STRETCH_KEYS = ("dz_bot", "dz_top", "str_bot", "str_top", "ztop")
+# the vertical grid of the 1024x1024x96 supercell workload (stretch_z = 1)
+CONFIG1_STRETCH = dict(dz_bot=50.0, dz_top=500.0, str_bot=500.0, str_top=17000.0, ztop=30000.0)
 
 
 def make(name, ni, nj, nk, nsteps, dt, bc, workdir="/tmp/cm1_fixture", opts=None, threads=1,
-         outdir=HERE, zd=15000.0):
+         outdir=HERE, zd=15000.0, dx=1000.0, dy=1000.0, dz=500.0, stretch=None):
     """Run the reference and write <outdir>/<name>.npz.  opts: namelist
     switches of the dynamical core (psolver, advwenos, advwenov, irdamp);
-    zd: the damping height when irdamp = 1."""
+    zd: the damping height when irdamp = 1; dx, dy, dz: the namelist grid
+    spacings; stretch: dict(dz_bot, dz_top, str_bot, str_top, ztop) for
+    stretch_z = 1 (param6)."""
     o = dict(OPT_DEFAULTS)
     o.update(opts or {})
     npt = o.pop("npt")
     ov = dict(nx=ni, ny=nj, nz=nk, dtl=dt, timax=nsteps * dt, tapfrq=nsteps * dt,
-              wbc=bc, ebc=bc, sbc=bc, nbc=bc, rdalpha=RDALPHA, zd=zd, **o)
+              wbc=bc, ebc=bc, sbc=bc, nbc=bc, rdalpha=RDALPHA, zd=zd, dx=dx, dy=dy, dz=dz, **o)
+    if stretch is not None:
+        ov.update(stretch_z=1, **{k: float(stretch[k]) for k in STRETCH_KEYS})
     if npt > 0:
         ov.update(iptra=1, npt=npt)
     o["npt"] = npt
@@ -53,7 +67,10 @@ def make(name, ni, nj, nk, nsteps, dt, bc, workdir="/tmp/cm1_fixture", opts=None
     W = refrun.read_grads(workdir, "cm1out_w.ctl")
     out = dict(meta=np.array([ni, nj, nk, nsteps, bc], np.int32), dt=np.array([dt], np.float32),
                opts=np.array([o[k] for k in OPT_KEYS], np.int32),
-               rdamp=np.array([RDALPHA, zd], np.float32))
+               rdamp=np.array([RDALPHA, zd], np.float32),
+               grid=np.array([dx, dy, dz, 0 if stretch is None else 1] +
+                             [0.0 if stretch is None else stretch[k] for k in STRETCH_KEYS],
+                             np.float32))
     for t, tag in ((0, "t0"), (1, "tN")):
         out[f"{tag}_u"] = U["u"][t]
         out[f"{tag}_v"] = V["v"][t]
@@ -69,13 +86,20 @@ def make(name, ni, nj, nk, nsteps, dt, bc, workdir="/tmp/cm1_fixture", opts=None
             out[f"{tag}_pt{n}"] = S[f"pt{n}"][t]
     for n in ("pi0", "th0", "qv0", "prs0"):
         out[n] = S[n][0, :, 0, 0]
+    # the reference's scalar-level heights (written in km) pin the grid set-up
+    out["zh"] = S["zhval"][0, :, 0, 0]
     path = os.path.join(outdir, f"{name}.npz")
     np.savez_compressed(path, **{k: np.ascontiguousarray(v) for k, v in out.items()})
     return path
 
 
 if __name__ == "__main__":
-    if len(sys.argv) > 1:
+    if len(sys.argv) > 1 and sys.argv[1] == "stretched":
+        # dt = 1.5 s: the reference's dtsmall picks nsound = 4; dt = 3 s: nsound = 8
+        for dt, tag in ((1.5, "dt1.5"), (3.0, "dt3")):
+            print(make(f"ref_stretch_{tag}_24x20x96_10", 24, 20, 96, 10, dt, 2,
+                       dx=250.0, dy=250.0, dz=500.0, stretch=CONFIG1_STRETCH))
+    elif len(sys.argv) > 1:
         a = sys.argv[1:]
         opts = dict(zip(OPT_KEYS[:4], (int(v) for v in a[7:11])))
         zd = float(a[11]) if len(a) > 11 else 15000.0

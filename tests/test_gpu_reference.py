@@ -27,7 +27,7 @@ TOL = 1.0e-5
 @pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
 def test_gpu_matches_reference_fixture(require_gpu, path):
     cfgd, cols, fields, nsteps, target, z = load_fixture(path)
-    cfgd["nsound"] = nsound_for(cfgd)
+    cfgd["nsound"] = nsound_for(cfgd, cols["zf"])
     g = make_gpu(cfgd, cols, fields)
     g.step(nsteps)
     for name, ref in target.items():

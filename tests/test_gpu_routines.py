@@ -18,13 +18,16 @@ import pytest
 
 import pyoracle
 from cm1_amd import _native as N
-from common import case, make_gpu, make_oracle
+from common import SMALL_STRETCH, case, make_gpu, make_oracle
 
 pytestmark = pytest.mark.gpu
 
 
-def pair(**kw):
-    cfgd, cols, fields = case(**kw)
+GRIDS = dict(uniform=None, stretched=SMALL_STRETCH)
+
+
+def pair(grid="uniform", **kw):
+    cfgd, cols, fields = case(stretch=GRIDS[grid], **kw)
     g = make_gpu(cfgd, cols, fields)
     o = make_oracle(cfgd, cols, fields)
     g.step(2)
@@ -63,20 +66,22 @@ def test_bc_fills(require_gpu, bc):
         assert nbad(g, o, "dum3") == 0, kind
 
 
+@pytest.mark.parametrize("grid", list(GRIDS))
 @pytest.mark.parametrize("weno", [0, 1], ids=["flx5", "weno5"])
 @pytest.mark.parametrize("nrk", [1, 3])
-def test_advu_advv_advw(require_gpu, weno, nrk):
-    cfgd, g, o = pair(advwenov=weno, advwenos=weno)
+def test_advu_advv_advw(require_gpu, weno, nrk, grid):
+    cfgd, g, o = pair(advwenov=weno, advwenos=weno, grid=grid)
     for name, ten in (("cm1g_advu", "uten"), ("cm1g_advv", "vten"), ("cm1g_advw", "wten")):
         gcall(g, name, nrk)
         getattr(o.L, name.replace("cm1g_", "cm1o_"))(o.h, nrk)
         check(g, o, [ten])
 
 
+@pytest.mark.parametrize("grid", list(GRIDS))
 @pytest.mark.parametrize("pdef", [0, 1], ids=["plain", "pdef"])
 @pytest.mark.parametrize("weno", [0, 1], ids=["flx5", "weno5"])
-def test_advs(require_gpu, pdef, weno):
-    cfgd, g, o = pair(advwenos=weno, advwenov=weno)
+def test_advs(require_gpu, pdef, weno, grid):
+    cfgd, g, o = pair(advwenos=weno, advwenov=weno, grid=grid)
     nrk = 3
     dttmp = np.float32(cfgd["dt"]) / np.float32(4 - nrk)
     weps = 1.0e-20
@@ -103,8 +108,9 @@ def test_advs(require_gpu, pdef, weno):
 @pytest.mark.parametrize("psolver,bc", [(3, 2), (3, 1), (2, 2)], ids=["sound-open", "sound-periodic",
                                                                        "sounde-open"])
 @pytest.mark.parametrize("nrk", [1, 2, 3])
-def test_sound(require_gpu, psolver, bc, nrk):
-    cfgd, g, o = pair(psolver=psolver, bc=bc, nsound=6 if psolver == 3 else 8)
+@pytest.mark.parametrize("grid", list(GRIDS))
+def test_sound(require_gpu, psolver, bc, nrk, grid):
+    cfgd, g, o = pair(psolver=psolver, bc=bc, nsound=6 if psolver == 3 else 8, grid=grid)
     nk, nj, ni = cfgd["nk"], cfgd["nj"], cfgd["ni"]
     # the solve2 operands of sound: thv (t11), ppterm (t22), piadv (sadv)
     # and the large-step tendencies, as solve2 prepares them
@@ -132,8 +138,9 @@ def test_sound(require_gpu, psolver, bc, nrk):
     assert all(v == 0 for v in bad.values()), bad
 
 
-def test_kessler_branch(require_gpu):
-    cfgd, g, o = pair()
+@pytest.mark.parametrize("grid", list(GRIDS))
+def test_kessler_branch(require_gpu, grid):
+    cfgd, g, o = pair(grid=grid)
     nk, nj, ni = cfgd["nk"], cfgd["nj"], cfgd["ni"]
     # cloud and rain in the lower half of the domain so autoconversion,
     # accretion, evaporation, fallout and saturation adjustment all act
@@ -209,8 +216,9 @@ def test_kessler_branch_signed_zeros(require_gpu):
     assert g.scalars()["train"] == o.scalars()["train"]
 
 
-def test_getdiv_getdivx_cfl(require_gpu):
-    cfgd, g, o = pair()
+@pytest.mark.parametrize("grid", list(GRIDS))
+def test_getdiv_getdivx_cfl(require_gpu, grid):
+    cfgd, g, o = pair(grid=grid)
     gcall(g, "cm1g_getdiv", N.FID["dum4"])
     o.L.cm1o_getdiv(o.h, pyoracle.FID["dum4"])
     check(g, o, ["dum4", "rru", "rrv", "rrw", "divx"])

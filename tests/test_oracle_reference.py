@@ -23,7 +23,11 @@ STAG = dict(PROG)
 @pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
 def test_oracle_matches_reference_fixture(path):
     cfgd, cols, fields, nsteps, target, z = load_fixture(path)
-    cfgd["nsound"] = nsound_for(cfgd)
+    cfgd["nsound"] = nsound_for(cfgd, cols["zf"])
+    if "zh" in z:
+        # the vertical grid restated in single precision equals the
+        # reference's own (its written scalar-level heights)
+        assert np.array_equal(cols["zh"][1:cfgd["nk"] + 1], z["zh"])
     o = make_oracle(cfgd, cols, fields)
     # base state derived by the oracle equals the reference's prs0
     assert np.array_equal(o.kf("prs0")[1:cfgd["nk"] + 1], z["prs0"])
