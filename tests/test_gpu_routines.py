@@ -110,7 +110,10 @@ def test_advs(require_gpu, pdef, weno, grid):
 @pytest.mark.parametrize("nrk", [1, 2, 3])
 @pytest.mark.parametrize("grid", list(GRIDS))
 def test_sound(require_gpu, psolver, bc, nrk, grid):
-    cfgd, g, o = pair(psolver=psolver, bc=bc, nsound=6 if psolver == 3 else 8, grid=grid)
+    # sounde's substep is bounded by the thinnest level (dtsmall with min_dz,
+    # src/misclibs.F:838-846): 44 substeps at dt = 6 s on the 100 m levels
+    ns = 6 if psolver == 3 else (8 if grid == "uniform" else 44)
+    cfgd, g, o = pair(psolver=psolver, bc=bc, nsound=ns, grid=grid)
     nk, nj, ni = cfgd["nk"], cfgd["nj"], cfgd["ni"]
     # the solve2 operands of sound: thv (t11), ppterm (t22), piadv (sadv)
     # and the large-step tendencies, as solve2 prepares them
