@@ -141,20 +141,6 @@ def bench_surrogate(args):
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    def harvest_during(m, nsteps, at, capacity, keep):
-        buf = torch.empty((capacity, 12), dtype=torch.float32, device="cuda")
-        used = 0
-        for s_ in range(nsteps):
-            if s_ in at and used < capacity:
-                N.check(L.cm1g_set_harvest(m.h, S._ptr(buf[used:]), capacity - used, keep), "cm1g_set_harvest")
-                m.step(1)
-                c = S.counts(m)
-                N.check(L.cm1g_set_harvest(m.h, None, 0, 1), "cm1g_set_harvest")
-                used += min(c["harvested"], capacity - used)
-            else:
-                m.step(1)
-        return buf[:used]
-
     def train(pairs):
         norm = S.fit_norm(pairs)
         x, t = S.normalise(pairs, norm)
@@ -198,7 +184,7 @@ def bench_surrogate(args):
     n1 = args.spinup
     at = {n1 // 2 + q * (n1 - n1 // 2) // 6 for q in range(6)}
     t0 = time.perf_counter()
-    pairs = harvest_during(m, n1, at, 16_000_000, 50)
+    pairs = S.harvest_spread(m, n1, at, 16_000_000, 50)
     state = m.save_state()
     log(f"surrogate (a): {n1} plain steps in {time.perf_counter() - t0:.1f} s, {pairs.shape[0]} pairs")
     net_a, norm_a, losses_a, tt_a = train(pairs)
@@ -215,7 +201,7 @@ def bench_surrogate(args):
     cfgd, cols, fields = workload_spinup()
     mb = model.setup(model.make_config(**cfgd), cols, fields)
     nb = 300
-    pairs = harvest_during(mb, nb, set(range(nb // 2, nb)), 12_000_000, 20)
+    pairs = S.harvest_spread(mb, nb, set(range(nb // 2, nb)), 12_000_000, 20)
     state = mb.save_state()
     net_b, norm_b, losses_b, tt_b = train(pairs)
     del pairs

@@ -116,6 +116,14 @@ SIGNATURES = {
     "cm1g_getdivx": (C.c_int, [C.c_void_p]),
     "cm1g_maxmin": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int32),
                               C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
+    "cm1g_put_2d_array": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "cm1g_get_2d_array": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "cm1g_k_fallout": (C.c_int, [C.c_void_p]),
+    "cm1g_fallout": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "cm1g_geterain": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
+                                C.POINTER(C.c_double)]),
+    "cm1g_kessler": (C.c_int, [C.c_void_p]),
+    "cm1g_satadj": (C.c_int, [C.c_void_p]),
     "cm1g_state_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "cm1g_state_save": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "cm1g_state_load": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),

@@ -115,6 +115,12 @@ void graphs_clear(cm1g_model *m);
 // the host side of one large step (solve1, solve2, mp_driver, solve3)
 int step_body(cm1g_model *m);
 int satadj_check(cm1g_model *m);
+// a valid 3-D field id of the model
+bool field_ok(const cm1g_model *m, int id);
+// CM1G_E_HIP if a launch since the last check failed
+int check_launch();
+// the rain budget of the fallout pass's columns added to train (src/kessler.F:470-473)
+void train_accumulate(cm1g_model *m);
 // every field pointer and the scalar updates' partner, and back
 std::vector<float *> field_map(const cm1g_model *m);
 void set_field_map(cm1g_model *m, const std::vector<float *> &v);

@@ -397,30 +397,22 @@ static void harvest_pairs(cm1g_model *m) {
   launch<64>(m, dim3(1), dim3(1), [=] __device__(int, int, int) { cnt[3] += 1ULL; });
 }
 
-void kessler_branch_run(cm1g_model *m) {
+// fallout: per column, one downward pass for the fall speed (k_fallout,
+// src/kessler.F:368-396), its maximum Courant number (taken over the speeds
+// before the fill, as the reference does; the maximum does not depend on the
+// order) and the fill of near-empty levels from the level above
+// (src/kessler.F:446-448); then a variable number of sedimentation substeps
+// (src/kessler.F:404-477).  Writes qr, rain, prate, the filled speeds into
+// qten(nqr) and each column's rain budget into the reduction workspace.
+static void fallout_run(cm1g_model *m) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   const float dt = m->c.dt;
-  float *th3d = m->f[CM1G_TH3D], *pp3d = m->f[CM1G_PP3D], *rho = m->f[CM1G_RHO];
-  float *prs = m->f[CM1G_PRS];
-  float *qv = m->f[CM1G_Q3D0], *qc = m->f[CM1G_Q3D1], *qr = m->f[CM1G_Q3D2];
+  const float *rho = m->f[CM1G_RHO];
+  float *qr = m->f[CM1G_Q3D2];
   float *vr = m->f[CM1G_QTEN2];
   float *rain = m->d2[CM1G_D_RAIN], *prate = m->d2[CM1G_D_PRATE];
   double *colbud = m->dred;
-  int *iflag = m->iflag;
-
-  // The reference's prelude (src/mp_driver.F:100-148) stores T, a copy of
-  // T in thten, a copy of qv in qten(nqv) and a copy of rho in dum3; in the
-  // Kessler branch these feed only budget diagnostics (geterain, the rr
-  // argument of kessler/satadj) and the other schemes' tendency checks, and
-  // kessler recomputes T itself, so nothing on this path reads them.
-  pdefq3_run(m, rho, qv, qc, qr);
-  // fallout: per column, one downward pass for the fall speed (k_fallout,
-  // src/kessler.F:368-396), its maximum Courant number (taken over the
-  // speeds before the fill, as the reference does; the maximum does not
-  // depend on the order) and the fill of near-empty levels from the level
-  // above (src/kessler.F:446-448); then a variable number of sedimentation
-  // substeps
   for2(m, 1, ni, 1, nj, [=] __device__(int i, int j) {
     const float rdz = P.rdz;
     float crmax = 0.0f, vabove = 0.0f;
@@ -463,6 +455,24 @@ void kessler_branch_run(cm1g_model *m) {
     prate[IX2D(P, i, j)] = pr;
     colbud[(i - 1) + (j - 1) * ni] = bud;
   });
+}
+
+void kessler_branch_run(cm1g_model *m) {
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj, nk = P.nk;
+  const float dt = m->c.dt;
+  float *th3d = m->f[CM1G_TH3D], *pp3d = m->f[CM1G_PP3D], *rho = m->f[CM1G_RHO];
+  float *prs = m->f[CM1G_PRS];
+  float *qv = m->f[CM1G_Q3D0], *qc = m->f[CM1G_Q3D1], *qr = m->f[CM1G_Q3D2];
+  int *iflag = m->iflag;
+
+  // The reference's prelude (src/mp_driver.F:100-148) stores T, a copy of
+  // T in thten, a copy of qv in qten(nqv) and a copy of rho in dum3; in the
+  // Kessler branch these feed only budget diagnostics (geterain, the rr
+  // argument of kessler/satadj) and the other schemes' tendency checks, and
+  // kessler recomputes T itself, so nothing on this path reads them.
+  pdefq3_run(m, rho, qv, qc, qr);
+  fallout_run(m);
   if (m->harv) {
     // the state entering kessler (after sedimentation)
     float *b0 = m->sx, *b1 = m->sy, *b2 = m->sz, *b3 = m->tx, *b4 = m->ty, *b5 = m->tz;
@@ -552,4 +562,101 @@ extern "C" int cm1g_surrogate_counts(cm1g_model *m, int64_t *out6) {
   HIPCHK(hipMemcpy(h, m->dcount, sizeof(h), hipMemcpyDeviceToHost));
   for (int q = 0; q < 6; q++) out6[q] = (int64_t)h[q];
   return 0;
+}
+
+// ------------------------------------------------- routine-level entries
+#define ARGCHK(cond, msg)  \
+  do {                     \
+    if (!(cond)) {         \
+      cm1g_set_error(msg); \
+      return CM1G_E_ARG;   \
+    }                      \
+  } while (0)
+
+// The routines of kessler_module one by one, for a host that calls them
+// itself (src/mp_driver.F:192-232): each on the model's fields th3d, pp3d,
+// prs, rho, q3d(nqv..nqr) and qten(nqr) as the fall speed.
+extern "C" int cm1g_k_fallout(cm1g_model *m) {
+  ARGCHK(m && m->c.imoist == 1, "cm1g_k_fallout: needs a moist model");
+  m->synced = 0;
+  const KP P = m->P;
+  const float *rho = m->f[CM1G_RHO], *qr = m->f[CM1G_Q3D2];
+  float *vr = m->f[CM1G_QTEN2];
+  // src/kessler.F:382-390
+  for3(m, 1, P.ni, 1, P.nj, 1, P.nk, [=] __device__(int i, int j, int k) {
+    const int c = IXD(P, i, j, k);
+    float v = 0.0f;
+    if (qr[c] > 1.0e-12f) v = 14.34f * pow_f(rho[c] * qr[c], 0.1346f) * sqrtf(1.15f / rho[c]);
+    vr[c] = v;
+  });
+  return check_launch();
+}
+
+extern "C" int cm1g_fallout(cm1g_model *m, double *train) {
+  ARGCHK(m && train && m->c.imoist == 1, "cm1g_fallout: bad argument");
+  m->synced = 0;
+  HIPCHK(hipMemcpyAsync(m->dscal + 3, train, sizeof(double), hipMemcpyHostToDevice, m->stream));
+  fallout_run(m);
+  train_accumulate(m);
+  HIPCHK(hipMemcpyAsync(train, m->dscal + 3, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return check_launch();
+}
+
+extern "C" int cm1g_geterain(cm1g_model *m, float dt, float cpx, float lx1, int t_id, int rr_id, int q_id,
+                             double *erain) {
+  ARGCHK(m && erain && field_ok(m, t_id) && field_ok(m, rr_id) && field_ok(m, q_id),
+         "cm1g_geterain: bad argument");
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj;
+  const float *t = m->f[t_id], *rr = m->f[rr_id], *q = m->f[q_id], *vr = m->f[CM1G_QTEN2];
+  double *bud = m->dred;
+  // src/kessler.F:545-552: per row j, a double sum over i of REAL products
+  launch<64>(m, dim3((nj + 63) / 64), dim3(64), [=] __device__(int bx, int, int) {
+    const int j = 1 + bx * 64 + (int)threadIdx.x;
+    if (j > nj) return;
+    double b = 0.0;
+    for (int i = 1; i <= ni; i++) {
+      const int c = IXD(P, i, j, 1);
+      b = b + (double)(dt * vr[c] * rr[c] * q[c] * (cpx * t[c] - lx1) * P.ruh[i + 2] * P.rvh[j + 2]);
+    }
+    bud[j - 1] = b;
+  });
+  std::vector<double> h(nj);
+  HIPCHK(hipMemcpyAsync(h.data(), bud, nj * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  double e = *erain;
+  for (int j = 0; j < nj; j++) e = e + h[j] * (double)P.dx * (double)P.dy;
+  *erain = e;
+  return check_launch();
+}
+
+extern "C" int cm1g_kessler(cm1g_model *m) {
+  ARGCHK(m && m->c.imoist == 1, "cm1g_kessler: needs a moist model");
+  m->synced = 0;
+  const KP P = m->P;
+  const float dt = m->c.dt;
+  float *th3d = m->f[CM1G_TH3D], *pp3d = m->f[CM1G_PP3D], *prs = m->f[CM1G_PRS];
+  float *qv = m->f[CM1G_Q3D0], *qc = m->f[CM1G_Q3D1], *qr = m->f[CM1G_Q3D2];
+  const float *rho = m->f[CM1G_RHO];
+  for3(m, 1, P.ni, 1, P.nj, 1, P.nk, [=] __device__(int i, int j, int k) {
+    kessler_cell(P, dt, IXD(P, i, j, k), k, th3d, pp3d, prs, qv, qc, qr, rho);
+  });
+  return check_launch();
+}
+
+extern "C" int cm1g_satadj(cm1g_model *m) {
+  ARGCHK(m && m->c.imoist == 1, "cm1g_satadj: needs a moist model");
+  m->synced = 0;
+  const KP P = m->P;
+  float *th3d = m->f[CM1G_TH3D], *pp3d = m->f[CM1G_PP3D], *prs = m->f[CM1G_PRS];
+  float *qv = m->f[CM1G_Q3D0], *qc = m->f[CM1G_Q3D1], *qr = m->f[CM1G_Q3D2];
+  const float *rho = m->f[CM1G_RHO];
+  int *iflag = m->iflag;
+  for3(m, 1, P.ni, 1, P.nj, 1, P.nk, [=] __device__(int i, int j, int k) {
+    satadj_cell(P, IXD(P, i, j, k), k, th3d, pp3d, prs, qv, qc, qr, rho, iflag);
+  });
+  int rc = check_launch();
+  if (rc) return rc;
+  return satadj_check(m);
 }

@@ -371,7 +371,7 @@ extern "C" int cm1g_base_derive(cm1g_model *m) {
   return 0;
 }
 
-static bool field_ok(const cm1g_model *m, int id) {
+bool field_ok(const cm1g_model *m, int id) {
   return m && id >= 0 && id < CM1G_NFIELDS && m->f[id] != nullptr;
 }
 
@@ -446,6 +446,50 @@ extern "C" int cm1g_get_2d(cm1g_model *m, int id, float *host) {
   size_t n = (size_t)m->P.pxy + (size_t)(m->P.px + m->P.py) * (m->c.nk + 3);
   HIPCHK(hipStreamSynchronize(m->stream));
   HIPCHK(hipMemcpy(host, m->d2[id], n * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// 2-D fields in the reference's own shapes: rain and prate (ib:ie,jb:je),
+// radbcw/radbce (jb:je,kb:ke), radbcs/radbcn (ib:ie,kb:ke), ngxy = 3,
+// ngz = 1, first index fastest.  Device positions: rain/prate at
+// IX2D(i,j); radbcw/e at (j+2) + k*py; radbcs/n at (i+2) + k*px.
+static void ref2d_map(const cm1g_model *m, int id, std::vector<size_t> &dev) {
+  const KP &P = m->P;
+  dev.clear();
+  if (id == CM1G_D_RAIN || id == CM1G_D_PRATE) {
+    for (int j = -2; j <= P.nj + 3; j++)
+      for (int i = -2; i <= P.ni + 3; i++) dev.push_back((size_t)IX2D(P, i, j));
+  } else if (id == CM1G_D_RADBCW || id == CM1G_D_RADBCE) {
+    for (int k = 0; k <= P.nk + 1; k++)
+      for (int j = -2; j <= P.nj + 3; j++) dev.push_back((size_t)(j + 2) + (size_t)k * P.py);
+  } else {
+    for (int k = 0; k <= P.nk + 1; k++)
+      for (int i = -2; i <= P.ni + 3; i++) dev.push_back((size_t)(i + 2) + (size_t)k * P.px);
+  }
+}
+
+extern "C" int cm1g_put_2d_array(cm1g_model *m, int id, const float *host) {
+  ARGCHK(m && host && id >= 0 && id < CM1G_D_NFIELDS, "cm1g_put_2d_array: bad argument");
+  const size_t n = (size_t)m->P.pxy + (size_t)(m->P.px + m->P.py) * (m->c.nk + 3);
+  std::vector<float> buf(n);
+  std::vector<size_t> dev;
+  ref2d_map(m, id, dev);
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(buf.data(), m->d2[id], n * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t q = 0; q < dev.size(); q++) buf[dev[q]] = host[q];
+  HIPCHK(hipMemcpy(m->d2[id], buf.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int cm1g_get_2d_array(cm1g_model *m, int id, float *host) {
+  ARGCHK(m && host && id >= 0 && id < CM1G_D_NFIELDS, "cm1g_get_2d_array: bad argument");
+  const size_t n = (size_t)m->P.pxy + (size_t)(m->P.px + m->P.py) * (m->c.nk + 3);
+  std::vector<float> buf(n);
+  std::vector<size_t> dev;
+  ref2d_map(m, id, dev);
+  HIPCHK(hipStreamSynchronize(m->stream));
+  HIPCHK(hipMemcpy(buf.data(), m->d2[id], n * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t q = 0; q < dev.size(); q++) host[q] = buf[dev[q]];
   return 0;
 }
 
@@ -585,7 +629,7 @@ extern "C" int cm1g_sync(cm1g_model *m) {
   return satadj_check(m);
 }
 
-static int check_launch() {
+int check_launch() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     cm1g_set_error(std::string("kernel launch: ") + hipGetErrorString(e));
@@ -686,7 +730,7 @@ __device__ __forceinline__ void train_sum(int nj, double tem, const double *rows
 }
 // train += the fallout's column budgets of this call (the end of
 // src/kessler.F:fallout)
-static void train_accumulate(cm1g_model *m) {
+void train_accumulate(cm1g_model *m) {
   const int ni = m->P.ni, nj = m->P.nj;
   const double *colbud = m->dred;
   double *rows = m->dred + (size_t)ni * nj, *dscal = m->dscal;

@@ -200,6 +200,43 @@ module cm1gpu
       integer(c_int), value :: id, stag
       real(c_float), intent(out) :: a(*)
     end function
+    integer(c_int) function cm1g_put_2d_array(m, id, a) bind(C, name="cm1g_put_2d_array")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      integer(c_int), value :: id
+      real(c_float), intent(in) :: a(*)
+    end function
+    integer(c_int) function cm1g_get_2d_array(m, id, a) bind(C, name="cm1g_get_2d_array")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      integer(c_int), value :: id
+      real(c_float), intent(out) :: a(*)
+    end function
+    integer(c_int) function cm1g_k_fallout(m) bind(C, name="cm1g_k_fallout")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_fallout(m, train) bind(C, name="cm1g_fallout")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: m
+      real(c_double), intent(inout) :: train
+    end function
+    integer(c_int) function cm1g_geterain(m, dt, cpx, lx1, t_id, rr_id, q_id, erain) &
+        bind(C, name="cm1g_geterain")
+      import :: c_int, c_ptr, c_float, c_double
+      type(c_ptr), value :: m
+      real(c_float), value :: dt, cpx, lx1
+      integer(c_int), value :: t_id, rr_id, q_id
+      real(c_double), intent(inout) :: erain
+    end function
+    integer(c_int) function cm1g_kessler(m) bind(C, name="cm1g_kessler")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_satadj(m) bind(C, name="cm1g_satadj")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
     integer(c_int) function cm1g_sync(m) bind(C, name="cm1g_sync")
       import :: c_int, c_ptr
       type(c_ptr), value :: m

@@ -129,6 +129,29 @@ def harvest(models, steps, capacity, keep_one_in=20):
     return buf[:used]
 
 
+def harvest_spread(model, nsteps, at, capacity, keep_one_in=20):
+    """Run a plain-Kessler model `nsteps` large steps, collecting the
+    (inputs, changes over kessler) pairs of the steps whose index (0-based)
+    is in `at`, so the training data spread over the run; returns a torch
+    cuda tensor [n, 12] (at most `capacity` rows)."""
+    import torch
+    L = N.lib()
+    buf = torch.empty((capacity, 12), dtype=torch.float32, device="cuda")
+    used = 0
+    for s in range(nsteps):
+        if s in at and used < capacity:
+            # the count restarts at every cm1g_set_harvest
+            N.check(L.cm1g_set_harvest(model.h, _ptr(buf[used:]), capacity - used, keep_one_in),
+                    "cm1g_set_harvest")
+            model.step(1)
+            got = counts(model)["harvested"]
+            N.check(L.cm1g_set_harvest(model.h, None, 0, 1), "cm1g_set_harvest")
+            used += min(got, capacity - used)
+        else:
+            model.step(1)
+    return buf[:used]
+
+
 def fit_norm(pairs, margin=0.1):
     """Normalisation of raw pairs [n, 12]: per-column mean and standard
     deviation of inputs and changes (a constant column, such as dt, gets a

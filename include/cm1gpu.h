@@ -136,6 +136,12 @@ int cm1g_put_array(cm1g_model *m, int id, int stag, const float *host);
 int cm1g_get_array(cm1g_model *m, int id, int stag, float *host);
 int cm1g_put_2d(cm1g_model *m, int id, const float *host);
 int cm1g_get_2d(cm1g_model *m, int id, float *host);
+/* The same for a 2-D host array in the reference's own shape: rain and
+ * prate (ib:ie,jb:je) (one slice of rain(ib:ie,jb:je,nrain)), radbcw and
+ * radbce (jb:je,kb:ke), radbcs and radbcn (ib:ie,kb:ke) (src/cm1.F:
+ * 747-777, src/bc.F:2054-2172), first index fastest. */
+int cm1g_put_2d_array(cm1g_model *m, int id, const float *host);
+int cm1g_get_2d_array(cm1g_model *m, int id, float *host);
 /* Device address of a field (for zero-copy use by a caller that already
  * holds device memory, e.g. a Fortran host with its own device mirrors).
  * The address is valid until the next call that runs kernels on the handle:
@@ -271,6 +277,25 @@ int cm1g_sound(cm1g_model *m, int nrk, float dttmp);
 /* Kessler branch of mp_driver: pdefq x3, k_fallout, fallout, kessler,
  * satadj (src/mp_driver.F:190-232, src/kessler.F). */
 int cm1g_kessler_branch(cm1g_model *m);
+/* The routines of kessler_module one by one, on the model's th3d, pp3d,
+ * prs, rho and q3d(nqv..nqr) fields, with qten(nqr) (CM1G_QTEN2) as the
+ * fall speed, in the order mp_driver calls them (src/mp_driver.F:195-232):
+ * k_fallout(rho,qr3d,vr), src/kessler.F:368-396: vr into CM1G_QTEN2.
+ * fallout(dt,train,...,rain,prate,rr,rho,q3d,vq), src/kessler.F:404-477:
+ *   sedimentation of qr, rain/prate, and *train (host in/out) += the
+ *   surface rain budget.
+ * geterain(dt,cpx,lx1,erain,ruh,rvh,t,rr,q3d,vr), src/kessler.F:530-562:
+ *   *erain (host in/out) += the rain-energy budget of fields t_id, rr_id,
+ *   q_id with the fall speed CM1G_QTEN2.
+ * kessler(dt,...), src/kessler.F:8-138 (the process budgets tauto, taccr,
+ *   tevar are not formed).
+ * satadj(4,dt,...), src/kessler.F:146-360 (tcond, tevac not formed);
+ *   CM1G_E_SATADJ where the adjustment does not converge. */
+int cm1g_k_fallout(cm1g_model *m);
+int cm1g_fallout(cm1g_model *m, double *train);
+int cm1g_geterain(cm1g_model *m, float dt, float cpx, float lx1, int t_id, int rr_id, int q_id, double *erain);
+int cm1g_kessler(cm1g_model *m);
+int cm1g_satadj(cm1g_model *m);
 /* pdefq(rmax,asq,...,rho,q3d), src/misclibs.F:1037-1117 */
 int cm1g_pdefq(cm1g_model *m, float rmax, int q_id);
 /* Boundary fills, src/bc.F: kind 0 bcs :180, 1 bcp :372, 2 bcu :528,
