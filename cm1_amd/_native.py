@@ -116,6 +116,9 @@ SIGNATURES = {
     "cm1g_getdivx": (C.c_int, [C.c_void_p]),
     "cm1g_maxmin": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int32),
                               C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
+    "cm1g_maxmin_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                      C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_float),
+                                      C.POINTER(C.c_int32)]),
     "cm1g_put_2d_array": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cm1g_get_2d_array": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cm1g_k_fallout": (C.c_int, [C.c_void_p]),

@@ -482,6 +482,15 @@ module cm1gpu
       real(c_float), intent(out) :: vmax, vmin
       integer(c_int32_t), intent(out) :: locmax(3), locmin(3)
     end function
+    ! no counterpart in the reference: maxmin of a device field without allocation
+    integer(c_int) function cm1g_maxmin_noalloc(field, ni, nj, nk, stag, stream, vmax, locmax, vmin, locmin) &
+        bind(C, name="cm1g_maxmin_noalloc")
+      import :: c_int, c_ptr, c_float, c_int32_t
+      type(c_ptr), value :: field, stream
+      integer(c_int), value :: ni, nj, nk, stag
+      real(c_float), intent(out) :: vmax, vmin
+      integer(c_int32_t), intent(out) :: locmax(3), locmin(3)
+    end function
     integer(c_int) function cm1g_scalars(m, out4) bind(C, name="cm1g_scalars")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: m

@@ -315,6 +315,20 @@ int cm1g_getdivx(cm1g_model *m);
  * 214) and their (i,j,k): among equal extremes the smallest k, then j, then
  * i, as maxmin's strict comparisons pick them.  Reduced on the device. */
 int cm1g_maxmin(cm1g_model *m, int id, int stag, float *vmax, int32_t *locmax, float *vmin, int32_t *locmin);
+/* No counterpart in the reference.  cm1g_maxmin for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same result for a device
+ * field `field` in this library's layout (as cm1g_field_ptr returns it) of
+ * an ni x nj x nk grid, with no model handle and no allocation.  Its
+ * partial results live in a file-scope device array of 64 KiB per GPU,
+ * held until the process ends; ni, nj and nk must each lie in
+ * 1..CM1G_NOALLOC_MAXN and the interior below 2^32-1 points, else
+ * CM1G_E_ARG.  Runs on `stream` (a hipStream_t, NULL for the default
+ * stream) and waits for it.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU race. */
+#define CM1G_NOALLOC_MAXN 16384
+int cm1g_maxmin_noalloc(const float *field, int ni, int nj, int nk, int stag, void *stream,
+                        float *vmax, int32_t *locmax, float *vmin, int32_t *locmin);
 /* dtsmall: nsound for dt (src/misclibs.F:818-915).  min_dz is the smallest
  * level thickness, min over k of zf(k+1)-zf(k) (src/param.F:8522-8541),
  * which psolver 2 uses; min_dz <= 0 takes cfg->dz (an unstretched grid). */
