@@ -305,47 +305,93 @@ def bench_cbl(args):
     return 0
 
 
-def cpu_baseline(threads):
-    """Reference CM1 (FP32 OpenMP build) on a 256x256x96 sample of the case."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    try:
-        import refrun
-    except Exception as e:  # pragma: no cover
-        return dict(value=None, unit="cell-updates/s", cores=threads, kind="reference",
-                    sample=f"unavailable: {e}")
-    if not os.path.exists(refrun.REF_EXE):
-        return dict(value=None, unit="cell-updates/s", cores=threads, kind="reference",
-                    sample="oracle/_ref/cm1_sp.exe not built")
-    ni, nj, nk, nsteps = 256, 256, 96, 6
-    ov = dict(nx=ni, ny=nj, nz=nk, dx=250.0, dy=250.0, dz=500.0, dtl=1.5, timax=nsteps * 1.5,
-              tapfrq=1.0e9, stretch_z=1, ztop=STRETCH["ztop"], str_bot=STRETCH["str_bot"],
-              str_top=STRETCH["str_top"], dz_bot=STRETCH["dz_bot"], dz_top=STRETCH["dz_top"])
-    wd = os.path.join("/tmp", f"cm1_cpu_baseline_{os.getpid()}")
-    try:
-        out, wall = refrun.run_reference(wd, ov, threads=threads, timeout=900,
-                                         extra_env={"OMP_PROC_BIND": "spread", "OMP_PLACES": "cores"})
-    except Exception as e:
-        return dict(value=None, unit="cell-updates/s", cores=threads, kind="reference",
-                    sample=f"run failed: {str(e)[:200]}")
-    ts = refrun.step_times(out)
-    if len(ts) > 1:
-        ts = ts[1:]
-    t = float(np.median(ts)) if ts else float("nan")
-    cpu_model = "unknown CPU"
+def host_cpu():
+    """The host's CPU model, sockets and physical cores (/proc/cpuinfo), the
+    CPUs this process may run on, and the CPU quota of its cgroup."""
+    model, pairs = "unknown CPU", set()
+    phys = None
     try:
         with open("/proc/cpuinfo") as f:
             for ln in f:
-                if ln.startswith("model name"):
-                    cpu_model = ln.split(":", 1)[1].strip()
-                    break
+                if ln.startswith("model name") and model == "unknown CPU":
+                    model = ln.split(":", 1)[1].strip()
+                elif ln.startswith("physical id"):
+                    phys = ln.split(":", 1)[1].strip()
+                elif ln.startswith("core id"):
+                    pairs.add((phys, ln.split(":", 1)[1].strip()))
     except OSError:
         pass
-    return dict(value=ni * nj * nk / t, unit="cell-updates/s", cores=threads, kind="reference",
-                cpu=cpu_model,
-                sample=(f"reference CM1 FP32 OpenMP build, {ni}x{nj}x{nk} subdomain of the same case "
-                        f"(dx=250 m, stretched dz, dt=1.5 s, Kessler, open BCs), {nsteps} steps, "
-                        f"median per-step time {t:.3f} s excluding step 1; nsound chosen by the "
-                        f"reference (4, vs 8 on the GPU)"))
+    sockets = len({p for p, _ in pairs}) or None
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    return dict(cpu=model, sockets=sockets, physical_cores=len(pairs) or None, logical_cpus=os.cpu_count(),
+                affinity_cpus=affinity, cgroup_cpu_quota=quota)
+
+
+def cpu_baseline(threads=None):
+    """Reference CM1 (FP32 OpenMP build) on a 256x256x96 sample of the case,
+    plus a one-thread run of the 64x64x40 supercell."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    host = host_cpu()
+    if threads is None:
+        # every CPU this job may use: the cgroup quota where one is set (the
+        # host's other cores belong to other jobs), else the affinity set
+        threads = int(host["cgroup_cpu_quota"] or host["affinity_cpus"])
+        threads = max(1, min(threads, host["affinity_cpus"]))
+    base = dict(unit="cell-updates/s", cores=threads, threads=threads, kind="reference", **host)
+    try:
+        import refrun
+    except Exception as e:  # pragma: no cover
+        return dict(base, value=None, sample=f"unavailable: {e}")
+    if not os.path.exists(refrun.REF_EXE):
+        return dict(base, value=None, sample="oracle/_ref/cm1_sp.exe not built")
+    env = {"OMP_PROC_BIND": "spread", "OMP_PLACES": "cores"}
+
+    def timed(ov, nthreads, tag):
+        wd = os.path.join("/tmp", f"cm1_cpu_baseline_{tag}_{os.getpid()}")
+        out, _ = refrun.run_reference(wd, ov, threads=nthreads, timeout=900, extra_env=env)
+        ts = refrun.step_times(out)
+        if len(ts) > 1:
+            ts = ts[1:]
+        return float(np.median(ts)) if ts else float("nan")
+
+    # the 1024x1024x96 case on a 256x256x96 horizontal sample: same dx,
+    # stretched grid, Kessler, open boundaries; dt = 3 s makes the
+    # reference's own dtsmall choose nsound = 8 (15 acoustic substeps per
+    # step, as the GPU runs at dt = 1.5 s with nsound fixed at 8), so a step
+    # does the same work
+    ni, nj, nk, nsteps, dt = 256, 256, 96, 6, 3.0
+    ov = dict(nx=ni, ny=nj, nz=nk, dx=250.0, dy=250.0, dz=500.0, dtl=dt, timax=nsteps * dt,
+              tapfrq=1.0e9, stretch_z=1, ztop=STRETCH["ztop"], str_bot=STRETCH["str_bot"],
+              str_top=STRETCH["str_top"], dz_bot=STRETCH["dz_bot"], dz_top=STRETCH["dz_top"])
+    try:
+        t = timed(ov, threads, "sample")
+    except Exception as e:
+        return dict(base, value=None, sample=f"run failed: {str(e)[:200]}")
+    res = dict(base, value=ni * nj * nk / t,
+               sample=(f"reference CM1 FP32 OpenMP build ({threads} threads, OMP_PROC_BIND=spread, "
+                       f"OMP_PLACES=cores), {ni}x{nj}x{nk} horizontal sample of the 1024x1024x96 case "
+                       f"(dx=250 m, stretched dz, Kessler, open BCs, dt={dt:g} s so that its dtsmall picks "
+                       f"nsound=8: 15 acoustic substeps per step as on the GPU), {nsteps} steps, median "
+                       f"per-step time {t:.3f} s excluding step 1"))
+    # the 64x64x40 supercell (dx = 1 km, dz = 500 m, dt = 6 s) on one thread
+    try:
+        t1 = timed(dict(nx=64, ny=64, nz=40, timax=8 * 6.0, tapfrq=1.0e9), 1, "config0")
+        res["config0_1thread"] = dict(value=64 * 64 * 40 / t1, unit="cell-updates/s",
+                                      ms_per_step=round(t1 * 1000.0, 2), steps=8)
+    except Exception as e:
+        res["config0_1thread"] = dict(value=None, error=str(e)[:200])
+    return res
 
 
 def measured_traffic(kernel):
@@ -369,6 +415,8 @@ def main():
     ap.add_argument("--nj", type=int, default=1024)
     ap.add_argument("--nk", type=int, default=96)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="threads of the CPU baseline (default: the CPUs this job may use)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (reference kernel for HBM counters)")
     ap.add_argument("--ensemble", type=int, default=0,
@@ -455,8 +503,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(threads)
+        cpu = cpu_baseline(args.cpu_threads)
 
     line = {
         "metric": "cell-updates/s per RK3 step (FP32)",

@@ -108,6 +108,7 @@ struct cm1g_model {
   unsigned long long *dcount;   // [0] pairs harvested [1] surrogate local steps [2] fallback cells
                                 // [3] harvest calls [4] cells out of the trained range [5] unusable proposals
   int *colflag;                 // per column: the surrogate fell back to plain Kessler
+  int *actlist;                 // the surrogate's list of cells kessler can act on (ni*nj*nk)
 };
 
 // drop the captured steps (after anything that changes launch arguments)

@@ -245,34 +245,66 @@ __device__ __forceinline__ bool sur_project(const KessAct &a, float qv, float qc
   return true;
 }
 
-static void surrogate_kessler(cm1g_model *m) {
+// The surrogate in place of kessler, followed by satadj (nrk = 4).  Three
+// launches: (1) a pass that lists the cells where one of kessler's
+// processes can act (wavefront ballots, one atomic add per wavefront; the
+// list order does not matter, every cell's result is its own); (2) the
+// fused network over the listed cells only, writing projected proposals and
+// flagging columns that must fall back; (3) one pointwise pass that applies
+// the proposal (or plain kessler in a flagged column) and then satadj, as
+// the plain path's fused kessler + satadj pass does.
+static void surrogate_kessler_satadj(cm1g_model *m) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   const float dt = m->c.dt;
   float *th3d = m->f[CM1G_TH3D], *pp3d = m->f[CM1G_PP3D], *rho = m->f[CM1G_RHO], *prs = m->f[CM1G_PRS];
   float *qv = m->f[CM1G_Q3D0], *qc = m->f[CM1G_Q3D1], *qr = m->f[CM1G_Q3D2];
   float *pqv = m->sy, *pqc = m->sz, *pqr = m->tx;  // the projected proposals
-  int *colflag = m->colflag;
-  unsigned long long *cnt = m->dcount;
+  int *colflag = m->colflag, *list = m->actlist, *iflag = m->iflag;
+  unsigned long long *cnt = m->dcount, *nact = m->dcount + 6;
   const float *zh = m->kf[CM1G_K_ZH];
   const float rztop = 1.0f / m->hk[CM1G_K_ZF][nk + 1];
   const float *prm = mlp_params(m->sur), *nm = mlp_norm(m->sur);
   for2(m, 1, ni, 1, nj, [=] __device__(int i, int j) {
     colflag[(i - 1) + (j - 1) * ni] = 0;
-    if (i == 1 && j == 1) cnt[1] += 1ULL;
+    if (i == 1 && j == 1) {
+      cnt[1] += 1ULL;
+      *nact = 0ULL;
+    }
   });
   const int64_t n = (int64_t)ni * nj * nk;
-  // (n < 2^31: cm1g_set_surrogate checks it)
-  auto cell = [=] __device__(int64_t s64, int &i, int &j, int &k) {
-    const int s = (int)s64, r = s / ni;
+  // (n < 2^31: cm1g_set_surrogate checks it); cell s: i fastest, then j, k
+  auto cell = [=] __device__(int s, int &i, int &j, int &k) {
+    const int r = s / ni;
     i = 1 + (s - r * ni);
     j = 1 + (r % nj);
     k = 1 + r / nj;
   };
-  // raw values of a cell: theta, pi', p, rho, qv, qc, qr
-  auto load = [=] __device__(int64_t s, float *r) {
+  launch<256>(m, dim3((unsigned)((n + 255) / 256)), dim3(256), [=] __device__(int bx, int, int) {
+    const int s = bx * 256 + (int)threadIdx.x;
+    bool act = false;
+    if (s < n) {
+      int i, j, k;
+      cell(s, i, j, k);
+      const int c = IXD(P, i, j, k);
+      const float c_ = qc[c], r_ = qr[c];
+      act = c_ > 0.001f || (c_ > 0.0f && r_ > 0.0f);
+      if (!act && r_ > 0.0f) {
+        const KessAct a = kess_active(P, k, th3d[c], pp3d[c], prs[c], qv[c], c_, r_);
+        act = a.er;
+      }
+    }
+    const unsigned long long mask = __ballot(act);
+    const int lane = (int)(threadIdx.x & 63);
+    unsigned long long base = 0ULL;
+    if (lane == 0 && mask) base = atomicAdd(nact, (unsigned long long)__popcll(mask));
+    base = __shfl(base, 0, 64);
+    if (act) list[base + (unsigned long long)__popcll(mask & ((1ULL << lane) - 1ULL))] = s;
+  });
+  // raw values of a listed cell: theta, pi', p, rho, qv, qc, qr
+  auto load = [=] __device__(int64_t t, float *r) {
     int i, j, k;
-    cell(s, i, j, k);
+    cell(list[t], i, j, k);
     const int c = IXD(P, i, j, k);
     r[0] = th3d[c];
     r[1] = pp3d[c];
@@ -282,18 +314,18 @@ static void surrogate_kessler(cm1g_model *m) {
     r[5] = qc[c];
     r[6] = qr[c];
   };
-  auto prep = [=] __device__(int64_t s, const float *r, float *x) {
+  auto prep = [=] __device__(int64_t t, const float *r, float *x) {
     int i, j, k;
-    cell(s, i, j, k);
+    cell(list[t], i, j, k);
     const float raw[MLP_IN] = {(P.th0[k] + r[0]) * (P.pi0[k] + r[1]), r[2], r[3], r[4], r[5], r[6], dt,
                                zh[k] * rztop};
     for (int q = 0; q < MLP_IN; q++) x[q] = (raw[q] - nm[q]) / nm[MLP_IN + q];
   };
-  // cells kessler changes: outside the trusted input range, or with an
-  // unusable proposal, the column falls back
-  auto out = [=] __device__(int64_t s, const float *y, const float *r) {
+  // outside the trusted input range, or with an unusable proposal, the
+  // column falls back
+  auto out = [=] __device__(int64_t t, const float *y, const float *r) {
     int i, j, k;
-    cell(s, i, j, k);
+    cell(list[t], i, j, k);
     const int c = IXD(P, i, j, k);
     const KessAct a = kess_active(P, k, r[0], r[1], r[2], r[4], r[5], r[6]);
     if (!(a.ar || a.cr || a.er)) return;
@@ -319,37 +351,39 @@ static void surrogate_kessler(cm1g_model *m) {
   const int64_t groups = (n + 16 * MLP_G * MLP_WAVES - 1) / (16 * MLP_G * MLP_WAVES);
   const int nbx = (int)(groups < 1024 ? groups : 1024);
   launch<512>(m, dim3(nbx), dim3(512),
-              [=] __device__(int bx, int, int) { mlp_tiles(prm, n, load, prep, out, bx, nbx); });
+              [=] __device__(int bx, int, int) { mlp_tiles(prm, n, nact, load, prep, out, bx, nbx); });
   // apply the projected proposal as kessler applies its own result (the
   // theta, pi', p update after evaporation above 1e-7), or plain kessler in
-  // flagged columns
+  // flagged columns; then satadj
   for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
     const Consts &K = P.K;
     const int c = IXD(P, i, j, k);
     if (colflag[(i - 1) + (j - 1) * ni]) {
       kessler_cell(P, dt, c, k, th3d, pp3d, prs, qv, qc, qr, rho);
       atomicAdd(&cnt[2], 1ULL);
-      return;
+    } else {
+      const KessAct a = kess_active(P, k, th3d[c], pp3d[c], prs[c], qv[c], qc[c], qr[c]);
+      if (a.ar || a.cr || a.er) {
+        const float qvnew = pqv[c], qcnew = pqc[c], qrnew = pqr[c];
+        const float er = qvnew - qv[c];
+        if (er > 1.0e-7f) {
+          float th = th3d[c], pp = pp3d[c];
+          const float cpml = K.cp + K.cpv * qvnew + K.cpl * (qcnew + qrnew);
+          const float cvml = K.cv + K.cvv * qvnew + K.cpl * (qcnew + qrnew);
+          const float rm = K.rd + K.rv * qvnew;
+          th = th - er * ((K.lv1 - K.lv2 * a.t) / (K.cpdcv * cvml * (P.pi0[k] + pp)) -
+                          (P.th0[k] + th) * (K.rv / cvml) * (1.0f - K.rovcp * cpml / rm));
+          pp = pow_f(rho[c] * rm * (P.th0[k] + th) * K.rp00, K.rddcv) - P.pi0[k];
+          th3d[c] = th;
+          pp3d[c] = pp;
+          prs[c] = K.p00 * pow_f(P.pi0[k] + pp, K.cpdrd);
+        }
+        qv[c] = qvnew;
+        qc[c] = qcnew;
+        qr[c] = qrnew;
+      }
     }
-    const KessAct a = kess_active(P, k, th3d[c], pp3d[c], prs[c], qv[c], qc[c], qr[c]);
-    if (!(a.ar || a.cr || a.er)) return;
-    const float qvnew = pqv[c], qcnew = pqc[c], qrnew = pqr[c];
-    const float er = qvnew - qv[c];
-    if (er > 1.0e-7f) {
-      float th = th3d[c], pp = pp3d[c];
-      const float cpml = K.cp + K.cpv * qvnew + K.cpl * (qcnew + qrnew);
-      const float cvml = K.cv + K.cvv * qvnew + K.cpl * (qcnew + qrnew);
-      const float rm = K.rd + K.rv * qvnew;
-      th = th - er * ((K.lv1 - K.lv2 * a.t) / (K.cpdcv * cvml * (P.pi0[k] + pp)) -
-                      (P.th0[k] + th) * (K.rv / cvml) * (1.0f - K.rovcp * cpml / rm));
-      pp = pow_f(rho[c] * rm * (P.th0[k] + th) * K.rp00, K.rddcv) - P.pi0[k];
-      th3d[c] = th;
-      pp3d[c] = pp;
-      prs[c] = K.p00 * pow_f(P.pi0[k] + pp, K.cpdrd);
-    }
-    qv[c] = qvnew;
-    qc[c] = qcnew;
-    qr[c] = qrnew;
+    satadj_cell(P, c, k, th3d, pp3d, prs, qv, qc, qr, rho, iflag);
   });
 }
 
@@ -497,7 +531,8 @@ void kessler_branch_run(cm1g_model *m) {
     return;
   }
   if (m->sur) {
-    surrogate_kessler(m);
+    surrogate_kessler_satadj(m);
+    return;
   } else {
     // kessler (pointwise)
     for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
@@ -528,6 +563,15 @@ extern "C" int cm1g_set_surrogate(cm1g_model *m, cm1g_mlp *net) {
   if (net && (int64_t)m->c.ni * m->c.nj * m->c.nk >= (1LL << 31)) {
     cm1g_set_error("cm1g_set_surrogate: more than 2^31 cells");
     return CM1G_E_ARG;
+  }
+  if (net && !m->actlist) {
+    // the list of cells the network runs on: at most every cell
+    HIPCHK(hipStreamSynchronize(m->stream));
+    if (hipMalloc((void **)&m->actlist, (size_t)m->c.ni * m->c.nj * m->c.nk * sizeof(int)) != hipSuccess) {
+      m->actlist = nullptr;
+      cm1g_set_error("cm1g_set_surrogate: cannot allocate the cell list");
+      return CM1G_E_NOMEM;
+    }
   }
   m->sur = net;
   graphs_clear(m);

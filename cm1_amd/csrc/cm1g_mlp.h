@@ -163,10 +163,16 @@ __device__ __forceinline__ void mlp_hidden(bf16x4 (&x)[MLP_G][8], const bf16x8s 
 // matrix work.  Workgroup bx of nbx stages the weights once; its
 // wavefronts then run independently.
 constexpr int MLP_KEEP = 8;
+// n_dev, if not null, holds a count (in device memory, written by an
+// earlier launch) that caps n: the samples are then 0..min(n, *n_dev)-1.
 template <class LOAD, class PREP, class OUT>
-__device__ void mlp_tiles(const float *__restrict__ prm, int64_t n, const LOAD &load, const PREP &prep,
-                          const OUT &out, int bx, int nbx) {
+__device__ void mlp_tiles(const float *__restrict__ prm, int64_t n, const unsigned long long *n_dev,
+                          const LOAD &load, const PREP &prep, const OUT &out, int bx, int nbx) {
   __shared__ MlpLds S;
+  if (n_dev) {
+    const int64_t nd = (int64_t)*n_dev;
+    n = nd < n ? nd : n;
+  }
   const int tid = (int)threadIdx.x, l = tid & 63, w = tid >> 6;
   unsigned short *w1 = (unsigned short *)S.w1, *w2 = (unsigned short *)S.w2, *w3 = (unsigned short *)S.w3,
                  *w4 = (unsigned short *)S.w4;

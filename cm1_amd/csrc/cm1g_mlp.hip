@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(256) k_adam(float *__restrict__ p, const float
 // fused inference over a row-major [n x 8] input, [n x 4] output
 template <class LOAD, class PREP, class OUT>
 __global__ void __launch_bounds__(512) k_mlp(const float *prm, int64_t n, LOAD load, PREP prep, OUT out) {
-  mlp_tiles(prm, n, load, prep, out, (int)blockIdx.x, (int)gridDim.x);
+  mlp_tiles(prm, n, nullptr, load, prep, out, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ------------------------------------------------------------ C ABI

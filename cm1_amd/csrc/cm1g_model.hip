@@ -236,6 +236,7 @@ extern "C" int cm1g_destroy(cm1g_model *m) {
   if (m->iflag) hipFree(m->iflag);
   if (m->dcount) hipFree(m->dcount);
   if (m->colflag) hipFree(m->colflag);
+  if (m->actlist) hipFree(m->actlist);
   for (hipEvent_t e : m->kt_ev) hipEventDestroy(e);
   if (m->stream) hipStreamDestroy(m->stream);
   delete m;

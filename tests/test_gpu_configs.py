@@ -71,21 +71,30 @@ def test_config4_surrogate_on_the_stretched_grid(require_gpu):
     import torch
     from cm1_amd import surrogate as S
     torch.cuda.init()
-    cfgd, cols, fields = workloads.config1(ni=96, nj=96)
+    cfgd, cols, fields = workloads.config1(ni=192, nj=192)
     assert cfgd["nk"] == 96 and cfgd["dt"] == 1.5 and cfgd["nsound"] == 8
     m = make_gpu(cfgd, cols, fields)
     # plain spin-up until rain reaches the ground (at most 4000 steps,
-    # 100 minutes), then 400 more steps with pairs harvested on every
-    # fourth, then training (10,000 Adam steps of 65,536 pairs)
-    steps = 0
-    while m.scalars()["train"] <= 0.0 and steps < 4000:
+    # 100 minutes) and has kept falling for 200 more steps, then 400 steps
+    # with pairs harvested on every fourth, then training (10,000 Adam
+    # steps of 65,536 pairs)
+    steps, first = 0, None
+    while steps < 4000:
+        r_before = m.scalars()["train"]
         m.step(100)
         steps += 100
-        print(f"spin-up step {steps}: rain {m.scalars()['train']:.3e}", flush=True)
-    assert m.scalars()["train"] > 0.0, steps
+        r_after = m.scalars()["train"]
+        print(f"spin-up step {steps}: rain {r_after:.3e}", flush=True)
+        if r_after > 0.0 and first is None:
+            first = steps
+        if first is not None and steps >= first + 200 and r_after > r_before:
+            break
+    assert first is not None and m.scalars()["train"] > 0.0, steps
+    r0h = m.scalars()["train"]
     pairs = S.harvest_spread(m, 400, set(range(0, 400, 4)), capacity=12_000_000, keep_one_in=20)
     state = m.save_state()
-    print(f"{pairs.shape[0]} pairs harvested over 400 steps", flush=True)
+    print(f"{pairs.shape[0]} pairs harvested over 400 steps (rain over them "
+          f"{m.scalars()['train'] - r0h:.3e})", flush=True)
     norm = S.fit_norm(pairs)
     x, t = S.normalise(pairs, norm)
     net = S.Net(seed=1)
@@ -99,6 +108,7 @@ def test_config4_surrogate_on_the_stretched_grid(require_gpu):
     r0 = m.scalars()["train"]
     m.step(nsteps)
     rain_plain = m.scalars()["train"] - r0
+    assert rain_plain > 0.0, "no rain reached the ground in the 60 compared steps"
 
     m.load_state(state)
     S.use(m, net)
@@ -114,7 +124,6 @@ def test_config4_surrogate_on_the_stretched_grid(require_gpu):
     ncell = m.ni * m.nj * m.nk * nsteps
     print(f"rain over {nsteps} steps: plain {rain_plain:.6e} surrogate {rain_sur:.6e} (rel error {err:.2e}); "
           f"fallback cells {fb} of {ncell}; counts {c1}")
-    assert rain_plain > 0.0
     assert err <= 0.02, err
     assert c1["surrogate_steps"] - c0["surrogate_steps"] == nsteps
     assert c1["unusable"] == c0["unusable"]

@@ -45,7 +45,9 @@ def test_training_kernels_match_oracle(require_gpu, rows):
     for (name, _), a, b in zip(R.SIZES, offs[:-1], offs[1:]):
         e = rel(g[a:b], g_r[a:b])
         print(f"{name}: rel L2 {e:.2e}")
-        assert e < 5.0e-3, (name, e)
+        # measured at most 7.2e-5 (W1, 300 rows): BF16 operand rounding
+        # shared with the oracle, FP32 against float64 accumulation
+        assert e < 1.0e-4, (name, e)
     net.close()
 
 
@@ -59,7 +61,9 @@ def test_fused_inference_matches_oracle(require_gpu):
     y_r = R.forward(p, x.astype(np.float64), rnd=True)[0]
     e = rel(y, y_r)
     print(f"fused forward vs oracle: rel L2 {e:.2e}")
-    assert e < 5.0e-3, e
+    # measured 9.5e-5: the fused kernel also rounds each layer's
+    # activations to BF16, as the oracle does, with FP32 accumulation
+    assert e < 1.5e-4, e
     net.close()
 
 
