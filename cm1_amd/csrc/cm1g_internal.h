@@ -292,6 +292,34 @@ __device__ __forceinline__ bool col_tile_b(const ColGrid &g, int b, int &i, int 
   return true;
 }
 
+// ---------------------------------------------------------- reductions
+// Reduction over a 256-thread workgroup in the pairing order of a halving
+// tree, v(t) op v(t+128), then op v(t+64), then op v(t+32) ... v(t+1): the
+// two steps that cross wavefronts go through LDS, the six inside the first
+// wavefront are shuffles.  The order is fixed, so sums are deterministic;
+// the result is valid in thread 0.  lds: 256 elements of T.
+template <class T, class OP>
+__device__ __forceinline__ T block_reduce256(T x, T *lds, OP op) {
+  const int t = (int)threadIdx.x;
+  lds[t] = x;
+  __syncthreads();
+  if (t < 128) lds[t] = op(lds[t], lds[t + 128]);
+  __syncthreads();
+  T v = x;
+  if (t < 64) {
+    v = op(lds[t], lds[t + 64]);
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) v = op(v, __shfl_down(v, s, 64));
+  }
+  return v;
+}
+struct OpAdd {
+  template <class T> __device__ __forceinline__ T operator()(T a, T b) const { return a + b; }
+};
+struct OpMaxF {
+  __device__ __forceinline__ float operator()(float a, float b) const { return fmaxf(a, b); }
+};
+
 // ---------------------------------------------------------------- modules
 // bc fills (cm1g_bc.hip)
 void bc_fill(cm1g_model *m, int kind, float *a);

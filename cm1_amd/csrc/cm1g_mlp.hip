@@ -149,13 +149,8 @@ __global__ void __launch_bounds__(256) k_mse(const float *__restrict__ Y, const 
     dY[i] = 2.0f * d / (float)count;
     e2 = d * d;
   }
-  r[threadIdx.x] = e2;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) r[threadIdx.x] += r[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = r[0];
+  e2 = block_reduce256(e2, r, OpAdd());
+  if (threadIdx.x == 0) part[blockIdx.x] = e2;
 }
 
 // out[0] = sum(part[0..n)) / count, in a fixed order
@@ -164,13 +159,8 @@ __global__ void __launch_bounds__(256) k_loss_final(const float *__restrict__ pa
   __shared__ double r[256];
   double s = 0.0;
   for (int i = (int)threadIdx.x; i < n; i += 256) s += (double)part[i];
-  r[threadIdx.x] = s;
-  __syncthreads();
-  for (int q = 128; q > 0; q >>= 1) {
-    if ((int)threadIdx.x < q) r[threadIdx.x] += r[threadIdx.x + q];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = (float)(r[0] / (double)count);
+  s = block_reduce256(s, r, OpAdd());
+  if (threadIdx.x == 0) out[0] = (float)(s / (double)count);
 }
 
 // out[c] = sum over rows of D[r*width + c]: one workgroup per column
@@ -180,13 +170,8 @@ __global__ void __launch_bounds__(256) k_colsum(const float *__restrict__ D, int
   const int c = (int)blockIdx.x;
   float s = 0.0f;
   for (int64_t q = threadIdx.x; q < rows; q += 256) s += D[q * width + c];
-  r[threadIdx.x] = s;
-  __syncthreads();
-  for (int q = 128; q > 0; q >>= 1) {
-    if ((int)threadIdx.x < q) r[threadIdx.x] += r[threadIdx.x + q];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[c] = r[0];
+  s = block_reduce256(s, r, OpAdd());
+  if (threadIdx.x == 0) out[c] = s;
 }
 
 __global__ void __launch_bounds__(256) k_adam(float *__restrict__ p, const float *__restrict__ g,

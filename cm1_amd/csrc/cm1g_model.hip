@@ -654,19 +654,11 @@ __device__ __forceinline__ void mass_level_sums(const KP &P, const float *rho, c
       a3 += (double)(rho[c] * P.ruh[i + 2] * P.rvh[j + 2] * P.rmh[k]);
       a4 += (double)(P.pi0[k] + pp3d[c]);
     }
-  r3[threadIdx.x] = a3;
-  r4[threadIdx.x] = a4;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      r3[threadIdx.x] += r3[threadIdx.x + s];
-      r4[threadIdx.x] += r4[threadIdx.x + s];
-    }
-    __syncthreads();
-  }
+  a3 = block_reduce256(a3, r3, OpAdd());
+  a4 = block_reduce256(a4, r4, OpAdd());
   if (threadIdx.x == 0) {
-    out[2 * (k * MASS_NB + b)] = r3[0];
-    out[2 * (k * MASS_NB + b) + 1] = r4[0];
+    out[2 * (k * MASS_NB + b)] = a3;
+    out[2 * (k * MASS_NB + b) + 1] = a4;
   }
 }
 
@@ -1376,19 +1368,11 @@ __global__ void __launch_bounds__(256) k_budget(KP P, const float *rho, const fl
     a1 += (double)rho[c] * vol;
     if (moist) a2 += (double)rho[c] * vol * ((double)qv[c] + (double)qc[c] + (double)qr[c]);
   }
-  r1[threadIdx.x] = a1;
-  r2[threadIdx.x] = a2;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      r1[threadIdx.x] += r1[threadIdx.x + s];
-      r2[threadIdx.x] += r2[threadIdx.x + s];
-    }
-    __syncthreads();
-  }
+  a1 = block_reduce256(a1, r1, OpAdd());
+  a2 = block_reduce256(a2, r2, OpAdd());
   if (threadIdx.x == 0) {
-    out[2 * k] = r1[0];
-    out[2 * k + 1] = r2[0];
+    out[2 * k] = a1;
+    out[2 * k + 1] = a2;
   }
 }
 

@@ -646,13 +646,8 @@ __device__ __forceinline__ void cfl_levels(const KP &P, float dt, const float *u
       float wsp = sqrtf((a * a + bb * bb) + c * c);
       cfl = fmaxf(cfl, wsp);
     }
-  red[threadIdx.x] = cfl;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[1 + (k - 1) * CFL_NB + b] = red[0];
+  cfl = block_reduce256(cfl, red, OpMaxF());
+  if (threadIdx.x == 0) out[1 + (k - 1) * CFL_NB + b] = cfl;
 }
 
 // maximum over the per-level partial maxima (exact in any order)

@@ -67,3 +67,43 @@ def test_dropin_program_equals_reference(require_gpu, name, tmp_path):
                 bad[f"{tag}_{key}"] = n
     print(f"{name}: {wall:.1f} s; differing values {bad}")
     assert not bad, bad
+
+
+def test_dropin_restart_round_trip(require_gpu, tmp_path):
+    """The reference's restart files (src/restart_write.F, restart_format 1)
+    in the drop-in program: a continuous 10-step run writes restarts at 30 s
+    and 60 s equal, in every field record (the s, u, v, w files), to the
+    reference executable's; a run restarted (irst = 1) from the reference
+    executable's 30 s restart writes a 60 s restart equal to them too."""
+    if not os.path.exists(EXE):
+        pytest.skip("drop-in program not built (cm1_amd/fortran/build_dropin.py needs the reference sources)")
+    if not os.path.exists(refrun.REF_EXE):
+        pytest.skip("reference executable not built")
+    import shutil
+    import subprocess
+    ov = dict(nx=24, ny=20, nz=18, dtl=6.0, timax=60.0, tapfrq=60.0, rstfrq=30.0, wbc=2, ebc=2, sbc=2,
+              nbc=2)
+    ref, gpu, rst = str(tmp_path / "ref"), str(tmp_path / "gpu"), str(tmp_path / "rst")
+    refrun.run_reference(ref, ov, threads=1)
+    refrun.run_reference(gpu, ov, threads=1, exe=EXE)
+
+    def same(a, b, n, kind):
+        fa = os.path.join(a, f"cm1rst_{n:06d}_{kind}.dat")
+        fb = os.path.join(b, f"cm1rst_{n:06d}_{kind}.dat")
+        with open(fa, "rb") as x, open(fb, "rb") as y:
+            return x.read() == y.read()
+
+    for n in (1, 2):
+        for kind in "suvw":
+            assert same(ref, gpu, n, kind), (n, kind)
+    # restart the drop-in program from the reference's 30 s files
+    os.makedirs(rst)
+    for kind in "xsuvw":
+        shutil.copy(os.path.join(ref, f"cm1rst_000001_{kind}.dat"), rst)
+    with open(os.path.join(rst, "namelist.input"), "w") as f:
+        f.write(refrun.namelist_text(dict(ov, irst=1, rstnum=1)))
+    r = subprocess.run([EXE], cwd=rst, capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, OMP_NUM_THREADS="1", OMP_STACKSIZE="512M"))
+    assert r.returncode == 0 and "Program terminated normally" in r.stdout + r.stderr, (r.stdout + r.stderr)[-2000:]
+    for kind in "suvw":
+        assert same(ref, rst, 2, kind), kind
