@@ -49,7 +49,7 @@ def bench_ensemble(args):
     import cm1_amd._native as N
     import ctypes as C
     L = N.lib()
-    work = workload_ensemble(args.ensemble)
+    work = workloads.ensemble_small(args.ensemble) if args.small else workload_ensemble(args.ensemble)
     members = []
     for cfgd, cols, fields in work:
         members.append(model.setup(model.make_config(**cfgd), cols, fields))
@@ -87,9 +87,10 @@ def bench_ensemble(args):
     e.step(max(1, args.warmup))
     ms_fixed = ens_timed(args.steps)
     print(f"ensemble: batched, 4-row tiles {ms_fixed:.2f} ms/step", file=sys.stderr, flush=True)
-    best, tune_ms = e.tune(max(2, args.steps // 4))
+    best, order, tune_ms = e.tune_schedule(max(2, args.steps // 4))
     ms_tuned = ens_timed(args.steps)
-    print(f"ensemble: batched, tuned ({best}-row tiles) {ms_tuned:.2f} ms/step", file=sys.stderr, flush=True)
+    print(f"ensemble: batched, tuned ({best}-row tiles, member order {order}) {ms_tuned:.2f} ms/step",
+          file=sys.stderr, flush=True)
     e.close()
     shapes = sorted({(m.ni, m.nj, m.nk) for m in members})
     line = {
@@ -105,14 +106,17 @@ def bench_ensemble(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"supercell ensemble, {len(members)} members, (nx,ny) in "
-                               "{96,128,192,256}, nz in {40,56,64}, dx=1km dt=6s nsound=6 Kessler",
+        "config": {"workload": (f"supercell ensemble, {len(members)} members, " +
+                                ("(nx,ny) in {24,32,48,64}, nz in {24,32,40}" if args.small else
+                                 "(nx,ny) in {96,128,192,256}, nz in {40,56,64}") +
+                                ", dx=1km dt=6s nsound=6 Kessler"),
                    "members": len(members), "cells": cells, "shapes": len(shapes),
                    "ms_per_step_one_by_one": round(ms_seq, 3),
                    "ms_per_step_batched_fixed_tiles": round(ms_fixed, 3),
                    "ms_per_step_batched_tuned": round(ms_tuned, 3),
                    "tuned_tile_rows": best,
-                   "tuner_ms_per_step": {str(k): round(v, 3) for k, v in tune_ms.items()},
+                   "tuned_member_order": ["as given", "deepest columns first", "largest first"][order],
+                   "tuner_ms_per_step": {f"order{o}_ty{h}": round(v, 3) for (o, h), v in tune_ms.items()},
                    "gain_batched_tuned_vs_one_by_one": round(ms_seq / ms_tuned, 3)},
     }
     print(json.dumps(line))
@@ -419,6 +423,8 @@ def main():
                     help="threads of the CPU baseline (default: the CPUs this job may use)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (reference kernel for HBM counters)")
+    ap.add_argument("--small", action="store_true",
+                    help="with --ensemble: members from the small-grid draw (24-64 columns, 24-40 levels)")
     ap.add_argument("--ensemble", type=int, default=0,
                     help="benchmark an ensemble of this many members instead (batched steps)")
     ap.add_argument("--cbl", action="store_true",

@@ -127,6 +127,9 @@ SIGNATURES = {
                                 C.POINTER(C.c_double)]),
     "cm1g_kessler": (C.c_int, [C.c_void_p]),
     "cm1g_satadj": (C.c_int, [C.c_void_p]),
+    "cm1g_ensemble_set_order": (C.c_int, [C.c_void_p, C.c_int]),
+    "cm1g_ensemble_tune_schedule": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                              C.POINTER(C.c_float)]),
     "cm1g_state_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "cm1g_state_save": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "cm1g_state_load": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),

@@ -12,11 +12,16 @@
 // for bit.  As with one model, the scalar updates exchange buffers, so the
 // ensemble keeps one graph per arrangement of the members' buffers.
 //
-// The column kernels' tile height (rows of 64 columns per workgroup) is the
-// tuning knob: small members fill the chip better with short tiles.  It does
-// not change any result, only which workgroup computes which column.
+// Two schedule knobs are tuned: the column kernels' tile height (rows of 64
+// columns per workgroup; small members fill the chip better with short
+// tiles) and the order in which the members' workgroups are laid out in a
+// merged launch (workgroups are dispatched in that order, so laying out the
+// deepest columns first leaves short workgroups for the tail of each
+// launch).  Neither changes any result, only which workgroup computes which
+// column and when.
 #include "cm1g_internal.h"
 
+#include <algorithm>
 #include <chrono>
 
 #define ARGCHK(cond, msg)          \
@@ -29,6 +34,7 @@
 
 struct cm1g_ensemble {
   std::vector<cm1g_model *> mem;
+  std::vector<int> order;  // layout position -> member
   hipStream_t stream = nullptr;
   struct Graph {
     std::vector<std::vector<float *>> before, after;
@@ -64,6 +70,7 @@ extern "C" int cm1g_ensemble_create(cm1g_model *const *members, int n, cm1g_ense
     for (int j = 0; j < i; j++) ARGCHK(members[i] != members[j], "cm1g_ensemble_create: a member appears twice");
   cm1g_ensemble *e = new cm1g_ensemble();
   e->mem.assign(members, members + n);
+  for (int i = 0; i < n; i++) e->order.push_back(i);
   // members' own queued work (set-up) finishes before the ensemble's
   for (auto *m : e->mem) HIPCHK(hipStreamSynchronize(m->stream));
   if (hipStreamCreate(&e->stream) != hipSuccess) {
@@ -148,17 +155,17 @@ static int ens_build(cm1g_ensemble *e, cm1g_ensemble::Graph &g) {
     const size_t fs = recs[0][t].fn.size();
     BatchSlot *sl = (BatchSlot *)(hb.data() + sloff[t]);
     long long cur = 0;
-    for (int i = 0; i < n; i++) {
-      const LaunchRec &r = recs[i][t];
-      std::memcpy(hb.data() + fnoff[t] + (size_t)i * fs, r.fn.data(), fs);
+    for (int p = 0; p < n; p++) {
+      const LaunchRec &r = recs[e->order[p]][t];
+      std::memcpy(hb.data() + fnoff[t] + (size_t)p * fs, r.fn.data(), fs);
       // each member starts on a multiple of 8 workgroups, so the column
       // kernels' workgroup-to-XCD mapping holds within the member
       const long long start = (cur + NXCD - 1) / NXCD * NXCD;
       const long long nb = (long long)r.grid.x * r.grid.y * r.grid.z;
-      sl[i].start = (int)start;
-      sl[i].n = (int)nb;
-      sl[i].gx = (int)r.grid.x;
-      sl[i].gy = (int)r.grid.y;
+      sl[p].start = (int)start;
+      sl[p].n = (int)nb;
+      sl[p].gx = (int)r.grid.x;
+      sl[p].gy = (int)r.grid.y;
       cur = start + nb;
     }
     if (cur >= (1LL << 31)) {
@@ -223,6 +230,75 @@ extern "C" int cm1g_ensemble_set_tiles(cm1g_ensemble *e, int ty) {
     if (rc) return rc;
   }
   return 0;
+}
+
+extern "C" int cm1g_ensemble_set_order(cm1g_ensemble *e, int mode) {
+  ARGCHK(e && mode >= 0 && mode <= 2, "cm1g_ensemble_set_order: mode must be 0, 1 or 2");
+  const int n = (int)e->mem.size();
+  std::vector<int> o(n);
+  for (int i = 0; i < n; i++) o[i] = i;
+  auto cells = [&](int i) { return (long long)e->mem[i]->c.ni * e->mem[i]->c.nj * e->mem[i]->c.nk; };
+  if (mode == 1)  // deepest columns first, then the larger members
+    std::stable_sort(o.begin(), o.end(), [&](int a, int b) {
+      if (e->mem[a]->c.nk != e->mem[b]->c.nk) return e->mem[a]->c.nk > e->mem[b]->c.nk;
+      return cells(a) > cells(b);
+    });
+  else if (mode == 2)  // the larger members first
+    std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return cells(a) > cells(b); });
+  if (o != e->order) {
+    ens_graphs_clear(e);
+    e->order = o;
+  }
+  return 0;
+}
+
+// Time `steps` ensemble steps of the current schedule (after one untimed
+// step that captures its graph).
+static int ens_time(cm1g_ensemble *e, int steps, float *ms) {
+  int rc = cm1g_ensemble_step(e);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(e->stream));
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int s = 0; s < steps; s++) {
+    rc = cm1g_ensemble_step(e);
+    if (rc) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(e->stream));
+  *ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+  return 0;
+}
+
+// Every tile height (1, 2, 4) with every member order (0 as given, 1 deepest
+// columns first, 2 largest first): `steps` timed steps each, after one
+// warm-up schedule; keeps the fastest.  ms9[3*order + height index].
+extern "C" int cm1g_ensemble_tune_schedule(cm1g_ensemble *e, int steps, int *ty_best, int *order_best,
+                                           float *ms9) {
+  ARGCHK(e && ty_best && order_best && steps >= 1, "cm1g_ensemble_tune_schedule: bad argument");
+  static const int cand[3] = {1, 2, 4};
+  float best = 0.0f, ms = 0.0f;
+  int bty = 4, bo = 0;
+  int rc = cm1g_ensemble_set_tiles(e, cand[0]);
+  if (!rc) rc = cm1g_ensemble_set_order(e, 0);
+  if (!rc) rc = ens_time(e, steps, &ms);  // the device's warm-up
+  if (rc) return rc;
+  for (int o = 0; o < 3; o++)
+    for (int c = 0; c < 3; c++) {
+      rc = cm1g_ensemble_set_order(e, o);
+      if (!rc) rc = cm1g_ensemble_set_tiles(e, cand[c]);
+      if (!rc) rc = ens_time(e, steps, &ms);
+      if (rc) return rc;
+      if (ms9) ms9[3 * o + c] = ms;
+      if ((o == 0 && c == 0) || ms < best) {
+        best = ms;
+        bty = cand[c];
+        bo = o;
+      }
+    }
+  *ty_best = bty;
+  *order_best = bo;
+  rc = cm1g_ensemble_set_order(e, bo);
+  if (!rc) rc = cm1g_ensemble_set_tiles(e, bty);
+  return rc;
 }
 
 // Try each tile height for `steps` timed ensemble steps (after one untimed

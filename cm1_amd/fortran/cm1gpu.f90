@@ -237,6 +237,19 @@ module cm1gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: m
     end function
+    integer(c_int) function cm1g_ensemble_set_order(e, mode) bind(C, name="cm1g_ensemble_set_order")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: mode
+    end function
+    integer(c_int) function cm1g_ensemble_tune_schedule(e, steps, ty_best, order_best, ms9) &
+        bind(C, name="cm1g_ensemble_tune_schedule")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: e
+      integer(c_int), value :: steps
+      integer(c_int), intent(out) :: ty_best, order_best
+      real(c_float), intent(out) :: ms9(9)
+    end function
     integer(c_int) function cm1g_sync(m) bind(C, name="cm1g_sync")
       import :: c_int, c_ptr
       type(c_ptr), value :: m

@@ -193,6 +193,18 @@ class Ensemble:
         N.check(self.L.cm1g_ensemble_tune(self.h, int(steps), C.byref(best), ms), "cm1g_ensemble_tune")
         return best.value, {1: ms[0], 2: ms[1], 4: ms[2]}
 
+    def set_order(self, mode):
+        N.check(self.L.cm1g_ensemble_set_order(self.h, int(mode)), "cm1g_ensemble_set_order")
+
+    def tune_schedule(self, steps):
+        """Autotune tile height and member order; returns
+        (best height, best order, {(order, height): ms/step})."""
+        ty, od = C.c_int(), C.c_int()
+        ms = (C.c_float * 9)()
+        N.check(self.L.cm1g_ensemble_tune_schedule(self.h, int(steps), C.byref(ty), C.byref(od), ms),
+                "cm1g_ensemble_tune_schedule")
+        return ty.value, od.value, {(o, h): ms[3 * o + q] for o in range(3) for q, h in enumerate((1, 2, 4))}
+
     def close(self):
         if self.h:
             self.L.cm1g_ensemble_destroy(self.h)

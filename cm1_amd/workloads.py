@@ -64,11 +64,30 @@ def ensemble_member(ni, nj, nk, qvs, umax, amp):
     return cfgd, cols, fields
 
 
+def ensemble_small_shapes(n=96, seed=2027):
+    """Small members, where one member alone fills a fraction of the chip:
+    (nx, ny) from {24, 32, 48, 64}, nz from {24, 32, 40}, the same
+    perturbations as the 48-member draw."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        ni, nj = (int(rng.choice([24, 32, 48, 64])) for _ in range(2))
+        nk = int(rng.choice([24, 32, 40]))
+        qvs, umax, amp = float(rng.uniform(0.9, 1.1)), float(rng.uniform(10, 30)), float(rng.uniform(1, 3))
+        out.append((ni, nj, nk, qvs, umax, amp))
+    return out
+
+
 def ensemble(n=48, seed=2025):
     """Batched supercell ensemble: n members with (nx, ny) drawn from
     {96,128,192,256} and nz from {40,56,64}, qv scaled by U(0.9,1.1), shear
     u = U(10,30) tanh(z/3 km), a centred U(1,3) K bubble."""
     return [ensemble_member(*s) for s in ensemble_shapes(n, seed)]
+
+
+def ensemble_small(n=96, seed=2027):
+    """Batched ensemble of small members (see ensemble_small_shapes)."""
+    return [ensemble_member(*s) for s in ensemble_small_shapes(n, seed)]
 
 
 def spinup(ni=128, nj=128, nk=40, seed=7):

@@ -208,6 +208,16 @@ int cm1g_ensemble_set_tiles(cm1g_ensemble *e, int ty);
  * three timings.  The 4*(steps+1) steps it takes are ordinary steps of the
  * run. */
 int cm1g_ensemble_tune(cm1g_ensemble *e, int steps, int *ty_best, float *ms_per_step);
+/* The order of the members' workgroups in every merged launch (dispatched
+ * in that order): 0 as given, 1 deepest columns (largest nk) first, then
+ * the larger members, 2 the larger members first.  Never changes a result. */
+int cm1g_ensemble_set_order(cm1g_ensemble *e, int mode);
+/* Autotuner over both schedule knobs: every tile height (1, 2, 4) with
+ * every member order (0, 1, 2), `steps` timed steps each after one warm-up
+ * schedule; keeps the fastest in *ty_best / *order_best and, if ms9 is
+ * non-NULL, stores the nine timings as ms9[3*order + k] (k: 0 for height
+ * 1, 1 for 2, 2 for 4).  Its 10*(steps+1) steps are ordinary steps. */
+int cm1g_ensemble_tune_schedule(cm1g_ensemble *e, int steps, int *ty_best, int *order_best, float *ms9);
 
 /* --- learned Kessler surrogate (no counterpart in the reference) --------
  * A per-cell MLP, 8 -> 128 -> 128 -> 128 -> 4 with SiLU hidden layers, BF16

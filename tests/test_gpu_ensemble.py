@@ -54,6 +54,39 @@ def test_ensemble_equals_members_alone(require_gpu, common, ty):
     e.close()
 
 
+@pytest.mark.parametrize("order", [1, 2], ids=["deepest-first", "largest-first"])
+def test_member_order_changes_no_result(require_gpu, order):
+    """The order of the members' workgroups in the merged launches is a
+    schedule choice only: every member still equals itself stepped alone."""
+    cases = members(dict(bc=2))
+    ens_models = [make_gpu(*c) for c in cases]
+    solo = [make_gpu(*c) for c in cases]
+    e = model.Ensemble(ens_models)
+    e.set_order(order)
+    e.step(3)
+    for s in solo:
+        s.step(3)
+    e.sync()
+    check_equal(ens_models, solo, NAMES)
+    e.close()
+
+
+def test_schedule_tuner(require_gpu):
+    """The autotuner over tile height and member order takes ordinary steps
+    (10 * (steps + 1) of them) and keeps one of the nine schedules."""
+    cases = members(dict(bc=2))
+    ens_models = [make_gpu(*c) for c in cases]
+    solo = [make_gpu(*c) for c in cases]
+    e = model.Ensemble(ens_models)
+    ty, od, ms = e.tune_schedule(1)
+    assert ty in (1, 2, 4) and od in (0, 1, 2) and len(ms) == 9 and all(v > 0.0 for v in ms.values())
+    for s in solo:
+        s.step(20)
+    e.sync()
+    check_equal(ens_models, solo, NAMES)
+    e.close()
+
+
 def test_ensemble_tune(require_gpu):
     cases = members(dict(bc=2))
     ens_models = [make_gpu(*c) for c in cases]
