@@ -502,7 +502,10 @@ def main():
             roof = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(gbs / HBM_PEAK_GBS, 4), traffic=measured_traffic(name.value.decode()),
                         kernel=name.value.decode(), avg_ms=round(avg_ms, 4),
-                        launches=nl.value, bytes_per_launch=nbytes.value)
+                        launches=nl.value, bytes_per_launch=nbytes.value,
+                        fields_per_cell=bytes_model.SOUND_SUBSTEP,
+                        achieved_fused_distinct=round(gbs * bytes_model.SOUND_SUBSTEP_FUSED /
+                                                      bytes_model.SOUND_SUBSTEP, 1))
     # whole-step algorithmic rate (frozen byte table)
     step_bytes = cells * bytes_model.bytes_per_cell_step(cfgd["nsound"])
     step_gbs = step_bytes / (ms * 1e-3) / 1e9

@@ -42,7 +42,9 @@ STEP_BASE = 534
 # the fused acoustic substep kernel (sound_sub) evaluates pk1, pk2, qk and
 # mm on the fly, so it touches 23 distinct fields: u, v, w, ppd, pp3d, rru,
 # rrv, rrw read and written (16) and uten, vten, wten, thv, ppten, ppterm,
-# piadv read (7)
+# piadv read (7).  The roofline's algorithmic bytes of one launch are the
+# routine's 27 fields (SOUND_SUBSTEP); the 23-field figure is reported
+# beside it.
 SOUND_SUBSTEP_FUSED = 23
 
 

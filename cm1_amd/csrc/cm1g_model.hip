@@ -1429,9 +1429,11 @@ extern "C" int cm1g_ktimer_stop(cm1g_model *m, double *total_ms, int64_t *launch
   const char *nm = "";
   double fields = 0.0;
   if (m->kt_which == CM1G_KT_SOUND_SUBSTEP) {
-    // u, v, w, ppd, pp3d, rru, rrv, rrw read and written (16); uten, vten,
-    // wten, thv, ppten, ppterm, piadv read (7)
-    fields = 23.0;
+    // one substep of sound (src/sound.F:307-835): u, v, w, ppd, pp3d, rru,
+    // rrv, rrw read and written (16); uten, vten, wten, thv, ppten, ppterm,
+    // piadv, pk1, pk2, qk, mm read (11).  The kernel evaluates pk1, pk2, qk
+    // and mm on the fly instead of reading them, so it touches 23 of them.
+    fields = 27.0;
     nm = "sound_sub";
   }
   *total_ms = tot;

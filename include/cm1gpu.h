@@ -368,8 +368,9 @@ int cm1g_scalars(cm1g_model *m, double *out4);
  * started, every launch of the selected kernel is bracketed by HIP events
  * on the model's stream.  cm1g_ktimer_stop synchronizes the stream and
  * returns the summed kernel time, the launch count and the algorithmic
- * bytes of one launch (distinct full-size fields read plus written, 4 B
- * per interior cell each, scratch excluded). */
+ * bytes of one launch (the distinct full-size fields the reference routine
+ * reads plus writes, 4 B per interior cell each, scratch excluded: 27 for
+ * an acoustic substep, src/sound.F:307-835). */
 enum { CM1G_KT_NONE = 0, CM1G_KT_SOUND_SUBSTEP = 1 };
 int cm1g_ktimer_start(cm1g_model *m, int which);
 int cm1g_ktimer_stop(cm1g_model *m, double *total_ms, int64_t *launches,
