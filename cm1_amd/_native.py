@@ -34,7 +34,13 @@ KT_SOUND_SUBSTEP = 1
 BC = {"s": 0, "p": 1, "u": 2, "v": 3, "w": 4, "w0": 5}
 
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "device allocation failed",
-          4: "unsupported", 5: "satadj did not converge", 6: "CFL >= 1.5"}
+          4: "unsupported", 5: "satadj did not converge", 6: "CFL >= 1.5",
+          7: "output file could not be written"}
+# GrADS output switches of cm1g_write_grads (the reference's output_* names)
+OUT = {n: 1 << i for i, n in enumerate(
+    ("rain", "zh", "th", "thpert", "prs", "prspert", "pi", "pipert", "rho", "rhopert", "qv",
+     "qvpert", "q", "buoyancy", "uinterp", "vinterp", "winterp", "basestate", "u", "upert", "v",
+     "vpert", "w"))}
 
 
 class Config(C.Structure):
@@ -136,6 +142,8 @@ SIGNATURES = {
     "cm1g_dtsmall": (C.c_int, [C.POINTER(Config), C.c_float, C.POINTER(C.c_int32)]),
     "cm1g_budgets": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "cm1g_scalars": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "cm1g_write_grads": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_uint32, C.c_int, C.c_int,
+                                   C.c_double, C.c_int]),
 }
 
 

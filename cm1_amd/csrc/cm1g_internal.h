@@ -109,8 +109,15 @@ struct cm1g_model {
                                 // [3] harvest calls [4] cells out of the trained range [5] unusable proposals
   int *colflag;                 // per column: the surrogate fell back to plain Kessler
   int *actlist;                 // the surrogate's list of cells kessler can act on (ni*nj*nk)
+  // GrADS output staging (cm1g_write_grads): two device / pinned host
+  // buffer pairs of io_cap floats, and the event of each buffer's copy
+  float *io_dev[2], *io_host[2];
+  hipEvent_t io_ev[2];
+  size_t io_cap;
 };
 
+// release the GrADS output staging (cm1g_io.hip)
+void io_free(cm1g_model *m);
 // drop the captured steps (after anything that changes launch arguments)
 void graphs_clear(cm1g_model *m);
 // the host side of one large step (solve1, solve2, mp_driver, solve3)

@@ -237,6 +237,7 @@ extern "C" int cm1g_destroy(cm1g_model *m) {
   if (m->dcount) hipFree(m->dcount);
   if (m->colflag) hipFree(m->colflag);
   if (m->actlist) hipFree(m->actlist);
+  io_free(m);
   for (hipEvent_t e : m->kt_ev) hipEventDestroy(e);
   if (m->stream) hipStreamDestroy(m->stream);
   delete m;

@@ -25,7 +25,32 @@ module cm1gpu
   ! return codes
   integer(c_int), parameter :: CM1G_OK = 0, CM1G_E_ARG = 1, CM1G_E_HIP = 2, &
                                CM1G_E_NOMEM = 3, CM1G_E_UNSUPPORTED = 4, &
-                               CM1G_E_SATADJ = 5, CM1G_E_CFL = 6
+                               CM1G_E_SATADJ = 5, CM1G_E_CFL = 6, CM1G_E_IO = 7
+  ! GrADS output switches of cm1g_write_grads (the namelist's output_* names)
+  integer(c_int32_t), parameter :: CM1G_OUT_RAIN      = 1
+  integer(c_int32_t), parameter :: CM1G_OUT_ZH        = 2
+  integer(c_int32_t), parameter :: CM1G_OUT_TH        = 4
+  integer(c_int32_t), parameter :: CM1G_OUT_THPERT    = 8
+  integer(c_int32_t), parameter :: CM1G_OUT_PRS       = 16
+  integer(c_int32_t), parameter :: CM1G_OUT_PRSPERT   = 32
+  integer(c_int32_t), parameter :: CM1G_OUT_PI        = 64
+  integer(c_int32_t), parameter :: CM1G_OUT_PIPERT    = 128
+  integer(c_int32_t), parameter :: CM1G_OUT_RHO       = 256
+  integer(c_int32_t), parameter :: CM1G_OUT_RHOPERT   = 512
+  integer(c_int32_t), parameter :: CM1G_OUT_QV        = 1024
+  integer(c_int32_t), parameter :: CM1G_OUT_QVPERT    = 2048
+  integer(c_int32_t), parameter :: CM1G_OUT_Q         = 4096
+  integer(c_int32_t), parameter :: CM1G_OUT_BUOYANCY  = 8192
+  integer(c_int32_t), parameter :: CM1G_OUT_UINTERP   = 16384
+  integer(c_int32_t), parameter :: CM1G_OUT_VINTERP   = 32768
+  integer(c_int32_t), parameter :: CM1G_OUT_WINTERP   = 65536
+  integer(c_int32_t), parameter :: CM1G_OUT_BASESTATE = 131072
+  integer(c_int32_t), parameter :: CM1G_OUT_U         = 262144
+  integer(c_int32_t), parameter :: CM1G_OUT_UPERT     = 524288
+  integer(c_int32_t), parameter :: CM1G_OUT_V         = 1048576
+  integer(c_int32_t), parameter :: CM1G_OUT_VPERT     = 2097152
+  integer(c_int32_t), parameter :: CM1G_OUT_W         = 4194304
+  integer(c_int32_t), parameter :: CM1G_OUT_ALL       = 8388607
   ! staggering of host arrays
   integer(c_int), parameter :: CM1G_STAG_S = 0, CM1G_STAG_U = 1, CM1G_STAG_V = 2, &
                                CM1G_STAG_W = 3
@@ -384,6 +409,15 @@ module cm1gpu
     integer(c_int) function cm1g_solve3(m) bind(C, name="cm1g_solve3")
       import :: c_int, c_ptr
       type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_write_grads(m, dir, nwrite, mask, iorigin, stretch_z, mtime, nstep) &
+        bind(C, name="cm1g_write_grads")
+      import :: c_int, c_ptr, c_char, c_int32_t, c_double
+      type(c_ptr), value :: m
+      character(kind=c_char), dimension(*), intent(in) :: dir
+      integer(c_int), value :: nwrite, iorigin, stretch_z, nstep
+      integer(c_int32_t), value :: mask
+      real(c_double), value :: mtime
     end function
     integer(c_int) function cm1g_budgets(m, tmass, tmois) bind(C, name="cm1g_budgets")
       import :: c_int, c_ptr, c_double

@@ -5,6 +5,7 @@ between host and device only through :meth:`Model.put` / :meth:`Model.get`
 (the residency protocol a Fortran host uses around its non-ported code).
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -145,6 +146,18 @@ class Model:
         a, b = C.c_double(), C.c_double()
         N.check(self.L.cm1g_budgets(self.h, C.byref(a), C.byref(b)), "cm1g_budgets")
         return a.value, b.value
+
+    def write_grads(self, directory, nwrite, switches=None, iorigin=2, stretch_z=0, mtime=0.0,
+                    nstep=0):
+        """GrADS output time ``nwrite`` of the variables named in ``switches``
+        (the reference's output_* names, default all) into ``directory``."""
+        names = N.OUT if switches is None else switches
+        mask = 0
+        for n in names:
+            mask |= N.OUT[n]
+        N.check(self.L.cm1g_write_grads(self.h, os.fsencode(directory), int(nwrite), mask,
+                                        int(iorigin), int(stretch_z), float(mtime), int(nstep)),
+                "cm1g_write_grads")
 
     def scalars(self):
         a = (C.c_double * 4)()

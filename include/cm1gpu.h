@@ -40,7 +40,8 @@ enum {
   CM1G_E_NOMEM = 3,    /* device allocation failed */
   CM1G_E_UNSUPPORTED = 4,
   CM1G_E_SATADJ = 5,   /* saturation adjustment did not converge (src/kessler.F:253-257) */
-  CM1G_E_CFL = 6       /* cflmax >= 1.5 (src/cm1.F:3020-3024) */
+  CM1G_E_CFL = 6,      /* cflmax >= 1.5 (src/cm1.F:3020-3024) */
+  CM1G_E_IO = 7        /* an output file could not be written */
 };
 
 /* Namelist subset read by the hot path (module input, src/input.F:261-351). */
@@ -359,6 +360,54 @@ int cm1g_budgets(cm1g_model *m, double *tmass, double *tmois);
 int cm1g_state_bytes(cm1g_model *m, int64_t *nbytes);
 int cm1g_state_save(cm1g_model *m, void *host, int64_t nbytes);
 int cm1g_state_load(cm1g_model *m, const void *host, int64_t nbytes);
+
+/* GrADS output (writeout, src/writeout.F:20-9530, with output_format = 1,
+ * output_filetype = 1, outunits = 1: x, y, z in km).  Appends output time
+ * nwrite (1, 2, ...; 1 starts the files afresh) of the variables the mask
+ * selects to cm1out_s.dat, cm1out_u.dat, cm1out_v.dat and cm1out_w.dat in
+ * the existing directory dir, in the reference's file order, one record of
+ * ni*nj ((ni+1)*nj for u, ni*(nj+1) for v) REAL*4 values per level, i
+ * fastest; rewrites the descriptors cm1out_{s,u,v,w}.ctl for nwrite times
+ * (src/writeout.F:9278-9530); and writes record nwrite of
+ * cm1out_metadata.dat/.ctl (mtime, nstep, dt, nwrite; src/writeout.F:
+ * 9536-9757).  Mask bits are the reference's output_* switches (param9);
+ * passive tracers pt1.. are written whenever npt > 0 (iptra = 1), and
+ * CM1G_OUT_Q writes qc and qr of a moist model.  iorigin (param2: 1 or 2)
+ * and stretch_z (param6: 0, or 1 for a descriptor that lists the levels)
+ * only shape the descriptors.  Every value is formed on the device with the
+ * reference's expression (th0+th', prs-prs0, the buoyancy, the averages
+ * to scalar points) and streamed to the files one variable at a time; the
+ * call waits for the model's queued work and returns when the files are
+ * closed.  The staging (two device and two pinned host buffers of the
+ * largest variable) is allocated on first use and kept with the handle. */
+enum {
+  CM1G_OUT_RAIN = 1u << 0,       /* rain, prate (2-D) */
+  CM1G_OUT_ZH = 1u << 1,         /* zhval */
+  CM1G_OUT_TH = 1u << 2,         /* th = th0 + th' */
+  CM1G_OUT_THPERT = 1u << 3,     /* thpert */
+  CM1G_OUT_PRS = 1u << 4,        /* prs */
+  CM1G_OUT_PRSPERT = 1u << 5,    /* prspert = prs - prs0 */
+  CM1G_OUT_PI = 1u << 6,         /* pi = pi0 + pi' */
+  CM1G_OUT_PIPERT = 1u << 7,     /* pipert */
+  CM1G_OUT_RHO = 1u << 8,        /* rho */
+  CM1G_OUT_RHOPERT = 1u << 9,    /* rhopert = rho - rho0 */
+  CM1G_OUT_QV = 1u << 10,        /* qv */
+  CM1G_OUT_QVPERT = 1u << 11,    /* qvpert = qv - qv0 */
+  CM1G_OUT_Q = 1u << 12,         /* qc, qr */
+  CM1G_OUT_BUOYANCY = 1u << 13,  /* g th'/th0 + g repsm1 (qv-qv0) - g qc - g qr */
+  CM1G_OUT_UINTERP = 1u << 14,   /* u, v, w averaged to scalar points */
+  CM1G_OUT_VINTERP = 1u << 15,
+  CM1G_OUT_WINTERP = 1u << 16,
+  CM1G_OUT_BASESTATE = 1u << 17, /* pi0, th0, prs0, qv0; u0; v0 */
+  CM1G_OUT_U = 1u << 18,
+  CM1G_OUT_UPERT = 1u << 19,
+  CM1G_OUT_V = 1u << 20,
+  CM1G_OUT_VPERT = 1u << 21,
+  CM1G_OUT_W = 1u << 22,
+  CM1G_OUT_ALL = (1u << 23) - 1u
+};
+int cm1g_write_grads(cm1g_model *m, const char *dir, int nwrite, uint32_t mask, int iorigin,
+                     int stretch_z, double mtime, int nstep);
 
 /* Scalars carried between steps: [0] mass1, [1] train (domain rain
  * accumulation, src/kessler.F:472), [2] cflmax, [3] mass2. */

@@ -1,0 +1,84 @@
+"""Golden vectors of the reference's GrADS output (src/writeout.F).
+
+Runs ``oracle/_ref/cm1_sp.exe`` with the namelist of a state fixture (so
+from the same t=0 state) and writes output at t = 0 and at the end, with
+every output switch the device writer supports turned on:
+
+* ``grads_config0_64x64x40_300``: the 64x64x40 supercell of
+  ``ref_config0_64x64x40_60.npz`` over 300 large steps (30 minutes, long
+  enough for rain to reach the ground);
+* ``grads_stretch_24x20x96_10``: the stretched 96-level grid of
+  ``ref_stretch_dt3_24x20x96_10.npz`` over 10 steps (its descriptors list
+  the levels).
+
+Each fixture stores:
+
+* ``{s,u,v,w}_digest``: the SHA-256 of every record of ``cm1out_{s,u,v,w}.dat``
+  (one record per level of each variable, both output times, in file order);
+* ``{s,u,v,w,metadata}_ctl``: the descriptor files, byte for byte;
+* ``metadata_dat``: ``cm1out_metadata.dat`` (mtime, nstep, dt, nwrite);
+* ``rain``, ``prate``: the two 2-D records of the last output time as values, so a
+  failing comparison can say by how much.
+
+Usage: python tests/golden/make_grads_fixture.py
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, HERE)
+import refrun  # noqa: E402
+from make_ref_fixture import CONFIG1_STRETCH, RDALPHA, STRETCH_KEYS  # noqa: E402
+
+# output switches (param9) the device writer supports, all on
+SWITCHES = ("rain", "zh", "basestate", "th", "thpert", "prs", "prspert", "pi", "pipert",
+            "rho", "rhopert", "qv", "qvpert", "q", "buoyancy", "u", "upert", "uinterp", "v",
+            "vpert", "vinterp", "w", "winterp")
+
+
+def digests(path, reclen):
+    data = open(path, "rb").read()
+    assert len(data) % reclen == 0, (path, len(data), reclen)
+    return np.array([hashlib.sha256(data[o:o + reclen]).hexdigest().encode()
+                     for o in range(0, len(data), reclen)], dtype="S64")
+
+
+def text(path):
+    return np.frombuffer(open(path, "rb").read(), dtype=np.uint8)
+
+
+def make(name, ni, nj, nk, nsteps, dt, dx, stretch=None, workdir="/tmp/cm1_grads_fixture",
+         outdir=HERE):
+    """The namelist make_ref_fixture.make writes for the same arguments, with
+    output at t = 0 and t = nsteps*dt and the writer's switches on."""
+    ov = dict(nx=ni, ny=nj, nz=nk, dtl=dt, timax=nsteps * dt, tapfrq=nsteps * dt,
+              wbc=2, ebc=2, sbc=2, nbc=2, rdalpha=RDALPHA, zd=15000.0,
+              dx=dx, dy=dx, dz=500.0, psolver=3, advwenos=0, advwenov=0, irdamp=0)
+    if stretch is not None:
+        ov.update(stretch_z=1, **{k: float(stretch[k]) for k in STRETCH_KEYS})
+    ov.update({"output_" + k: 1 for k in SWITCHES})
+    refrun.run_reference(workdir, ov, threads=8)
+    out = dict(meta=np.array([ni, nj, nk, nsteps, 2], np.int32), dt=np.array([dt], np.float32),
+               switches=np.array([s.encode() for s in SWITCHES], dtype="S16"))
+    recl = dict(s=ni * nj * 4, u=(ni + 1) * nj * 4, v=ni * (nj + 1) * 4, w=ni * nj * 4)
+    for f, r in recl.items():
+        out[f"{f}_digest"] = digests(os.path.join(workdir, f"cm1out_{f}.dat"), r)
+        out[f"{f}_ctl"] = text(os.path.join(workdir, f"cm1out_{f}.ctl"))
+    out["metadata_ctl"] = text(os.path.join(workdir, "cm1out_metadata.ctl"))
+    out["metadata_dat"] = text(os.path.join(workdir, "cm1out_metadata.dat"))
+    S = refrun.read_grads(workdir, "cm1out_s.ctl")
+    out["rain"] = S["rain"][1, 0]
+    out["prate"] = S["prate"][1, 0]
+    path = os.path.join(outdir, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    return path
+
+
+if __name__ == "__main__":
+    print(make("grads_config0_64x64x40_300", 64, 64, 40, 300, 6.0, 1000.0))
+    print(make("grads_stretch_24x20x96_10", 24, 20, 96, 10, 3.0, 250.0, stretch=CONFIG1_STRETCH))

@@ -1,0 +1,85 @@
+"""Device GrADS writer (cm1g_write_grads) against the reference's own output.
+
+Golden vectors (tests/golden/make_grads_fixture.py): the reference
+executable, started from the t=0 state of a state fixture with every output
+switch the writer supports on, writes output at t=0 and after N large steps;
+the fixture keeps the SHA-256 of every record of cm1out_{s,u,v,w}.dat, the
+descriptor files and cm1out_metadata.dat.  The GPU model starts from the
+same state, writes output time 1, steps N times and writes output time 2:
+
+* every record (each level of each variable, both times, including the 2-D
+  rain and prate after rain has reached the ground, and the derived
+  th0+th', prs-prs0, rho-rho0, qv-qv0, buoyancy and the averages to scalar
+  points) has the reference's digest, i.e. equals it bit for bit;
+* the descriptors (.ctl) and the metadata file equal the reference's byte
+  for byte, for the unstretched 64x64x40 grid (300 steps) and for the
+  stretched 96-level grid (10 steps), whose descriptors list the levels.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from common import load_fixture, make_gpu, nsound_for
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = [("grads_config0_64x64x40_300", "ref_config0_64x64x40_60", 0),
+         ("grads_stretch_24x20x96_10", "ref_stretch_dt3_24x20x96_10", 1)]
+
+
+def ctl_records(text):
+    """(name, level) of each record of one output time, from a descriptor."""
+    lines = text.splitlines()
+    i0 = next(i for i, l in enumerate(lines) if l.startswith("vars"))
+    recs = []
+    for l in lines[i0 + 1:i0 + 1 + int(lines[i0].split()[1])]:
+        name, nl = l.split()[:2]
+        recs += [(name, k) for k in range(1, max(1, int(nl)) + 1)]
+    return recs
+
+
+@pytest.mark.parametrize("case,state,stretch", CASES, ids=[c[0] for c in CASES])
+def test_grads_output_equals_the_reference(require_gpu, tmp_path, case, state, stretch):
+    g = np.load(os.path.join(GOLD, case + ".npz"))
+    ni, nj, nk, nsteps, ntimes = [int(v) for v in g["meta"]]
+    dt = float(g["dt"][0])
+    switches = [s.decode() for s in g["switches"]]
+    cfgd, cols, fields, _, _, _ = load_fixture(os.path.join(GOLD, state + ".npz"))
+    assert (cfgd["ni"], cfgd["nj"], cfgd["nk"], cfgd["dt"]) == (ni, nj, nk, dt)
+    cfgd["nsound"] = nsound_for(cfgd, cols["zf"])
+    m = make_gpu(cfgd, cols, fields)
+    out = str(tmp_path)
+    m.write_grads(out, 1, switches, iorigin=2, stretch_z=stretch, mtime=0.0, nstep=0)
+    m.step(nsteps)
+    m.write_grads(out, 2, switches, iorigin=2, stretch_z=stretch, mtime=nsteps * dt, nstep=nsteps)
+    print(f"{case}: {nsteps} steps, output written to {out}", flush=True)
+
+    recl = dict(s=ni * nj * 4, u=(ni + 1) * nj * 4, v=ni * (nj + 1) * 4, w=ni * nj * 4)
+    for f, r in recl.items():
+        ref_ctl = g[f"{f}_ctl"].tobytes()
+        got_ctl = open(os.path.join(out, f"cm1out_{f}.ctl"), "rb").read()
+        assert got_ctl == ref_ctl, (f, got_ctl.decode(), ref_ctl.decode())
+        data = open(os.path.join(out, f"cm1out_{f}.dat"), "rb").read()
+        ref = [d.decode() for d in g[f"{f}_digest"]]
+        assert len(data) == r * len(ref), (f, len(data), r * len(ref))
+        names = ctl_records(ref_ctl.decode())
+        bad = []
+        for n, want in enumerate(ref):
+            if hashlib.sha256(data[n * r:(n + 1) * r]).hexdigest() != want:
+                bad.append((n // len(names) + 1,) + names[n % len(names)])
+        print(f"cm1out_{f}.dat: {len(ref)} records, {len(bad)} differ", flush=True)
+        assert not bad, (f, bad[:20])
+    assert open(os.path.join(out, "cm1out_metadata.ctl"), "rb").read() == g["metadata_ctl"].tobytes()
+    assert open(os.path.join(out, "cm1out_metadata.dat"), "rb").read() == g["metadata_dat"].tobytes()
+    # the 2-D records of the last time as values (rain has reached the
+    # ground in the 300-step case)
+    s = np.fromfile(os.path.join(out, "cm1out_s.dat"), dtype="<f4")
+    per_t = s.size // ntimes
+    rain = s[per_t:per_t + ni * nj].reshape(nj, ni)
+    prate = s[per_t + ni * nj:per_t + 2 * ni * nj].reshape(nj, ni)
+    assert np.array_equal(rain, g["rain"]) and np.array_equal(prate, g["prate"])
+    if nsteps >= 300:
+        assert g["rain"].max() > 0.0 and g["prate"].max() > 0.0
+    m.close()
