@@ -41,6 +41,11 @@ OUT = {n: 1 << i for i, n in enumerate(
     ("rain", "zh", "th", "thpert", "prs", "prspert", "pi", "pipert", "rho", "rhopert", "qv",
      "qvpert", "q", "buoyancy", "uinterp", "vinterp", "winterp", "basestate", "u", "upert", "v",
      "vpert", "w"))}
+# statistics switches of cm1g_statpack (the reference's stat_* names)
+STAT = {n: 1 << i for i, n in enumerate(
+    ("w", "wlevs", "u", "v", "pipert", "prspert", "thpert", "q", "sfcprs", "wsp", "cfl", "tmass",
+     "tmois", "qmass"))}
+STAT_MAXN = 128
 
 
 class Config(C.Structure):
@@ -144,6 +149,8 @@ SIGNATURES = {
     "cm1g_scalars": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "cm1g_write_grads": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_uint32, C.c_int, C.c_int,
                                    C.c_double, C.c_int]),
+    "cm1g_statpack": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_uint32, C.c_double,
+                                C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
 }
 
 

@@ -114,9 +114,12 @@ struct cm1g_model {
   float *io_dev[2], *io_host[2];
   hipEvent_t io_ev[2];
   size_t io_cap;
+  // statpack (cm1g_statpack): max/min keys, picked values and budget sums
+  void *stat_buf;
+  size_t stat_cap;
 };
 
-// release the GrADS output staging (cm1g_io.hip)
+// release the output staging and the statpack buffer (cm1g_io.hip)
 void io_free(cm1g_model *m);
 // drop the captured steps (after anything that changes launch arguments)
 void graphs_clear(cm1g_model *m);

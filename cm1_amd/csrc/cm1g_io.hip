@@ -289,7 +289,7 @@ int write_metadata(const cm1g_model *m, const std::string &dir, int nwrite, doub
 
 }  // namespace
 
-void io_free(cm1g_model *m) {
+static void staging_free(cm1g_model *m) {
   for (int b = 0; b < 2; b++) {
     if (m->io_dev[b]) hipFree(m->io_dev[b]);
     if (m->io_host[b]) hipHostFree(m->io_host[b]);
@@ -299,6 +299,13 @@ void io_free(cm1g_model *m) {
     m->io_ev[b] = nullptr;
   }
   m->io_cap = 0;
+}
+
+void io_free(cm1g_model *m) {
+  staging_free(m);
+  if (m->stat_buf) hipFree(m->stat_buf);
+  m->stat_buf = nullptr;
+  m->stat_cap = 0;
 }
 
 extern "C" int cm1g_write_grads(cm1g_model *m, const char *dir, int nwrite, uint32_t mask, int iorigin,
@@ -320,11 +327,11 @@ extern "C" int cm1g_write_grads(cm1g_model *m, const char *dir, int nwrite, uint
     cap = std::max(cap, (size_t)s.nx * s.ny * s.nlev);
   }
   if (cap > m->io_cap) {
-    io_free(m);
+    staging_free(m);
     for (int b = 0; b < 2; b++) {
       if (hipMalloc(&m->io_dev[b], cap * sizeof(float)) != hipSuccess ||
           hipHostMalloc(&m->io_host[b], cap * sizeof(float), hipHostMallocDefault) != hipSuccess) {
-        io_free(m);
+        staging_free(m);
         cm1g_set_error("cm1g_write_grads: cannot allocate the staging buffers");
         return CM1G_E_NOMEM;
       }

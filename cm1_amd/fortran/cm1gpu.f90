@@ -51,6 +51,23 @@ module cm1gpu
   integer(c_int32_t), parameter :: CM1G_OUT_VPERT     = 2097152
   integer(c_int32_t), parameter :: CM1G_OUT_W         = 4194304
   integer(c_int32_t), parameter :: CM1G_OUT_ALL       = 8388607
+  ! statistics switches of cm1g_statpack (the namelist's stat_* names)
+  integer(c_int32_t), parameter :: CM1G_STAT_W       = 1
+  integer(c_int32_t), parameter :: CM1G_STAT_WLEVS   = 2
+  integer(c_int32_t), parameter :: CM1G_STAT_U       = 4
+  integer(c_int32_t), parameter :: CM1G_STAT_V       = 8
+  integer(c_int32_t), parameter :: CM1G_STAT_PIPERT  = 16
+  integer(c_int32_t), parameter :: CM1G_STAT_PRSPERT = 32
+  integer(c_int32_t), parameter :: CM1G_STAT_THPERT  = 64
+  integer(c_int32_t), parameter :: CM1G_STAT_Q       = 128
+  integer(c_int32_t), parameter :: CM1G_STAT_SFCPRS  = 256
+  integer(c_int32_t), parameter :: CM1G_STAT_WSP     = 512
+  integer(c_int32_t), parameter :: CM1G_STAT_CFL     = 1024
+  integer(c_int32_t), parameter :: CM1G_STAT_TMASS   = 2048
+  integer(c_int32_t), parameter :: CM1G_STAT_TMOIS   = 4096
+  integer(c_int32_t), parameter :: CM1G_STAT_QMASS   = 8192
+  integer(c_int32_t), parameter :: CM1G_STAT_ALL    = 16383
+  integer(c_int), parameter :: CM1G_STAT_MAXN = 128
   ! staggering of host arrays
   integer(c_int), parameter :: CM1G_STAG_S = 0, CM1G_STAG_U = 1, CM1G_STAG_V = 2, &
                                CM1G_STAG_W = 3
@@ -418,6 +435,17 @@ module cm1gpu
       integer(c_int), value :: nwrite, iorigin, stretch_z, nstep
       integer(c_int32_t), value :: mask
       real(c_double), value :: mtime
+    end function
+    integer(c_int) function cm1g_statpack(m, dir, nrec, mask, mtime, rstat, nstat) &
+        bind(C, name="cm1g_statpack")
+      import :: c_int, c_ptr, c_char, c_int32_t, c_double, c_float
+      type(c_ptr), value :: m
+      character(kind=c_char), dimension(*), intent(in) :: dir
+      integer(c_int), value :: nrec
+      integer(c_int32_t), value :: mask
+      real(c_double), value :: mtime
+      real(c_float), dimension(*), intent(out) :: rstat
+      integer(c_int32_t), intent(out) :: nstat
     end function
     integer(c_int) function cm1g_budgets(m, tmass, tmois) bind(C, name="cm1g_budgets")
       import :: c_int, c_ptr, c_double

@@ -159,6 +159,20 @@ class Model:
                                         int(iorigin), int(stretch_z), float(mtime), int(nstep)),
                 "cm1g_write_grads")
 
+    def statpack(self, directory, nrec, switches=None, mtime=0.0):
+        """Statistics record ``nrec`` of the switches named in ``switches``
+        (the reference's stat_* names, default all) appended to
+        ``directory``/cm1out_stats.dat; returns the record's values."""
+        names = N.STAT if switches is None else switches
+        mask = 0
+        for n in names:
+            mask |= N.STAT[n]
+        r = (C.c_float * N.STAT_MAXN)()
+        n = C.c_int32()
+        N.check(self.L.cm1g_statpack(self.h, os.fsencode(directory), int(nrec), mask, float(mtime), r,
+                                     C.byref(n)), "cm1g_statpack")
+        return np.array(r[:n.value], np.float32)
+
     def scalars(self):
         a = (C.c_double * 4)()
         N.check(self.L.cm1g_scalars(self.h, a), "cm1g_scalars")

@@ -409,6 +409,42 @@ enum {
 int cm1g_write_grads(cm1g_model *m, const char *dir, int nwrite, uint32_t mask, int iorigin,
                      int stretch_z, double mtime, int nstep);
 
+/* statpack (src/statpack.F:22-660, with output_format = 1): one record of
+ * domain statistics of the resident state, in the reference's order,
+ * written as record nrec (1, 2, ...; 1 starts the file afresh) of
+ * cm1out_stats.dat in the existing directory dir, with the descriptor
+ * cm1out_stats.ctl for nrec records (src/statpack.F:1460-1530).  Mask bits
+ * are the reference's stat_* switches (param10); the record always starts
+ * with mtime, and tracer maxima/minima follow whenever npt > 0 (iptra = 1).
+ * Every maximum and minimum is maxmin's (src/maxmin.F): the value found
+ * first in the reference's scan order, the level of w's and the wind
+ * speed's extremes from its position; the mass totals (calcmass, totmois,
+ * totq, src/misclibs.F:3134-3330) add the FP32 cell terms in double in the
+ * reference's order.  All reductions run on the device; only the record
+ * comes back.  rstat (may be NULL) receives the record (at most
+ * CM1G_STAT_MAXN values), nstat (may be NULL) its length. */
+enum {
+  CM1G_STAT_W = 1u << 0,        /* wmax, wmin, zwmax, zwmin */
+  CM1G_STAT_WLEVS = 1u << 1,    /* w extremes at 0.5, 1, 2.5, 5, 10 km (model top >= 10 km) */
+  CM1G_STAT_U = 1u << 2,        /* umax, umin, sumax, sumin */
+  CM1G_STAT_V = 1u << 3,        /* vmax, vmin, svmax, svmin */
+  CM1G_STAT_PIPERT = 1u << 4,   /* ppimax, ppimin */
+  CM1G_STAT_PRSPERT = 1u << 5,  /* ppmax, ppmin (prs - p00 pi0^(cp/rd)) */
+  CM1G_STAT_THPERT = 1u << 6,   /* thpmax, thpmin, sthpmax, sthpmin */
+  CM1G_STAT_Q = 1u << 7,        /* max/min of qv, qc, qr; pratemax, pratemin */
+  CM1G_STAT_SFCPRS = 1u << 8,   /* sprsmax, sprsmin, psfcmax, psfcmin */
+  CM1G_STAT_WSP = 1u << 9,      /* wspmax, wspmin, zwspmax, zwspmin, swspmax, swspmin */
+  CM1G_STAT_CFL = 1u << 10,     /* cflmax (calccfl, src/misclibs.F:2584-2718) */
+  CM1G_STAT_TMASS = 1u << 11,   /* tmass */
+  CM1G_STAT_TMOIS = 1u << 12,   /* tmois */
+  CM1G_STAT_QMASS = 1u << 13,   /* massqv, massqc, massqr */
+  CM1G_STAT_ALL = (1u << 14) - 1u
+};
+#define CM1G_STAT_MAXPAIRS 48
+#define CM1G_STAT_MAXN 128
+int cm1g_statpack(cm1g_model *m, const char *dir, int nrec, uint32_t mask, double mtime, float *rstat,
+                  int32_t *nstat);
+
 /* Scalars carried between steps: [0] mass1, [1] train (domain rain
  * accumulation, src/kessler.F:472), [2] cflmax, [3] mass2. */
 int cm1g_scalars(cm1g_model *m, double *out4);

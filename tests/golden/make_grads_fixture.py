@@ -1,8 +1,11 @@
-"""Golden vectors of the reference's GrADS output (src/writeout.F).
+"""Golden vectors of the reference's GrADS output (src/writeout.F) and
+statistics (src/statpack.F).
 
 Runs ``oracle/_ref/cm1_sp.exe`` with the namelist of a state fixture (so
 from the same t=0 state) and writes output at t = 0 and at the end, with
-every output switch the device writer supports turned on:
+every output switch the device writer supports turned on, and statistics at
+t = 0, halfway and at the end with every statistics switch the device
+statpack supports turned on:
 
 * ``grads_config0_64x64x40_300``: the 64x64x40 supercell of
   ``ref_config0_64x64x40_60.npz`` over 300 large steps (30 minutes, long
@@ -18,7 +21,9 @@ Each fixture stores:
 * ``{s,u,v,w,metadata}_ctl``: the descriptor files, byte for byte;
 * ``metadata_dat``: ``cm1out_metadata.dat`` (mtime, nstep, dt, nwrite);
 * ``rain``, ``prate``: the two 2-D records of the last output time as values, so a
-  failing comparison can say by how much.
+  failing comparison can say by how much;
+* ``stats_dat``, ``stats_ctl``: ``cm1out_stats.dat`` (three records) and its
+  descriptor.
 
 Usage: python tests/golden/make_grads_fixture.py
 """
@@ -39,6 +44,9 @@ from make_ref_fixture import CONFIG1_STRETCH, RDALPHA, STRETCH_KEYS  # noqa: E40
 SWITCHES = ("rain", "zh", "basestate", "th", "thpert", "prs", "prspert", "pi", "pipert",
             "rho", "rhopert", "qv", "qvpert", "q", "buoyancy", "u", "upert", "uinterp", "v",
             "vpert", "vinterp", "w", "winterp")
+# statistics switches (param10) the device statpack supports, all on
+STATS = ("w", "wlevs", "u", "v", "pipert", "prspert", "thpert", "q", "sfcprs", "wsp", "cfl", "tmass",
+         "tmois", "qmass")
 
 
 def digests(path, reclen):
@@ -57,20 +65,25 @@ def make(name, ni, nj, nk, nsteps, dt, dx, stretch=None, workdir="/tmp/cm1_grads
     """The namelist make_ref_fixture.make writes for the same arguments, with
     output at t = 0 and t = nsteps*dt and the writer's switches on."""
     ov = dict(nx=ni, ny=nj, nz=nk, dtl=dt, timax=nsteps * dt, tapfrq=nsteps * dt,
+              statfrq=nsteps * dt / 2,
               wbc=2, ebc=2, sbc=2, nbc=2, rdalpha=RDALPHA, zd=15000.0,
               dx=dx, dy=dx, dz=500.0, psolver=3, advwenos=0, advwenov=0, irdamp=0)
     if stretch is not None:
         ov.update(stretch_z=1, **{k: float(stretch[k]) for k in STRETCH_KEYS})
     ov.update({"output_" + k: 1 for k in SWITCHES})
+    ov.update({"stat_" + k: 1 for k in STATS})
     refrun.run_reference(workdir, ov, threads=8)
     out = dict(meta=np.array([ni, nj, nk, nsteps, 2], np.int32), dt=np.array([dt], np.float32),
-               switches=np.array([s.encode() for s in SWITCHES], dtype="S16"))
+               switches=np.array([s.encode() for s in SWITCHES], dtype="S16"),
+               stats=np.array([s.encode() for s in STATS], dtype="S16"))
     recl = dict(s=ni * nj * 4, u=(ni + 1) * nj * 4, v=ni * (nj + 1) * 4, w=ni * nj * 4)
     for f, r in recl.items():
         out[f"{f}_digest"] = digests(os.path.join(workdir, f"cm1out_{f}.dat"), r)
         out[f"{f}_ctl"] = text(os.path.join(workdir, f"cm1out_{f}.ctl"))
     out["metadata_ctl"] = text(os.path.join(workdir, "cm1out_metadata.ctl"))
     out["metadata_dat"] = text(os.path.join(workdir, "cm1out_metadata.dat"))
+    out["stats_ctl"] = text(os.path.join(workdir, "cm1out_stats.ctl"))
+    out["stats_dat"] = text(os.path.join(workdir, "cm1out_stats.dat"))
     S = refrun.read_grads(workdir, "cm1out_s.ctl")
     out["rain"] = S["rain"][1, 0]
     out["prate"] = S["prate"][1, 0]

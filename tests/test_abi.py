@@ -85,3 +85,16 @@ def test_output_switches_match_header():
     f90 = open(os.path.join(ROOT, "cm1_amd", "fortran", "cm1gpu.f90")).read()
     fort = {n.lower(): int(v) for n, v in re.findall(r"CM1G_OUT_(\w+?)\s*= (\d+)", f90) if n != "ALL"}
     assert fort == hdr
+
+
+def test_statistics_switches_match_header():
+    """CM1G_STAT_* bits of the header, the ctypes mirror and the Fortran module."""
+    from cm1_amd import _native
+    text = open(HEADER).read()
+    hdr = {n.lower(): 1 << int(b) for n, b in re.findall(r"CM1G_STAT_(\w+) = 1u << (\d+)", text)}
+    assert hdr == _native.STAT
+    f90 = open(os.path.join(ROOT, "cm1_amd", "fortran", "cm1gpu.f90")).read()
+    fort = {n.lower(): int(v) for n, v in re.findall(r"CM1G_STAT_(\w+?)\s*= (\d+)", f90)
+            if n not in ("ALL", "MAXN")}
+    assert fort == hdr
+    assert int(re.search(r"#define CM1G_STAT_MAXN (\d+)", text).group(1)) == _native.STAT_MAXN

@@ -1,4 +1,5 @@
-"""Device GrADS writer (cm1g_write_grads) against the reference's own output.
+"""Device GrADS writer (cm1g_write_grads) and statpack (cm1g_statpack)
+against the reference's own output.
 
 Golden vectors (tests/golden/make_grads_fixture.py): the reference
 executable, started from the t=0 state of a state fixture with every output
@@ -13,7 +14,13 @@ same state, writes output time 1, steps N times and writes output time 2:
   points) has the reference's digest, i.e. equals it bit for bit;
 * the descriptors (.ctl) and the metadata file equal the reference's byte
   for byte, for the unstretched 64x64x40 grid (300 steps) and for the
-  stretched 96-level grid (10 steps), whose descriptors list the levels.
+  stretched 96-level grid (10 steps), whose descriptors list the levels;
+* the three statistics records (t=0, halfway, end: 55 values each, the
+  maxima/minima of w, u, v, pi', p', th', q, prate, surface pressure, wind
+  speed, w at five heights, the levels of the w and wind-speed extremes,
+  the Courant number, and the dry-air, moisture and per-species mass
+  totals) equal the reference's cm1out_stats.dat bit for bit, and the stats
+  descriptor byte for byte.
 """
 import hashlib
 import os
@@ -51,8 +58,13 @@ def test_grads_output_equals_the_reference(require_gpu, tmp_path, case, state, s
     cfgd["nsound"] = nsound_for(cfgd, cols["zf"])
     m = make_gpu(cfgd, cols, fields)
     out = str(tmp_path)
+    stats = [s.decode() for s in g["stats"]]
     m.write_grads(out, 1, switches, iorigin=2, stretch_z=stretch, mtime=0.0, nstep=0)
-    m.step(nsteps)
+    rec = [m.statpack(out, 1, stats, mtime=0.0)]
+    m.step(nsteps // 2)
+    rec.append(m.statpack(out, 2, stats, mtime=(nsteps // 2) * dt))
+    m.step(nsteps - nsteps // 2)
+    rec.append(m.statpack(out, 3, stats, mtime=nsteps * dt))
     m.write_grads(out, 2, switches, iorigin=2, stretch_z=stretch, mtime=nsteps * dt, nstep=nsteps)
     print(f"{case}: {nsteps} steps, output written to {out}", flush=True)
 
@@ -82,4 +94,17 @@ def test_grads_output_equals_the_reference(require_gpu, tmp_path, case, state, s
     assert np.array_equal(rain, g["rain"]) and np.array_equal(prate, g["prate"])
     if nsteps >= 300:
         assert g["rain"].max() > 0.0 and g["prate"].max() > 0.0
+    # statistics: values and file against the reference's
+    ref_stats = np.frombuffer(g["stats_dat"].tobytes(), dtype="<f4").reshape(3, -1)
+    names = ctl_records(g["stats_ctl"].tobytes().decode())
+    got = np.array(rec)
+    assert got.shape == ref_stats.shape, (got.shape, ref_stats.shape)
+    bad = [(t, names[n][0], float(got[t, n]), float(ref_stats[t, n]))
+           for t in range(3) for n in range(got.shape[1])
+           if got[t, n].view(np.uint32) != ref_stats[t, n].view(np.uint32)]
+    print(f"statistics: {got.size} values, {len(bad)} differ {bad[:8]}", flush=True)
+    assert not bad, bad[:20]
+    assert open(os.path.join(out, "cm1out_stats.dat"), "rb").read() == g["stats_dat"].tobytes()
+    assert open(os.path.join(out, "cm1out_stats.ctl"), "rb").read() == g["stats_ctl"].tobytes()
+
     m.close()
