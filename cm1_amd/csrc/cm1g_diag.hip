@@ -68,6 +68,16 @@ static void mm_unpack(const unsigned long long h[2], int izz, int jzz, float *vm
   unpack(h[1], false, vmin, locmin);
 }
 
+// maxmin reports the value it found at the winning position: read it back
+// there, so a zero extreme keeps the sign of the point the scan met first
+// (the keys hold -0 and +0 as one value).  f: the field's origin, px/pxy
+// its strides.
+static int mm_value_at(const float *f, int px, int pxy, const int32_t *loc, float *v, hipStream_t s) {
+  const size_t c = (size_t)(loc[0] + CM1G_IOFF) + (size_t)(loc[1] + 2) * px + (size_t)loc[2] * pxy;
+  HIPCHK(hipMemcpyAsync(v, f + c, sizeof(float), hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
 extern "C" int cm1g_maxmin(cm1g_model *m, int id, int stag, float *vmax, int32_t *locmax, float *vmin,
                            int32_t *locmin) {
   ARGCHK(m && id >= 0 && id < CM1G_NFIELDS && m->f[id], "cm1g_maxmin: bad field id");
@@ -107,6 +117,12 @@ extern "C" int cm1g_maxmin(cm1g_model *m, int id, int stag, float *vmax, int32_t
   HIPCHK(hipMemcpyAsync(h, key, sizeof(h), hipMemcpyDeviceToHost, m->stream));
   HIPCHK(hipStreamSynchronize(m->stream));
   mm_unpack(h, izz, jzz, vmax, locmax, vmin, locmin);
+  if (h[0] != 0ULL) {
+    int rc = mm_value_at(f, P.px, P.pxy, locmax, vmax, m->stream);
+    if (!rc) rc = mm_value_at(f, P.px, P.pxy, locmin, vmin, m->stream);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(m->stream));
+  }
   return 0;
 }
 
@@ -219,6 +235,12 @@ extern "C" int cm1g_maxmin_noalloc(const float *field, int ni, int nj, int nk, i
   HIPCHK(hipMemcpyAsync(h, part + 2 * MM_NOALLOC_BLOCKS, sizeof(h), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   mm_unpack(h, izz, jzz, vmax, locmax, vmin, locmin);
+  if (h[0] != 0ULL) {
+    int rc = mm_value_at(field, px, px * py, locmax, vmax, s);
+    if (!rc) rc = mm_value_at(field, px, px * py, locmin, vmin, s);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+  }
   return 0;
 }
 

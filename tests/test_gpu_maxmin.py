@@ -44,3 +44,26 @@ def test_maxmin_of_a_stepped_field(require_gpu):
         want = R.maxmin(f, a.shape[2], a.shape[1], a.shape[0])
         got = g.maxmin(name, stag)
         assert got == want, (name, got, want)
+
+
+def test_maxmin_keeps_the_sign_of_a_zero_extreme(require_gpu):
+    """maxmin reports the value at the point its scan met first: with every
+    value zero and -0.0 at (1,1,1), both extremes are -0.0 there; with -0.0
+    elsewhere, +0.0 at (1,1,1)."""
+    cfgd, cols, fields = case(ni=24, nj=20, nk=12)
+    g = make_gpu(cfgd, cols, fields)
+    ni, nj, nk = cfgd["ni"], cfgd["nj"], cfgd["nk"]
+    for pos in ((0, 0, 0), (3, 4, 5)):
+        box = np.zeros((g.pz, g.py, ni + 7), np.float32)
+        inner = np.zeros((nk, nj, ni), np.float32)
+        inner[pos] = -0.0
+        box[1:nk + 1, 3:nj + 3, 3:ni + 3] = inner
+        g.put("dum3", box)
+        (vmax, lmax), (vmin, lmin) = g.maxmin("dum3")
+        f = np.zeros((nk + 1, nj + 1, ni + 1), np.float32)
+        f[1:, 1:, 1:] = inner
+        (rmax, rlmax), (rmin, rlmin) = R.maxmin(f, ni, nj, nk)
+        assert lmax == rlmax == (1, 1, 1) and lmin == rlmin == (1, 1, 1)
+        first_negative = pos == (0, 0, 0)
+        assert np.signbit(vmax) == np.signbit(rmax) == first_negative
+        assert np.signbit(vmin) == np.signbit(rmin) == first_negative
