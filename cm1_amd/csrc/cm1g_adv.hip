@@ -384,8 +384,14 @@ __device__ __forceinline__ void advs_col(const KP &P, const AdvsArgs &a, const C
 // (src/solve2.F:1416-1426, :1518-1528).
 template <bool WENO, bool SADD, bool TH, bool MODE1>
 static void advs_launch(cm1g_model *m, const KP &P, const AdvsArgs &a, const ColGrid &cg) {
-  launch<256>(m, col_blocks(cg), col_threads(cg),
-              [=] __device__(int bx, int, int) { advs_col<WENO, SADD, TH, MODE1>(P, a, cg, bx); });
+  // 8 waves per SIMD (64 registers) for the fifth-order scalar passes: the
+  // 4096 workgroups of a 1024 x 1024 grid then run in exactly two rounds (7
+  // waves per SIMD leave a third round 29% full).  Measured per launch: q
+  // 849 against 943 us, the pdef pass 793 against 866 us (both without
+  // spilling), theta 1158 against 1185 us (20 bytes per lane spilled)
+  constexpr int W = (CM1G_COL_WAVES >= 1 && !WENO && (!TH || CM1G_COL_WAVES >= 2)) ? 8 : 0;
+  launch<256, W>(m, col_blocks(cg), col_threads(cg),
+                 [=] __device__(int bx, int, int) { advs_col<WENO, SADD, TH, MODE1>(P, a, cg, bx); });
 }
 
 void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float *s0,

@@ -168,9 +168,9 @@ __global__ void __launch_bounds__(NT) k_one(F f) {
   f((int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
 }
 
-template <int NT, class F>
-__global__ void __launch_bounds__(NT) k_many(const F *__restrict__ fs, const BatchSlot *__restrict__ slots,
-                                             int nmem) {
+// The member search and functor call of k_many (below).
+template <class F>
+__device__ __forceinline__ void many_body(const F *__restrict__ fs, const BatchSlot *__restrict__ slots, int nmem) {
   // the search is uniform across the workgroup: keep it in scalar registers
   // so the member's functor is read with scalar loads
   const int b = (int)blockIdx.x;
@@ -191,6 +191,26 @@ __global__ void __launch_bounds__(NT) k_many(const F *__restrict__ fs, const Bat
   f(l % gx, (l / gx) % gy, l / (gx * gy));
 }
 
+template <int NT, class F>
+__global__ void __launch_bounds__(NT) k_many(const F *__restrict__ fs, const BatchSlot *__restrict__ slots,
+                                             int nmem) {
+  many_body(fs, slots, nmem);
+}
+
+// The same two launchers compiled for at least W waves per SIMD (the
+// register allocation is capped to fit W): used where the grid's wavefront
+// count is a multiple of what W waves per SIMD hold on the chip, so no
+// partly filled last round is left.
+template <int NT, int W, class F>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_one_w(F f) {
+  f((int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+}
+template <int NT, int W, class F>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W)))
+k_many_w(const F *__restrict__ fs, const BatchSlot *__restrict__ slots, int nmem) {
+  many_body(fs, slots, nmem);
+}
+
 // One recorded launch: the functor's bytes, its grid and workgroup, and the
 // launcher of the k_many instance for its type (which also identifies it).
 struct LaunchRec {
@@ -199,17 +219,28 @@ struct LaunchRec {
   std::vector<unsigned char> fn;
 };
 
-template <int NT, class F>
+template <int NT, int W, class F>
 void launch_many(hipStream_t s, const void *fs, const BatchSlot *slots, int nmem, int nblocks, dim3 block) {
-  hipLaunchKernelGGL((k_many<NT, F>), dim3(nblocks), block, 0, s, (const F *)fs, slots, nmem);
+  if constexpr (W > 0)
+    hipLaunchKernelGGL((k_many_w<NT, W, F>), dim3(nblocks), block, 0, s, (const F *)fs, slots, nmem);
+  else
+    hipLaunchKernelGGL((k_many<NT, F>), dim3(nblocks), block, 0, s, (const F *)fs, slots, nmem);
 }
 
-template <int NT, class F>
+// Scalar advection column kernels compiled for 8 waves per SIMD: 0 none
+// (the compiler's choice), 1 the fifth-order passes of q and the pdef pass,
+// 2 also theta's.
+#ifndef CM1G_COL_WAVES
+#define CM1G_COL_WAVES 2
+#endif
+
+// W > 0: the kernel is compiled for at least W waves per SIMD
+template <int NT, int W = 0, class F>
 inline void launch(cm1g_model *m, dim3 g, dim3 b, const F &f) {
   static_assert(std::is_trivially_copyable<F>::value, "launch: functor must be trivially copyable");
   if (m->rec) {
     LaunchRec r;
-    r.many = &launch_many<NT, F>;
+    r.many = &launch_many<NT, W, F>;
     r.grid = g;
     r.block = b;
     r.fn.resize(sizeof(F));
@@ -217,7 +248,10 @@ inline void launch(cm1g_model *m, dim3 g, dim3 b, const F &f) {
     m->rec->push_back(std::move(r));
     return;
   }
-  hipLaunchKernelGGL((k_one<NT, F>), g, b, 0, m->stream, f);
+  if constexpr (W > 0)
+    hipLaunchKernelGGL((k_one_w<NT, W, F>), g, b, 0, m->stream, f);
+  else
+    hipLaunchKernelGGL((k_one<NT, F>), g, b, 0, m->stream, f);
 }
 
 // Element-wise launches.  Point i = 1 sits at the start of a 128-byte line

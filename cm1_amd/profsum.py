@@ -7,7 +7,8 @@ Usage:
       # ``bench.py --copy-calibration`` runs first (1 GiB each)
 
 Kernel names are shortened to the device function each launcher runs
-(``k_one<NT, F>``: ``F``'s host function and lambda number)."""
+(``k_one<NT, F>`` and ``k_one_w<NT, W, F>``: ``F``'s host function and
+lambda number)."""
 import csv
 import re
 import sys
@@ -16,7 +17,7 @@ from collections import defaultdict
 
 def short(name):
     """k_one<256, for3<cm1g_solve2::{lambda(int, int, int)#1}>(...)> -> for3<cm1g_solve2 λ1>."""
-    m = re.match(r"void k_one<\d+, (.*)>\(", name)
+    m = re.match(r"void k_one(?:_w)?<\d+, (?:\d+, )?(.*)>\(", name)
     if not m:
         return name.split("(")[0]
     body = m.group(1)
