@@ -62,6 +62,7 @@ struct cm1g_model {
   float *sx, *sy, *sz, *tx, *ty, *tz;   // advection fluxes / pdef work
   float *thb;                   // ping-pong partner for th3d / q3d updates
   float *scr1, *scr2;           // u/v partner arrays of the acoustic substeps
+  float *scoef;                 // per-level acoustic solver coefficients, 8 x (nk+3)
   double *dred;                 // reduction workspace (per-k partial sums etc.)
   float *fred;                  // float reduction workspace
   double *dscal;                // device scalars: [0] mass1 [1] mass2 [2] tem(pp shift) [3] train

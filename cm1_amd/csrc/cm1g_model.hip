@@ -149,6 +149,7 @@ extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
   }
   for (int i = 0; i < 6 && !rc; i++) rc = dalloc(&m->h[i], (i < 3 ? P.px : P.py));
   for (int i = 0; i < CM1G_D_NFIELDS && !rc; i++) rc = dalloc(&m->d2[i], (size_t)P.pxy + (size_t)(P.px + P.py) * pz);
+  if (!rc) rc = dalloc(&m->scoef, 8 * (size_t)pz);
   if (!rc) rc = dalloc(&m->dtu0, P.pxy);
   if (!rc) rc = dalloc(&m->dtv0, P.pxy);
   float **scr[] = {&m->sx, &m->sy, &m->sz, &m->tx, &m->ty, &m->tz,
@@ -228,7 +229,7 @@ extern "C" int cm1g_destroy(cm1g_model *m) {
   for (int i = 0; i < CM1G_K_NFIELDS; i++) { if (m->kf[i]) hipFree(m->kf[i]); free(m->hk[i]); }
   for (int i = 0; i < 6; i++) if (m->h[i]) hipFree(m->h[i]);
   for (int i = 0; i < CM1G_D_NFIELDS; i++) if (m->d2[i]) hipFree(m->d2[i]);
-  float *scr[] = {m->dtu0, m->dtv0, m->sx, m->sy, m->sz, m->tx, m->ty, m->tz,
+  float *scr[] = {m->scoef, m->dtu0, m->dtv0, m->sx, m->sy, m->sz, m->tx, m->ty, m->tz,
                   m->scr1, m->scr2, m->thb, m->fred};
   for (auto p : scr) if (p) hipFree(p);
   if (m->dred) hipFree(m->dred);

@@ -45,6 +45,9 @@ struct SubArgs {
   // is what the reference's copies before the substep loop set
   // (src/sound.F:191-250)
   const float *wR, *ppR, *wa;
+  // per level k: tem, r1, r2, temm, c1(k), c1(k+1), c2(k), c2(k+1) of the
+  // solver coefficients (src/sound.F:117-180), 8 floats at coef + 8k
+  const float *coef;
   int first;
   int nring;                      // trailing workgroups that do the averaging ring
 };
@@ -124,6 +127,37 @@ __device__ __forceinline__ void sound_ring_point(const KP &P, const SubArgs &a, 
   put_w_avg(a, c, a.wR[c]);
 }
 
+// Operands of one level of the upward sweep of sound_sub (raw loads).
+struct SubLoads {
+  float uXc, uXe, utc, ute, vXc, vXn, vtc, vtn;
+  float pdc, pdw, pde, pds, pdn;
+  float thc, thw, the, ths, thn;
+  float rruc, rrvc;
+  float paw, pae, pas, pan, pa_p;
+  float ppterm, ppten, wten, pp_c, w_p;
+  float rw, re, rs, rn;
+};
+
+__device__ __forceinline__ void sub_load(const KP &P, const SubArgs &a, int c, int k, int i, int j, bool bw,
+                                         bool be, bool bs, bool bn, SubLoads &L) {
+  const int px = P.px, pxy = P.pxy;
+  L.uXc = a.uX[c]; L.uXe = a.uX[c + 1]; L.utc = a.uten[c]; L.ute = a.uten[c + 1];
+  L.vXc = a.vX[c]; L.vXn = a.vX[c + px]; L.vtc = a.vten[c]; L.vtn = a.vten[c + px];
+  L.pdc = a.ppdX[c]; L.pdw = a.ppdX[c - 1]; L.pde = a.ppdX[c + 1]; L.pds = a.ppdX[c - px];
+  L.pdn = a.ppdX[c + px];
+  L.thc = a.thv[c]; L.thw = a.thv[c - 1]; L.the = a.thv[c + 1]; L.ths = a.thv[c - px]; L.thn = a.thv[c + px];
+  L.rruc = a.first ? 0.0f : a.rru[c];
+  L.rrvc = a.first ? 0.0f : a.rrv[c];
+  L.paw = a.piadv[c - 1]; L.pae = a.piadv[c + 1]; L.pas = a.piadv[c - px]; L.pan = a.piadv[c + px];
+  L.pa_p = a.piadv[c + pxy];
+  L.ppterm = a.ppterm[c]; L.ppten = a.ppten[c]; L.wten = a.wten[c]; L.pp_c = a.ppR[c];
+  L.w_p = (k == P.nk) ? a.w3d[c + pxy] : a.wR[c + pxy];
+  L.rw = bw ? a.radbcw[(j + 2) + k * P.py] : 0.0f;
+  L.re = be ? a.radbce[(j + 2) + k * P.py] : 0.0f;
+  L.rs = bs ? a.radbcs[(i + 2) + k * px] : 0.0f;
+  L.rn = bn ? a.radbcn[(i + 2) + k * px] : 0.0f;
+}
+
 // One acoustic substep for a column (i,j): u/v forward step (also the
 // faces i+1 / j+1 the divergence needs, recomputed from the same inputs),
 // explicit pi forcing fpk (src/sound.F:484-505), Thomas forward sweep of the
@@ -134,9 +168,8 @@ __device__ __forceinline__ void sound_ring_point(const KP &P, const SubArgs &a, 
 // written to uY/vY (pi'+damping likewise ppdX/ppdY) so that neighbouring
 // columns see the old values.
 //
-// Each level issues all of its loads before any store, so a level costs one
-// memory round trip; the downward sweep also prefetches the next level's
-// operands before storing the current one.
+// Both sweeps load the next level's operands before the current level is
+// evaluated and stored, so two levels' loads are in flight per column.
 //
 // The a.nring workgroups after the column tiles do the averaging ring (one
 // thread per ring point and level); they fill the CUs the last tiles leave
@@ -172,24 +205,23 @@ __device__ __forceinline__ void sound_sub(const KP &P, const SubArgs &a, const C
   float pa_m = a.piadv[c - pxy], pa_c = a.piadv[c];
   float th_m = a.thv[c - pxy];
   float w_c = a.wR[c];
+  SubLoads nx;
+  sub_load(P, a, c, 1, i, j, bw, be, bs, bn, nx);
   for (int k = 1; k <= nk; k++, c += pxy) {
-    // ---- loads
-    const float uXc = a.uX[c], uXe = a.uX[c + 1], utc = a.uten[c], ute = a.uten[c + 1];
-    const float vXc = a.vX[c], vXn = a.vX[c + px], vtc = a.vten[c], vtn = a.vten[c + px];
-    const float pdc = a.ppdX[c], pdw = a.ppdX[c - 1], pde = a.ppdX[c + 1], pds = a.ppdX[c - px],
-                pdn = a.ppdX[c + px];
-    const float thc = a.thv[c], thw = a.thv[c - 1], the = a.thv[c + 1], ths = a.thv[c - px],
-                thn = a.thv[c + px];
-    const float rruc = a.first ? 0.0f : a.rru[c], rrvc = a.first ? 0.0f : a.rrv[c];
-    const float paw = a.piadv[c - 1], pae = a.piadv[c + 1], pas = a.piadv[c - px], pan = a.piadv[c + px];
-    const float pa_p = a.piadv[c + pxy];
-    const float ppterm = a.ppterm[c], ppten = a.ppten[c], wten = a.wten[c], pp_c = a.ppR[c];
-    const float w_p = (k == nk) ? a.w3d[c + pxy] : a.wR[c + pxy];
-    float rw = 0.0f, re = 0.0f, rs = 0.0f, rn = 0.0f;
-    if (bw) rw = a.radbcw[(j + 2) + k * P.py];
-    if (be) re = a.radbce[(j + 2) + k * P.py];
-    if (bs) rs = a.radbcs[(i + 2) + k * px];
-    if (bn) rn = a.radbcn[(i + 2) + k * px];
+    // ---- this level's operands; the next level's are loaded before this
+    // level is evaluated and stored
+    const SubLoads L = nx;
+    if (k < nk) sub_load(P, a, c + pxy, k + 1, i, j, bw, be, bs, bn, nx);
+    const float uXc = L.uXc, uXe = L.uXe, utc = L.utc, ute = L.ute;
+    const float vXc = L.vXc, vXn = L.vXn, vtc = L.vtc, vtn = L.vtn;
+    const float pdc = L.pdc, pdw = L.pdw, pde = L.pde, pds = L.pds, pdn = L.pdn;
+    const float thc = L.thc, thw = L.thw, the = L.the, ths = L.ths, thn = L.thn;
+    const float rruc = L.rruc, rrvc = L.rrvc;
+    const float paw = L.paw, pae = L.pae, pas = L.pas, pan = L.pan;
+    const float pa_p = L.pa_p;
+    const float ppterm = L.ppterm, ppten = L.ppten, wten = L.wten, pp_c = L.pp_c;
+    const float w_p = L.w_p;
+    const float rw = L.rw, re = L.re, rs = L.rs, rn = L.rn;
     // ---- u, v (ssopenbc* then src/sound.F:307-345)
     float uo0 = uXc, uo1 = uXe, vo0 = vXc, vo1 = vXn;
     if (bw) uo0 = uo0 + dts * (-rw * (uXe - uo0) * rdx * P.uh[1 + 2] + utc);
@@ -201,11 +233,9 @@ __device__ __forceinline__ void sound_sub(const KP &P, const SubArgs &a, const C
     const float vn0 = vo0 + dtv_c * (vtc - tem2 * (pdc - pds) * vf_c * (thc + ths));
     const float vn1 = vo1 + dtv_n * (vtn - tem2 * (pdn - pdc) * vf_n * (thn + thc));
     // ---- coefficients (src/sound.F:117-180)
-    const float tem = dts * rdz * P.mh[k];
-    const float r2 = dts * rdz * P.mh[k] * P.rr0[k] * P.rf0[k + 1];
-    const float r1 = dts * rdz * P.mh[k] * P.rr0[k] * P.rf0[k];
-    const float temm = dts * rdz * cp * P.mf[k];
-    const float c1a = P.c1[k], c1b = P.c1[k + 1], c2a = P.c2[k], c2b = P.c2[k + 1];
+    const float *cf = a.coef + 8 * k;
+    const float tem = cf[0], r1 = cf[1], r2 = cf[2], temm = cf[3];
+    const float c1a = cf[4], c1b = cf[5], c2a = cf[6], c2b = cf[7];
     const float qk = tem * ppterm;
     const float pk2 = (k == nk) ? 0.0f : r2 * (-c2b * pa_p + (1.0f - c1b) * pa_c);
     const float pk1 = (k == 1) ? 0.0f : r1 * (+c1a * pa_m + (c2a - 1.0f) * pa_c);
@@ -293,10 +323,9 @@ __device__ __forceinline__ void sound_sub(const KP &P, const SubArgs &a, const C
       q_term = a.ppterm[cb];
       q_rrw = a.first ? a.wa[cb] * a.beta : a.rrw[cb];
     }
-    const float tem = dts * rdz * P.mh[k];
-    const float r2 = dts * rdz * P.mh[k] * P.rr0[k] * P.rf0[k + 1];
-    const float r1 = dts * rdz * P.mh[k] * P.rr0[k] * P.rf0[k];
-    const float c1a = P.c1[k], c1b = P.c1[k + 1], c2a = P.c2[k], c2b = P.c2[k + 1];
+    const float *cf = a.coef + 8 * k;
+    const float tem = cf[0], r1 = cf[1], r2 = cf[2];
+    const float c1a = cf[4], c1b = cf[5], c2a = cf[6], c2b = cf[7];
     const float qk = tem * pterm;
     const float pk2 = (k == nk) ? 0.0f : r2 * (-c2b * pa_p + (1.0f - c1b) * pa_c2);
     const float pk1 = (k == 1) ? 0.0f : r1 * (+c1a * pa_m2 + (c2a - 1.0f) * pa_c2);
@@ -382,6 +411,26 @@ void sound_run(cm1g_model *m, int nrk, bool fold1) {
   {
     const int nthr = (int)blk.x, npts = (ni + nj + 1) * nk;
     a.nring = (npts + nthr - 1) / nthr;
+  }
+  // the per-level solver coefficients of this stage's dts, with the
+  // reference's expressions (src/sound.F:117-180)
+  {
+    float *cf = m->scoef;
+    const float rdz = P.rdz, cp = P.K.cp;
+    launch<64>(m, dim3((nk + 63) / 64), dim3(64), [=] __device__(int bx, int, int) {
+      const int k = 1 + bx * 64 + (int)threadIdx.x;
+      if (k > nk) return;
+      float *o = cf + 8 * k;
+      o[0] = dts * rdz * P.mh[k];
+      o[1] = dts * rdz * P.mh[k] * P.rr0[k] * P.rf0[k];
+      o[2] = dts * rdz * P.mh[k] * P.rr0[k] * P.rf0[k + 1];
+      o[3] = dts * rdz * cp * P.mf[k];
+      o[4] = P.c1[k];
+      o[5] = P.c1[k + 1];
+      o[6] = P.c2[k];
+      o[7] = P.c2[k + 1];
+    });
+    a.coef = cf;
   }
   float *ppdb[2] = {ppd, m->tx};
   float *uprev = nullptr, *vprev = nullptr;

@@ -31,7 +31,8 @@ def short(name):
     h = re.match(r"([\w:]+)(?:\(.*?\))?::\{lambda\(int, int(?:, int)?\)#(\d+)\}", body)
     if h:
         host = h.group(1).split("::")[-1]
-        names = {("sound_run", "1"): "sound_sub", ("advs_run", "1"): "pdef_xy", ("advs_run", "2"): "pdef_z",
+        names = {("sound_run", "1"): "sound_coef", ("sound_run", "2"): "sound_sub",
+                 ("advs_run", "1"): "pdef_xy", ("advs_run", "2"): "pdef_z",
                  ("cm1g_solve2", "2"): "massflux_div", ("advm_run", "1"): "ten_edges"}
         return names.get((host, h.group(2)), f"{host} λ{h.group(2)}")
     return body.split("(")[0]
