@@ -493,6 +493,6 @@ void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float
 // (model fields) and divx; uten/vten/wten are updated in place, or, with
 // from_ten1, set to uten1/vten1/wten1 plus advection (column kernels in
 // cm1g_advm.hip).
-void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own) {
-  advm_run(m, nrk, mask, buoy, from_ten1, own);
+void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own, const float *const *src3) {
+  advm_run(m, nrk, mask, buoy, from_ten1, own, src3);
 }

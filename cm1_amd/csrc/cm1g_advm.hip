@@ -290,7 +290,7 @@ static void advm_launch(cm1g_model *m, const KP &P, const AdvmArgs &g, const Col
               [=] __device__(int bx, int, int) { advm_col<STAG, WENO, OWN>(P, g, cg, bx); });
 }
 
-void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own) {
+void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own, const float *const *src3) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj;
   const bool weno = (m->c.advwenov == 1) || (m->c.advwenov == 2 && nrk == 3);
@@ -314,7 +314,7 @@ void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own
                 });
   }
   if (mask & 1) {
-    g.a = m->f[CM1G_U3D];
+    g.a = src3 ? src3[0] : m->f[CM1G_U3D];
     g.ten = m->f[CM1G_UTEN];
     g.ten_in = from_ten1 ? m->f[CM1G_UTEN1] : g.ten;
     g.i1 = ui1;
@@ -327,7 +327,7 @@ void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own
     kern(m, P, g, cg);
   }
   if (mask & 2) {
-    g.a = m->f[CM1G_V3D];
+    g.a = src3 ? src3[1] : m->f[CM1G_V3D];
     g.ten = m->f[CM1G_VTEN];
     g.ten_in = from_ten1 ? m->f[CM1G_VTEN1] : g.ten;
     g.i1 = 1;
@@ -340,7 +340,7 @@ void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own
     kern(m, P, g, cg);
   }
   if (mask & 4) {
-    g.a = m->f[CM1G_W3D];
+    g.a = src3 ? src3[2] : m->f[CM1G_W3D];
     g.ten = m->f[CM1G_WTEN];
     g.ten_in = from_ten1 ? m->f[CM1G_WTEN1] : g.ten;
     g.i1 = 1;

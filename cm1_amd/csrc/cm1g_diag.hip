@@ -81,6 +81,7 @@ static int mm_value_at(const float *f, int px, int pxy, const int32_t *loc, floa
 extern "C" int cm1g_maxmin(cm1g_model *m, int id, int stag, float *vmax, int32_t *locmax, float *vmin,
                            int32_t *locmin) {
   ARGCHK(m && id >= 0 && id < CM1G_NFIELDS && m->f[id], "cm1g_maxmin: bad field id");
+  { const int rc_ = unswap(m); if (rc_) return rc_; }
   ARGCHK(vmax && locmax && vmin && locmin, "cm1g_maxmin: null argument");
   ARGCHK(stag >= CM1G_STAG_S && stag <= CM1G_STAG_W, "cm1g_maxmin: bad staggering");
   const KP P = m->P;

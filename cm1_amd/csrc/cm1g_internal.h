@@ -97,6 +97,12 @@ struct cm1g_model {
   // arrays: the caller may then write them where the library cannot see
   // it, so synced is never set again on this handle
   int ext_writers;
+  // 1 after a step that ended by exchanging the a-arrays with the 3-D
+  // arrays (solve3) instead of copying: the a-arrays hold the state, the
+  // 3-D arrays are stale until the first RK stage rewrites them (it reads
+  // the a-arrays in their place), or until unswap() copies them back for a
+  // caller that reads them
+  int swapped;
   // non-null while an ensemble records this member's step: launches are
   // appended here instead of run
   std::vector<struct LaunchRec> *rec;
@@ -122,6 +128,9 @@ struct cm1g_model {
 
 // release the output staging and the statpack buffer (cm1g_io.hip)
 void io_free(cm1g_model *m);
+// make the 3-D arrays equal the a-arrays again after an exchanging solve3
+// (entry points that read or write fields by id call this first)
+int unswap(cm1g_model *m);
 // drop the captured steps (after anything that changes launch arguments)
 void graphs_clear(cm1g_model *m);
 // the host side of one large step (solve1, solve2, mp_driver, solve3)
@@ -372,8 +381,11 @@ void bc_fill_n(cm1g_model *m, int kind, float *const *arrs, int n);
 void radbc4(cm1g_model *m, const float *ua, const float *u3d, const float *va,
             const float *v3d, float dt);
 // advection (cm1g_adv.hip)
-void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1 = 0, int own = 0);
-void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own);  // cm1g_advm.hip
+// src3 (optional): the u, v, w arrays to advect instead of u3d, v3d, w3d
+void adv_uvw(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1 = 0, int own = 0,
+             const float *const *src3 = nullptr);
+void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own,
+              const float *const *src3 = nullptr);  // cm1g_advm.hip
 void advs_run(cm1g_model *m, int nrk, const float *s_in, float sadd, const float *s0,
               const float *sten0, const float *thterm_div, float *out_sten, float *out_new,
               int pdef, int pdefweno, float dt, double weps, int smeps_flush);

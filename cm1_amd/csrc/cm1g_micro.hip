@@ -622,6 +622,7 @@ extern "C" int cm1g_surrogate_counts(cm1g_model *m, int64_t *out6) {
 // prs, rho, q3d(nqv..nqr) and qten(nqr) as the fall speed.
 extern "C" int cm1g_k_fallout(cm1g_model *m) {
   ARGCHK(m && m->c.imoist == 1, "cm1g_k_fallout: needs a moist model");
+  { const int rc_ = unswap(m); if (rc_) return rc_; }
   m->synced = 0;
   const KP P = m->P;
   const float *rho = m->f[CM1G_RHO], *qr = m->f[CM1G_Q3D2];
@@ -638,6 +639,7 @@ extern "C" int cm1g_k_fallout(cm1g_model *m) {
 
 extern "C" int cm1g_fallout(cm1g_model *m, double *train) {
   ARGCHK(m && train && m->c.imoist == 1, "cm1g_fallout: bad argument");
+  { const int rc_ = unswap(m); if (rc_) return rc_; }
   m->synced = 0;
   HIPCHK(hipMemcpyAsync(m->dscal + 3, train, sizeof(double), hipMemcpyHostToDevice, m->stream));
   fallout_run(m);
@@ -651,6 +653,7 @@ extern "C" int cm1g_geterain(cm1g_model *m, float dt, float cpx, float lx1, int 
                              double *erain) {
   ARGCHK(m && erain && field_ok(m, t_id) && field_ok(m, rr_id) && field_ok(m, q_id),
          "cm1g_geterain: bad argument");
+  { const int rc_ = unswap(m); if (rc_) return rc_; }
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj;
   const float *t = m->f[t_id], *rr = m->f[rr_id], *q = m->f[q_id], *vr = m->f[CM1G_QTEN2];
@@ -677,6 +680,7 @@ extern "C" int cm1g_geterain(cm1g_model *m, float dt, float cpx, float lx1, int 
 
 extern "C" int cm1g_kessler(cm1g_model *m) {
   ARGCHK(m && m->c.imoist == 1, "cm1g_kessler: needs a moist model");
+  { const int rc_ = unswap(m); if (rc_) return rc_; }
   m->synced = 0;
   const KP P = m->P;
   const float dt = m->c.dt;
@@ -691,6 +695,7 @@ extern "C" int cm1g_kessler(cm1g_model *m) {
 
 extern "C" int cm1g_satadj(cm1g_model *m) {
   ARGCHK(m && m->c.imoist == 1, "cm1g_satadj: needs a moist model");
+  { const int rc_ = unswap(m); if (rc_) return rc_; }
   m->synced = 0;
   const KP P = m->P;
   float *th3d = m->f[CM1G_TH3D], *pp3d = m->f[CM1G_PP3D], *prs = m->f[CM1G_PRS];

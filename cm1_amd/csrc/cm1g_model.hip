@@ -20,6 +20,12 @@ extern "C" const char *cm1g_last_error(void) { return g_err.c_str(); }
       return CM1G_E_ARG;               \
     }                                  \
   } while (0)
+// fields read or written by id see the 3-D arrays equal the a-arrays
+#define UNSWAP(m)                      \
+  do {                                 \
+    const int rc_ = unswap(m);         \
+    if (rc_) return rc_;               \
+  } while (0)
 
 static void set_constants(Consts &K) {
   K.g = 9.81f; K.rd = 287.04f; K.cp = 1005.7f; K.cpv = 1870.0f; K.xlv = 2501000.0f;
@@ -383,6 +389,7 @@ static float *box_origin(const cm1g_model *m, int id) { return m->f[id] + (CM1G_
 
 extern "C" int cm1g_put_field(cm1g_model *m, int id, const float *host) {
   ARGCHK(field_ok(m, id) && host, "cm1g_put_field: bad field id or null pointer");
+  UNSWAP(m);
   m->synced = 0;
   const size_t w = (size_t)m->P.ni + 7, rows = (size_t)m->P.py * (size_t)(m->P.nk + 3);
   HIPCHK(hipStreamSynchronize(m->stream));
@@ -393,6 +400,7 @@ extern "C" int cm1g_put_field(cm1g_model *m, int id, const float *host) {
 
 extern "C" int cm1g_get_field(cm1g_model *m, int id, float *host) {
   ARGCHK(field_ok(m, id) && host, "cm1g_get_field: bad field id or null pointer");
+  UNSWAP(m);
   const size_t w = (size_t)m->P.ni + 7, rows = (size_t)m->P.py * (size_t)(m->P.nk + 3);
   HIPCHK(hipStreamSynchronize(m->stream));
   HIPCHK(hipMemcpy2D(host, w * sizeof(float), box_origin(m, id), (size_t)m->P.px * sizeof(float),
@@ -414,6 +422,7 @@ static int array_shape(const cm1g_model *m, int stag, size_t *nx, size_t *ny, si
 
 extern "C" int cm1g_put_array(cm1g_model *m, int id, int stag, const float *host) {
   ARGCHK(field_ok(m, id) && host, "cm1g_put_array: bad field id or null pointer");
+  UNSWAP(m);
   m->synced = 0;
   size_t nx, ny, nz;
   ARGCHK(array_shape(m, stag, &nx, &ny, &nz) == 0, "cm1g_put_array: bad staggering");
@@ -427,6 +436,7 @@ extern "C" int cm1g_put_array(cm1g_model *m, int id, int stag, const float *host
 
 extern "C" int cm1g_get_array(cm1g_model *m, int id, int stag, float *host) {
   ARGCHK(field_ok(m, id) && host, "cm1g_get_array: bad field id or null pointer");
+  UNSWAP(m);
   size_t nx, ny, nz;
   ARGCHK(array_shape(m, stag, &nx, &ny, &nz) == 0, "cm1g_get_array: bad staggering");
   HIPCHK(hipStreamSynchronize(m->stream));
@@ -499,9 +509,11 @@ extern "C" int cm1g_get_2d_array(cm1g_model *m, int id, float *host) {
 // ------------------------------------------------------------ checkpoint
 // Arrays a step carries into the next: the time-level "a" arrays and their
 // "3d" copies (halos included), ppx, the previous step's diabatic
-// tendencies, rho/prs/rr/rf, the tracers, and the 2-D fields (rain, prate,
-// and the radiative-boundary speeds of the last stage, which the next
-// step's first acoustic substeps use).
+// tendencies, rho/prs/rr/rf, the tracers, the scalar updates' partner
+// buffer (it rotates into the scalar fields, carrying the ghost cells no
+// kernel writes), and the 2-D fields (rain, prate, and the
+// radiative-boundary speeds of the last stage, which the next step's first
+// acoustic substeps use).
 struct StateHeader {
   char magic[8];
   int32_t ni, nj, nk, px, py, npt, imoist, n3d, n2d, reserved;
@@ -539,12 +551,24 @@ extern "C" int cm1g_state_bytes(cm1g_model *m, int64_t *nbytes) {
   ARGCHK(m && nbytes, "cm1g_state_bytes: null argument");
   int ids[64];
   const int n3 = state_ids(m, ids);
-  *nbytes = (int64_t)(sizeof(StateHeader) + n3 * state_box(m) + CM1G_D_NFIELDS * state_2d(m));
+  *nbytes = (int64_t)(sizeof(StateHeader) + (n3 + 1) * state_box(m) + CM1G_D_NFIELDS * state_2d(m));
+  return 0;
+}
+
+// the logical box of the partner buffer, to or from the host
+static int partner_box(cm1g_model *m, void *host, bool to_host) {
+  const size_t w = (size_t)m->P.ni + 7, rows = (size_t)m->P.py * (size_t)(m->P.nk + 3);
+  float *d = m->thb + (CM1G_IOFF - 2);
+  const size_t dp = (size_t)m->P.px * sizeof(float), hp = w * sizeof(float);
+  HIPCHK(hipStreamSynchronize(m->stream));
+  if (to_host) HIPCHK(hipMemcpy2D(host, hp, d, dp, hp, rows, hipMemcpyDeviceToHost));
+  else HIPCHK(hipMemcpy2D(d, dp, host, hp, hp, rows, hipMemcpyHostToDevice));
   return 0;
 }
 
 extern "C" int cm1g_state_save(cm1g_model *m, void *host, int64_t nbytes) {
   ARGCHK(m && host, "cm1g_state_save: null argument");
+  UNSWAP(m);
   int64_t need = 0;
   cm1g_state_bytes(m, &need);
   ARGCHK(nbytes >= need, "cm1g_state_save: buffer too small");
@@ -552,7 +576,7 @@ extern "C" int cm1g_state_save(cm1g_model *m, void *host, int64_t nbytes) {
   const int n3 = state_ids(m, ids);
   StateHeader h;
   memset(&h, 0, sizeof(h));
-  memcpy(h.magic, "CM1GST01", 8);
+  memcpy(h.magic, "CM1GST02", 8);
   h.ni = m->P.ni; h.nj = m->P.nj; h.nk = m->P.nk; h.px = m->P.px; h.py = m->P.py;
   h.npt = m->c.npt; h.imoist = m->c.imoist; h.n3d = n3; h.n2d = CM1G_D_NFIELDS;
   HIPCHK(hipStreamSynchronize(m->stream));
@@ -565,6 +589,11 @@ extern "C" int cm1g_state_save(cm1g_model *m, void *host, int64_t nbytes) {
     int rc = cm1g_get_field(m, ids[n], (float *)p);
     if (rc) return rc;
   }
+  {
+    int rc = partner_box(m, p, true);
+    if (rc) return rc;
+    p += state_box(m);
+  }
   for (int n = 0; n < CM1G_D_NFIELDS; n++, p += state_2d(m))
     HIPCHK(hipMemcpy(p, m->d2[n], state_2d(m), hipMemcpyDeviceToHost));
   return 0;
@@ -572,6 +601,7 @@ extern "C" int cm1g_state_save(cm1g_model *m, void *host, int64_t nbytes) {
 
 extern "C" int cm1g_state_load(cm1g_model *m, const void *host, int64_t nbytes) {
   ARGCHK(m && host, "cm1g_state_load: null argument");
+  UNSWAP(m);  // the load then writes every field
   m->synced = 0;
   int64_t need = 0;
   cm1g_state_bytes(m, &need);
@@ -580,7 +610,7 @@ extern "C" int cm1g_state_load(cm1g_model *m, const void *host, int64_t nbytes) 
   memcpy(&h, host, sizeof(h));
   int ids[64];
   const int n3 = state_ids(m, ids);
-  ARGCHK(memcmp(h.magic, "CM1GST01", 8) == 0, "cm1g_state_load: not a cm1gpu state");
+  ARGCHK(memcmp(h.magic, "CM1GST02", 8) == 0, "cm1g_state_load: not a cm1gpu state");
   ARGCHK(h.ni == m->P.ni && h.nj == m->P.nj && h.nk == m->P.nk && h.px == m->P.px &&
              h.py == m->P.py && h.npt == m->c.npt && h.imoist == m->c.imoist && h.n3d == n3 &&
              h.n2d == CM1G_D_NFIELDS,
@@ -589,6 +619,11 @@ extern "C" int cm1g_state_load(cm1g_model *m, const void *host, int64_t nbytes) 
   for (int n = 0; n < n3; n++, p += state_box(m)) {
     int rc = cm1g_put_field(m, ids[n], (const float *)p);
     if (rc) return rc;
+  }
+  {
+    int rc = partner_box(m, (void *)p, false);
+    if (rc) return rc;
+    p += state_box(m);
   }
   for (int n = 0; n < CM1G_D_NFIELDS; n++, p += state_2d(m))
     HIPCHK(hipMemcpy(m->d2[n], p, state_2d(m), hipMemcpyHostToDevice));
@@ -600,6 +635,7 @@ extern "C" int cm1g_state_load(cm1g_model *m, const void *host, int64_t nbytes) 
 
 extern "C" int cm1g_field_ptr(cm1g_model *m, int id, float **dptr) {
   ARGCHK(field_ok(m, id) && dptr, "cm1g_field_ptr: bad field id");
+  UNSWAP(m);
   m->synced = 0;
   // writes through the address are invisible to the library: the first
   // acoustic stage stops relying on the a-arrays equalling the 3d-arrays
@@ -738,6 +774,7 @@ void train_accumulate(cm1g_model *m) {
 // ------------------------------------------------------------------ getset
 extern "C" int cm1g_getset(cm1g_model *m, int fresh_start) {
   ARGCHK(m, "cm1g_getset: null model");
+  UNSWAP(m);
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   float **f = m->f;
@@ -857,6 +894,32 @@ __device__ __forceinline__ void massflux_div(const KP &P, const float *u3d, cons
   }
 }
 
+// the (a-array, 3-D array) pairs solve3 hands over
+static std::vector<const int *> swap_pairs(const cm1g_model *m) {
+  static const int base[5][2] = {{CM1G_UA, CM1G_U3D}, {CM1G_VA, CM1G_V3D}, {CM1G_WA, CM1G_W3D},
+                                 {CM1G_PPI, CM1G_PP3D}, {CM1G_THA, CM1G_TH3D}};
+  static const int q[3][2] = {{CM1G_QA0, CM1G_Q3D0}, {CM1G_QA1, CM1G_Q3D1}, {CM1G_QA2, CM1G_Q3D2}};
+  static const int pt[8][2] = {{CM1G_PTA0 + 0, CM1G_PT3D0 + 0}, {CM1G_PTA0 + 1, CM1G_PT3D0 + 1},
+                               {CM1G_PTA0 + 2, CM1G_PT3D0 + 2}, {CM1G_PTA0 + 3, CM1G_PT3D0 + 3},
+                               {CM1G_PTA0 + 4, CM1G_PT3D0 + 4}, {CM1G_PTA0 + 5, CM1G_PT3D0 + 5},
+                               {CM1G_PTA0 + 6, CM1G_PT3D0 + 6}, {CM1G_PTA0 + 7, CM1G_PT3D0 + 7}};
+  std::vector<const int *> v;
+  for (auto &p : base) v.push_back(p);
+  if (m->c.imoist == 1)
+    for (auto &p : q) v.push_back(p);
+  for (int n = 0; n < m->c.npt; n++) v.push_back(pt[n]);
+  return v;
+}
+
+int unswap(cm1g_model *m) {
+  if (!m->swapped) return 0;
+  const size_t bytes = m->ntot * sizeof(float);
+  for (const int *pr : swap_pairs(m))
+    HIPCHK(hipMemcpyAsync(m->f[pr[1]], m->f[pr[0]], bytes, hipMemcpyDeviceToDevice, m->stream));
+  m->swapped = 0;
+  return 0;
+}
+
 extern "C" int cm1g_solve1(cm1g_model *m) {
   ARGCHK(m, "cm1g_solve1: null model");
   const KP P = m->P;
@@ -953,18 +1016,27 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
     float *th3d = f[CM1G_TH3D];
     const float *qv = f[CM1G_Q3D0], *qc = f[CM1G_Q3D1], *qr = f[CM1G_Q3D2];
     const float dttmp = dt / (float)(3 + 1 - nrk);
+    // after an exchanging solve3 the first stage reads the state from the
+    // a-arrays wherever it would read the (equal) 3-D arrays before writing
+    // them; its writes go to the 3-D arrays, which it rewrites completely
+    const bool from_a = (nrk == 1 && m->swapped);
+    const float *th_r = from_a ? f[CM1G_THA] : th3d, *pp_r = from_a ? f[CM1G_PPI] : pp3d;
+    const float *qv_r = from_a ? f[CM1G_QA0] : qv, *qc_r = from_a ? f[CM1G_QA1] : qc,
+                *qr_r = from_a ? f[CM1G_QA2] : qr;
+    const float *u_r = from_a ? f[CM1G_UA] : u3d, *v_r = from_a ? f[CM1G_VA] : v3d,
+                *w_r = from_a ? f[CM1G_WA] : w3d;
     // thermodynamic prep (src/solve2.F:302-450), with sadv = pi0 + pi'
     // (src/solve2.F:1027-1050) over the interior and its 1-wide ring
     for3(m, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
       const Consts &K = P.K;
       int c = IXD(P, i, j, k);
-      sadv[c] = P.pi0[k] + pp3d[c];
+      sadv[c] = P.pi0[k] + pp_r[c];
       if (i < 1 || i > ni || j < 1 || j > nj) return;
-      float th = th3d[c];
+      float th = th_r[c];
       if (moist) {
-        float ql = qc[c];
-        ql = ql + qr[c];
-        float qvv = fmaxf(qv[c], 0.0f);
+        float ql = qc_r[c];
+        ql = ql + qr_r[c];
+        float qvv = fmaxf(qv_r[c], 0.0f);
         float qli = fmaxf(0.0f, ql + 0.0f);
         float cpli = K.cpl * fmaxf(0.0f, ql) + K.cpi * fmaxf(0.0f, 0.0f);
         dum8[c] = K.g * (th * P.rth0[k] + K.repsm1 * (qvv - P.qv0[k]) - (qli - P.qc0[k] - P.qi0[k]));
@@ -972,11 +1044,11 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
         float cpm = K.cp + K.cpv * qvv + cpli;
         float cvm = 1.0f / (K.cv + K.cvv * qvv + cpli);
         thterm[c] = (P.th0[k] + th) * (K.rd + K.rv * qvv - K.rovcp * cpm) * cvm;
-        t22[c] = (P.pi0[k] + pp3d[c]) * K.rovcp * cpm * cvm;
+        t22[c] = (P.pi0[k] + pp_r[c]) * K.rovcp * cpm * cvm;
       } else {
         dum8[c] = K.g * th * P.rth0[k];
         t11[c] = P.th0[k] + th;
-        t22[c] = (P.pi0[k] + pp3d[c]) * K.rddcv;
+        t22[c] = (P.pi0[k] + pp_r[c]) * K.rddcv;
       }
     });
     bc_fill(m, BC_P, t11);
@@ -984,25 +1056,29 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
     {
       const ColGrid cg = col_grid_box(0, ni + 2, 0, nj + 2, m->tile_rows);
       launch<256>(m, col_blocks(cg), col_threads(cg), [=] __device__(int bx, int, int) {
-        massflux_div(P, u3d, v3d, w3d, rru, rrv, rrw, divx, cg, bx);
+        massflux_div(P, u_r, v_r, w_r, rru, rrv, rrw, divx, cg, bx);
       });
     }
     // tendencies: the pre-RK values (src/solve2.F:459-471) plus advection,
     // each component's own mass flux formed from its velocity
-    adv_uvw(m, nrk, 7, 1, 1, 1);
+    {
+      const float *src3[3] = {u_r, v_r, w_r};
+      adv_uvw(m, nrk, 7, 1, 1, 1, from_a ? src3 : nullptr);
+    }
     if (m->c.psolver == 2) sounde_run(m, nrk);
     else sound_run(m, nrk, nrk == 1 && m->synced);
     if (nrk == 3) cfl_run(m, dt, u3d, v3d, w3d);
     if (m->c.irbc == 4 && (P.ibw || P.ibe || P.ibs || P.ibn))
       radbc4(m, f[CM1G_UA], u3d, f[CM1G_VA], v3d, dttmp);
     getdiv_both(m, moist);
-    // theta (src/solve2.F:1310-1428)
-    advs_run(m, nrk, th3d, 1.0f, f[CM1G_THA], f[CM1G_THTEN1], moist ? t11 : nullptr, nullptr,
-             th3d, 0, 0, dttmp, (double)(10.0f * P.K.epsilon), 1);
+    // theta (src/solve2.F:1310-1428); from the a-arrays, the update writes
+    // the 3-D array directly instead of a partner
+    advs_run(m, nrk, from_a ? f[CM1G_THA] : th3d, 1.0f, f[CM1G_THA], f[CM1G_THTEN1], moist ? t11 : nullptr,
+             nullptr, th3d, 0, 0, dttmp, (double)(10.0f * P.K.epsilon), 1);
     if (moist) {
       for (int n = 0; n < 3; n++) {
-        advs_run(m, nrk, f[CM1G_Q3D0 + n], 0.0f, f[CM1G_QA0 + n], f[CM1G_QTEN0 + n], nullptr, nullptr,
-                 f[CM1G_Q3D0 + n], nrk == 3 ? 1 : 0, n == 0 ? 0 : 1, dttmp,
+        advs_run(m, nrk, from_a ? f[CM1G_QA0 + n] : f[CM1G_Q3D0 + n], 0.0f, f[CM1G_QA0 + n], f[CM1G_QTEN0 + n],
+                 nullptr, nullptr, f[CM1G_Q3D0 + n], nrk == 3 ? 1 : 0, n == 0 ? 0 : 1, dttmp,
                  (double)(0.01f * P.K.epsilon), 1);
       }
     }
@@ -1050,11 +1126,12 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
     }
     // passive tracers (src/solve2.F:1902-1970)
     for (int n = 0; n < m->c.npt; n++) {
-      advs_run(m, nrk, f[CM1G_PT3D0 + n], 0.0f, f[CM1G_PTA0 + n], f[CM1G_PTTEN0 + n], nullptr, nullptr,
-               f[CM1G_PT3D0 + n], (nrk == 3 && m->c.pdtra == 1) ? 1 : 0, 1, dttmp,
-               (double)(1.0f * P.K.epsilon), 0);
+      advs_run(m, nrk, from_a ? f[CM1G_PTA0 + n] : f[CM1G_PT3D0 + n], 0.0f, f[CM1G_PTA0 + n],
+               f[CM1G_PTTEN0 + n], nullptr, nullptr, f[CM1G_PT3D0 + n],
+               (nrk == 3 && m->c.pdtra == 1) ? 1 : 0, 1, dttmp, (double)(1.0f * P.K.epsilon), 0);
       if (nrk <= 2) bc_fill(m, BC_S, f[CM1G_PT3D0 + n]);
     }
+    if (from_a) m->swapped = 0;
     // finish the dry-mass adjustment (src/solve2.F:1973-2030)
     if (nrk == 3 && m->c.apmasscon == 1) {
       const double *dscal = m->dscal;
@@ -1085,6 +1162,7 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
 // --------------------------------------------------------------- mp_driver
 extern "C" int cm1g_mp_driver(cm1g_model *m) {
   ARGCHK(m, "cm1g_mp_driver: null model");
+  UNSWAP(m);
   m->synced = 0;
   if (m->c.imoist != 1) return 0;
   if (m->c.ptype == 0) {
@@ -1099,6 +1177,7 @@ extern "C" int cm1g_mp_driver(cm1g_model *m) {
 // ------------------------------------------------------------------ solve3
 extern "C" int cm1g_solve3(cm1g_model *m) {
   ARGCHK(m, "cm1g_solve3: null model");
+  UNSWAP(m);
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   const float dt = m->c.dt;
@@ -1126,22 +1205,30 @@ extern "C" int cm1g_solve3(cm1g_model *m) {
     float *ppi = f[CM1G_PPI], *tha = f[CM1G_THA];
     float *qa0 = f[CM1G_QA0], *qa1 = f[CM1G_QA1], *qa2 = f[CM1G_QA2];
     const float *q0 = f[CM1G_Q3D0], *q1 = f[CM1G_Q3D1], *q2 = f[CM1G_Q3D2];
+    // exchange instead of copying unless a caller holds field addresses
+    const bool copy = m->ext_writers != 0;
     for3(m, 0, ni + 2, 0, nj + 2, 1, nk, [=] __device__(int i, int j, int k) {
       const int c = IXD(P, i, j, k);
-      ua[c] = u3d[c];
-      va[c] = v3d[c];
-      wa[c] = w3d[c];
+      if (copy) {
+        ua[c] = u3d[c];
+        va[c] = v3d[c];
+        wa[c] = w3d[c];
+      }
       if (i > ni + 1 || j > nj + 1) return;
       const float pp = pp3d[c], th = th3d[c];
-      ppi[c] = pp;
-      tha[c] = th;
       float qv = 0.0f, qc = 0.0f;
       if (moist) {
         qv = q0[c];
         qc = q1[c];
-        qa0[c] = qv;
-        qa1[c] = qc;
-        qa2[c] = q2[c];
+      }
+      if (copy) {
+        ppi[c] = pp;
+        tha[c] = th;
+        if (moist) {
+          qa0[c] = qv;
+          qa1[c] = qc;
+          qa2[c] = q2[c];
+        }
       }
       const float rh = rho[c];
       rr[c] = 1.0f / rh;
@@ -1162,12 +1249,20 @@ extern "C" int cm1g_solve3(cm1g_model *m) {
     });
   }
   bc_fill(m, BC_P, ppx);
-  for (int n = 0; n < m->c.npt; n++) {
-    float *pta = f[CM1G_PTA0 + n];
-    const float *pt3d = f[CM1G_PT3D0 + n];
-    for3(m, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
-      pta[IXD(P, i, j, k)] = pt3d[IXD(P, i, j, k)];
-    });
+  if (m->ext_writers) {
+    for (int n = 0; n < m->c.npt; n++) {
+      float *pta = f[CM1G_PTA0 + n];
+      const float *pt3d = f[CM1G_PT3D0 + n];
+      for3(m, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
+        pta[IXD(P, i, j, k)] = pt3d[IXD(P, i, j, k)];
+      });
+    }
+  } else {
+    // the step's end state becomes the a-arrays by exchanging buffers
+    // (src/solve3.F's copies ua = u3d ... pta = pt3d); the next first
+    // stage reads them in place of the 3-D arrays
+    for (const int *pr : swap_pairs(m)) std::swap(f[pr[0]], f[pr[1]]);
+    m->swapped = 1;
   }
   // ua..qa, pta now equal u3d..q3d, pt3d where the next stage reads them
   m->synced = !m->ext_writers;
@@ -1188,12 +1283,14 @@ std::vector<float *> field_map(const cm1g_model *m) {
   std::vector<float *> v(m->f, m->f + CM1G_NFIELDS);
   v.push_back(m->thb);
   v.push_back((float *)(uintptr_t)m->synced);  // decides the first acoustic stage's launches
+  v.push_back((float *)(uintptr_t)m->swapped);  // decides what the first stage reads
   return v;
 }
 void set_field_map(cm1g_model *m, const std::vector<float *> &v) {
   for (int i = 0; i < CM1G_NFIELDS; i++) m->f[i] = v[i];
   m->thb = v[CM1G_NFIELDS];
   m->synced = (int)(uintptr_t)v[CM1G_NFIELDS + 1];
+  m->swapped = (int)(uintptr_t)v[CM1G_NFIELDS + 2];
 }
 
 void graphs_clear(cm1g_model *m) {
@@ -1273,6 +1370,7 @@ extern "C" int cm1g_step(cm1g_model *m, float *cflmax) {
 extern "C" int cm1g_advs(cm1g_model *m, int nrk, int s_id, int s0_id, int sten_id, int pdef,
                          int pdefweno, float dt, double weps) {
   ARGCHK(field_ok(m, s_id) && field_ok(m, s0_id) && field_ok(m, sten_id), "cm1g_advs: bad field id");
+  UNSWAP(m);
   m->synced = 0;
   ARGCHK(nrk >= 1 && nrk <= 3, "cm1g_advs: nrk must be 1..3");
   // the model's rru/rrv/rrw are used as given (no rho0 scaling here)
@@ -1285,21 +1383,25 @@ extern "C" int cm1g_advs(cm1g_model *m, int nrk, int s_id, int s0_id, int sten_i
 }
 extern "C" int cm1g_advu(cm1g_model *m, int nrk) {
   ARGCHK(m && nrk >= 1 && nrk <= 3, "cm1g_advu: bad argument");
+  UNSWAP(m);
   adv_uvw(m, nrk, 1, 0);
   return check_launch();
 }
 extern "C" int cm1g_advv(cm1g_model *m, int nrk) {
   ARGCHK(m && nrk >= 1 && nrk <= 3, "cm1g_advv: bad argument");
+  UNSWAP(m);
   adv_uvw(m, nrk, 2, 0);
   return check_launch();
 }
 extern "C" int cm1g_advw(cm1g_model *m, int nrk) {
   ARGCHK(m && nrk >= 1 && nrk <= 3, "cm1g_advw: bad argument");
+  UNSWAP(m);
   adv_uvw(m, nrk, 4, 0);
   return check_launch();
 }
 extern "C" int cm1g_sound(cm1g_model *m, int nrk, float dttmp) {
   ARGCHK(m && nrk >= 1 && nrk <= 3, "cm1g_sound: bad argument");
+  UNSWAP(m);
   m->synced = 0;
   (void)dttmp;
   if (m->c.psolver == 2) sounde_run(m, nrk);
@@ -1308,6 +1410,7 @@ extern "C" int cm1g_sound(cm1g_model *m, int nrk, float dttmp) {
 }
 extern "C" int cm1g_kessler_branch(cm1g_model *m) {
   ARGCHK(m, "null model");
+  UNSWAP(m);
   m->synced = 0;
   kessler_branch_run(m);
   train_accumulate(m);
@@ -1315,17 +1418,20 @@ extern "C" int cm1g_kessler_branch(cm1g_model *m) {
 }
 extern "C" int cm1g_pdefq(cm1g_model *m, float rmax, int q_id) {
   ARGCHK(field_ok(m, q_id), "cm1g_pdefq: bad field id");
+  UNSWAP(m);
   pdefq_run(m, rmax, m->f[CM1G_RHO], m->f[q_id]);
   return check_launch();
 }
 extern "C" int cm1g_bc(cm1g_model *m, int id, int kind) {
   ARGCHK(field_ok(m, id) && kind >= 0 && kind <= 5, "cm1g_bc: bad argument");
+  UNSWAP(m);
   m->synced = 0;
   bc_fill(m, kind, m->f[id]);
   return check_launch();
 }
 extern "C" int cm1g_calccflquick(cm1g_model *m, float dt, int u_id, int v_id, int w_id, float *cflmax) {
   ARGCHK(field_ok(m, u_id) && field_ok(m, v_id) && field_ok(m, w_id) && cflmax, "cm1g_calccflquick: bad argument");
+  UNSWAP(m);
   cfl_run(m, dt, m->f[u_id], m->f[v_id], m->f[w_id]);
   HIPCHK(hipMemcpyAsync(cflmax, m->fred, sizeof(float), hipMemcpyDeviceToHost, m->stream));
   HIPCHK(hipStreamSynchronize(m->stream));
@@ -1334,11 +1440,13 @@ extern "C" int cm1g_calccflquick(cm1g_model *m, float dt, int u_id, int v_id, in
 extern "C" int cm1g_getdiv(cm1g_model *m, int div_id) {
   ARGCHK(field_ok(m, div_id) && div_id != CM1G_RRU && div_id != CM1G_RRV && div_id != CM1G_RRW,
          "cm1g_getdiv: bad field id");
+  UNSWAP(m);
   getdiv_both(m, 1, 0, m->f[div_id]);
   return check_launch();
 }
 extern "C" int cm1g_getdivx(cm1g_model *m) {
   ARGCHK(m, "null model");
+  UNSWAP(m);
   getdiv_both(m, 0, 1);
   scale_mass_fluxes(m);
   return check_launch();

@@ -507,8 +507,10 @@ void sounde_run(cm1g_model *m, int nrk) {
   const float *dtu0 = m->dtu0, *dtv0 = m->dtv0;
   const float rdx = P.rdx, rdy = P.rdy, rdz = P.rdz, cp = P.K.cp, smeps = P.K.smeps;
 
-  // prep copies (src/sounde.F:181-233)
-  if (nrk == 1) {
+  // prep copies (src/sounde.F:181-233); the first stage skips u3d = ua ...
+  // because they are equal, except after an exchanging solve3, when the
+  // 3-D arrays still hold the previous step
+  if (nrk == 1 && !m->swapped) {
     for3(m, 0, ni + 1, 0, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
       int c = IXD(P, i, j, k);
       ppd[c] = ppx[c];

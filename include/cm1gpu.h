@@ -147,11 +147,13 @@ int cm1g_get_2d_array(cm1g_model *m, int id, float *host);
  * holds device memory, e.g. a Fortran host with its own device mirrors).
  * The address is valid until the next call that runs kernels on the handle:
  * the scalar updates exchange th3d, q3d and pt3d with partner buffers, so
- * query it again after each step or routine call.  Once the address of
+ * query it again after each step or routine call (a step also ends by
+ * exchanging the buffers of ua..qa, pta with those of u3d..q3d, pt3d instead
+ * of copying them, src/solve3.F's ua = u3d ...).  Once the address of
  * ua, va, wa, ppi, u3d, v3d, w3d or pp3d has been handed out, the caller may
- * write those arrays, and every later step reads u3d..pp3d as the reference
- * does instead of assuming they equal ua..ppi after solve3 (correct for any
- * writes, at the cost of the first stage's copies). */
+ * write those arrays, and every later step copies instead of exchanging and
+ * reads u3d..pp3d as the reference does instead of assuming they equal
+ * ua..ppi after solve3 (correct for any writes, at the cost of the copies). */
 int cm1g_field_ptr(cm1g_model *m, int id, float **dptr);
 /* Wait for all queued work of this handle; CM1G_E_SATADJ if a saturation
  * adjustment of any step so far failed to converge (src/kessler.F:343-356). */
@@ -355,8 +357,9 @@ int cm1g_budgets(cm1g_model *m, double *tmass, double *tmois);
  * the state into a host buffer of that size; cm1g_state_load restores it
  * into a model created with the same configuration (checked), after which
  * stepping continues bit for bit as if uninterrupted.  Layout: an 88-byte
- * header, then 3-D fields as logical boxes (cm1g_put_field order), then the
- * 2-D fields in device order. */
+ * header, then 3-D fields as logical boxes (cm1g_put_field order), the box
+ * of the scalar updates' partner buffer, then the 2-D fields in device
+ * order. */
 int cm1g_state_bytes(cm1g_model *m, int64_t *nbytes);
 int cm1g_state_save(cm1g_model *m, void *host, int64_t nbytes);
 int cm1g_state_load(cm1g_model *m, const void *host, int64_t nbytes);
