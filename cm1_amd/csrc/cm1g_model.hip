@@ -1088,6 +1088,9 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
       float *qvw = f[CM1G_Q3D0], *qcw = f[CM1G_Q3D1];
       float *qpten = f[CM1G_QPTEN], *qtten = f[CM1G_QTTEN], *qvten = f[CM1G_QVTEN], *qcten = f[CM1G_QCTEN];
       const int diab = (moist && m->c.ptype >= 1);
+      // with the dry-mass adjustment the pass below recomputes prs from the
+      // shifted pi', so only rho (for the mass sums) is stored here
+      const int keep_prs = (m->c.apmasscon != 1);
       for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
         const Consts &K = P.K;
         int c = IXD(P, i, j, k);
@@ -1107,7 +1110,7 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
           th3d[c] = th;
         }
         float p = K.p00 * pow_f(P.pi0[k] + pp, K.cpdrd);
-        prs[c] = p;
+        if (keep_prs) prs[c] = p;
         if (moist)
           rho[c] = p / ((P.th0[k] + th) * (P.pi0[k] + pp) * (K.rd + fmaxf(0.0f, qvw[c]) * K.rv));
         else
