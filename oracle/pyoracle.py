@@ -29,11 +29,16 @@ for _n in range(8):
     FID[f"pta{_n}"] = F_PTA0 + _n
     FID[f"pt3d{_n}"] = F_PTA0 + 8 + _n
     FID[f"ptten{_n}"] = F_PTA0 + 16 + _n
+# LES subgrid turbulence (after the tracers)
+for _i, _n in enumerate(["tkea", "tke3d", "tketen", "kmh", "kmv", "khh", "khv", "nm", "defv",
+                          "defh", "dissten", "lenscl"]):
+    FID[_n] = F_PTA0 + 24 + _i
 KFIELDS = ["mh", "rmh", "mf", "rmf", "c1", "c2", "zh", "zf", "pi0", "th0", "prs0",
            "qv0", "rho0", "rr0", "rf0", "rrf0", "thv0", "rth0", "qc0", "qi0", "u0", "v0",
            "tauh", "tauf"]
 KID = {n: i for i, n in enumerate(KFIELDS)}
-D2 = {"rain": 0, "prate": 1, "radbcw": 2, "radbce": 3, "radbcs": 4, "radbcn": 5}
+D2 = {"rain": 0, "prate": 1, "radbcw": 2, "radbce": 3, "radbcs": 4, "radbcn": 5,
+      "ust": 6, "u1": 7, "v1": 8, "s1": 9, "thflux": 10, "znt": 11}
 
 
 _INTS = ("ni", "nj", "nk", "wbc", "ebc", "sbc", "nbc", "imoist", "ptype", "psolver",
@@ -41,7 +46,9 @@ _INTS = ("ni", "nj", "nk", "wbc", "ebc", "sbc", "nbc", "imoist", "ptype", "psolv
 _REALS = ("dx", "dy", "dz", "dt", "alph", "kdiv", "csmax")
 
 
-_TAIL = (("irdamp", "i"), ("rdalpha", "r"), ("zd", "r"))
+_TAIL = (("irdamp", "i"), ("rdalpha", "r"), ("zd", "r"),
+         ("cm1setup", "i"), ("doimpl", "i"), ("bbc", "i"), ("isfcflx", "i"),
+         ("cnst_shflx", "r"), ("cnst_znt", "r"))
 
 
 class Config(C.Structure):
@@ -74,7 +81,7 @@ def _load(fp64=False):
     lib.cm1o_base_derive.argtypes = [vp]
     lib.cm1o_getset.argtypes = [vp, C.c_int]
     lib.cm1o_step.argtypes = [vp]
-    for name in ("cm1o_solve1", "cm1o_solve2", "cm1o_mp_driver", "cm1o_solve3"):
+    for name in ("cm1o_solve1", "cm1o_solve2", "cm1o_mp_driver", "cm1o_solve3", "cm1o_sfc_and_turb"):
         getattr(lib, name).argtypes = [vp]
     lib.cm1o_bc.argtypes = [vp, C.c_int, C.c_int]
     lib.cm1o_advs.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]

@@ -22,6 +22,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_EXE = os.path.join(HERE, "_ref", "cm1_sp.exe")
+# the reference's run directory (its data tables), read by fixture scripts only
+REF_RUN = "/root/reference/run"
 
 
 def _fmt(v):
@@ -145,13 +147,17 @@ def namelist_text(overrides=None):
 
 
 def run_reference(workdir, overrides=None, threads=None, exe=REF_EXE, timeout=3600,
-                  extra_env=None):
-    """Run the reference in ``workdir``; returns (stdout text, wall seconds)."""
+                  extra_env=None, tables=()):
+    """Run the reference in ``workdir``; returns (stdout text, wall seconds).
+    ``tables``: data files of the reference's run directory the case reads
+    (e.g. LANDUSE.TBL for a surface model), copied into ``workdir``."""
     if not os.path.exists(exe):
         raise FileNotFoundError(exe)
     if os.path.exists(workdir):
         shutil.rmtree(workdir)
     os.makedirs(workdir)
+    for t in tables:
+        shutil.copy(os.path.join(REF_RUN, t), os.path.join(workdir, t))
     with open(os.path.join(workdir, "namelist.input"), "w") as f:
         f.write(namelist_text(overrides))
     env = dict(os.environ)

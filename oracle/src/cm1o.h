@@ -45,6 +45,11 @@ typedef struct cm1o_config {
   float csmax;                    /* 350 (src/constants.F) */
   int32_t irdamp;                 /* 1 = Rayleigh damping above zd (src/solve1.F:414-426) */
   float rdalpha, zd;              /* damping rate (1/s) and height (m) */
+  /* LES subgrid turbulence (src/turb.F, src/turbtend.F): cm1setup 0 none,
+   * 1 LES with sgsmodel 1 (TKE), tconfig 1, bcturbs 1, horizturb 0; doimpl
+   * 0/1; bbc 3 with sfcmodel 1, set_znt 1, set_flx 1; tbc 1 */
+  int32_t cm1setup, doimpl, bbc, isfcflx;
+  float cnst_shflx, cnst_znt;
 } cm1o_config;
 
 /* Field identifiers for cm1o_field(). */
@@ -59,7 +64,9 @@ enum {
   F_T11, F_T12, F_T13, F_T22, F_T23, F_T33,
   F_QA0, F_QA1, F_QA2, F_Q3D0, F_Q3D1, F_Q3D2, F_QTEN0, F_QTEN1, F_QTEN2,
   F_PTA0, F_PT3D0 = F_PTA0 + 8, F_PTTEN0 = F_PT3D0 + 8,
-  F_NFIELDS = F_PTTEN0 + 8
+  F_TKEA = F_PTTEN0 + 8, F_TKE3D, F_TKETEN, F_KMH, F_KMV, F_KHH, F_KHV, F_NM, F_DEFV, F_DEFH,
+  F_DISSTEN, F_LENSCL,
+  F_NFIELDS
 };
 
 /* 1-D (k) metric/base-state identifiers for cm1o_kfield(). */
@@ -70,7 +77,8 @@ enum {
 };
 
 /* 2-D fields */
-enum { D_RAIN, D_PRATE, D_RADBCW, D_RADBCE, D_RADBCS, D_RADBCN, D_N2D };
+enum { D_RAIN, D_PRATE, D_RADBCW, D_RADBCE, D_RADBCS, D_RADBCN,
+       D_UST, D_U1, D_V1, D_S1, D_THFLUX, D_ZNT, D_N2D };
 
 typedef struct cm1o_model cm1o_model;
 
@@ -97,6 +105,9 @@ void cm1o_solve1(cm1o_model *m);
 void cm1o_solve2(cm1o_model *m);
 void cm1o_mp_driver(cm1o_model *m);
 void cm1o_solve3(cm1o_model *m);
+/* LES subgrid turbulence set-up of a step (src/turb.F:12, sfc_and_turb):
+ * surface layer, N^2, deformation, eddy viscosities, stresses */
+void cm1o_sfc_and_turb(cm1o_model *m);
 
 /* Individual routines on the model's own arrays (for unit parity tests). */
 void cm1o_bc(cm1o_model *m, int id, int kind);  /* kind 0 s,1 p,2 u,3 v,4 w(flag1),5 w(flag0) */

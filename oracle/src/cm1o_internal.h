@@ -22,6 +22,7 @@ static inline double o64_pow(double x, double y) { return pow(x, y); }
 static inline double o64_copysign(double x, double y) { return copysign(x, y); }
 static inline long o64_lround(double x) { return lround(x); }
 static inline double o64_cos(double x) { return cos(x); }
+static inline double o64_log(double x) { return log(x); }
 #define float double
 #define fabsf o64_fabs
 #define fmaxf o64_fmax
@@ -32,6 +33,7 @@ static inline double o64_cos(double x) { return cos(x); }
 #define copysignf o64_copysign
 #define lroundf o64_lround
 #define cosf o64_cos
+#define logf o64_log
 #endif
 
 #include "cm1o.h"
@@ -115,6 +117,8 @@ void o_advv(cm1o_model *M, int nrk, const float *rru, const float *rrv, const fl
             float *vten, const float *rrw, const float *divx);
 void o_advw(cm1o_model *M, int nrk, const float *rru, const float *rrv, const float *rrw,
             const float *w3d, float *wten, const float *divx);
+void o_advw_w(cm1o_model *M, int nrk, int doweno, const float *rru, const float *rrv, const float *rrw,
+              const float *w3d, float *wten, const float *divx);
 
 /* sound.F */
 void o_sound(cm1o_model *M, int nrk, float dttmp);
@@ -133,6 +137,13 @@ void o_kessler(cm1o_model *M, float dt, float *tmp, float *rho, float *pp3d, flo
                float *prs, float *qv3d, float *qc3d, float *qr3d);
 int  o_satadj(cm1o_model *M, int nrk, float dt, float *rho, float *pp3d, float *prs,
               float *th3d, float *qv, float *qc, float *qr);
+
+/* turb.F, turbtend.F, sfcphys.F (LES subgrid turbulence) */
+void o_bc2d(cm1o_model *M, float *s);   /* k = 1 slice or a 2-D array, periodic */
+void o_sfc_and_turb(cm1o_model *M);
+void o_turb_tend(cm1o_model *M);
+void o_tke_step(cm1o_model *M, int nrk, float dttmp);
+void o_tke_copy(cm1o_model *M);
 
 /* solve*.F */
 void o_solve1(cm1o_model *M);

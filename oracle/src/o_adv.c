@@ -558,6 +558,14 @@ void o_advv(cm1o_model *M, int nrk, const float *rru, const float *rrv, const fl
 /* ---------------------------------------------------------------- advw */
 void o_advw(cm1o_model *M, int nrk, const float *rru, const float *rrv, const float *rrw,
             const float *w3d, float *wten, const float *divx) {
+  const int doweno = (M->c.advwenov == 1) || (M->c.advwenov == 2 && nrk == 3);
+  o_advw_w(M, nrk, doweno, rru, rrv, rrw, w3d, wten, divx);
+}
+
+/* advw with the WENO switch given: the velocity's (advwenov) for w, the
+ * scalars' (advwenos) for the TKE (src/solve2.F:1746-1749) */
+void o_advw_w(cm1o_model *M, int nrk, int doweno, const float *rru, const float *rrv, const float *rrw,
+              const float *w3d, float *wten, const float *divx) {
   const int ni = M->ni, nj = M->nj, nk = M->nk;
   const float rdx = M->rdx, rdy = M->rdy, rdz = M->rdz;
   const float *mf = M->kf[K_MF], *rrf0 = M->kf[K_RRF0], *c1 = M->kf[K_C1], *c2 = M->kf[K_C2];
@@ -565,7 +573,6 @@ void o_advw(cm1o_model *M, int nrk, const float *rru, const float *rrv, const fl
   int i, j, k;
   int doitw = (M->c.wbc == 2 && M->ibw == 1), doite = (M->c.ebc == 2 && M->ibe == 1);
   int doits = (M->c.sbc == 2 && M->ibs == 1), doitn = (M->c.nbc == 2 && M->ibn == 1);
-  int doweno = (M->c.advwenov == 1) || (M->c.advwenov == 2 && nrk == 3);
   double weps = (double)(100.0f * M->k.epsilon);
   if (doweno) o_hadv_weno5(M, 4, rru, rrv, dumx, dumy, w3d, 0, weps);
   else hadv_flx5(M, 4, rru, rrv, dumx, dumy, w3d);

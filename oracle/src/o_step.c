@@ -61,6 +61,9 @@ cm1o_model *cm1o_create(const cm1o_config *cfg) {
   M->s7 = (float *)calloc(M->ntot, sizeof(float)); M->s8 = (float *)calloc(M->ntot, sizeof(float));
   M->flag = (unsigned char *)calloc(M->ntot, 1);
   M->qmag[0] = M->qmag[1] = M->qmag[2] = 0.01f;
+  /* roughness length, specified (src/init_surface.F:251) */
+  if (M->c.cm1setup == 1)
+    for (i = 0; i < M->pxy; i++) M->d2[D_ZNT][i] = fmaxf(1.0e-12f, cfg->cnst_znt);
   /* horizontal metrics of an unstretched grid: uh=dx/(xf(i+1)-xf(i)) = 1 */
   for (i = 0; i < M->px; i++) { M->h[0][i] = 1.0f; M->h[1][i] = 1.0f; M->h[2][i] = 1.0f; }
   for (j = 0; j < M->py; j++) { M->h[3][j] = 1.0f; M->h[4][j] = 1.0f; M->h[5][j] = 1.0f; }
@@ -237,6 +240,16 @@ void cm1o_getset(cm1o_model *M, int fresh_start) {
     for (n = 0; n < 3; n++) M->f[F_Q3D0 + n][t] = M->f[F_QA0 + n][t];
     for (n = 0; n < M->c.npt; n++) M->f[F_PT3D0 + n][t] = M->f[F_PTA0 + n][t];
   }
+  if (M->c.cm1setup == 1) {
+    /* subgrid TKE (src/init3d.F:2160, 2250-2266, 2322-2331): w-level
+     * boundaries, tkea(1) = tkea(2), tke3d = tkea */
+    float *tkea = M->f[F_TKEA];
+    o_bcw(M, tkea, 1);
+    for (j = 1; j <= nj; j++)
+      for (i = 1; i <= ni; i++) A3(tkea, i, j, 1) = A3(tkea, i, j, 2);
+    o_bc2d(M, tkea + M->pxy);
+    for (t = 0; t < (size_t)M->ntot; t++) M->f[F_TKE3D][t] = tkea[t];
+  }
   if (fresh_start) {
     for (k = 1; k <= nk; k++)
       for (j = 1; j <= nj; j++)
@@ -320,6 +333,8 @@ void o_solve1(cm1o_model *M) {
     for (k = 1; k <= nk; k++)
       for (j = 1; j <= nj; j++)
         for (i = 1; i <= ni; i++) A3(M->f[F_PTTEN0 + n], i, j, k) = 0.0f;
+  /* subgrid turbulence tendencies (src/solve1.F:315-380, 1118-1240) */
+  if (M->c.cm1setup == 1) o_turb_tend(M);
   {
     const float rdt = 1.0f / dt;
     const float tem = 0.0001f * K->tsmall;
@@ -514,6 +529,9 @@ void o_solve2(cm1o_model *M) {
       }
     }
 
+    /* TKE advection and step (src/solve2.F:1709-1800) */
+    if (M->c.cm1setup == 1) o_tke_step(M, nrk, dttmp);
+
     /* pressure and density */
     if (M->c.imoist == 1) {
       if (nrk == 3 && M->c.ptype >= 1) {
@@ -649,6 +667,7 @@ void o_solve3(cm1o_model *M) {
     for (k = 1; k <= nk; k++)
       for (j = 0; j <= nj + 1; j++)
         for (i = 0; i <= ni + 1; i++) A3(M->f[F_PTA0 + n], i, j, k) = A3(M->f[F_PT3D0 + n], i, j, k);
+  if (M->c.cm1setup == 1) o_tke_copy(M);
   for (k = 1; k <= nk; k++)
     for (j = 0; j <= nj + 1; j++)
       for (i = 0; i <= ni + 1; i++) {
@@ -675,6 +694,7 @@ void o_solve3(cm1o_model *M) {
 }
 
 void cm1o_step(cm1o_model *M) {
+  if (M->c.cm1setup == 1) o_sfc_and_turb(M);
   o_solve1(M);
   o_solve2(M);
   if (M->c.imoist == 1) o_mp_driver(M);
@@ -683,6 +703,7 @@ void cm1o_step(cm1o_model *M) {
 
 /* ----------------------------------------------------- unit entry points */
 void cm1o_solve1(cm1o_model *M) { o_solve1(M); }
+void cm1o_sfc_and_turb(cm1o_model *M) { o_sfc_and_turb(M); }
 void cm1o_solve2(cm1o_model *M) { o_solve2(M); }
 void cm1o_mp_driver(cm1o_model *M) { if (M->c.imoist == 1) o_mp_driver(M); }
 void cm1o_solve3(cm1o_model *M) { o_solve3(M); }

@@ -144,3 +144,54 @@ def nsound_for(cfgd, zf=None):
     if cfgd["dt"] / float(np.float32(ns)) > float(dtsm):
         ns += 2
     return ns
+
+
+# LES subgrid-turbulence switches of the device path (the keys the oracle
+# and cm1g_set_turbulence take), for the reference's dry CBL case
+TURB_KEYS = ("cm1setup", "doimpl", "bbc", "isfcflx", "cnst_shflx", "cnst_znt")
+
+
+def load_turb_fixture(path):
+    """Reference t=0 state of an LES fixture (make_ref_fixture.make_turb) ->
+    (cfgd, cols, fields, nsteps, nout, z); cfgd carries the turbulence
+    switches (TURB_KEYS) besides the dynamical-core configuration."""
+    z = np.load(path)
+    ni, nj, nk, nsteps, bc, nout = [int(v) for v in z["meta"]]
+    dt = float(z["dt"][0])
+    psolver, advwenos, _, _, npt, doimpl = [int(v) for v in z["opts"]]
+    dx, dy, dz = [float(v) for v in z["grid"]]
+    shflx, znt = [float(v) for v in z["surface"]]
+    cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=bc, ebc=bc, sbc=bc, nbc=bc, imoist=0, ptype=0,
+                psolver=psolver, advwenos=advwenos, advwenov=0, apmasscon=1, irbc=4,
+                nsound=0, npt=npt, pdtra=1, dx=dx, dy=dy, dz=dz, dt=dt, alph=0.6,
+                kdiv=0.1, csmax=350.0, irdamp=0, rdalpha=3.3333333333e-3, zd=15000.0,
+                cm1setup=1, doimpl=doimpl, bbc=3, isfcflx=1, cnst_shflx=shflx, cnst_znt=znt)
+    cols, _ = hinit.vertical_grid(nk, dz, None)
+    for n in ("pi0", "th0", "qv0"):
+        c = np.zeros(nk + 3, np.float32)
+        c[1:nk + 1] = z[n]
+        cols[n] = c
+    for n, key in (("u0", "t0_u"), ("v0", "t0_v")):
+        c = np.zeros(nk + 3, np.float32)
+        c[1:nk + 1] = z[key][:, 0, 0]
+        cols[n] = c
+    f = {n: hinit.blank(ni, nj, nk) for n in ("ua", "va", "wa", "tha", "ppi", "tkea")}
+    f["ua"][1:nk + 1, 3:nj + 3, 3:ni + 4] = z["t0_u"]
+    f["va"][1:nk + 1, 3:nj + 4, 3:ni + 3] = z["t0_v"]
+    f["wa"][1:nk + 2, 3:nj + 3, 3:ni + 3] = z["t0_w"]
+    f["tha"][1:nk + 1, 3:nj + 3, 3:ni + 3] = z["t0_th"]
+    f["ppi"][1:nk + 1, 3:nj + 3, 3:ni + 3] = z["t0_pi"]
+    f["tkea"][1:nk + 2, 3:nj + 3, 3:ni + 3] = z["t0_tke"]
+    for n in range(npt):
+        f[f"pta{n}"] = hinit.blank(ni, nj, nk)
+        f[f"pta{n}"][1:nk + 1, 3:nj + 3, 3:ni + 3] = z[f"t0_pt{n + 1}"]
+    cfgd["nsound"] = nsound_for(cfgd)
+    return cfgd, cols, f, nsteps, nout, z
+
+
+# fixture key -> (model field, staggering) of the turbulence outputs
+TURB_OUT = {"u": ("ua", "u"), "v": ("va", "v"), "w": ("wa", "w"), "th": ("tha", None),
+            "pi": ("ppi", None), "tke": ("tkea", "w"), "rho": ("rho", None), "prs": ("prs", None),
+            "kmh": ("kmh", "w"), "kmv": ("kmv", "w"), "khh": ("khh", "w"), "khv": ("khv", "w"),
+            "lenscl": ("lenscl", "w"), "nm": ("nm", "w"), "defv": ("defv", "w"),
+            "defh": ("defh", "w"), "dissten": ("dissten", "w")}

@@ -93,8 +93,69 @@ def make(name, ni, nj, nk, nsteps, dt, bc, workdir="/tmp/cm1_fixture", opts=None
     return path
 
 
+# the reference's dry convective boundary layer switches (run/config_files/
+# les_ConvBoundLayer/namelist.input) as far as the device path carries
+# them: TKE subgrid model, semi-slip surface with the neutral log-law
+# friction velocity over a specified roughness, specified surface heat
+# flux, the mixed-layer sounding with uniform 10 m/s wind and +-0.25 K
+# random theta noise
+TURB_NAMELIST = dict(cm1setup=1, sgsmodel=1, tconfig=1, bcturbs=1, horizturb=0, ramp_sgs=0,
+                     bbc=3, tbc=1, isfcflx=1, sfcmodel=1, set_flx=1, cnst_shflx=0.24,
+                     set_znt=1, cnst_znt=0.1, lu0=3, xland0=1.0, imoist=0, ptype=0,
+                     isnd=14, iwnd=6, iinit=0, irandp=1)
+TURB_OUT = ("tke", "km", "kh", "nm", "def", "dissten")
+
+
+def make_turb(name, ni, nj, nk, nsteps, nout, dt, doimpl, dx=20.0, dz=20.0, npt=2, psolver=2,
+              advwenos=1, workdir="/tmp/cm1_fixture_turb", outdir=HERE, threads=4):
+    """Dry LES with the TKE scheme: the reference from t=0 over nsteps large
+    steps with output every nsteps/nout steps.  Stores the t=0 state and,
+    at every output time, u, v, w, theta', pi', tke (``o{n}_*``); at the
+    last time also rho, prs, the tracers and the turbulence diagnostics
+    (km, kh, lenscl, nm, defv, defh, dissten)."""
+    ov = dict(nx=ni, ny=nj, nz=nk, dtl=dt, timax=nsteps * dt, tapfrq=nsteps * dt / nout,
+              wbc=1, ebc=1, sbc=1, nbc=1, dx=dx, dy=dx, dz=dz, psolver=psolver,
+              advwenos=advwenos, advwenov=0, irdamp=0, doimpl=doimpl, iptra=1 if npt else 0,
+              npt=max(npt, 1), **TURB_NAMELIST)
+    ov.update({"output_" + k: 1 for k in TURB_OUT})
+    refrun.run_reference(workdir, ov, threads=threads, tables=("LANDUSE.TBL",))
+    S = refrun.read_grads(workdir, "cm1out_s.ctl")
+    U = refrun.read_grads(workdir, "cm1out_u.ctl")
+    V = refrun.read_grads(workdir, "cm1out_v.ctl")
+    W = refrun.read_grads(workdir, "cm1out_w.ctl")
+    out = dict(meta=np.array([ni, nj, nk, nsteps, 1, nout], np.int32), dt=np.array([dt], np.float32),
+               opts=np.array([psolver, advwenos, 0, 0, npt, doimpl], np.int32),
+               grid=np.array([dx, dx, dz], np.float32),
+               surface=np.array([TURB_NAMELIST["cnst_shflx"], TURB_NAMELIST["cnst_znt"]], np.float32))
+    for t in range(nout + 1):
+        tag = "t0" if t == 0 else f"o{t}"
+        out[f"{tag}_u"] = U["u"][t]
+        out[f"{tag}_v"] = V["v"][t]
+        out[f"{tag}_w"] = W["w"][t]
+        out[f"{tag}_th"] = S["thpert"][t]
+        out[f"{tag}_pi"] = S["pipert"][t]
+        out[f"{tag}_tke"] = W["tke"][t]
+    for n in range(1, npt + 1):
+        out[f"t0_pt{n}"] = S[f"pt{n}"][0]
+        out[f"o{nout}_pt{n}"] = S[f"pt{n}"][nout]
+    for k in ("rho", "prs"):
+        out[f"o{nout}_{k}"] = S[k][nout]
+    for k in ("kmh", "kmv", "khh", "khv", "lenscl", "nm", "defv", "defh", "dissten"):
+        out[f"o{nout}_{k}"] = W[k][nout]
+    for n in ("pi0", "th0", "qv0", "prs0"):
+        out[n] = S[n][0, :, 0, 0]
+    out["zh"] = S["zhval"][0, :, 0, 0]
+    path = os.path.join(outdir, f"{name}.npz")
+    np.savez_compressed(path, **{k: np.ascontiguousarray(v) for k, v in out.items()})
+    return path
+
+
 if __name__ == "__main__":
-    if len(sys.argv) > 1 and sys.argv[1] == "stretched":
+    if len(sys.argv) > 1 and sys.argv[1] == "turb":
+        # 400 steps of 0.25 s: the subgrid TKE appears after about 300
+        for doimpl in (1, 0):
+            print(make_turb(f"ref_les_tke_impl{doimpl}_16x12x24_400", 16, 12, 24, 400, 4, 0.25, doimpl))
+    elif len(sys.argv) > 1 and sys.argv[1] == "stretched":
         # dt = 1.5 s: the reference's dtsmall picks nsound = 4; dt = 3 s: nsound = 8
         for dt, tag in ((1.5, "dt1.5"), (3.0, "dt3")):
             print(make(f"ref_stretch_{tag}_24x20x96_10", 24, 20, 96, 10, dt, 2,
