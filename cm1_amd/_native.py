@@ -25,11 +25,18 @@ for _n in range(8):
     FID[f"pta{_n}"] = _PTA0 + _n
     FID[f"pt3d{_n}"] = _PTA0 + 8 + _n
     FID[f"ptten{_n}"] = _PTA0 + 16 + _n
+# LES subgrid turbulence (after the tracers; allocated by cm1g_set_turbulence)
+TURB_FIELDS = ["tkea", "tke3d", "tketen", "kmh", "kmv", "khh", "khv", "nm", "defv", "defh",
+               "dissten", "lenscl"]
+for _i, _n in enumerate(TURB_FIELDS):
+    FID[_n] = _PTA0 + 24 + _i
+NFIELDS = _PTA0 + 24 + len(TURB_FIELDS)
 KFIELDS = ["mh", "rmh", "mf", "rmf", "c1", "c2", "zh", "zf", "pi0", "th0", "prs0",
            "qv0", "rho0", "rr0", "rf0", "rrf0", "thv0", "rth0", "qc0", "qi0", "u0", "v0",
            "tauh", "tauf"]
 KID = {n: i for i, n in enumerate(KFIELDS)}
-D2 = {"rain": 0, "prate": 1, "radbcw": 2, "radbce": 3, "radbcs": 4, "radbcn": 5}
+D2 = {"rain": 0, "prate": 1, "radbcw": 2, "radbce": 3, "radbcs": 4, "radbcn": 5,
+      "ust": 6, "u1": 7, "v1": 8, "s1": 9, "thflux": 10, "znt": 11}
 KT_SOUND_SUBSTEP = 1
 BC = {"s": 0, "p": 1, "u": 2, "v": 3, "w": 4, "w0": 5}
 
@@ -57,6 +64,12 @@ class Config(C.Structure):
         [("irdamp", C.c_int32), ("rdalpha", C.c_float), ("zd", C.c_float)]
 
 
+class TurbConfig(C.Structure):
+    """cm1g_turb_config: the LES subgrid-turbulence switches."""
+    _fields_ = [(n, C.c_int32) for n in ("cm1setup", "doimpl", "bbc", "isfcflx")] + \
+        [(n, C.c_float) for n in ("cnst_shflx", "cnst_znt")]
+
+
 class CM1GError(RuntimeError):
     pass
 
@@ -82,6 +95,8 @@ SIGNATURES = {
     "cm1g_field_ptr": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "cm1g_sync": (C.c_int, [C.c_void_p]),
     "cm1g_getset": (C.c_int, [C.c_void_p, C.c_int]),
+    "cm1g_set_turbulence": (C.c_int, [C.c_void_p, C.POINTER(TurbConfig)]),
+    "cm1g_sfc_and_turb": (C.c_int, [C.c_void_p]),
     "cm1g_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "cm1g_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "cm1g_set_tiles": (C.c_int, [C.c_void_p, C.c_int]),

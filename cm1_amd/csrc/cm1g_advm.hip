@@ -353,3 +353,27 @@ void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own
     kern(m, P, g, cg);
   }
 }
+
+void advw_tke(cm1g_model *m, int nrk, const float *rru, const float *rrv, const float *rrw, const float *a,
+              const float *ten_in, float *ten) {
+  const KP P = m->P;
+  const bool weno = (m->c.advwenos == 1) || (m->c.advwenos == 2 && nrk == 3);
+  AdvmArgs g;
+  g.rru = rru;
+  g.rrv = rrv;
+  g.rrw = rrw;
+  g.divx = m->f[CM1G_DIVX];
+  g.dum8 = m->f[CM1G_DUM8];
+  g.weps = (double)(100.0f * P.K.epsilon);
+  g.buoy = 0;
+  g.a = a;
+  g.ten_in = ten_in;
+  g.ten = ten;
+  g.i1 = 1;
+  g.i2 = P.ni;
+  g.j1 = 1;
+  g.j2 = P.nj;
+  const ColGrid cg = col_grid(P.ni, P.nj, m->tile_rows);
+  auto *kern = weno ? advm_launch<4, true, false> : advm_launch<4, false, false>;
+  kern(m, P, g, cg);
+}

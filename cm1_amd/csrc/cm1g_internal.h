@@ -124,7 +124,26 @@ struct cm1g_model {
   // statpack (cm1g_statpack): max/min keys, picked values and budget sums
   void *stat_buf;
   size_t stat_cap;
+  // LES subgrid turbulence (cm1g_set_turbulence; cm1setup 0: off)
+  cm1g_turb_config turb;
+  float *tcol;                  // per level: tkekm's grid scale (k = 2..nk)
+  float ustlog;                 // alog(za/znt) of the log-law friction velocity
 };
+
+// zeroed device allocation of n floats
+int dalloc(float **p, size_t n);
+// LES subgrid turbulence (cm1g_turb.hip): sfc_and_turb, the solve1
+// tendencies, the TKE of one RK stage, solve3's tkea = tke3d, getset's TKE
+void turb_sfc(cm1g_model *m);
+void turb_tend(cm1g_model *m);
+void tke_stage(cm1g_model *m, int nrk, float dttmp);
+void tke_copy(cm1g_model *m);
+int tke_init(cm1g_model *m);
+// ten = ten_in + advection of a on w levels with the scalars' WENO switch
+// and the mass fluxes given (advw as solve2 calls it for the TKE,
+// src/solve2.F:1746-1749)
+void advw_tke(cm1g_model *m, int nrk, const float *rru, const float *rrv, const float *rrw, const float *a,
+              const float *ten_in, float *ten);
 
 // release the output staging and the statpack buffer (cm1g_io.hip)
 void io_free(cm1g_model *m);

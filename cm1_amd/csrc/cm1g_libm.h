@@ -1,4 +1,4 @@
-// Single-precision exp and pow with the host C library's rounding.
+// Single-precision exp, pow and log with the host C library's rounding.
 //
 // The reference's REAL exp() and ** lower to the host C library's expf and
 // powf, which (glibc >= 2.28, x86-64 with FMA) implement the table-driven
@@ -176,6 +176,42 @@ CM1_HD inline float powf_host(float x, float y) {
   kd -= shift;
   const double r = ylogx - kd;
   return exp2_finish(r, ki, sign_bias);
+}
+
+// logf: the same 16 subintervals and 1/c as pow's log2 step, with ln(c)
+// and ln(1+r) ~ r + C2 r^2 + C1 r^3 + C0 r^4 in double precision, rounded
+// to float once (the method of the library's logf, FMA build)
+static constexpr double LOGF_LOGC[16] = {
+    -0x1.57bf7808caadep-2, -0x1.2bef0a7c06ddbp-2, -0x1.01eae7f513a67p-2, -0x1.b31d8a68224e9p-3,
+    -0x1.6574f0ac07758p-3, -0x1.1aa2bc79c81p-3,   -0x1.a4e76ce8c0e5ep-4, -0x1.1973c5a611cccp-4,
+    -0x1.252f438e10c1ep-5, 0x0p+0,                0x1.aa5aa5df25984p-5,  0x1.c5e53aa362eb4p-4,
+    0x1.526e57720db08p-3,  0x1.bc2860d22477p-3,   0x1.1058bc8a07ee1p-2,  0x1.4043057b6ee09p-2};
+constexpr double LOGF_C0 = -0x1.00ea348b88334p-2, LOGF_C1 = 0x1.5575b0be00b6ap-2,
+                 LOGF_C2 = -0x1.ffffef20a4123p-2, LOGF_LN2 = 0x1.62e42fefa39efp-1;
+
+CM1_HD inline float logf_host(float x) {
+  uint32_t ix = f2u(x);
+  if (ix == 0x3f800000u) return 0.0f;
+  if (ix - 0x00800000u >= 0x7f800000u - 0x00800000u) {
+    if (ix * 2 == 0) return -INFINITY;
+    if (ix == 0x7f800000u) return x;
+    if ((ix & 0x80000000u) || ix * 2 >= 0xff000000u) return NAN;
+    // subnormal: normalise
+    ix = f2u(x * 0x1p23f);
+    ix -= 23u << 23;
+  }
+  const uint32_t tmp = ix - 0x3f330000u;
+  const int i = (int)((tmp >> 19) % 16);
+  const int k = (int32_t)tmp >> 23;
+  const uint32_t iz = ix - (tmp & 0xff800000u);
+  const double z = (double)u2f(iz);
+  const double r = fmad(z, LOG2_INVC[i], -1.0);
+  const double y0 = fmad((double)k, LOGF_LN2, LOGF_LOGC[i]);
+  const double r2 = r * r;
+  double y = fmad(LOGF_C1, r, LOGF_C2);
+  y = fmad(LOGF_C0, r2, y);
+  y = fmad(y, r2, y0 + r);
+  return (float)y;
 }
 
 }  // namespace cm1libm

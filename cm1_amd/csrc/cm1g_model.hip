@@ -52,7 +52,7 @@ static void set_constants(Consts &K) {
   K.epsilon = 1.0e-18f;
 }
 
-static int dalloc(float **p, size_t n) {
+int dalloc(float **p, size_t n) {
   hipError_t e = hipMalloc((void **)p, n * sizeof(float));
   if (e != hipSuccess) {
     cm1g_set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -143,10 +143,11 @@ extern "C" int cm1g_create(const cm1g_config *cfg, cm1g_model **out) {
   }
   for (int i = 0; i < CM1G_NFIELDS && !rc; i++) {
     bool need = true;
-    if (i >= CM1G_PTA0 && i < CM1G_NFIELDS) {
+    if (i >= CM1G_PTA0 && i < CM1G_PTTEN0 + 8) {
       int t = (i - CM1G_PTA0) % 8;
       need = t < cfg->npt;
     }
+    if (i >= CM1G_TKEA) need = false;  // cm1g_set_turbulence allocates these
     if (need) rc = dalloc(&m->f[i], m->ntot);
   }
   for (int i = 0; i < CM1G_K_NFIELDS && !rc; i++) {
@@ -235,6 +236,7 @@ extern "C" int cm1g_destroy(cm1g_model *m) {
   for (int i = 0; i < CM1G_K_NFIELDS; i++) { if (m->kf[i]) hipFree(m->kf[i]); free(m->hk[i]); }
   for (int i = 0; i < 6; i++) if (m->h[i]) hipFree(m->h[i]);
   for (int i = 0; i < CM1G_D_NFIELDS; i++) if (m->d2[i]) hipFree(m->d2[i]);
+  if (m->tcol) hipFree(m->tcol);
   float *scr[] = {m->scoef, m->dtu0, m->dtv0, m->sx, m->sy, m->sz, m->tx, m->ty, m->tz,
                   m->scr1, m->scr2, m->thb, m->fred};
   for (auto p : scr) if (p) hipFree(p);
@@ -797,6 +799,10 @@ extern "C" int cm1g_getset(cm1g_model *m, int fresh_start) {
       HIPCHK(hipMemcpyAsync(f[CM1G_Q3D0 + n], f[CM1G_QA0 + n], bytes, hipMemcpyDeviceToDevice, m->stream));
   for (int n = 0; n < m->c.npt; n++)
     HIPCHK(hipMemcpyAsync(f[CM1G_PT3D0 + n], f[CM1G_PTA0 + n], bytes, hipMemcpyDeviceToDevice, m->stream));
+  if (m->turb.cm1setup == 1) {
+    const int rc = tke_init(m);
+    if (rc) return rc;
+  }
   float *rho = f[CM1G_RHO], *prs = f[CM1G_PRS], *rr = f[CM1G_RR], *rf = f[CM1G_RF];
   const float *pp3d = f[CM1G_PP3D], *th3d = f[CM1G_TH3D], *qv = f[CM1G_Q3D0];
   const int moist = m->c.imoist;
@@ -944,6 +950,9 @@ extern "C" int cm1g_solve1(cm1g_model *m) {
   // on the interior (:1716-1756)
   const float *ua = f[CM1G_UA], *va = f[CM1G_VA], *wa = f[CM1G_WA];
   const int rdamp = P.irdamp;
+  // with the subgrid turbulence its tendencies come between the ten1 set-up
+  // and ppten (src/solve1.F:1118-1240): this pass then stops after ten1
+  const int turb = (m->turb.cm1setup == 1);
   for3(m, 1, ni + 1, 1, nj + 1, 1, nk, [=] __device__(int i, int j, int k) {
     const Consts &K = P.K;
     const float tem = 0.0001f * K.tsmall;
@@ -959,7 +968,7 @@ extern "C" int cm1g_solve1(cm1g_model *m) {
     uten1[c] = u1;
     vten1[c] = v1;
     wten1[c] = w1;
-    if (i > ni || j > nj) {
+    if (i > ni || j > nj || turb) {
       thten1[c] = th1;
       return;
     }
@@ -992,6 +1001,24 @@ extern "C" int cm1g_solve1(cm1g_model *m) {
     ppten[c] = pt;
     thten1[c] = th1;
   });
+  if (turb) {
+    // the turbulence tendencies, then ppten from the final thten1 (dry:
+    // the turbulence is only set on dry models)
+    turb_tend(m);
+    for3(m, 1, ni, 1, nj, 1, nk, [=] __device__(int i, int j, int k) {
+      const Consts &K = P.K;
+      const float tem = 0.0001f * K.tsmall;
+      const int c = IXD(P, i, j, k);
+      const float th1 = thten1[c];
+      float pt;
+      if (fabsf(dt * th1) > tem) {
+        float thnew = tha[c] + dt * th1;
+        float pinew = pow_f(rho[c] * (P.th0[k] + thnew) * K.rd * K.rp00, K.rddcv) - P.pi0[k];
+        pt = (pinew - ppi[c]) * rdt;
+      } else pt = 0.0f;
+      ppten[c] = pt;
+    });
+  }
   return check_launch();
 }
 
@@ -1083,6 +1110,8 @@ extern "C" int cm1g_solve2(cm1g_model *m) {
       }
     }
     th3d = f[CM1G_TH3D];
+    // subgrid TKE: advection and update (src/solve2.F:1709-1800)
+    if (m->turb.cm1setup == 1) tke_stage(m, nrk, dttmp);
     // pressure and density (src/solve2.F:1536-1656)
     if (nrk == 3) {
       float *qvw = f[CM1G_Q3D0], *qcw = f[CM1G_Q3D1];
@@ -1269,10 +1298,13 @@ extern "C" int cm1g_solve3(cm1g_model *m) {
   }
   // ua..qa, pta now equal u3d..q3d, pt3d where the next stage reads them
   m->synced = !m->ext_writers;
+  if (m->turb.cm1setup == 1) tke_copy(m);
   return check_launch();
 }
 
 int step_body(cm1g_model *m) {
+  // the subgrid turbulence set-up of the step (src/cm1.F:3737)
+  if (m->turb.cm1setup == 1) turb_sfc(m);
   int rc = cm1g_solve1(m);
   if (!rc) rc = cm1g_solve2(m);
   if (!rc) rc = cm1g_mp_driver(m);

@@ -22,6 +22,12 @@ module cm1gpu
     real(c_float) :: rdalpha, zd
   end type cm1g_config
 
+  ! LES subgrid-turbulence switches (cm1g_set_turbulence)
+  type, bind(C) :: cm1g_turb_config
+    integer(c_int32_t) :: cm1setup, doimpl, bbc, isfcflx
+    real(c_float) :: cnst_shflx, cnst_znt
+  end type cm1g_turb_config
+
   ! return codes
   integer(c_int), parameter :: CM1G_OK = 0, CM1G_E_ARG = 1, CM1G_E_HIP = 2, &
                                CM1G_E_NOMEM = 3, CM1G_E_UNSUPPORTED = 4, &
@@ -156,6 +162,19 @@ module cm1gpu
   integer(c_int), parameter :: CM1G_PTTEN5   = 80
   integer(c_int), parameter :: CM1G_PTTEN6   = 81
   integer(c_int), parameter :: CM1G_PTTEN7   = 82
+  ! LES subgrid turbulence (allocated by cm1g_set_turbulence)
+  integer(c_int), parameter :: CM1G_TKEA      = 83
+  integer(c_int), parameter :: CM1G_TKE3D     = 84
+  integer(c_int), parameter :: CM1G_TKETEN    = 85
+  integer(c_int), parameter :: CM1G_KMH       = 86
+  integer(c_int), parameter :: CM1G_KMV       = 87
+  integer(c_int), parameter :: CM1G_KHH       = 88
+  integer(c_int), parameter :: CM1G_KHV       = 89
+  integer(c_int), parameter :: CM1G_NM        = 90
+  integer(c_int), parameter :: CM1G_DEFV      = 91
+  integer(c_int), parameter :: CM1G_DEFH      = 92
+  integer(c_int), parameter :: CM1G_DISSTEN   = 93
+  integer(c_int), parameter :: CM1G_LENSCL    = 94
 
   ! 1-D column ids (k = 0..nk+2)
   integer(c_int), parameter :: CM1G_K_MH    = 0
@@ -190,6 +209,12 @@ module cm1gpu
   integer(c_int), parameter :: CM1G_D_RADBCE  = 3
   integer(c_int), parameter :: CM1G_D_RADBCS  = 4
   integer(c_int), parameter :: CM1G_D_RADBCN  = 5
+  integer(c_int), parameter :: CM1G_D_UST     = 6
+  integer(c_int), parameter :: CM1G_D_U1      = 7
+  integer(c_int), parameter :: CM1G_D_V1      = 8
+  integer(c_int), parameter :: CM1G_D_S1      = 9
+  integer(c_int), parameter :: CM1G_D_THFLUX  = 10
+  integer(c_int), parameter :: CM1G_D_ZNT     = 11
 
   interface
     integer(c_int) function cm1g_create(cfg, m) bind(C, name="cm1g_create")
@@ -293,6 +318,15 @@ module cm1gpu
       real(c_float), intent(out) :: ms9(9)
     end function
     integer(c_int) function cm1g_sync(m) bind(C, name="cm1g_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_set_turbulence(m, t) bind(C, name="cm1g_set_turbulence")
+      import :: c_int, c_ptr, cm1g_turb_config
+      type(c_ptr), value :: m
+      type(cm1g_turb_config), intent(in) :: t
+    end function
+    integer(c_int) function cm1g_sfc_and_turb(m) bind(C, name="cm1g_sfc_and_turb")
       import :: c_int, c_ptr
       type(c_ptr), value :: m
     end function

@@ -76,6 +76,17 @@ class Model:
                 self.put_column(n, cols[n])
         N.check(self.L.cm1g_base_derive(self.h), "cm1g_base_derive")
 
+    def set_turbulence(self, cm1setup=1, doimpl=1, bbc=3, isfcflx=1, cnst_shflx=0.24, cnst_znt=0.1):
+        """LES subgrid turbulence (cm1g_set_turbulence): the TKE scheme with
+        the semi-slip surface and a specified surface heat flux; after
+        set_columns, before getset."""
+        t = N.TurbConfig(cm1setup=cm1setup, doimpl=doimpl, bbc=bbc, isfcflx=isfcflx,
+                         cnst_shflx=cnst_shflx, cnst_znt=cnst_znt)
+        N.check(self.L.cm1g_set_turbulence(self.h, C.byref(t)), "cm1g_set_turbulence")
+
+    def sfc_and_turb(self):
+        N.check(self.L.cm1g_sfc_and_turb(self.h), "cm1g_sfc_and_turb")
+
     # ---- 3-D fields in the logical (nk+3, nj+7, ni+7) box
     def put(self, name, arr):
         a = np.ascontiguousarray(arr, np.float32)
@@ -179,10 +190,13 @@ class Model:
         return dict(mass1=a[0], train=a[1], cflmax=a[2], mass2=a[3])
 
 
-def setup(cfg, cols, fields, fresh=True):
-    """Create a model, load columns and t=0 fields, run getset."""
+def setup(cfg, cols, fields, fresh=True, turb=None):
+    """Create a model, load columns and t=0 fields, run getset; turb: the
+    keywords of :meth:`Model.set_turbulence` (None: no subgrid turbulence)."""
     m = Model(cfg)
     m.set_columns(cols)
+    if turb:
+        m.set_turbulence(**turb)
     for name, arr in fields.items():
         m.put(name, arr)
     m.getset(fresh)

@@ -78,7 +78,12 @@ enum {
   CM1G_QA0, CM1G_QA1, CM1G_QA2, CM1G_Q3D0, CM1G_Q3D1, CM1G_Q3D2,
   CM1G_QTEN0, CM1G_QTEN1, CM1G_QTEN2,
   CM1G_PTA0, CM1G_PT3D0 = CM1G_PTA0 + 8, CM1G_PTTEN0 = CM1G_PT3D0 + 8,
-  CM1G_NFIELDS = CM1G_PTTEN0 + 8
+  /* LES subgrid turbulence (allocated by cm1g_set_turbulence): the TKE
+   * (tkea, tke3d, tketen), eddy viscosities and diffusivities, N^2,
+   * deformation, dissipation and length scale, all on w levels */
+  CM1G_TKEA = CM1G_PTTEN0 + 8, CM1G_TKE3D, CM1G_TKETEN, CM1G_KMH, CM1G_KMV, CM1G_KHH, CM1G_KHV,
+  CM1G_NM, CM1G_DEFV, CM1G_DEFH, CM1G_DISSTEN, CM1G_LENSCL,
+  CM1G_NFIELDS
 };
 
 /* 1-D vertical metric and base-state columns, index k = 0..nk+2. */
@@ -92,7 +97,12 @@ enum {
 
 /* 2-D fields: rain, prate are (i,j); radbcw/e are (j,k); radbcs/n are (i,k). */
 enum { CM1G_D_RAIN, CM1G_D_PRATE, CM1G_D_RADBCW, CM1G_D_RADBCE, CM1G_D_RADBCS,
-       CM1G_D_RADBCN, CM1G_D_NFIELDS };
+       CM1G_D_RADBCN,
+       /* surface layer of the turbulence (i,j): friction velocity, the
+        * lowest-level wind at scalar points and its speed, surface heat
+        * flux, roughness length */
+       CM1G_D_UST, CM1G_D_U1, CM1G_D_V1, CM1G_D_S1, CM1G_D_THFLUX, CM1G_D_ZNT,
+       CM1G_D_NFIELDS };
 
 typedef struct cm1g_model cm1g_model;
 
@@ -158,6 +168,40 @@ int cm1g_field_ptr(cm1g_model *m, int id, float **dptr);
 /* Wait for all queued work of this handle; CM1G_E_SATADJ if a saturation
  * adjustment of any step so far failed to converge (src/kessler.F:343-356). */
 int cm1g_sync(cm1g_model *m);
+
+/* --- LES subgrid turbulence ------------------------------------------- */
+
+/* The reference's switches of its dry convective boundary-layer LES
+ * (run/config_files/les_ConvBoundLayer) as far as the device path carries
+ * them: cm1setup = 1 with sgsmodel = 1 (Deardorff TKE), tconfig = 1,
+ * bcturbs = 1, horizturb = 0 and ramp_sgs = 0 are implied; doimpl
+ * chooses explicit (0) or implicit (1) vertical diffusion; bbc = 3 is the
+ * semi-slip surface with sfcmodel = 1 and set_znt = 1 (the neutral log-law
+ * friction velocity over the roughness length cnst_znt); isfcflx = 1 with
+ * set_flx = 1 is the specified surface kinematic heat flux cnst_shflx
+ * (K m/s); tbc = 1.  Dry (imoist = 0), doubly periodic grids only. */
+typedef struct cm1g_turb_config {
+  int32_t cm1setup;               /* 1: LES with the TKE scheme; 0: off */
+  int32_t doimpl;                 /* 0 explicit, 1 implicit vertical diffusion */
+  int32_t bbc;                    /* 3: semi-slip surface (src/turb.F:522-1124) */
+  int32_t isfcflx;                /* 1: surface heat flux */
+  float cnst_shflx;               /* surface kinematic heat flux (K m/s) */
+  float cnst_znt;                 /* roughness length (m) */
+} cm1g_turb_config;
+/* Turns the subgrid turbulence on for a created model (before cm1g_getset):
+ * allocates the TKE, viscosity and diagnostic fields, sets the roughness
+ * length (src/init_surface.F:251) and the per-level grid scale of tkekm
+ * (src/turb.F:3040-3046).  Replaces the namelist hand-over of param2 and
+ * param12 (src/param.F:189-277) for these switches.  CM1G_E_ARG for a
+ * combination outside the ones above. */
+int cm1g_set_turbulence(cm1g_model *m, const cm1g_turb_config *t);
+/* sfc_and_turb(getsfc=.true., ...), src/turb.F:12-1700, for the switches
+ * above: u1, v1, s1 and ust (getcecd, src/sfcphys.F:18-110), the
+ * specified heat flux (sfcflux, :313-330), N^2 (calcnm), rho S_ij in
+ * t11..t33 and the deformation (calcdef), km, kh, dissten, lenscl (tkekm),
+ * the surface values (tkebc) and the stresses (gettau, sfcstress).
+ * cm1g_step runs it at the start of every large step (src/cm1.F:3737). */
+int cm1g_sfc_and_turb(cm1g_model *m);
 
 /* --- the time step ---------------------------------------------------- */
 

@@ -57,8 +57,10 @@ def make_oracle(cfgd, cols, fields, libm=0, fp64=False):
 
 def make_gpu(cfgd, cols, fields):
     from cm1_amd import model
-    cfg = model.make_config(**cfgd)
-    return model.setup(cfg, cols, fields)
+    d = dict(cfgd)
+    turb = {k: d.pop(k) for k in TURB_KEYS if k in d}
+    cfg = model.make_config(**d)
+    return model.setup(cfg, cols, fields, turb=turb if turb.get("cm1setup") else None)
 
 
 def interior(a, nk, nj, ni, stag=None):

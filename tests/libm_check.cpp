@@ -1,6 +1,6 @@
-// Compares the restated single-precision exp/pow of cm1_amd/csrc/cm1g_libm.h
+// Compares the restated single-precision exp/pow/log of cm1_amd/csrc/cm1g_libm.h
 // (the functions the HIP kernels call) with the host C library's expf/powf.
-// usage: libm_check STRIDE  -> prints "exp N BAD" and "pow Y N BAD" lines
+// usage: libm_check STRIDE  -> prints "exp N BAD", "pow Y N BAD" and "log N BAD" lines
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -26,7 +26,7 @@ int main(int argc, char **argv) {
   std::printf("exp %ld %ld\n", n, bad);
   volatile float rd = 287.04f, cp = 1005.7f;
   const float cv = cp - rd;
-  const float ys[] = {0.1346f, 0.2046f, 0.525f, 0.875f, rd / cv, cp / rd, rd / cp};
+  const float ys[] = {0.1346f, 0.2046f, 0.525f, 0.875f, rd / cv, cp / rd, rd / cp, 0.33333333f};
   for (float y : ys) {
     n = 0;
     bad = 0;
@@ -43,5 +43,16 @@ int main(int argc, char **argv) {
     }
     std::printf("pow %a %ld %ld\n", y, n, bad);
   }
+  // log: every STRIDE-th positive float plus edge inputs
+  n = 0;
+  bad = 0;
+  for (uint64_t u = 1; u < 0x7f800000ull; u += stride) {
+    const float x = fl((uint32_t)u);
+    n++;
+    if (bits(cm1libm::logf_host(x)) != bits(logf(x))) bad++;
+  }
+  const float lx[] = {1.0f, 0.0f, -0.0f, INFINITY, 1e-45f, 0.99999994f, 1.0000001f, 3.4e38f};
+  for (float x : lx) { n++; if (bits(cm1libm::logf_host(x)) != bits(logf(x))) bad++; }
+  std::printf("log %ld %ld\n", n, bad);
   return 0;
 }

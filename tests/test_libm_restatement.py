@@ -1,4 +1,4 @@
-"""The single-precision exp/pow the HIP kernels call (cm1_amd/csrc/cm1g_libm.h)
+"""The single-precision exp/pow/log the HIP kernels call (cm1_amd/csrc/cm1g_libm.h)
 against the host C library's expf/powf, which the reference's REAL exp() and
 ** call.  The header is compiled for the CPU here (it is host+device code)
 and compared on every 127th float of exp's working range (-100, 88) and
@@ -25,7 +25,7 @@ def test_exp_pow_match_host_library(tmp_path):
     out = subprocess.run([exe, "127"], check=True, capture_output=True, text=True).stdout
     print(out)
     lines = [l.split() for l in out.strip().splitlines()]
-    assert len(lines) == 8
+    assert len(lines) == 10
     for parts in lines:
         n, bad = int(parts[-2]), int(parts[-1])
         assert n > 1_000_000 and bad == 0, parts
