@@ -274,7 +274,7 @@ def bench_cbl(args):
     t0 = time.perf_counter()
     cfgd, cols, fields = workload_cbl()
     cfg = model.make_config(**cfgd)
-    m = model.setup(cfg, cols, fields)
+    m = model.setup(cfg, cols, fields, turb=workloads.CBL_TURB)
     del fields
     print(f"cbl: set-up {time.perf_counter() - t0:.1f} s, nsound {cfg.nsound}", file=sys.stderr, flush=True)
     tm0, _ = m.budgets()
@@ -300,8 +300,9 @@ def bench_cbl(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "dry CBL 1536x1536x128 dx=dz=20m dt=0.25s periodic sounde WENO5 6 tracers "
-                               "(no SGS turbulence / surface flux)",
+        "config": {"workload": "dry CBL 1536x1536x128 dx=dz=20m dt=0.25s periodic sounde WENO5 6 tracers, "
+                               "TKE subgrid turbulence (implicit vertical diffusion), semi-slip surface, "
+                               "0.24 K m/s surface heat flux",
                    "nsound": cfg.nsound, "cflmax": cfl,
                    "dry_mass_rel_change": (tm1 - tm0) / tm0},
     }

@@ -104,12 +104,19 @@ def spinup(ni=128, nj=128, nk=40, seed=7):
     return cfgd, cols, fields
 
 
+# the subgrid turbulence of the CBL workload: the TKE scheme with implicit
+# vertical diffusion, the semi-slip surface (log-law friction velocity over
+# a 0.1 m roughness length) and the 0.24 K m/s surface heat flux, the
+# switches of the reference's run/config_files/les_ConvBoundLayer
+CBL_TURB = dict(cm1setup=1, doimpl=1, bbc=3, isfcflx=1, cnst_shflx=0.24, cnst_znt=0.1)
+
+
 def cbl(ni=1536, nj=1536, nk=128, seed=2026):
     """Dry convective boundary layer: 1536x1536x128, dx=dy=dz=20 m,
     doubly periodic, dt=0.25 s, explicit acoustic solver (psolver=2), WENO5
     scalar advection on every stage, 300 K mixed layer to 1 km then 8 K/km,
     u=10 m/s, +-0.1 K noise below 500 m, six Gaussian tracer blobs
-    (sigma 200 m)."""
+    (sigma 200 m); its subgrid turbulence and surface flux are CBL_TURB."""
     cols, fields, dzn = hinit.cbl_state(ni, nj, nk, 20.0, 20.0, 20.0, seed=seed)
     cfgd = dict(ni=ni, nj=nj, nk=nk, wbc=1, ebc=1, sbc=1, nbc=1, imoist=0, ptype=0,
                 psolver=2, advwenos=1, advwenov=0, apmasscon=1, irbc=4, nsound=0, npt=6,
