@@ -354,10 +354,10 @@ void advm_run(cm1g_model *m, int nrk, int mask, int buoy, int from_ten1, int own
   }
 }
 
-void advw_tke(cm1g_model *m, int nrk, const float *rru, const float *rrv, const float *rrw, const float *a,
-              const float *ten_in, float *ten) {
+void advw_tke(cm1g_model *m, int nrk, int advweno, const float *rru, const float *rrv, const float *rrw,
+              const float *a, const float *ten_in, float *ten) {
   const KP P = m->P;
-  const bool weno = (m->c.advwenos == 1) || (m->c.advwenos == 2 && nrk == 3);
+  const bool weno = (advweno == 1) || (advweno == 2 && nrk == 3);
   AdvmArgs g;
   g.rru = rru;
   g.rrv = rrv;

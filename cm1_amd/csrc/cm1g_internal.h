@@ -128,6 +128,9 @@ struct cm1g_model {
   cm1g_turb_config turb;
   float *tcol;                  // per level: tkekm's grid scale (k = 2..nk)
   float ustlog;                 // alog(za/znt) of the log-law friction velocity
+  // pinned host staging of cm1g_put_array / cm1g_get_array
+  float *pin;
+  size_t pin_cap;
 };
 
 // zeroed device allocation of n floats
@@ -142,8 +145,8 @@ int tke_init(cm1g_model *m);
 // ten = ten_in + advection of a on w levels with the scalars' WENO switch
 // and the mass fluxes given (advw as solve2 calls it for the TKE,
 // src/solve2.F:1746-1749)
-void advw_tke(cm1g_model *m, int nrk, const float *rru, const float *rrv, const float *rrw, const float *a,
-              const float *ten_in, float *ten);
+void advw_tke(cm1g_model *m, int nrk, int advweno, const float *rru, const float *rrv, const float *rrw,
+              const float *a, const float *ten_in, float *ten);
 
 // release the output staging and the statpack buffer (cm1g_io.hip)
 void io_free(cm1g_model *m);

@@ -237,6 +237,7 @@ extern "C" int cm1g_destroy(cm1g_model *m) {
   for (int i = 0; i < 6; i++) if (m->h[i]) hipFree(m->h[i]);
   for (int i = 0; i < CM1G_D_NFIELDS; i++) if (m->d2[i]) hipFree(m->d2[i]);
   if (m->tcol) hipFree(m->tcol);
+  if (m->pin) hipHostFree(m->pin);
   float *scr[] = {m->scoef, m->dtu0, m->dtv0, m->sx, m->sy, m->sz, m->tx, m->ty, m->tz,
                   m->scr1, m->scr2, m->thb, m->fred};
   for (auto p : scr) if (p) hipFree(p);
@@ -422,6 +423,20 @@ static int array_shape(const cm1g_model *m, int stag, size_t *nx, size_t *ny, si
   return 0;
 }
 
+// pinned staging of the array moves (allocated on first use, nx*ny*nz
+// floats of the largest staggering)
+static int pinned_stage(cm1g_model *m, size_t n, float **out) {
+  if (m->pin_cap < n) {
+    if (m->pin) HIPCHK(hipHostFree(m->pin));
+    m->pin = nullptr;
+    m->pin_cap = 0;
+    HIPCHK(hipHostMalloc((void **)&m->pin, n * sizeof(float), hipHostMallocDefault));
+    m->pin_cap = n;
+  }
+  *out = m->pin;
+  return 0;
+}
+
 extern "C" int cm1g_put_array(cm1g_model *m, int id, int stag, const float *host) {
   ARGCHK(field_ok(m, id) && host, "cm1g_put_array: bad field id or null pointer");
   UNSWAP(m);
@@ -429,10 +444,21 @@ extern "C" int cm1g_put_array(cm1g_model *m, int id, int stag, const float *host
   size_t nx, ny, nz;
   ARGCHK(array_shape(m, stag, &nx, &ny, &nz) == 0, "cm1g_put_array: bad staggering");
   HIPCHK(hipStreamSynchronize(m->stream));
-  const size_t dpitch = (size_t)m->P.px * sizeof(float), plane = (size_t)m->P.pxy;
-  for (size_t k = 0; k < nz; k++)
-    HIPCHK(hipMemcpy2D(box_origin(m, id) + k * plane, dpitch, host + k * nx * ny, nx * sizeof(float),
-                       nx * sizeof(float), ny, hipMemcpyHostToDevice));
+  // staged through pinned memory, then one pitched 3-D copy: host rows of
+  // nx, ny rows a level; device rows of px floats, py rows a level
+  float *stage = nullptr;
+  {
+    const int rc = pinned_stage(m, (size_t)(m->P.ni + 8) * (m->P.nj + 8) * (m->P.nk + 3), &stage);
+    if (rc) return rc;
+  }
+  std::memcpy(stage, host, nx * ny * nz * sizeof(float));
+  hipMemcpy3DParms p = {};
+  p.srcPtr = make_hipPitchedPtr((void *)stage, nx * sizeof(float), nx, ny);
+  p.dstPtr = make_hipPitchedPtr((void *)box_origin(m, id), (size_t)m->P.px * sizeof(float), (size_t)m->P.px,
+                                (size_t)m->P.py);
+  p.extent = make_hipExtent(nx * sizeof(float), ny, nz);
+  p.kind = hipMemcpyHostToDevice;
+  HIPCHK(hipMemcpy3D(&p));
   return 0;
 }
 
@@ -442,10 +468,19 @@ extern "C" int cm1g_get_array(cm1g_model *m, int id, int stag, float *host) {
   size_t nx, ny, nz;
   ARGCHK(array_shape(m, stag, &nx, &ny, &nz) == 0, "cm1g_get_array: bad staggering");
   HIPCHK(hipStreamSynchronize(m->stream));
-  const size_t spitch = (size_t)m->P.px * sizeof(float), plane = (size_t)m->P.pxy;
-  for (size_t k = 0; k < nz; k++)
-    HIPCHK(hipMemcpy2D(host + k * nx * ny, nx * sizeof(float), box_origin(m, id) + k * plane, spitch,
-                       nx * sizeof(float), ny, hipMemcpyDeviceToHost));
+  float *stage = nullptr;
+  {
+    const int rc = pinned_stage(m, (size_t)(m->P.ni + 8) * (m->P.nj + 8) * (m->P.nk + 3), &stage);
+    if (rc) return rc;
+  }
+  hipMemcpy3DParms p = {};
+  p.srcPtr = make_hipPitchedPtr((void *)box_origin(m, id), (size_t)m->P.px * sizeof(float), (size_t)m->P.px,
+                                (size_t)m->P.py);
+  p.dstPtr = make_hipPitchedPtr((void *)stage, nx * sizeof(float), nx, ny);
+  p.extent = make_hipExtent(nx * sizeof(float), ny, nz);
+  p.kind = hipMemcpyDeviceToHost;
+  HIPCHK(hipMemcpy3D(&p));
+  std::memcpy(host, stage, nx * ny * nz * sizeof(float));
   return 0;
 }
 
@@ -464,14 +499,15 @@ extern "C" int cm1g_get_2d(cm1g_model *m, int id, float *host) {
   return 0;
 }
 
-// 2-D fields in the reference's own shapes: rain and prate (ib:ie,jb:je),
-// radbcw/radbce (jb:je,kb:ke), radbcs/radbcn (ib:ie,kb:ke), ngxy = 3,
-// ngz = 1, first index fastest.  Device positions: rain/prate at
-// IX2D(i,j); radbcw/e at (j+2) + k*py; radbcs/n at (i+2) + k*px.
+// 2-D fields in the reference's own shapes: rain, prate and the surface
+// layer's ust..znt (ib:ie,jb:je), radbcw/radbce (jb:je,kb:ke),
+// radbcs/radbcn (ib:ie,kb:ke), ngxy = 3, ngz = 1, first index fastest.
+// Device positions: (i,j) fields at IX2D(i,j); radbcw/e at (j+2) + k*py;
+// radbcs/n at (i+2) + k*px.
 static void ref2d_map(const cm1g_model *m, int id, std::vector<size_t> &dev) {
   const KP &P = m->P;
   dev.clear();
-  if (id == CM1G_D_RAIN || id == CM1G_D_PRATE) {
+  if (id == CM1G_D_RAIN || id == CM1G_D_PRATE || id >= CM1G_D_UST) {
     for (int j = -2; j <= P.nj + 3; j++)
       for (int i = -2; i <= P.ni + 3; i++) dev.push_back((size_t)IX2D(P, i, j));
   } else if (id == CM1G_D_RADBCW || id == CM1G_D_RADBCE) {

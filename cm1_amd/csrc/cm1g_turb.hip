@@ -238,9 +238,11 @@ struct TurbtArgs {
   const float *t, *rho, *rf, *kmh, *kmv, *khv, *nm, *diss, *defv, *defh;
   float *tten, *cx, *cy;
   int doimpl;
+  int rhs;     // 1: tten = the TKE source terms + transport; 0: tten += transport
   float dt;
 };
 __device__ __forceinline__ float tke_rhs(const TurbtArgs &a, int c) {
+  if (!a.rhs) return a.tten[c];
   return -a.khv[c] * a.nm[c] - a.diss[c] + a.kmv[c] * fmaxf(0.0f, (a.defv[c] + a.defh[c]));
 }
 // the horizontal part at level k: turbx + turby
@@ -597,13 +599,15 @@ __device__ void turbs_col(const KP &P, const TurbsArgs &a, int i, int j) {
 
 }  // namespace
 
-void turb_tend(cm1g_model *m) {
+namespace {
+
+// turbt; rhs = 1 also forms the TKE's source terms (src/solve1.F:320-332)
+void run_turbt(cm1g_model *m, int rhs) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj;
   float **f = m->f;
   const int doimpl = m->turb.doimpl;
   const float dt = m->c.dt;
-  // tketen and turbt
   {
     TurbtArgs a;
     a.t = f[CM1G_TKEA];
@@ -620,10 +624,19 @@ void turb_tend(cm1g_model *m) {
     a.cx = f[CM1G_DUM1];
     a.cy = f[CM1G_DUM2];
     a.doimpl = doimpl;
+    a.rhs = rhs;
     a.dt = dt;
     for2(m, 1, ni, 1, nj, [=] __device__(int i, int j) { turbt_col(P, a, i, j); });
   }
-  // turbu, turbv, turbw
+}
+
+// turbu (mask 1), turbv (2), turbw (4) added to uten1, vten1, wten1
+void run_turbuvw(cm1g_model *m, int mask) {
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj;
+  float **f = m->f;
+  const int doimpl = m->turb.doimpl;
+  const float dt = m->c.dt;
   {
     TurbuvArgs a;
     a.w = f[CM1G_WA];
@@ -640,15 +653,15 @@ void turb_tend(cm1g_model *m) {
     a.tb = f[CM1G_T12];
     a.tz = f[CM1G_T13];
     a.ten = f[CM1G_UTEN1];
-    for2(m, 1, ni + 1, 1, nj, [=] __device__(int i, int j) { turbuv_col<0>(P, a, i, j); });
+    if (mask & 1) for2(m, 1, ni + 1, 1, nj, [=] __device__(int i, int j) { turbuv_col<0>(P, a, i, j); });
     a.vel = f[CM1G_VA];
     a.ta = f[CM1G_T12];
     a.tb = f[CM1G_T22];
     a.tz = f[CM1G_T23];
     a.ten = f[CM1G_VTEN1];
-    for2(m, 1, ni, 1, nj + 1, [=] __device__(int i, int j) { turbuv_col<1>(P, a, i, j); });
+    if (mask & 2) for2(m, 1, ni, 1, nj + 1, [=] __device__(int i, int j) { turbuv_col<1>(P, a, i, j); });
   }
-  {
+  if (mask & 4) {
     TurbwArgs a;
     a.w = f[CM1G_WA];
     a.rho = f[CM1G_RHO];
@@ -664,28 +677,47 @@ void turb_tend(cm1g_model *m) {
     a.dt = dt;
     for2(m, 1, ni, 1, nj, [=] __device__(int i, int j) { turbw_col(P, a, i, j); });
   }
-  // turbs: theta relative to th0r with the surface flux, then the tracers
-  for (int n = -1; n < m->c.npt; n++) {
+}
+
+// turbs of s into sten; theta: s is tha, taken relative to th0r as solve1's
+// sadv (src/solve1.F:1184-1206)
+void run_turbs(cm1g_model *m, const float *s, int theta, float *sten, int iflux, int dosfcflx) {
+  const KP P = m->P;
+  const int ni = P.ni, nj = P.nj;
+  float **f = m->f;
+  {
     TurbsArgs a;
-    a.s.s = n < 0 ? f[CM1G_THA] : f[CM1G_PTA0 + n];
+    a.s.s = s;
     a.s.th0 = P.th0;
     a.s.th0r = P.K.th0r;
-    a.s.theta = n < 0;
+    a.s.theta = theta;
     a.rho = f[CM1G_RHO];
     a.rr = f[CM1G_RR];
     a.rf = f[CM1G_RF];
     a.khh = f[CM1G_KHH];
     a.khv = f[CM1G_KHV];
     a.sflux = m->d2[CM1G_D_THFLUX];
-    a.sten = n < 0 ? f[CM1G_THTEN1] : f[CM1G_PTTEN0 + n];
+    a.sten = sten;
     a.d1 = f[CM1G_DUM1];
     a.d2 = f[CM1G_DUM2];
-    a.iflux = n < 0 ? 1 : 0;
-    a.dosfcflx = m->turb.isfcflx == 1;
-    a.doimpl = doimpl;
-    a.dt = dt;
+    a.iflux = iflux;
+    a.dosfcflx = dosfcflx;
+    a.doimpl = m->turb.doimpl;
+    a.dt = m->c.dt;
     for2(m, 1, ni, 1, nj, [=] __device__(int i, int j) { turbs_col(P, a, i, j); });
   }
+}
+
+}  // namespace
+
+void turb_tend(cm1g_model *m) {
+  float **f = m->f;
+  run_turbt(m, 1);
+  run_turbuvw(m, 7);
+  // theta with the surface flux, then the tracers
+  const int dosfc = (m->turb.isfcflx == 1);
+  run_turbs(m, f[CM1G_THA], 1, f[CM1G_THTEN1], 1, dosfc);
+  for (int n = 0; n < m->c.npt; n++) run_turbs(m, f[CM1G_PTA0 + n], 0, f[CM1G_PTTEN0 + n], 0, dosfc);
 }
 
 // --------------------------------------------------------- TKE of a stage
@@ -715,7 +747,7 @@ void tke_stage(cm1g_model *m, int nrk, float dttmp) {
     }
   });
   // wten = tketen + advection of tke3d (advw with the scalars' WENO switch)
-  advw_tke(m, nrk, sru, srv, srw, tke3d, f[CM1G_TKETEN], wten);
+  advw_tke(m, nrk, m->c.advwenos, sru, srv, srw, tke3d, f[CM1G_TKETEN], wten);
   for3(m, 1, ni, 1, nj, 2, nk, [=] __device__(int i, int j, int k) {
     const int c = IXD(P, i, j, k);
     float t = tkea[c] + dttmp * wten[c];
@@ -806,5 +838,69 @@ extern "C" int cm1g_sfc_and_turb(cm1g_model *m) {
     return CM1G_E_ARG;
   }
   turb_sfc(m);
+  return check_launch();
+}
+
+// ------------------------------------------------- routine-level entries
+static int turb_ready(cm1g_model *m, const char *who) {
+  if (!m || m->turb.cm1setup != 1) {
+    cm1g_set_error(std::string(who) + ": turbulence not set (cm1g_set_turbulence)");
+    return CM1G_E_ARG;
+  }
+  return CM1G_OK;
+}
+
+extern "C" int cm1g_turbs(cm1g_model *m, int iflux, int dosfcflx, int s_id, int sten_id) {
+  int rc = turb_ready(m, "cm1g_turbs");
+  if (rc) return rc;
+  if (!field_ok(m, s_id) || !field_ok(m, sten_id) || (iflux != 0 && iflux != 1)) {
+    cm1g_set_error("cm1g_turbs: bad field id or iflux");
+    return CM1G_E_ARG;
+  }
+  if ((rc = unswap(m))) return rc;
+  run_turbs(m, m->f[s_id], 0, m->f[sten_id], iflux, dosfcflx != 0);
+  return check_launch();
+}
+
+extern "C" int cm1g_turbt(cm1g_model *m) {
+  int rc = turb_ready(m, "cm1g_turbt");
+  if (rc) return rc;
+  run_turbt(m, 0);
+  return check_launch();
+}
+
+extern "C" int cm1g_turbu(cm1g_model *m) {
+  int rc = turb_ready(m, "cm1g_turbu");
+  if (rc) return rc;
+  if ((rc = unswap(m))) return rc;
+  run_turbuvw(m, 1);
+  return check_launch();
+}
+
+extern "C" int cm1g_turbv(cm1g_model *m) {
+  int rc = turb_ready(m, "cm1g_turbv");
+  if (rc) return rc;
+  if ((rc = unswap(m))) return rc;
+  run_turbuvw(m, 2);
+  return check_launch();
+}
+
+extern "C" int cm1g_turbw(cm1g_model *m) {
+  int rc = turb_ready(m, "cm1g_turbw");
+  if (rc) return rc;
+  if ((rc = unswap(m))) return rc;
+  run_turbuvw(m, 4);
+  return check_launch();
+}
+
+extern "C" int cm1g_advw_scalar(cm1g_model *m, int nrk, int advweno, int a_id, int ten_id) {
+  int rc;
+  if (!m || nrk < 1 || nrk > 3 || advweno < 0 || advweno > 2 || !field_ok(m, a_id) || !field_ok(m, ten_id)) {
+    cm1g_set_error("cm1g_advw_scalar: bad argument");
+    return CM1G_E_ARG;
+  }
+  if ((rc = unswap(m))) return rc;
+  float **f = m->f;
+  advw_tke(m, nrk, advweno, f[CM1G_RRU], f[CM1G_RRV], f[CM1G_RRW], f[a_id], f[ten_id], f[ten_id]);
   return check_launch();
 }

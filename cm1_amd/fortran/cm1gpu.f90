@@ -330,6 +330,32 @@ module cm1gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: m
     end function
+    integer(c_int) function cm1g_turbs(m, iflux, dosfcflx, s_id, sten_id) bind(C, name="cm1g_turbs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: iflux, dosfcflx, s_id, sten_id
+    end function
+    integer(c_int) function cm1g_turbt(m) bind(C, name="cm1g_turbt")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_turbu(m) bind(C, name="cm1g_turbu")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_turbv(m) bind(C, name="cm1g_turbv")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_turbw(m) bind(C, name="cm1g_turbw")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function cm1g_advw_scalar(m, nrk, advweno, a_id, ten_id) bind(C, name="cm1g_advw_scalar")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), value :: nrk, advweno, a_id, ten_id
+    end function
     integer(c_int) function cm1g_getset(m, fresh_start) bind(C, name="cm1g_getset")
       import :: c_int, c_ptr
       type(c_ptr), value :: m

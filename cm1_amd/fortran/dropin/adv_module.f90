@@ -5,9 +5,9 @@
 ! routine reads are copied to the device model, the tendency it writes is
 ! copied back.  The work arrays of the reference (dumx..subs, advx..advz,
 ! mass, flag, the MPI buffers) stay untouched: the GPU keeps its own.
+! advw also runs solve2's TKE call (iamw = .false.) through cm1g_advw_scalar.
 ! Branches outside the GPU path (budgets and diagnostics, hyperdiffusion in
-! advs, advection orders other than 5th/WENO5, the TKE call of advw) stop
-! the run through stopcm1.
+! advs, advection orders other than 5th/WENO5) stop the run through stopcm1.
 module adv_module
   implicit none
   private
@@ -221,12 +221,25 @@ contains
     integer, intent(in), dimension(ibib:ieib,jbib:jeib,kmaxib) :: hflxw, hflxe, hflxs, hflxn
     logical, intent(in) :: iamw
 
-    call dg_need(iamw, 'advw: advection of a w-level scalar (TKE) is not on the GPU path')
     call dg_need(.not. dowbud, 'advw: tendency budgets (dowbud) are not on the GPU path')
     call dg_need(hadvorder == 5 .and. vadvorder == 5, 'advw: advection orders other than 5')
-    call dg_need(advweno == advwenov, 'advw: a WENO switch other than advwenov')
     call adv_columns(mh, rho0, rr0, rf0, rrf0, c1, c2)
     call dg_col_w(CM1G_K_MF, mf)
+    if (.not. iamw) then
+      ! a w-level scalar, the subgrid TKE (src/solve2.F:1746-1749): the mass
+      ! fluxes as getdivx left them, the scalars' WENO switch
+      call dg_put(CM1G_RRU, CM1G_STAG_U, rru)
+      call dg_put(CM1G_RRV, CM1G_STAG_V, rrv)
+      call dg_put(CM1G_RRW, CM1G_STAG_W, rrw)
+      call dg_put(CM1G_DIVX, CM1G_STAG_S, divx)
+      call dg_put(CM1G_DUM9, CM1G_STAG_W, w3d)
+      call dg_put(CM1G_WTEN, CM1G_STAG_W, wten)
+      call dg_check(cm1g_advw_scalar(dg_model(), int(nrk, c_int), int(advweno, c_int), CM1G_DUM9, CM1G_WTEN), &
+                    'cm1g_advw_scalar')
+      call dg_get(CM1G_WTEN, CM1G_STAG_W, wten)
+      return
+    endif
+    call dg_need(advweno == advwenov, 'advw: a WENO switch other than advwenov')
     call momentum_in(rru, rrv, rrw, divx, u3d, v3d, w3d, rho)
     call dg_put(CM1G_WTEN, CM1G_STAG_W, wten)
     call dg_check(cm1g_advw(dg_model(), int(nrk, c_int)), 'cm1g_advw')

@@ -1,5 +1,6 @@
 ! Shared state of the drop-in replacements of CM1's adv_module, sound_module,
-! sounde_module and kessler_module (the other files of this directory): one
+! sounde_module, kessler_module and turbtend_module (the other files of this
+! directory): one
 ! device model for the run, created at the first call from the namelist
 ! scalars of module input (src/input.F:261-351) and the constants of module
 ! constants (src/constants.F), and the host<->device moves around each call.
@@ -26,10 +27,13 @@ contains
     use input, only : ni, nj, nk, wbc, ebc, sbc, nbc, imoist, ptype, psolver, advwenos, advwenov, &
                       apmasscon, irbc, nsound, iptra, npt, pdtra, dx, dy, dz, dtl, alph, kdiv,     &
                       irdamp, rdalpha, zd, terrain_flag, axisymm, stretch_x, stretch_y, adapt_dt,  &
-                      weno_order, eqtset, do_ib, ihail
+                      weno_order, eqtset, do_ib, ihail, cm1setup, sgsmodel, tconfig, bcturbs,      &
+                      horizturb, doimpl, bbc, tbc, isfcflx, sfcmodel, set_flx, set_znt, ramp_sgs,  &
+                      cnst_shflx, cnst_znt
     use constants, only : csmax
     type(c_ptr) :: m
     type(cm1g_config) :: c
+    type(cm1g_turb_config) :: t
     if (.not. c_associated(gm)) then
       call dg_need(.not. terrain_flag, 'terrain (terrain_flag) is not on the GPU path')
       call dg_need(axisymm == 0, 'axisymmetric grids (axisymm) are not on the GPU path')
@@ -68,6 +72,20 @@ contains
       c%rdalpha = rdalpha
       c%zd = zd
       call dg_check(cm1g_create(c, gm), 'cm1g_create')
+      ! the LES subgrid turbulence of the dry CBL branch set, when the
+      ! namelist asks for it (other subgrid models keep the CPU routines'
+      ! callers from reaching the GPU drop-ins: those stop)
+      if (cm1setup == 1 .and. sgsmodel == 1 .and. tconfig == 1 .and. bcturbs == 1 .and.     &
+          horizturb == 0 .and. bbc == 3 .and. tbc == 1 .and. isfcflx == 1 .and. sfcmodel == 1 &
+          .and. set_flx == 1 .and. set_znt == 1 .and. ramp_sgs == 0) then
+        t%cm1setup = 1
+        t%doimpl = doimpl
+        t%bbc = bbc
+        t%isfcflx = isfcflx
+        t%cnst_shflx = cnst_shflx
+        t%cnst_znt = cnst_znt
+        call dg_check(cm1g_set_turbulence(gm, t), 'cm1g_set_turbulence')
+      endif
     endif
     m = gm
   end function dg_model

@@ -202,6 +202,24 @@ int cm1g_set_turbulence(cm1g_model *m, const cm1g_turb_config *t);
  * the surface values (tkebc) and the stresses (gettau, sfcstress).
  * cm1g_step runs it at the start of every large step (src/cm1.F:3737). */
 int cm1g_sfc_and_turb(cm1g_model *m);
+/* The turbulence tendencies solve1 calls (src/solve1.F:1118-1240), with the
+ * model's arrays as solve1 passes them (src/turbtend.F, the branch set
+ * above): turbs(iflux,...,sflux,...,s,sten,khh,khv,...) adds the subgrid
+ * flux divergence of field s_id to field sten_id, sflux being the 2-D
+ * field CM1G_D_THFLUX (src/turbtend.F:18-698); turbt adds the TKE's own
+ * transport to tketen from tkea (:700-1050); turbu, turbv, turbw add the
+ * stress divergence to uten1, vten1, wten1 from ua, va, wa, t11..t33
+ * (:1052-2892). */
+int cm1g_turbs(cm1g_model *m, int iflux, int dosfcflx, int s_id, int sten_id);
+int cm1g_turbt(cm1g_model *m);
+int cm1g_turbu(cm1g_model *m);
+int cm1g_turbv(cm1g_model *m);
+int cm1g_turbw(cm1g_model *m);
+/* advw(...,w3d=a,wten=ten,...,advweno,...,iamw=.false.) as solve2 calls it
+ * for the TKE (src/solve2.F:1746-1749, src/adv.F:2778-3610): ten += the
+ * advection of the w-level field a_id with the model's rru, rrv, rrw and
+ * divx as stored, WENO5 when advweno is 1 (or 2 on the last stage). */
+int cm1g_advw_scalar(cm1g_model *m, int nrk, int advweno, int a_id, int ten_id);
 
 /* --- the time step ---------------------------------------------------- */
 

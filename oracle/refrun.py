@@ -147,10 +147,12 @@ def namelist_text(overrides=None):
 
 
 def run_reference(workdir, overrides=None, threads=None, exe=REF_EXE, timeout=3600,
-                  extra_env=None, tables=()):
+                  extra_env=None, tables=(), files=None):
     """Run the reference in ``workdir``; returns (stdout text, wall seconds).
     ``tables``: data files of the reference's run directory the case reads
-    (e.g. LANDUSE.TBL for a surface model), copied into ``workdir``."""
+    (e.g. LANDUSE.TBL for a surface model), copied into ``workdir``;
+    ``files``: {name: bytes} written into ``workdir`` (the same tables taken
+    from a fixture where the reference's directory is absent)."""
     if not os.path.exists(exe):
         raise FileNotFoundError(exe)
     if os.path.exists(workdir):
@@ -158,6 +160,9 @@ def run_reference(workdir, overrides=None, threads=None, exe=REF_EXE, timeout=36
     os.makedirs(workdir)
     for t in tables:
         shutil.copy(os.path.join(REF_RUN, t), os.path.join(workdir, t))
+    for name, data in (files or {}).items():
+        with open(os.path.join(workdir, name), "wb") as f:
+            f.write(data)
     with open(os.path.join(workdir, "namelist.input"), "w") as f:
         f.write(namelist_text(overrides))
     env = dict(os.environ)

@@ -150,11 +150,41 @@ def make_turb(name, ni, nj, nk, nsteps, nout, dt, doimpl, dx=20.0, dz=20.0, npt=
     return path
 
 
+def make_turb_restart(name, ni, nj, nk, nsteps, nout, dt, doimpl, nrst, dx=20.0, dz=20.0, npt=2, psolver=2,
+                      advwenos=1, workdir="/tmp/cm1_fixture_turb_rst", outdir=HERE, threads=4):
+    """The run of make_turb with restart files written after nrst steps:
+    stores every cm1rst_000001_*.dat as bytes."""
+    import glob as _glob
+    ov = dict(nx=ni, ny=nj, nz=nk, dtl=dt, timax=nsteps * dt, tapfrq=nsteps * dt / nout,
+              rstfrq=nrst * dt, wbc=1, ebc=1, sbc=1, nbc=1, dx=dx, dy=dx, dz=dz, psolver=psolver,
+              advwenos=advwenos, advwenov=0, irdamp=0, doimpl=doimpl, iptra=1 if npt else 0,
+              npt=max(npt, 1), **TURB_NAMELIST)
+    ov.update({"output_" + k: 1 for k in TURB_OUT})
+    refrun.run_reference(workdir, ov, threads=threads, tables=("LANDUSE.TBL",))
+    out = {os.path.basename(f).replace(".", "_"): np.frombuffer(open(f, "rb").read(), dtype=np.uint8)
+           for f in sorted(_glob.glob(os.path.join(workdir, "cm1rst_000001_*.dat")))}
+    out["meta"] = np.array([nrst], np.int32)
+    path = os.path.join(outdir, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    return path
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "turb":
         # 400 steps of 0.25 s: the subgrid TKE appears after about 300
         for doimpl in (1, 0):
             print(make_turb(f"ref_les_tke_impl{doimpl}_16x12x24_400", 16, 12, 24, 400, 4, 0.25, doimpl))
+        # the reference's restart files after 300 of the 400 steps (the state
+        # a run of the program restarted from them continues exactly; the
+        # drop-in program restarts there instead of stepping from t=0)
+        for doimpl in (1, 0):
+            print(make_turb_restart(f"les_rst_impl{doimpl}_16x12x24_300", 16, 12, 24, 400, 4, 0.25, doimpl, 300))
+        # the land-use table the surface model reads at start-up (input data
+        # of the reference's les_ConvBoundLayer case), for runs of the
+        # program where the reference's directory is absent
+        with open(os.path.join(refrun.REF_RUN, "LANDUSE.TBL"), "rb") as f:
+            tbl = np.frombuffer(f.read(), dtype=np.uint8)
+        np.savez_compressed(os.path.join(HERE, "les_inputs.npz"), landuse_tbl=tbl)
     elif len(sys.argv) > 1 and sys.argv[1] == "stretched":
         # dt = 1.5 s: the reference's dtsmall picks nsound = 4; dt = 3 s: nsound = 8
         for dt, tag in ((1.5, "dt1.5"), (3.0, "dt3")):

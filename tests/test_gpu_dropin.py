@@ -107,3 +107,55 @@ def test_dropin_restart_round_trip(require_gpu, tmp_path):
     assert r.returncode == 0 and "Program terminated normally" in r.stdout + r.stderr, (r.stdout + r.stderr)[-2000:]
     for kind in "suvw":
         assert same(ref, rst, 2, kind), kind
+
+
+LES = ["impl1", "impl0"]
+
+
+@pytest.mark.parametrize("impl", LES)
+def test_dropin_program_les_equals_reference(require_gpu, impl, tmp_path):
+    """The drop-in program with turbtend_module replaced as well (turbs,
+    turbt, turbu, turbv, turbw on the GPU, advw's TKE call through
+    cm1g_advw_scalar) on the dry LES namelist of the fixture, restarted from
+    the reference executable's restart files after 300 steps (the TKE is
+    active from there on): after the last 100 steps its GrADS output (u, v,
+    w, theta', pi', TKE, rho, prs, the tracers, km, kh) equals the reference
+    executable's bit for bit."""
+    if not os.path.exists(EXE):
+        pytest.skip("drop-in program not built (cm1_amd/fortran/build_dropin.py needs the reference sources)")
+    import sys
+    sys.path.insert(0, GOLDEN)
+    import make_ref_fixture as mrf
+    z = np.load(os.path.join(GOLDEN, f"ref_les_tke_{impl}_16x12x24_400.npz"))
+    rst = np.load(os.path.join(GOLDEN, f"les_rst_{impl}_16x12x24_300.npz"))
+    ni, nj, nk, nsteps, bc, nout = [int(v) for v in z["meta"]]
+    dt = float(z["dt"][0])
+    psolver, advwenos, _, _, npt, doimpl = [int(v) for v in z["opts"]]
+    dx, _, dz = [float(v) for v in z["grid"]]
+    ov = dict(nx=ni, ny=nj, nz=nk, dtl=dt, timax=nsteps * dt, tapfrq=nsteps * dt / nout,
+              wbc=1, ebc=1, sbc=1, nbc=1, dx=dx, dy=dx, dz=dz, psolver=psolver,
+              advwenos=advwenos, advwenov=0, irdamp=0, doimpl=doimpl, iptra=1, npt=npt,
+              irst=1, rstnum=1, **mrf.TURB_NAMELIST)
+    ov.update({"output_" + k: 1 for k in mrf.TURB_OUT})
+    files = {k[:-4] + ".dat": rst[k].tobytes() for k in rst.files if k.startswith("cm1rst")}
+    files["LANDUSE.TBL"] = np.load(os.path.join(GOLDEN, "les_inputs.npz"))["landuse_tbl"].tobytes()
+    work = str(tmp_path / "run")
+    print(f"drop-in LES run {impl}, from step {int(rst['meta'][0])}", flush=True)
+    out, wall = refrun.run_reference(work, ov, threads=1, exe=EXE, timeout=600, files=files)
+    S = refrun.read_grads(work, "cm1out_s.ctl")
+    U = refrun.read_grads(work, "cm1out_u.ctl")
+    V = refrun.read_grads(work, "cm1out_v.ctl")
+    W = refrun.read_grads(work, "cm1out_w.ctl")
+    got = {"u": U["u"], "v": V["v"], "w": W["w"], "th": S["thpert"], "pi": S["pipert"], "tke": W["tke"],
+           "rho": S["rho"], "prs": S["prs"], "kmh": W["kmh"], "khv": W["khv"]}
+    for m in range(1, npt + 1):
+        got[f"pt{m}"] = S[f"pt{m}"]
+    bad = {}
+    for key, arr in got.items():
+        want = z[f"o{nout}_{key}"]
+        c = int(np.count_nonzero(arr[nout].view(np.uint32) != want.view(np.uint32)))
+        if c:
+            bad[key] = c
+    print(f"drop-in LES {impl}: {wall:.1f} s; differing values {bad}", flush=True)
+    assert not bad, bad
+    assert float(z[f"o{nout}_tke"].max()) > 0.1
