@@ -96,7 +96,7 @@ SIGNATURES = {
     "cm1g_sync": (C.c_int, [C.c_void_p]),
     "cm1g_getset": (C.c_int, [C.c_void_p, C.c_int]),
     "cm1g_set_turbulence": (C.c_int, [C.c_void_p, C.POINTER(TurbConfig)]),
-    "cm1g_sfc_and_turb": (C.c_int, [C.c_void_p]),
+    "cm1g_sfc_and_turb": (C.c_int, [C.c_void_p, C.c_int]),
     "cm1g_turbs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "cm1g_turbt": (C.c_int, [C.c_void_p]),
     "cm1g_turbu": (C.c_int, [C.c_void_p]),

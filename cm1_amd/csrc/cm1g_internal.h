@@ -137,7 +137,7 @@ struct cm1g_model {
 int dalloc(float **p, size_t n);
 // LES subgrid turbulence (cm1g_turb.hip): sfc_and_turb, the solve1
 // tendencies, the TKE of one RK stage, solve3's tkea = tke3d, getset's TKE
-void turb_sfc(cm1g_model *m);
+void turb_sfc(cm1g_model *m, int getsfc);
 void turb_tend(cm1g_model *m);
 void tke_stage(cm1g_model *m, int nrk, float dttmp);
 void tke_copy(cm1g_model *m);

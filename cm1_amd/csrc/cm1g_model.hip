@@ -1340,7 +1340,7 @@ extern "C" int cm1g_solve3(cm1g_model *m) {
 
 int step_body(cm1g_model *m) {
   // the subgrid turbulence set-up of the step (src/cm1.F:3737)
-  if (m->turb.cm1setup == 1) turb_sfc(m);
+  if (m->turb.cm1setup == 1) turb_sfc(m, 1);
   int rc = cm1g_solve1(m);
   if (!rc) rc = cm1g_solve2(m);
   if (!rc) rc = cm1g_mp_driver(m);

@@ -67,7 +67,7 @@ struct ScalarIn {
 }  // namespace
 
 // ------------------------------------------------------------ sfc_and_turb
-void turb_sfc(cm1g_model *m) {
+void turb_sfc(cm1g_model *m, int getsfc) {
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   float **f = m->f;
@@ -87,8 +87,10 @@ void turb_sfc(cm1g_model *m) {
 
   // u1, v1, s1, ust (getcecd, set_znt = 1) and the specified heat flux on
   // the interior, and on the 3-wide periodic halo the values of the
-  // interior points they repeat (the bc2d fills of src/turb.F:1138-1242)
-  for2(m, -2, ni + 3, -2, nj + 3, [=] __device__(int i, int j) {
+  // interior points they repeat (the bc2d fills of src/turb.F:1138-1242);
+  // getsfc = 0 (the first step after a restart, src/cm1.F:3726-3728) keeps
+  // the stored ones
+  if (getsfc) for2(m, -2, ni + 3, -2, nj + 3, [=] __device__(int i, int j) {
     const int ii = wrap(i, ni), jj = wrap(j, nj);
     const float a = 0.5f * (ua[IXD(P, ii, jj, 1)] + ua[IXD(P, ii + 1, jj, 1)]);
     const float b = 0.5f * (va[IXD(P, ii, jj, 1)] + va[IXD(P, ii, jj + 1, 1)]);
@@ -99,6 +101,16 @@ void turb_sfc(cm1g_model *m) {
     s1[c] = s;
     ust[c] = fmaxf(s * KARMAN / ustlog, 1.0e-6f);
     if (i >= 1 && i <= ni && j >= 1 && j <= nj) thflux[c] = shflx;
+  });
+  // the bc2d fills run with getsfc = 0 as well: the stored interior values
+  // repeat on the halo
+  else for2(m, -2, ni + 3, -2, nj + 3, [=] __device__(int i, int j) {
+    if (i >= 1 && i <= ni && j >= 1 && j <= nj) return;
+    const int s = IX2D(P, wrap(i, ni), wrap(j, nj)), c = IX2D(P, i, j);
+    ust[c] = ust[s];
+    u1[c] = u1[s];
+    v1[c] = v1[s];
+    s1[c] = s1[s];
   });
 
   // calcdef's rho*S_ij (src/turb.F:4750-4830, no terrain) and calcnm's
@@ -832,12 +844,14 @@ extern "C" int cm1g_set_turbulence(cm1g_model *m, const cm1g_turb_config *t) {
   return CM1G_OK;
 }
 
-extern "C" int cm1g_sfc_and_turb(cm1g_model *m) {
+extern "C" int cm1g_sfc_and_turb(cm1g_model *m, int getsfc) {
   if (!m || m->turb.cm1setup != 1) {
     cm1g_set_error("cm1g_sfc_and_turb: turbulence not set (cm1g_set_turbulence)");
     return CM1G_E_ARG;
   }
-  turb_sfc(m);
+  int rc;
+  if ((rc = unswap(m))) return rc;
+  turb_sfc(m, getsfc != 0);
   return check_launch();
 }
 

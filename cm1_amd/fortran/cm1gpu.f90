@@ -326,9 +326,10 @@ module cm1gpu
       type(c_ptr), value :: m
       type(cm1g_turb_config), intent(in) :: t
     end function
-    integer(c_int) function cm1g_sfc_and_turb(m) bind(C, name="cm1g_sfc_and_turb")
+    integer(c_int) function cm1g_sfc_and_turb(m, getsfc) bind(C, name="cm1g_sfc_and_turb")
       import :: c_int, c_ptr
       type(c_ptr), value :: m
+      integer(c_int), value :: getsfc
     end function
     integer(c_int) function cm1g_turbs(m, iflux, dosfcflx, s_id, sten_id) bind(C, name="cm1g_turbs")
       import :: c_int, c_ptr

@@ -1,6 +1,6 @@
 ! Shared state of the drop-in replacements of CM1's adv_module, sound_module,
-! sounde_module, kessler_module and turbtend_module (the other files of this
-! directory): one
+! sounde_module, kessler_module, turbtend_module and turb_module (the other
+! files of this directory): one
 ! device model for the run, created at the first call from the namelist
 ! scalars of module input (src/input.F:261-351) and the constants of module
 ! constants (src/constants.F), and the host<->device moves around each call.
@@ -16,7 +16,8 @@ module cm1g_dropin
   use cm1gpu
   implicit none
   private
-  public :: dg_model, dg_check, dg_need, dg_put, dg_get, dg_col_s, dg_col_w, dg_put2, dg_get2
+  public :: dg_model, dg_check, dg_need, dg_put, dg_get, dg_col_s, dg_col_w, dg_put2, dg_get2, &
+            dg_set_turbulence
 
   type(c_ptr), save :: gm = c_null_ptr
 
@@ -27,13 +28,10 @@ contains
     use input, only : ni, nj, nk, wbc, ebc, sbc, nbc, imoist, ptype, psolver, advwenos, advwenov, &
                       apmasscon, irbc, nsound, iptra, npt, pdtra, dx, dy, dz, dtl, alph, kdiv,     &
                       irdamp, rdalpha, zd, terrain_flag, axisymm, stretch_x, stretch_y, adapt_dt,  &
-                      weno_order, eqtset, do_ib, ihail, cm1setup, sgsmodel, tconfig, bcturbs,      &
-                      horizturb, doimpl, bbc, tbc, isfcflx, sfcmodel, set_flx, set_znt, ramp_sgs,  &
-                      cnst_shflx, cnst_znt
+                      weno_order, eqtset, do_ib, ihail
     use constants, only : csmax
     type(c_ptr) :: m
     type(cm1g_config) :: c
-    type(cm1g_turb_config) :: t
     if (.not. c_associated(gm)) then
       call dg_need(.not. terrain_flag, 'terrain (terrain_flag) is not on the GPU path')
       call dg_need(axisymm == 0, 'axisymmetric grids (axisymm) are not on the GPU path')
@@ -72,23 +70,37 @@ contains
       c%rdalpha = rdalpha
       c%zd = zd
       call dg_check(cm1g_create(c, gm), 'cm1g_create')
-      ! the LES subgrid turbulence of the dry CBL branch set, when the
-      ! namelist asks for it (other subgrid models keep the CPU routines'
-      ! callers from reaching the GPU drop-ins: those stop)
-      if (cm1setup == 1 .and. sgsmodel == 1 .and. tconfig == 1 .and. bcturbs == 1 .and.     &
-          horizturb == 0 .and. bbc == 3 .and. tbc == 1 .and. isfcflx == 1 .and. sfcmodel == 1 &
-          .and. set_flx == 1 .and. set_znt == 1 .and. ramp_sgs == 0) then
-        t%cm1setup = 1
-        t%doimpl = doimpl
-        t%bbc = bbc
-        t%isfcflx = isfcflx
-        t%cnst_shflx = cnst_shflx
-        t%cnst_znt = cnst_znt
-        call dg_check(cm1g_set_turbulence(gm, t), 'cm1g_set_turbulence')
-      endif
+      call dg_set_turbulence(.false.)
     endif
     m = gm
   end function dg_model
+
+  ! the LES subgrid turbulence of the dry CBL branch set, when the namelist
+  ! asks for it (cm1g_set_turbulence; again after the vertical columns are
+  ! in, whose zh and rmf it reads).  With other subgrid switches the GPU
+  ! drop-ins of the turbulence stop the run.  required: stop if the
+  ! namelist's switches are outside the branch set.
+  subroutine dg_set_turbulence(required)
+    use input, only : cm1setup, sgsmodel, tconfig, bcturbs, horizturb, doimpl, bbc, tbc, isfcflx, &
+                      sfcmodel, set_flx, set_znt, ramp_sgs, cnst_shflx, cnst_znt
+    logical, intent(in) :: required
+    type(cm1g_turb_config) :: t
+    logical :: ok
+    ok = cm1setup == 1 .and. sgsmodel == 1 .and. tconfig == 1 .and. bcturbs == 1 .and.      &
+         horizturb == 0 .and. bbc == 3 .and. tbc == 1 .and. isfcflx == 1 .and. sfcmodel == 1 &
+         .and. set_flx == 1 .and. set_znt == 1 .and. ramp_sgs == 0
+    if (required) call dg_need(ok, 'subgrid turbulence: only cm1setup 1, sgsmodel 1, tconfig 1, bcturbs 1, '// &
+                               'horizturb 0, bbc 3, tbc 1, isfcflx 1, sfcmodel 1, set_flx 1, set_znt 1, '// &
+                               'ramp_sgs 0 are on the GPU path')
+    if (.not. ok) return
+    t%cm1setup = 1
+    t%doimpl = doimpl
+    t%bbc = bbc
+    t%isfcflx = isfcflx
+    t%cnst_shflx = cnst_shflx
+    t%cnst_znt = cnst_znt
+    call dg_check(cm1g_set_turbulence(gm, t), 'cm1g_set_turbulence')
+  end subroutine dg_set_turbulence
 
   subroutine dg_check(ierr, what)
     integer(c_int), intent(in) :: ierr

@@ -84,8 +84,8 @@ class Model:
                          cnst_shflx=cnst_shflx, cnst_znt=cnst_znt)
         N.check(self.L.cm1g_set_turbulence(self.h, C.byref(t)), "cm1g_set_turbulence")
 
-    def sfc_and_turb(self):
-        N.check(self.L.cm1g_sfc_and_turb(self.h), "cm1g_sfc_and_turb")
+    def sfc_and_turb(self, getsfc=True):
+        N.check(self.L.cm1g_sfc_and_turb(self.h, 1 if getsfc else 0), "cm1g_sfc_and_turb")
 
     # ---- 3-D fields in the logical (nk+3, nj+7, ni+7) box
     def put(self, name, arr):

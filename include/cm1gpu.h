@@ -199,9 +199,12 @@ int cm1g_set_turbulence(cm1g_model *m, const cm1g_turb_config *t);
  * above: u1, v1, s1 and ust (getcecd, src/sfcphys.F:18-110), the
  * specified heat flux (sfcflux, :313-330), N^2 (calcnm), rho S_ij in
  * t11..t33 and the deformation (calcdef), km, kh, dissten, lenscl (tkekm),
- * the surface values (tkebc) and the stresses (gettau, sfcstress).
- * cm1g_step runs it at the start of every large step (src/cm1.F:3737). */
-int cm1g_sfc_and_turb(cm1g_model *m);
+ * the surface values (tkebc) and the stresses (gettau, sfcstress);
+ * getsfc = 0 keeps the stored u1, v1, s1, ust and heat flux (the first call
+ * after a restart, src/cm1.F:3726-3728).  cm1g_step runs it (getsfc = 1) at
+ * the start of every large step, where the reference runs it at the end of
+ * the one before (src/cm1.F:3737). */
+int cm1g_sfc_and_turb(cm1g_model *m, int getsfc);
 /* The turbulence tendencies solve1 calls (src/solve1.F:1118-1240), with the
  * model's arrays as solve1 passes them (src/turbtend.F, the branch set
  * above): turbs(iflux,...,sflux,...,s,sten,khh,khv,...) adds the subgrid

@@ -114,9 +114,11 @@ LES = ["impl1", "impl0"]
 
 @pytest.mark.parametrize("impl", LES)
 def test_dropin_program_les_equals_reference(require_gpu, impl, tmp_path):
-    """The drop-in program with turbtend_module replaced as well (turbs,
-    turbt, turbu, turbv, turbw on the GPU, advw's TKE call through
-    cm1g_advw_scalar) on the dry LES namelist of the fixture, restarted from
+    """The drop-in program with turb_module and turbtend_module replaced as
+    well (sfc_and_turb, turbs, turbt, turbu, turbv, turbw on the GPU, advw's
+    TKE call through cm1g_advw_scalar; the first sfc_and_turb after the
+    restart keeps the restart's surface values, getsfc = .false.) on the dry
+    LES namelist of the fixture, restarted from
     the reference executable's restart files after 300 steps (the TKE is
     active from there on): after the last 100 steps its GrADS output (u, v,
     w, theta', pi', TKE, rho, prs, the tracers, km, kh) equals the reference
