@@ -150,7 +150,8 @@ int cm1g_get_2d(cm1g_model *m, int id, float *host);
 /* The same for a 2-D host array in the reference's own shape: rain and
  * prate (ib:ie,jb:je) (one slice of rain(ib:ie,jb:je,nrain)), radbcw and
  * radbce (jb:je,kb:ke), radbcs and radbcn (ib:ie,kb:ke) (src/cm1.F:
- * 747-777, src/bc.F:2054-2172), first index fastest. */
+ * 747-777, src/bc.F:2054-2172), and the surface fields CM1G_D_UST ..
+ * CM1G_D_ZNT (ib:ie,jb:je) (src/cm1.F:747-777), first index fastest. */
 int cm1g_put_2d_array(cm1g_model *m, int id, const float *host);
 int cm1g_get_2d_array(cm1g_model *m, int id, float *host);
 /* Device address of a field (for zero-copy use by a caller that already
@@ -200,8 +201,9 @@ int cm1g_set_turbulence(cm1g_model *m, const cm1g_turb_config *t);
  * specified heat flux (sfcflux, :313-330), N^2 (calcnm), rho S_ij in
  * t11..t33 and the deformation (calcdef), km, kh, dissten, lenscl (tkekm),
  * the surface values (tkebc) and the stresses (gettau, sfcstress);
- * getsfc = 0 keeps the stored u1, v1, s1, ust and heat flux (the first call
- * after a restart, src/cm1.F:3726-3728).  cm1g_step runs it (getsfc = 1) at
+ * getsfc = 0 keeps the stored interior u1, v1, s1, ust and heat flux (the
+ * first call after a restart, src/cm1.F:3726-3728) and refills the periodic
+ * halos of u1, v1, s1, ust from them (bc2d, src/turb.F:1134-1160).  cm1g_step runs it (getsfc = 1) at
  * the start of every large step, where the reference runs it at the end of
  * the one before (src/cm1.F:3737). */
 int cm1g_sfc_and_turb(cm1g_model *m, int getsfc);
