@@ -609,6 +609,171 @@ __device__ void turbs_col(const KP &P, const TurbsArgs &a, int i, int j) {
   }
 }
 
+// turbs of up to 8 scalars of one column in one pass: the solver's
+// forward coefficient d1 depends only on khv, rf and rr, so it is formed
+// once; each scalar keeps its own recursion (d2, the back-substituted
+// value) in registers and its own d2 column in scratch.  Every scalar's
+// arithmetic is turbs_col's, in the same order.
+constexpr int TURBS_MAX = 8;
+struct TurbsMultiArgs {
+  const float *s[TURBS_MAX];
+  float *sten[TURBS_MAX], *d2[TURBS_MAX];
+  const float *th0;
+  float th0r;
+  int theta0;   // scalar 0 is theta, taken relative to th0r
+  int flux0;    // scalar 0 takes the surface flux
+  const float *rho, *rr, *rf, *khh, *khv, *sflux;
+  float *d1;
+  int doimpl;
+  float dt;
+};
+template <int NS>
+__device__ void turbs_multi_col(const KP &P, const TurbsMultiArgs &a, int i, int j) {
+  const int nk = P.nk, pxy = P.pxy, px = P.px;
+  const float *rho = a.rho, *rr = a.rr, *rf = a.rf, *khh = a.khh, *khv = a.khv;
+  const float rdz = P.rdz;
+  auto sv = [&](int n, int c, int k) {
+    const float v = a.s[n][c];
+    if (n == 0 && a.theta0) {
+      const float tem = a.th0[k] - a.th0r;
+      return tem + v;
+    }
+    return v;
+  };
+  const int c1 = IXD(P, i, j, 1);
+  const float bot0 = a.flux0 ? a.sflux[IX2D(P, i, j)] * rf[c1] : 0.0f;
+  // the horizontal terms of every scalar at level k (turbs_h), their shared
+  // coefficient parts formed once: fx = ((q * ds) * rdx) * uf
+  auto horiz = [&](int k, float (&h)[NS]) {
+    const int c = IXD(P, i, j, k);
+    auto qx = [&](int cc) {
+      return -0.125f * (rho[cc] + rho[cc - 1]) * ((khh[cc] + khh[cc - 1]) + (khh[cc + pxy] + khh[cc - 1 + pxy]));
+    };
+    auto qy = [&](int cc) {
+      return -0.125f * (rho[cc] + rho[cc - px]) * ((khh[cc] + khh[cc - px]) + (khh[cc + pxy] + khh[cc - px + pxy]));
+    };
+    const float qxw = qx(c), qxe = qx(c + 1), qys = qy(c), qyn = qy(c + px);
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      const float sc = sv(n, c, k);
+      const float fxw = qxw * (sc - sv(n, c - 1, k)) * P.rdx * P.uf[i + 2];
+      const float fxe = qxe * (sv(n, c + 1, k) - sc) * P.rdx * P.uf[i + 3];
+      const float fys = qys * (sc - sv(n, c - px, k)) * P.rdy * P.vf[j + 2];
+      const float fyn = qyn * (sv(n, c + px, k) - sc) * P.rdy * P.vf[j + 3];
+      const float tx = -(fxe - fxw) * P.rdx * P.uh[i + 2];
+      const float ty = -(fyn - fys) * P.rdy * P.vh[j + 2];
+      h[n] = tx + ty;
+    }
+  };
+  auto add = [&](int k, const float (&tz)[NS]) {
+    const int c = IXD(P, i, j, k);
+    float h[NS];
+    horiz(k, h);
+    const float r = rr[c];
+#pragma unroll
+    for (int n = 0; n < NS; n++) a.sten[n][c] = a.sten[n][c] + (h[n] + tz[n]) * r;
+  };
+  if (!a.doimpl) {
+    float lo[NS];
+#pragma unroll
+    for (int n = 0; n < NS; n++) lo[n] = (n == 0) ? bot0 : 0.0f;
+    for (int k = 1; k <= nk; k++) {
+      float tz[NS];
+      const int c = IXD(P, i, j, k), cu = c + pxy;
+      const float kv = (k < nk) ? khv[cu] : 0.0f, rfu = (k < nk) ? rf[cu] : 0.0f;
+#pragma unroll
+      for (int n = 0; n < NS; n++) {
+        float hi = 0.0f;
+        if (k < nk) hi = -kv * (sv(n, cu, k + 1) - sv(n, c, k)) * rdz * P.mf[k + 1] * rfu;
+        tz[n] = -(hi - lo[n]) * rdz * P.mh[k];
+        lo[n] = hi;
+      }
+      add(k, tz);
+    }
+    return;
+  }
+  const float rdt = 1.0f / a.dt;
+  const float tema = -1.0f * a.dt * VIALPHA * rdz * rdz;
+  const float temb = a.dt * VIBETA * rdz * rdz;
+  const float temc = a.dt * rdz;
+  float *d1 = a.d1;
+  float d1p = 0.0f, d2p[NS], sm[NS], s0[NS], sp[NS];
+  // sm, s0, sp: the scalar at k-1, k, k+1
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    s0[n] = sv(n, c1, 1);
+    sm[n] = 0.0f;
+  }
+  for (int k = 1; k <= nk - 1; k++) {
+    const int c = IXD(P, i, j, k);
+    const float rrc = rr[c];
+    // r2 = khv(k+1) mf(k+1) rf(k+1) mh(k) rr(k), r1 = khv(k) mf(k) rf(k) mh(k) rr(k)
+    const float r2 = khv[c + pxy] * P.mf[k + 1] * rf[c + pxy] * P.mh[k] * rrc;
+    const float cfc = tema * r2;
+#pragma unroll
+    for (int n = 0; n < NS; n++) sp[n] = sv(n, c + pxy, k + 1);
+    if (k == 1) {
+      const float cfb = 1.0f - cfc;
+      const float tem = 1.0f / cfb;
+      d1p = -cfc * tem;
+#pragma unroll
+      for (int n = 0; n < NS; n++) {
+        const float bot = (n == 0) ? bot0 : 0.0f;
+        const float cfd = s0[n] + temb * (-r2 * s0[n] + r2 * sp[n]) + temc * bot * P.mh[1] * rrc;
+        d2p[n] = cfd * tem;
+      }
+    } else {
+      const float r1 = khv[c] * P.mf[k] * rf[c] * P.mh[k] * rrc;
+      const float cfa = tema * r1;
+      const float cfb = 1.0f - cfa - cfc;
+      const float tem = 1.0f / (cfa * d1p + cfb);
+      d1p = -cfc * tem;
+#pragma unroll
+      for (int n = 0; n < NS; n++) {
+        const float cfd = s0[n] + temb * (r1 * sm[n] - (r1 + r2) * s0[n] + r2 * sp[n]);
+        d2p[n] = (cfd - cfa * d2p[n]) * tem;
+      }
+    }
+    d1[c] = d1p;
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      a.d2[n][c] = d2p[n];
+      sm[n] = s0[n];
+      s0[n] = sp[n];
+    }
+  }
+  float up[NS];
+  {
+    const int k = nk, c = IXD(P, i, j, k);
+    const float rrc = rr[c];
+    const float r1 = khv[c] * P.mf[k] * rf[c] * P.mh[k] * rrc;
+    const float cfa = tema * r1;
+    const float cfb = 1.0f - cfa;
+    const float tem = 1.0f / (cfa * d1p + cfb);
+    const float rh = rho[c];
+    float tz[NS];
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      // the top flux is 0 (bcturbs = 1)
+      const float cfd = s0[n] + temb * (r1 * sm[n] - r1 * s0[n]) - temc * 0.0f * P.mh[nk] * rrc;
+      up[n] = (cfd - cfa * d2p[n]) * tem;
+      tz[n] = rh * (up[n] - s0[n]) * rdt;
+    }
+    add(k, tz);
+  }
+  for (int k = nk - 1; k >= 1; k--) {
+    const int c = IXD(P, i, j, k);
+    const float dd = d1[c], rh = rho[c];
+    float tz[NS];
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      up[n] = dd * up[n] + a.d2[n][c];
+      tz[n] = rh * (up[n] - sv(n, c, k)) * rdt;
+    }
+    add(k, tz);
+  }
+}
+
 }  // namespace
 
 namespace {
@@ -720,16 +885,73 @@ void run_turbs(cm1g_model *m, const float *s, int theta, float *sten, int iflux,
   }
 }
 
+template <int NS>
+void turbs_multi_launch(cm1g_model *m, const TurbsMultiArgs &a) {
+  const KP P = m->P;
+  for2(m, 1, P.ni, 1, P.nj, [=] __device__(int i, int j) { turbs_multi_col<NS>(P, a, i, j); });
+}
+
+// turbs of ns scalars (ns <= TURBS_MAX) in one launch: s[n] into sten[n];
+// scalar 0 is theta (relative to th0r, with the surface flux when
+// dosfcflx) when theta0; d1 in dum1, the scalars' d2 in dum2..dum9
+void run_turbs_multi(cm1g_model *m, int ns, const float *const *s, float *const *sten, int theta0,
+                     int dosfcflx) {
+  const KP P = m->P;
+  float **f = m->f;
+  TurbsMultiArgs a = {};
+  for (int n = 0; n < ns; n++) {
+    a.s[n] = s[n];
+    a.sten[n] = sten[n];
+    a.d2[n] = f[CM1G_DUM2 + n];
+  }
+  a.th0 = P.th0;
+  a.th0r = P.K.th0r;
+  a.theta0 = theta0;
+  a.flux0 = theta0 && dosfcflx;
+  a.rho = f[CM1G_RHO];
+  a.rr = f[CM1G_RR];
+  a.rf = f[CM1G_RF];
+  a.khh = f[CM1G_KHH];
+  a.khv = f[CM1G_KHV];
+  a.sflux = m->d2[CM1G_D_THFLUX];
+  a.d1 = f[CM1G_DUM1];
+  a.doimpl = m->turb.doimpl;
+  a.dt = m->c.dt;
+  switch (ns) {
+    case 1: turbs_multi_launch<1>(m, a); break;
+    case 2: turbs_multi_launch<2>(m, a); break;
+    case 3: turbs_multi_launch<3>(m, a); break;
+    case 4: turbs_multi_launch<4>(m, a); break;
+    case 5: turbs_multi_launch<5>(m, a); break;
+    case 6: turbs_multi_launch<6>(m, a); break;
+    case 7: turbs_multi_launch<7>(m, a); break;
+    case 8: turbs_multi_launch<8>(m, a); break;
+    default: break;
+  }
+}
+
 }  // namespace
 
 void turb_tend(cm1g_model *m) {
   float **f = m->f;
   run_turbt(m, 1);
   run_turbuvw(m, 7);
-  // theta with the surface flux, then the tracers
+  // theta (iflux = 1: the surface flux) and the tracers (iflux = 0), up
+  // to eight scalars a launch
   const int dosfc = (m->turb.isfcflx == 1);
-  run_turbs(m, f[CM1G_THA], 1, f[CM1G_THTEN1], 1, dosfc);
-  for (int n = 0; n < m->c.npt; n++) run_turbs(m, f[CM1G_PTA0 + n], 0, f[CM1G_PTTEN0 + n], 0, dosfc);
+  const float *s[1 + 8];
+  float *sten[1 + 8];
+  int ns = 0;
+  s[ns] = f[CM1G_THA];
+  sten[ns++] = f[CM1G_THTEN1];
+  for (int n = 0; n < m->c.npt; n++) {
+    s[ns] = f[CM1G_PTA0 + n];
+    sten[ns++] = f[CM1G_PTTEN0 + n];
+  }
+  for (int n0 = 0; n0 < ns; n0 += TURBS_MAX) {
+    const int cnt = (ns - n0 < TURBS_MAX) ? ns - n0 : TURBS_MAX;
+    run_turbs_multi(m, cnt, s + n0, sten + n0, n0 == 0, dosfc);
+  }
 }
 
 // --------------------------------------------------------- TKE of a stage

@@ -20,9 +20,9 @@ def short(name):
     m = re.match(r"void k_one(?:_w)?<\d+, (?:\d+, )?(.*)>\(", name)
     if not m:
         return name.split("(")[0]
-    body = m.group(1)
+    body = m.group(1).replace("(anonymous namespace)::", "")
     # for3<host::{lambda ...#n}>(...)::{lambda#1}  or  host(...)::{lambda(...)#n}
-    f = re.match(r"(for[23])<([\w:]+?)(?:\(.*?\))?::\{lambda\(int, int(?:, int)?\)#(\d+)\}", body)
+    f = re.match(r"(for[23])<([\w:]+?(?:<\d+>)?)(?:\(.*?\))?::\{lambda\(int, int(?:, int)?\)#(\d+)\}", body)
     if f:
         return f"{f.group(1)}<{f.group(2).split('::')[-1]} λ{f.group(3)}>"
     t = re.match(r"(\w+)<([^>]*)>\(", body)

@@ -41,16 +41,20 @@ def test_cbl_matches_oracle(require_gpu):
         assert g.interior(f"pta{t}").min() >= 0.0
 
 
-def test_cbl_les_matches_oracle(require_gpu):
+@pytest.mark.parametrize("doimpl", [1, 0])
+def test_cbl_les_matches_oracle(require_gpu, doimpl):
     """The same reduced CBL with the workload's subgrid turbulence (TKE
-    scheme, implicit vertical diffusion, semi-slip surface, 0.24 K m/s
-    surface heat flux) and an initial TKE of up to 0.6 m^2/s^2 below 600 m,
-    so that the TKE's length scale, production, transport and advection all
-    act from the first step: the GPU equals the oracle bit for bit after 12
-    steps, the TKE and eddy viscosities included."""
+    scheme, semi-slip surface, 0.24 K m/s surface heat flux; implicit
+    vertical diffusion as in the workload, and explicit) and an initial TKE
+    of up to 0.6 m^2/s^2 below 600 m, so that the TKE's length scale,
+    production, transport and advection all act from the first step: the
+    GPU equals the oracle bit for bit after 12 steps, the TKE, the eddy
+    viscosities and the six tracers (diffused with theta in one pass)
+    included."""
     from cm1_amd.workloads import CBL_TURB
     cfgd, cols, fields = cbl_case()
     cfgd.update(CBL_TURB)
+    cfgd["doimpl"] = doimpl
     nk, nj, ni = cfgd["nk"], cfgd["nj"], cfgd["ni"]
     tke = hinit.blank(ni, nj, nk)
     zf = cols["zf"]
