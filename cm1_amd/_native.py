@@ -48,6 +48,9 @@ OUT = {n: 1 << i for i, n in enumerate(
     ("rain", "zh", "th", "thpert", "prs", "prspert", "pi", "pipert", "rho", "rhopert", "qv",
      "qvpert", "q", "buoyancy", "uinterp", "vinterp", "winterp", "basestate", "u", "upert", "v",
      "vpert", "w"))}
+# the subgrid turbulence's output switches (need the turbulence set); not
+# part of the default "all"
+OUT_TURB = {n: 1 << (23 + i) for i, n in enumerate(("tke", "km", "kh", "dissten", "nm", "def"))}
 # statistics switches of cm1g_statpack (the reference's stat_* names)
 STAT = {n: 1 << i for i, n in enumerate(
     ("w", "wlevs", "u", "v", "pipert", "prspert", "thpert", "q", "sfcprs", "wsp", "cfl", "tmass",

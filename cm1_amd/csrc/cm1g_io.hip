@@ -30,7 +30,8 @@ enum OutKind {
   OK_AVG_J,      // 0.5*(a(i,j,k)+a(i,j+1,k))   vinterp
   OK_AVG_K,      // 0.5*(a(i,j,k)+a(i,j,k+1))   winterp
   OK_BUOY,       // g*th'/th0 + g*repsm1*(qv-qv0) - g*qc - g*qr
-  OK_2D          // a 2-D field (rain, prate)
+  OK_2D,         // a 2-D field (rain, prate)
+  OK_ZERO        // 0 (epst outside the reference's domain diagnostics)
 };
 
 struct OutVar {
@@ -123,6 +124,27 @@ std::vector<OutVar> out_list(const cm1g_model *m, uint32_t mask) {
   if (on(CM1G_OUT_BASESTATE))
     v.push_back({"v0", "base-state v (grid-relative)", "m/s", 'v', OK_COL, 0, CM1G_K_V0});
   if (on(CM1G_OUT_W)) v.push_back({"w", "vertical velocity", "m/s", 'w', OK_FIELD, CM1G_WA, 0});
+  // the subgrid turbulence (src/writeout.F:7788-7890, 8125-8205)
+  if (on(CM1G_OUT_TKE))
+    v.push_back({"tke", "subgrid turbulence kinetic energy", "m^2/s^2", 'w', OK_FIELD, CM1G_TKEA, 0});
+  if (on(CM1G_OUT_KM)) {
+    v.push_back({"kmh", "horizontal eddy viscosity for momentum", "m^2/s", 'w', OK_FIELD, CM1G_KMH, 0});
+    v.push_back({"kmv", "vertical eddy viscosity for momentum", "m^2/s", 'w', OK_FIELD, CM1G_KMV, 0});
+  }
+  if (on(CM1G_OUT_KH)) {
+    v.push_back({"khh", "horizontal eddy diffusivity for scalars", "m^2/s", 'w', OK_FIELD, CM1G_KHH, 0});
+    v.push_back({"khv", "vertical eddy diffusivity for scalars", "m^2/s", 'w', OK_FIELD, CM1G_KHV, 0});
+  }
+  if (on(CM1G_OUT_DISSTEN)) {
+    v.push_back({"lenscl", "lenscl", "m", 'w', OK_FIELD, CM1G_LENSCL, 0});
+    v.push_back({"dissten", "dissipation rate", "m^2/s^3", 'w', OK_FIELD, CM1G_DISSTEN, 0});
+  }
+  if (on(CM1G_OUT_NM)) v.push_back({"nm", "squared Brunt-Vaisala frequency", "1/s^2", 'w', OK_FIELD, CM1G_NM, 0});
+  if (on(CM1G_OUT_DEF)) {
+    v.push_back({"defv", "vertical deformation", "1/s^2", 'w', OK_FIELD, CM1G_DEFV, 0});
+    v.push_back({"defh", "horizontal deformation", "1/s^2", 'w', OK_FIELD, CM1G_DEFH, 0});
+    v.push_back({"epst", "epst", "1/s^2", 'w', OK_ZERO, 0, 0});
+  }
   return v;
 }
 
@@ -131,7 +153,8 @@ void pack(cm1g_model *m, const OutVar &ov, float *out) {
   const KP P = m->P;
   const Shape s = shape_of(m, ov.grid);
   const int kind = ov.kind, nx = s.nx, ny = s.ny;
-  const float *a = ov.kind == OK_2D ? m->d2[ov.fld] : (ov.kind == OK_COL ? nullptr : m->f[ov.fld]);
+  const float *a = ov.kind == OK_2D ? m->d2[ov.fld]
+                                    : ((ov.kind == OK_COL || ov.kind == OK_ZERO) ? nullptr : m->f[ov.fld]);
   const float *col = m->kf[ov.col];
   const float *qv = m->f[CM1G_QA0], *qc = m->f[CM1G_QA1], *qr = m->f[CM1G_QA2];
   const bool moist = m->c.imoist == 1;
@@ -157,6 +180,7 @@ void pack(cm1g_model *m, const OutVar &ov, float *out) {
         }
         break;
       }
+      case OK_ZERO: r = 0.0f; break;
       default: r = a[IX2D(P, i, j)]; break;
     }
     out[o] = r;
@@ -311,7 +335,10 @@ void io_free(cm1g_model *m) {
 extern "C" int cm1g_write_grads(cm1g_model *m, const char *dir, int nwrite, uint32_t mask, int iorigin,
                                 int stretch_z, double mtime, int nstep) {
   ARGCHK(m && dir && nwrite >= 1, "cm1g_write_grads: null argument or nwrite < 1");
-  ARGCHK((mask & ~(uint32_t)CM1G_OUT_ALL) == 0, "cm1g_write_grads: unknown output switch in mask");
+  ARGCHK((mask & ~(uint32_t)(CM1G_OUT_ALL | CM1G_OUT_TURB)) == 0,
+         "cm1g_write_grads: unknown output switch in mask");
+  ARGCHK(m->turb.cm1setup == 1 || (mask & CM1G_OUT_TURB) == 0,
+         "cm1g_write_grads: turbulence output needs the subgrid turbulence (cm1g_set_turbulence)");
   ARGCHK(iorigin == 1 || iorigin == 2, "cm1g_write_grads: iorigin must be 1 or 2");
   ARGCHK(m->c.imoist == 1 || (mask & (CM1G_OUT_QV | CM1G_OUT_QVPERT)) == 0,
          "cm1g_write_grads: qv output needs imoist = 1");

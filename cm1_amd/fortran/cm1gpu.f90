@@ -57,6 +57,12 @@ module cm1gpu
   integer(c_int32_t), parameter :: CM1G_OUT_VPERT     = 2097152
   integer(c_int32_t), parameter :: CM1G_OUT_W         = 4194304
   integer(c_int32_t), parameter :: CM1G_OUT_ALL       = 8388607
+  integer(c_int32_t), parameter :: CM1G_OUT_TKE       = 8388608
+  integer(c_int32_t), parameter :: CM1G_OUT_KM        = 16777216
+  integer(c_int32_t), parameter :: CM1G_OUT_KH        = 33554432
+  integer(c_int32_t), parameter :: CM1G_OUT_DISSTEN   = 67108864
+  integer(c_int32_t), parameter :: CM1G_OUT_NM        = 134217728
+  integer(c_int32_t), parameter :: CM1G_OUT_DEF       = 268435456
   ! statistics switches of cm1g_statpack (the namelist's stat_* names)
   integer(c_int32_t), parameter :: CM1G_STAT_W       = 1
   integer(c_int32_t), parameter :: CM1G_STAT_WLEVS   = 2

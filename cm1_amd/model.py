@@ -161,11 +161,12 @@ class Model:
     def write_grads(self, directory, nwrite, switches=None, iorigin=2, stretch_z=0, mtime=0.0,
                     nstep=0):
         """GrADS output time ``nwrite`` of the variables named in ``switches``
-        (the reference's output_* names, default all) into ``directory``."""
+        (the reference's output_* names, default all but the subgrid
+        turbulence's) into ``directory``."""
         names = N.OUT if switches is None else switches
         mask = 0
         for n in names:
-            mask |= N.OUT[n]
+            mask |= N.OUT[n] if n in N.OUT else N.OUT_TURB[n]
         N.check(self.L.cm1g_write_grads(self.h, os.fsencode(directory), int(nwrite), mask,
                                         int(iorigin), int(stretch_z), float(mtime), int(nstep)),
                 "cm1g_write_grads")

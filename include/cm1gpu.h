@@ -474,7 +474,18 @@ enum {
   CM1G_OUT_V = 1u << 20,
   CM1G_OUT_VPERT = 1u << 21,
   CM1G_OUT_W = 1u << 22,
-  CM1G_OUT_ALL = (1u << 23) - 1u
+  CM1G_OUT_ALL = (1u << 23) - 1u,  /* every switch above (no subgrid turbulence needed) */
+  /* the subgrid turbulence's fields, after w (src/writeout.F:7788-8200);
+   * they need cm1g_set_turbulence, and hold what cm1g_sfc_and_turb formed
+   * last (the reference writes after its end-of-step sfc_and_turb) */
+  CM1G_OUT_TKE = 1u << 23,         /* tke */
+  CM1G_OUT_KM = 1u << 24,          /* kmh, kmv */
+  CM1G_OUT_KH = 1u << 25,          /* khh, khv */
+  CM1G_OUT_DISSTEN = 1u << 26,     /* lenscl, dissten */
+  CM1G_OUT_NM = 1u << 27,          /* nm */
+  CM1G_OUT_DEF = 1u << 28,         /* defv, defh, epst (0: the reference forms epst
+                                      only for its domain diagnostics) */
+  CM1G_OUT_TURB = CM1G_OUT_TKE | CM1G_OUT_KM | CM1G_OUT_KH | CM1G_OUT_DISSTEN | CM1G_OUT_NM | CM1G_OUT_DEF
 };
 int cm1g_write_grads(cm1g_model *m, const char *dir, int nwrite, uint32_t mask, int iorigin,
                      int stretch_z, double mtime, int nstep);

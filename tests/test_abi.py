@@ -77,11 +77,13 @@ def test_fortran_module_compiles(tmp_path):
 
 
 def test_output_switches_match_header():
-    """CM1G_OUT_* bits of the header, the ctypes mirror and the Fortran module."""
+    """CM1G_OUT_* bits of the header, the ctypes mirror and the Fortran module
+    (the subgrid turbulence's switches included)."""
     from cm1_amd import _native
     text = open(HEADER).read()
     hdr = {n.lower(): 1 << int(b) for n, b in re.findall(r"CM1G_OUT_(\w+) = 1u << (\d+)", text)}
-    assert hdr == _native.OUT
+    assert hdr == {**_native.OUT, **_native.OUT_TURB}
+    assert not set(_native.OUT) & set(_native.OUT_TURB)
     f90 = open(os.path.join(ROOT, "cm1_amd", "fortran", "cm1gpu.f90")).read()
     fort = {n.lower(): int(v) for n, v in re.findall(r"CM1G_OUT_(\w+?)\s*= (\d+)", f90) if n != "ALL"}
     assert fort == hdr

@@ -28,7 +28,7 @@ import os
 import numpy as np
 import pytest
 
-from common import load_fixture, make_gpu, nsound_for
+from common import load_fixture, load_turb_fixture, make_gpu, nsound_for
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -67,24 +67,7 @@ def test_grads_output_equals_the_reference(require_gpu, tmp_path, case, state, s
     rec.append(m.statpack(out, 3, stats, mtime=nsteps * dt))
     m.write_grads(out, 2, switches, iorigin=2, stretch_z=stretch, mtime=nsteps * dt, nstep=nsteps)
     print(f"{case}: {nsteps} steps, output written to {out}", flush=True)
-
-    recl = dict(s=ni * nj * 4, u=(ni + 1) * nj * 4, v=ni * (nj + 1) * 4, w=ni * nj * 4)
-    for f, r in recl.items():
-        ref_ctl = g[f"{f}_ctl"].tobytes()
-        got_ctl = open(os.path.join(out, f"cm1out_{f}.ctl"), "rb").read()
-        assert got_ctl == ref_ctl, (f, got_ctl.decode(), ref_ctl.decode())
-        data = open(os.path.join(out, f"cm1out_{f}.dat"), "rb").read()
-        ref = [d.decode() for d in g[f"{f}_digest"]]
-        assert len(data) == r * len(ref), (f, len(data), r * len(ref))
-        names = ctl_records(ref_ctl.decode())
-        bad = []
-        for n, want in enumerate(ref):
-            if hashlib.sha256(data[n * r:(n + 1) * r]).hexdigest() != want:
-                bad.append((n // len(names) + 1,) + names[n % len(names)])
-        print(f"cm1out_{f}.dat: {len(ref)} records, {len(bad)} differ", flush=True)
-        assert not bad, (f, bad[:20])
-    assert open(os.path.join(out, "cm1out_metadata.ctl"), "rb").read() == g["metadata_ctl"].tobytes()
-    assert open(os.path.join(out, "cm1out_metadata.dat"), "rb").read() == g["metadata_dat"].tobytes()
+    compare_files(g, out, ni, nj)
     # the 2-D records of the last time as values (rain has reached the
     # ground in the 300-step case)
     s = np.fromfile(os.path.join(out, "cm1out_s.dat"), dtype="<f4")
@@ -108,3 +91,53 @@ def test_grads_output_equals_the_reference(require_gpu, tmp_path, case, state, s
     assert open(os.path.join(out, "cm1out_stats.ctl"), "rb").read() == g["stats_ctl"].tobytes()
 
     m.close()
+
+
+def test_les_grads_output_equals_the_reference(require_gpu, tmp_path):
+    """The dry LES (TKE scheme, implicit vertical diffusion, semi-slip
+    surface, 0.24 K m/s heat flux) from the reference's t=0 state: output at
+    t=0 and after 400 steps, the TKE active, with the subgrid turbulence's
+    switches on (tke, kmh, kmv, khh, khv, lenscl, dissten, nm, defv, defh,
+    epst) besides the dry ones.  As the reference writes after its
+    end-of-step sfc_and_turb, cm1g_sfc_and_turb runs before each write.
+    Every record, the descriptors and the metadata equal the reference's."""
+    g = np.load(os.path.join(GOLD, "grads_les_16x12x24_400.npz"))
+    ni, nj, nk, nsteps, ntimes = [int(v) for v in g["meta"]]
+    dt = float(g["dt"][0])
+    switches = [s.decode() for s in g["switches"]]
+    cfgd, cols, fields, n_ref, _, _ = load_turb_fixture(os.path.join(GOLD, "ref_les_tke_impl1_16x12x24_400.npz"))
+    assert (cfgd["ni"], cfgd["nj"], cfgd["nk"], cfgd["dt"], n_ref) == (ni, nj, nk, dt, nsteps)
+    m = make_gpu(cfgd, cols, fields)
+    out = str(tmp_path)
+    m.sfc_and_turb()
+    m.write_grads(out, 1, switches, iorigin=2, stretch_z=0, mtime=0.0, nstep=0)
+    m.step(nsteps)
+    m.sfc_and_turb()
+    m.write_grads(out, 2, switches, iorigin=2, stretch_z=0, mtime=nsteps * dt, nstep=nsteps)
+    print(f"LES: {nsteps} steps, output written to {out}", flush=True)
+    compare_files(g, out, ni, nj)
+    w = np.fromfile(os.path.join(out, "cm1out_w.dat"), dtype="<f4").reshape(ntimes, -1, nk + 1, nj, ni)
+    assert float(w[1, 1].max()) > 0.05 and np.array_equal(w[1, 1], g["tke"])
+    m.close()
+
+
+def compare_files(g, out, ni, nj):
+    """cm1out_{s,u,v,w}.ctl and the metadata files byte for byte; every
+    record of cm1out_{s,u,v,w}.dat against the reference's digest."""
+    recl = dict(s=ni * nj * 4, u=(ni + 1) * nj * 4, v=ni * (nj + 1) * 4, w=ni * nj * 4)
+    for f, r in recl.items():
+        ref_ctl = g[f"{f}_ctl"].tobytes()
+        got_ctl = open(os.path.join(out, f"cm1out_{f}.ctl"), "rb").read()
+        assert got_ctl == ref_ctl, (f, got_ctl.decode(), ref_ctl.decode())
+        data = open(os.path.join(out, f"cm1out_{f}.dat"), "rb").read()
+        ref = [d.decode() for d in g[f"{f}_digest"]]
+        assert len(data) == r * len(ref), (f, len(data), r * len(ref))
+        names = ctl_records(ref_ctl.decode())
+        bad = []
+        for n, want in enumerate(ref):
+            if hashlib.sha256(data[n * r:(n + 1) * r]).hexdigest() != want:
+                bad.append((n // len(names) + 1,) + names[n % len(names)])
+        print(f"cm1out_{f}.dat: {len(ref)} records, {len(bad)} differ", flush=True)
+        assert not bad, (f, bad[:20])
+    assert open(os.path.join(out, "cm1out_metadata.ctl"), "rb").read() == g["metadata_ctl"].tobytes()
+    assert open(os.path.join(out, "cm1out_metadata.dat"), "rb").read() == g["metadata_dat"].tobytes()
