@@ -55,6 +55,9 @@ OUT_TURB = {n: 1 << (23 + i) for i, n in enumerate(("tke", "km", "kh", "dissten"
 STAT = {n: 1 << i for i, n in enumerate(
     ("w", "wlevs", "u", "v", "pipert", "prspert", "thpert", "q", "sfcprs", "wsp", "cfl", "tmass",
      "tmois", "qmass"))}
+# the subgrid turbulence's statistics switches (need the turbulence set);
+# not part of the default "all"
+STAT_TURB = {n: 1 << (14 + i) for i, n in enumerate(("tke", "km", "kh"))}
 STAT_MAXN = 128
 
 

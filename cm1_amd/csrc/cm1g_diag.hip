@@ -364,7 +364,15 @@ std::string fpad(const std::string &s, size_t w) {
 extern "C" int cm1g_statpack(cm1g_model *m, const char *dir, int nrec, uint32_t mask, double mtime,
                              float *rstat, int32_t *nstat) {
   ARGCHK(m && dir && nrec >= 1, "cm1g_statpack: null argument or nrec < 1");
-  ARGCHK((mask & ~(uint32_t)CM1G_STAT_ALL) == 0, "cm1g_statpack: unknown statistics switch in mask");
+  ARGCHK((mask & ~(uint32_t)(CM1G_STAT_ALL | CM1G_STAT_TURB)) == 0,
+         "cm1g_statpack: unknown statistics switch in mask");
+  ARGCHK(m->turb.cm1setup == 1 || (mask & CM1G_STAT_TURB) == 0,
+         "cm1g_statpack: turbulence statistics need the subgrid turbulence (cm1g_set_turbulence)");
+  // with the semi-slip surface the reference's stat_wsp adds the 10 m wind
+  // and its stat_cfl the diffusion numbers (src/statpack.F:453-461, calccfl),
+  // which are not formed here
+  ARGCHK(m->turb.cm1setup != 1 || (mask & (CM1G_STAT_WSP | CM1G_STAT_CFL)) == 0,
+         "cm1g_statpack: stat_wsp and stat_cfl with the subgrid turbulence are not supported");
   const KP P = m->P;
   const int ni = P.ni, nj = P.nj, nk = P.nk;
   ARGCHK((int64_t)(ni + 1) * (nj + 1) * (nk + 1) < 0xffffffffLL, "cm1g_statpack: grid too large");
@@ -484,6 +492,27 @@ extern "C" int cm1g_statpack(cm1g_model *m, const char *dir, int nrec, uint32_t 
          "minimum surface precipitation rate", "kg/m2/s",
          [=] __device__(int i, int j, int) { return prate[IX2D(P, i, j)]; });
   }
+  // the subgrid turbulence (src/statpack.F:258-278): w-level fields, nk+1 levels
+  auto wpair = [&](int id, const char *nmax, const char *dmax, const char *nmin, const char *dmin,
+                   const char *unit) {
+    const float *a = m->f[id];
+    pair(ni, nj, nk + 1, nmax, dmax, nmin, dmin, unit,
+         [=] __device__(int i, int j, int k) { return a[IXD(P, i, j, k)]; });
+  };
+  if (on(CM1G_STAT_TKE))
+    wpair(CM1G_TKEA, "tkemax", "max subgrid tke", "tkemin", "min subgrid tke", "m^2/s^2");
+  if (on(CM1G_STAT_KM)) {
+    wpair(CM1G_KMH, "kmhmax", "max horiz eddy viscosity for momentum", "kmhmin",
+          "min horiz eddy viscosity for momentum", "m^2/s");
+    wpair(CM1G_KMV, "kmvmax", "max vert eddy viscosity for momentum", "kmvmin",
+          "min vert eddy viscosity for momentum", "m^2/s");
+  }
+  if (on(CM1G_STAT_KH)) {
+    wpair(CM1G_KHH, "khhmax", "max horiz eddy diffusivity for scalars", "khhmin",
+          "min horiz eddy diffusivity for scalars", "m^2/s");
+    wpair(CM1G_KHV, "khvmax", "max vert eddy diffusivity for scalars", "khvmin",
+          "min vert eddy diffusivity for scalars", "m^2/s");
+  }
   for (int n = 0; n < m->c.npt; n++) {  // passive tracers (iptra = 1)
     const float *pt = m->f[CM1G_PTA0 + n];
     const std::string s = std::to_string(n + 1);
@@ -512,6 +541,15 @@ extern "C" int cm1g_statpack(cm1g_model *m, const char *dir, int nrec, uint32_t 
     E.push_back({"zwspmin", "level of min grid-rel horiz wind speed", "m AGL", 3, 2 * p + 1});
     pair(ni, nj, 1, "swspmax", "max grid-rel horiz wind speed at l.m.l.", "swspmin",
          "min grid-rel horiz wind speed at l.m.l.", "m/s", [=] __device__(int i, int j, int) { return wsp(i, j, 1); });
+  }
+  if (m->turb.cm1setup == 1 && m->turb.bbc == 3) {
+    // with a surface model (bbc = 3, sfcmodel = 1) the record always holds
+    // the extremes of the diagnosed PBL depth (src/statpack.F:465-467);
+    // without a PBL scheme and outside the reference's LES test cases
+    // (testcase 1-7, src/turb.F:563-583) hpbl keeps the 100 m it is
+    // allocated with (src/cm1.F:2004)
+    pair(ni, nj, 1, "hpblmax", "max diagnosed pbl depth", "hpblmin", "min diagnosed pbl depth", "m",
+         [=] __device__(int, int, int) { return 100.0f; });
   }
   if (on(CM1G_STAT_CFL)) {
     // calccfl (src/misclibs.F:2584-2718) for nx > 1 and ny > 1: only the

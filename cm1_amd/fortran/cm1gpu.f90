@@ -79,6 +79,9 @@ module cm1gpu
   integer(c_int32_t), parameter :: CM1G_STAT_TMOIS   = 4096
   integer(c_int32_t), parameter :: CM1G_STAT_QMASS   = 8192
   integer(c_int32_t), parameter :: CM1G_STAT_ALL    = 16383
+  integer(c_int32_t), parameter :: CM1G_STAT_TKE     = 16384
+  integer(c_int32_t), parameter :: CM1G_STAT_KM      = 32768
+  integer(c_int32_t), parameter :: CM1G_STAT_KH      = 65536
   integer(c_int), parameter :: CM1G_STAT_MAXN = 128
   ! staggering of host arrays
   integer(c_int), parameter :: CM1G_STAG_S = 0, CM1G_STAG_U = 1, CM1G_STAG_V = 2, &

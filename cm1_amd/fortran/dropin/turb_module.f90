@@ -212,6 +212,8 @@ contains
     endif
     call dg_need(imoist == 0, 'sfc_and_turb: moist subgrid turbulence is not on the GPU path')
     call dg_need(imove == 0 .and. pertflx == 0, 'sfc_and_turb: imove or pertflx with turbulence')
+    call dg_need(testcase < 1 .or. testcase > 7, &
+                 'sfc_and_turb: the PBL depth of the LES test cases (testcase 1-7) is not on the GPU path')
     call dg_need(.not. dotdwrite .or. .not. dodomaindiag, &
                  'sfc_and_turb: the turbulence diagnostics of domaindiag are not on the GPU path')
     m = dg_model()

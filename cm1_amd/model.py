@@ -178,7 +178,7 @@ class Model:
         names = N.STAT if switches is None else switches
         mask = 0
         for n in names:
-            mask |= N.STAT[n]
+            mask |= N.STAT[n] if n in N.STAT else N.STAT_TURB[n]
         r = (C.c_float * N.STAT_MAXN)()
         n = C.c_int32()
         N.check(self.L.cm1g_statpack(self.h, os.fsencode(directory), int(nrec), mask, float(mtime), r,

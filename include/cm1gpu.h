@@ -519,7 +519,13 @@ enum {
   CM1G_STAT_TMASS = 1u << 11,   /* tmass */
   CM1G_STAT_TMOIS = 1u << 12,   /* tmois */
   CM1G_STAT_QMASS = 1u << 13,   /* massqv, massqc, massqr */
-  CM1G_STAT_ALL = (1u << 14) - 1u
+  CM1G_STAT_ALL = (1u << 14) - 1u,  /* every switch above (no subgrid turbulence needed) */
+  /* the subgrid turbulence's, after the q block (src/statpack.F:258-278);
+   * they need cm1g_set_turbulence */
+  CM1G_STAT_TKE = 1u << 14,     /* tkemax, tkemin */
+  CM1G_STAT_KM = 1u << 15,      /* kmhmax, kmhmin, kmvmax, kmvmin */
+  CM1G_STAT_KH = 1u << 16,      /* khhmax, khhmin, khvmax, khvmin */
+  CM1G_STAT_TURB = CM1G_STAT_TKE | CM1G_STAT_KM | CM1G_STAT_KH
 };
 #define CM1G_STAT_MAXPAIRS 48
 #define CM1G_STAT_MAXN 128

@@ -16,8 +16,8 @@ statpack supports turned on:
 * ``grads_les_16x12x24_400``: the dry LES of ``ref_les_tke_impl1_16x12x24_400.npz``
   (TKE scheme, implicit vertical diffusion, semi-slip surface, 0.24 K m/s
   heat flux) over 400 steps, the TKE active over the last ~100, with the
-  subgrid turbulence's output switches (tke, km, kh, dissten, nm, def) on
-  besides the dry ones (no statistics).
+  subgrid turbulence's output and statistics switches (output: tke, km,
+  kh, dissten, nm, def; statistics: tke, km, kh) on besides the dry ones.
 
 Each fixture stores:
 
@@ -101,6 +101,9 @@ def make(name, ni, nj, nk, nsteps, dt, dx, stretch=None, workdir="/tmp/cm1_grads
 LES_SWITCHES = ("zh", "basestate", "th", "thpert", "prs", "prspert", "pi", "pipert", "rho",
                 "rhopert", "buoyancy", "u", "upert", "uinterp", "v", "vpert", "vinterp", "w",
                 "winterp", "tke", "km", "kh", "dissten", "nm", "def")
+# (stat_wsp and stat_cfl are left out: with a surface model and a subgrid
+# model the reference adds wsp10/hpbl and the diffusion numbers ksh/ksv)
+LES_STATS = ("w", "u", "v", "pipert", "prspert", "thpert", "sfcprs", "tmass", "tke", "km", "kh")
 
 
 def make_les(name, state, workdir="/tmp/cm1_grads_fixture_les", outdir=HERE):
@@ -113,24 +116,28 @@ def make_les(name, state, workdir="/tmp/cm1_grads_fixture_les", outdir=HERE):
     psolver, advwenos, _, _, npt, doimpl = [int(v) for v in z["opts"]]
     dx, _, dz = [float(v) for v in z["grid"]]
     ov = dict(nx=ni, ny=nj, nz=nk, dtl=dt, timax=nsteps * dt, tapfrq=nsteps * dt,
+              statfrq=nsteps * dt / 2,
               wbc=1, ebc=1, sbc=1, nbc=1, dx=dx, dy=dx, dz=dz, psolver=psolver,
               advwenos=advwenos, advwenov=0, irdamp=0, doimpl=doimpl, iptra=1 if npt else 0,
               npt=max(npt, 1), **TURB_NAMELIST)
     ov.update({"output_" + k: 0 for k in ("rain", "qv", "q")})
     ov.update({"output_" + k: 1 for k in LES_SWITCHES})
+    ov.update({"stat_" + k: 1 for k in LES_STATS})
     refrun.run_reference(workdir, ov, threads=4, tables=("LANDUSE.TBL",))
     S = refrun.read_grads(workdir, "cm1out_s.ctl")
     # the same t = 0 state as the state fixture
     assert np.array_equal(S["thpert"][0], z["t0_th"]), "t = 0 state differs from " + state
     out = dict(meta=np.array([ni, nj, nk, nsteps, 2], np.int32), dt=np.array([dt], np.float32),
                switches=np.array([s.encode() for s in LES_SWITCHES], dtype="S16"),
-               stats=np.array([], dtype="S16"))
+               stats=np.array([s.encode() for s in LES_STATS], dtype="S16"))
     recl = dict(s=ni * nj * 4, u=(ni + 1) * nj * 4, v=ni * (nj + 1) * 4, w=ni * nj * 4)
     for f, r in recl.items():
         out[f"{f}_digest"] = digests(os.path.join(workdir, f"cm1out_{f}.dat"), r)
         out[f"{f}_ctl"] = text(os.path.join(workdir, f"cm1out_{f}.ctl"))
     out["metadata_ctl"] = text(os.path.join(workdir, "cm1out_metadata.ctl"))
     out["metadata_dat"] = text(os.path.join(workdir, "cm1out_metadata.dat"))
+    out["stats_ctl"] = text(os.path.join(workdir, "cm1out_stats.ctl"))
+    out["stats_dat"] = text(os.path.join(workdir, "cm1out_stats.dat"))
     W = refrun.read_grads(workdir, "cm1out_w.ctl")
     for k in ("tke", "kmh", "dissten"):
         out[k] = W[k][1]

@@ -94,7 +94,8 @@ def test_statistics_switches_match_header():
     from cm1_amd import _native
     text = open(HEADER).read()
     hdr = {n.lower(): 1 << int(b) for n, b in re.findall(r"CM1G_STAT_(\w+) = 1u << (\d+)", text)}
-    assert hdr == _native.STAT
+    assert hdr == {**_native.STAT, **_native.STAT_TURB}
+    assert not set(_native.STAT) & set(_native.STAT_TURB)
     f90 = open(os.path.join(ROOT, "cm1_amd", "fortran", "cm1gpu.f90")).read()
     fort = {n.lower(): int(v) for n, v in re.findall(r"CM1G_STAT_(\w+?)\s*= (\d+)", f90)
             if n not in ("ALL", "MAXN")}

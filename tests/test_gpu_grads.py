@@ -77,7 +77,50 @@ def test_grads_output_equals_the_reference(require_gpu, tmp_path, case, state, s
     assert np.array_equal(rain, g["rain"]) and np.array_equal(prate, g["prate"])
     if nsteps >= 300:
         assert g["rain"].max() > 0.0 and g["prate"].max() > 0.0
-    # statistics: values and file against the reference's
+    compare_stats(g, out, rec)
+    m.close()
+
+
+def test_les_grads_output_equals_the_reference(require_gpu, tmp_path):
+    """The dry LES (TKE scheme, implicit vertical diffusion, semi-slip
+    surface, 0.24 K m/s heat flux) from the reference's t=0 state: output at
+    t=0 and after 400 steps, the TKE active, with the subgrid turbulence's
+    switches on (tke, kmh, kmv, khh, khv, lenscl, dissten, nm, defv, defh,
+    epst) besides the dry ones, and statistics at t=0, 200 and 400 steps
+    with the turbulence's tkemax/min, kmh/kmv and khh/khv extremes.  As the
+    reference writes after its end-of-step sfc_and_turb, cm1g_sfc_and_turb
+    runs before each write.  Every record, the statistics, the descriptors
+    and the metadata equal the reference's."""
+    g = np.load(os.path.join(GOLD, "grads_les_16x12x24_400.npz"))
+    ni, nj, nk, nsteps, ntimes = [int(v) for v in g["meta"]]
+    dt = float(g["dt"][0])
+    switches = [s.decode() for s in g["switches"]]
+    cfgd, cols, fields, n_ref, _, _ = load_turb_fixture(os.path.join(GOLD, "ref_les_tke_impl1_16x12x24_400.npz"))
+    assert (cfgd["ni"], cfgd["nj"], cfgd["nk"], cfgd["dt"], n_ref) == (ni, nj, nk, dt, nsteps)
+    stats = [s.decode() for s in g["stats"]]
+    m = make_gpu(cfgd, cols, fields)
+    out = str(tmp_path)
+    m.sfc_and_turb()
+    m.write_grads(out, 1, switches, iorigin=2, stretch_z=0, mtime=0.0, nstep=0)
+    rec = [m.statpack(out, 1, stats, mtime=0.0)]
+    m.step(nsteps // 2)
+    m.sfc_and_turb()
+    rec.append(m.statpack(out, 2, stats, mtime=(nsteps // 2) * dt))
+    m.step(nsteps - nsteps // 2)
+    m.sfc_and_turb()
+    rec.append(m.statpack(out, 3, stats, mtime=nsteps * dt))
+    m.write_grads(out, 2, switches, iorigin=2, stretch_z=0, mtime=nsteps * dt, nstep=nsteps)
+    print(f"LES: {nsteps} steps, output written to {out}", flush=True)
+    compare_files(g, out, ni, nj)
+    compare_stats(g, out, rec)
+    w = np.fromfile(os.path.join(out, "cm1out_w.dat"), dtype="<f4").reshape(ntimes, -1, nk + 1, nj, ni)
+    assert float(w[1, 1].max()) > 0.05 and np.array_equal(w[1, 1], g["tke"])
+    m.close()
+
+
+def compare_stats(g, out, rec):
+    """The three statistics records (values) and cm1out_stats.dat/.ctl
+    against the reference's."""
     ref_stats = np.frombuffer(g["stats_dat"].tobytes(), dtype="<f4").reshape(3, -1)
     names = ctl_records(g["stats_ctl"].tobytes().decode())
     got = np.array(rec)
@@ -89,36 +132,6 @@ def test_grads_output_equals_the_reference(require_gpu, tmp_path, case, state, s
     assert not bad, bad[:20]
     assert open(os.path.join(out, "cm1out_stats.dat"), "rb").read() == g["stats_dat"].tobytes()
     assert open(os.path.join(out, "cm1out_stats.ctl"), "rb").read() == g["stats_ctl"].tobytes()
-
-    m.close()
-
-
-def test_les_grads_output_equals_the_reference(require_gpu, tmp_path):
-    """The dry LES (TKE scheme, implicit vertical diffusion, semi-slip
-    surface, 0.24 K m/s heat flux) from the reference's t=0 state: output at
-    t=0 and after 400 steps, the TKE active, with the subgrid turbulence's
-    switches on (tke, kmh, kmv, khh, khv, lenscl, dissten, nm, defv, defh,
-    epst) besides the dry ones.  As the reference writes after its
-    end-of-step sfc_and_turb, cm1g_sfc_and_turb runs before each write.
-    Every record, the descriptors and the metadata equal the reference's."""
-    g = np.load(os.path.join(GOLD, "grads_les_16x12x24_400.npz"))
-    ni, nj, nk, nsteps, ntimes = [int(v) for v in g["meta"]]
-    dt = float(g["dt"][0])
-    switches = [s.decode() for s in g["switches"]]
-    cfgd, cols, fields, n_ref, _, _ = load_turb_fixture(os.path.join(GOLD, "ref_les_tke_impl1_16x12x24_400.npz"))
-    assert (cfgd["ni"], cfgd["nj"], cfgd["nk"], cfgd["dt"], n_ref) == (ni, nj, nk, dt, nsteps)
-    m = make_gpu(cfgd, cols, fields)
-    out = str(tmp_path)
-    m.sfc_and_turb()
-    m.write_grads(out, 1, switches, iorigin=2, stretch_z=0, mtime=0.0, nstep=0)
-    m.step(nsteps)
-    m.sfc_and_turb()
-    m.write_grads(out, 2, switches, iorigin=2, stretch_z=0, mtime=nsteps * dt, nstep=nsteps)
-    print(f"LES: {nsteps} steps, output written to {out}", flush=True)
-    compare_files(g, out, ni, nj)
-    w = np.fromfile(os.path.join(out, "cm1out_w.dat"), dtype="<f4").reshape(ntimes, -1, nk + 1, nj, ni)
-    assert float(w[1, 1].max()) > 0.05 and np.array_equal(w[1, 1], g["tke"])
-    m.close()
 
 
 def compare_files(g, out, ni, nj):
