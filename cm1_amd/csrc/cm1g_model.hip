@@ -547,7 +547,7 @@ extern "C" int cm1g_get_2d_array(cm1g_model *m, int id, float *host) {
 // ------------------------------------------------------------ checkpoint
 // Arrays a step carries into the next: the time-level "a" arrays and their
 // "3d" copies (halos included), ppx, the previous step's diabatic
-// tendencies, rho/prs/rr/rf, the tracers, the scalar updates' partner
+// tendencies, rho/prs/rr/rf, the tracers, the subgrid TKE, the scalar updates' partner
 // buffer (it rotates into the scalar fields, carrying the ghost cells no
 // kernel writes), and the 2-D fields (rain, prate, and the
 // radiative-boundary speeds of the last stage, which the next step's first
@@ -574,6 +574,13 @@ static int state_ids(const cm1g_model *m, int *ids) {
   for (int t = 0; t < m->c.npt; t++) {
     ids[n++] = CM1G_PTA0 + t;
     ids[n++] = CM1G_PT3D0 + t;
+  }
+  // the subgrid TKE (the viscosities and the surface set-up are formed again
+  // from the state at the next step's start; the surface fields are among
+  // the 2-D fields)
+  if (m->turb.cm1setup == 1) {
+    ids[n++] = CM1G_TKEA;
+    ids[n++] = CM1G_TKE3D;
   }
   return n;
 }

@@ -422,8 +422,10 @@ int cm1g_budgets(cm1g_model *m, double *tmass, double *tmois);
  * layout): every array and scalar a step carries into the next, taken
  * between steps.  cm1g_state_bytes gives the size; cm1g_state_save copies
  * the state into a host buffer of that size; cm1g_state_load restores it
- * into a model created with the same configuration (checked), after which
- * stepping continues bit for bit as if uninterrupted.  Layout: an 88-byte
+ * into a model created with the same configuration (checked; a model with
+ * the subgrid turbulence also carries its TKE and surface fields, so load
+ * into a model with the same cm1g_set_turbulence), after which stepping
+ * continues bit for bit as if uninterrupted.  Layout: an 88-byte
  * header, then 3-D fields as logical boxes (cm1g_put_field order), the box
  * of the scalar updates' partner buffer, then the 2-D fields in device
  * order. */

@@ -31,6 +31,32 @@ def test_resume_is_bitwise(require_gpu, kw):
     assert a.budgets() == b.budgets()
 
 
+def test_les_resume_is_bitwise(require_gpu):
+    """The dry LES of the reference fixture, saved after 310 steps (TKE
+    active) and resumed in a fresh model with the same turbulence set-up:
+    the next 20 steps equal the uninterrupted run's bit for bit, the TKE and
+    the eddy viscosities included."""
+    import os
+    from common import load_turb_fixture
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    cfgd, cols, fields, _, _, _ = load_turb_fixture(os.path.join(gold, "ref_les_tke_impl1_16x12x24_400.npz"))
+    a = make_gpu(cfgd, cols, fields)
+    a.step(310)
+    buf = a.save_state()
+    a.step(20)
+    b = make_gpu(cfgd, cols, fields)
+    b.step(1)
+    b.load_state(buf)
+    b.step(20)
+    assert float(a.get("tkea").max()) > 1.0e-3
+    names = [n for n, _ in PROG if not n.startswith("qa")] + [f"pta{t}" for t in range(cfgd["npt"])] + \
+        ["tkea", "kmh", "khv"]
+    for n in names:
+        x, y = a.get(n), b.get(n)
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), n
+    assert a.scalars() == b.scalars()
+
+
 def test_load_refuses_other_configuration(require_gpu):
     cfgd, cols, fields = case(ni=40, nj=36, nk=30)
     a = make_gpu(cfgd, cols, fields)
