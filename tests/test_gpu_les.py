@@ -91,3 +91,40 @@ def test_les_graphs_equal_launches(require_gpu):
     assert float(interior(a.get("tkea"), nk, nj, ni, "w").max()) > 0.0
     a.close()
     b.close()
+
+
+def test_les_ensemble_equals_members_alone(require_gpu):
+    """Two LES members (the fixture's t=0 state, and the same with theta
+    perturbed below 200 m; both with an initial TKE so the scheme acts from
+    the first step) advanced together by the batched ensemble equal each
+    member stepped alone, bit for bit, the TKE and eddy viscosities
+    included."""
+    from cm1_amd import model
+    path = sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                         "ref_les_tke_impl1_*.npz")))[0]
+    cfgd, cols, fields, _, _, _ = load_turb_fixture(path)
+    nk, nj, ni = cfgd["nk"], cfgd["nj"], cfgd["ni"]
+    rng = np.random.default_rng(11)
+    tke = fields["tkea"].copy()
+    tke[2:nk // 2, 3:nj + 3, 3:ni + 3] = rng.uniform(0.05, 0.5, (nk // 2 - 2, nj, ni)).astype(np.float32)
+    f1 = dict(fields, tkea=tke)
+    tha = fields["tha"].copy()
+    tha[1:11, 3:nj + 3, 3:ni + 3] += rng.uniform(-0.2, 0.2, (10, nj, ni)).astype(np.float32)
+    f2 = dict(f1, tha=tha)
+    cases = [(cfgd, cols, f1), (cfgd, cols, f2)]
+    ens_models = [make_gpu(*c) for c in cases]
+    solo = [make_gpu(*c) for c in cases]
+    e = model.Ensemble(ens_models)
+    names = [("ua", "u"), ("va", "v"), ("wa", "w"), ("tha", None), ("ppi", None), ("rho", None),
+             ("tkea", "w"), ("kmh", "w"), ("khv", "w")] + [(f"pta{t}", None) for t in range(cfgd["npt"])]
+    for _ in range(3):
+        e.step(2)
+        for s in solo:
+            s.step(2)
+        e.sync()
+        for a, b in zip(ens_models, solo):
+            for n, stag in names:
+                x, y = a.interior(n, stag), b.interior(n, stag)
+                assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), n
+    assert float(solo[1].interior("tkea", "w").max()) > 0.01
+    e.close()
